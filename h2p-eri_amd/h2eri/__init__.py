@@ -1,0 +1,267 @@
+"""Python host mirror of the H2P-ERI C API, backed by libh2eri_amd.so.
+
+The method names follow the reference's C entry points (H2ERI_init,
+CMS_read_mol_file, H2ERI_process_shells, H2ERI_partition, H2ERI_build_H2,
+H2ERI_matvec, H2ERI_build_Coulomb, H2ERI_build_exchange).  Every compute call
+goes through the C-ABI of the HIP library; there is no Python or CPU
+fallback, and a library error raises H2ERIError.
+"""
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+
+from . import capi
+
+__all__ = ["H2ERI", "H2ERIError", "capi"]
+
+
+class H2ERIError(RuntimeError):
+    pass
+
+
+def _check(lib):
+    code = lib.h2eri_last_error()
+    if code:
+        msg = lib.h2eri_last_error_msg().decode(errors="replace")
+        lib.h2eri_clear_error()
+        raise H2ERIError(f"h2eri error {code}: {msg}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(capi.c_dbl_p)
+
+
+def _iarr(p, n):
+    if not p or n <= 0:
+        return np.zeros(0, dtype=np.int32)
+    return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+
+def _darr(p, n):
+    if not p or n <= 0:
+        return np.zeros(0, dtype=np.float64)
+    return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+
+class H2ERI:
+    """One H2 representation of the ERI tensor (handle of struct H2ERI)."""
+
+    def __init__(self, scr_tol=1e-10, ext_tol=1e-10, qr_tol=1e-6):
+        self.lib = capi.load()
+        self.h = capi.H2ERI_p()
+        self.lib.H2ERI_init(C.byref(self.h), scr_tol, ext_tol, qr_tol)
+        _check(self.lib)
+
+    # -- lifecycle --------------------------------------------------------
+    def close(self):
+        if self.h:
+            self.lib.H2ERI_destroy(self.h)
+            self.h = capi.H2ERI_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def s(self):
+        return self.h.contents
+
+    # -- construction (host) -------------------------------------------------
+    def read_mol(self, path):
+        s = self.s
+        self.lib.CMS_read_mol_file(os.fsencode(path), C.byref(C.c_int.from_buffer(s, capi.H2ERIStruct.natom.offset)),
+                                   C.byref(C.c_int.from_buffer(s, capi.H2ERIStruct.nshell.offset)),
+                                   C.cast(C.addressof(s) + capi.H2ERIStruct.shells.offset,
+                                          C.POINTER(C.POINTER(capi.Shell))))
+        _check(self.lib)
+        return self
+
+    def read_mol_text(self, text):
+        with tempfile.NamedTemporaryFile("w", suffix=".mol", delete=False) as f:
+            f.write(text)
+            path = f.name
+        try:
+            self.read_mol(path)
+        finally:
+            os.unlink(path)
+        return self
+
+    def process_shells(self):
+        self.lib.H2ERI_process_shells(self.h)
+        _check(self.lib)
+        return self
+
+    def partition(self):
+        self.lib.H2ERI_partition(self.h)
+        _check(self.lib)
+        return self
+
+    def build_h2(self, bd_jit=0):
+        self.lib.H2ERI_build_H2(self.h, bd_jit)
+        _check(self.lib)
+        return self
+
+    def build_from_mol(self, path=None, text=None):
+        if text is not None:
+            self.read_mol_text(text)
+        else:
+            self.read_mol(path)
+        return self.process_shells().partition().build_h2()
+
+    def partition_synthetic(self, coords, nbfp, max_leaf_points=300):
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        assert coords.ndim == 2 and coords.shape[0] == 3
+        nbfp = np.ascontiguousarray(nbfp, dtype=np.int32)
+        n = coords.shape[1]
+        assert nbfp.shape == (n,)
+        self.lib.H2ERI_partition_points_synthetic(self.h, n, _dptr(coords), nbfp.ctypes.data_as(capi.c_int_p),
+                                                  max_leaf_points)
+        _check(self.lib)
+        return self
+
+    def build_synthetic(self, seed=1, max_rank=40, lr_frac=0.3, diag_lr_frac=0.0):
+        self.lib.H2ERI_build_H2_synthetic(self.h, seed, max_rank, lr_frac, diag_lr_frac)
+        _check(self.lib)
+        return self
+
+    def save(self, path):
+        self.lib.H2ERI_save_rep(self.h, os.fsencode(path))
+        _check(self.lib)
+
+    @classmethod
+    def load(cls, path):
+        o = cls()
+        o.lib.H2ERI_load_rep(o.h, os.fsencode(path))
+        _check(o.lib)
+        return o
+
+    # -- device hot path -----------------------------------------------------
+    @property
+    def n(self):
+        return self.s.krnl_mat_size
+
+    @property
+    def nbf(self):
+        return self.s.num_bf
+
+    def set_precision(self, bits):
+        self.lib.H2ERI_set_precision(self.h, bits)
+        _check(self.lib)
+
+    def set_tile_bytes(self, nbytes):
+        self.lib.H2ERI_set_tile_bytes(self.h, float(nbytes))
+        _check(self.lib)
+
+    def plan_check(self, bits=64):
+        """Host-only build and bounds check of the device plan (no GPU needed)."""
+        r = self.lib.H2ERI_plan_check(self.h, bits)
+        _check(self.lib)
+        return r
+
+    def upload(self):
+        r = self.lib.H2ERI_upload(self.h)
+        _check(self.lib)
+        return r
+
+    def matvec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.shape == (self.n,)
+        y = np.empty_like(x)
+        self.lib.H2ERI_matvec(self.h, _dptr(x), _dptr(y))
+        _check(self.lib)
+        return y
+
+    def matvec_multi(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        assert X.ndim == 2 and X.shape[1] == self.n
+        Y = np.empty_like(X)
+        self.lib.H2ERI_matvec_multi(self.h, X.shape[0], _dptr(X), _dptr(Y))
+        _check(self.lib)
+        return Y
+
+    def matvec_dev(self, x_ptr, y_ptr, stream=None):
+        self.lib.H2ERI_matvec_dev(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), C.c_void_p(stream or 0))
+        _check(self.lib)
+
+    def matvec_multi_dev(self, nvec, x_ptr, y_ptr, stream=None):
+        self.lib.H2ERI_matvec_multi_dev(self.h, nvec, C.c_void_p(x_ptr), C.c_void_p(y_ptr), C.c_void_p(stream or 0))
+        _check(self.lib)
+
+    def build_coulomb(self, D):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        assert D.shape == (self.nbf, self.nbf)
+        J = np.empty_like(D)
+        self.lib.H2ERI_build_Coulomb(self.h, _dptr(D), _dptr(J))
+        _check(self.lib)
+        return J
+
+    def build_coulomb_dev(self, d_ptr, j_ptr, stream=None):
+        self.lib.H2ERI_build_Coulomb_dev(self.h, C.c_void_p(d_ptr), C.c_void_p(j_ptr), C.c_void_p(stream or 0))
+        _check(self.lib)
+
+    def build_exchange(self, D):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        assert D.shape == (self.nbf, self.nbf)
+        K = np.empty_like(D)
+        self.lib.H2ERI_build_exchange(self.h, _dptr(D), _dptr(K))
+        _check(self.lib)
+        return K
+
+    def set_phase_timing(self, on=True):
+        self.lib.H2ERI_set_phase_timing(self.h, 1 if on else 0)
+
+    def phase_ms(self):
+        out = np.zeros(7)
+        self.lib.H2ERI_get_phase_ms(self.h, _dptr(out))
+        return dict(zip(["up", "B", "y1", "down", "D", "gather", "total"], out.tolist()))
+
+    def matvec_bytes(self):
+        out = np.zeros(6)
+        self.lib.H2ERI_matvec_bytes(self.h, _dptr(out))
+        _check(self.lib)
+        return dict(zip(["alg", "ref_accounting", "U", "B", "D", "vec"], out.tolist()))
+
+    def print_statistic(self):
+        self.lib.H2ERI_print_statistic(self.h)
+
+    # -- read-only views of the host representation ---------------------------
+    def tree(self):
+        s = self.s
+        nn, L, nl = s.n_node, s.max_level + 1, s.n_leaf_node
+        return dict(
+            n_node=nn, n_leaf_node=nl, max_level=s.max_level, min_adm_level=s.min_adm_level,
+            krnl_mat_size=s.krnl_mat_size, num_sp=s.num_sp, num_bf=s.num_bf, nshell=s.nshell,
+            parent=_iarr(s.parent, nn), children=_iarr(s.children, 8 * nn).reshape(nn, 8),
+            n_child=_iarr(s.n_child, nn), node_level=_iarr(s.node_level, nn),
+            pt_cluster=_iarr(s.pt_cluster, 2 * nn).reshape(nn, 2),
+            mat_cluster=_iarr(s.mat_cluster, 2 * nn).reshape(nn, 2),
+            level_n_node=_iarr(s.level_n_node, L), level_nodes=_iarr(s.level_nodes, L * nl).reshape(L, nl),
+            height_n_node=_iarr(s.height_n_node, L), height_nodes=_iarr(s.height_nodes, L * nl).reshape(L, nl),
+            r_adm_pairs=_iarr(s.r_adm_pairs, 2 * s.n_r_adm_pair).reshape(-1, 2),
+            r_inadm_pairs=_iarr(s.r_inadm_pairs, 2 * s.n_r_inadm_pair).reshape(-1, 2),
+            sp_bfp_sidx=_iarr(s.sp_bfp_sidx, s.num_sp + 1),
+            shell_bf_sidx=_iarr(s.shell_bf_sidx, s.nshell + 1) if s.shell_bf_sidx else None,
+            sp_shell_idx=_iarr(s.sp_shell_idx, 2 * s.num_sp) if s.sp_shell_idx else None,
+        )
+
+    @staticmethod
+    def _blk(p):
+        if not p:
+            return None
+        b = p.contents
+        if b.ld >= 0:
+            return ("dense", _darr(b.data, b.nrow * b.ncol).reshape(b.nrow, b.ncol))
+        k = -b.ld
+        d = _darr(b.data, (b.nrow + b.ncol) * k)
+        return ("lowrank", d[: b.nrow * k].reshape(b.nrow, k), d[b.nrow * k:].reshape(b.ncol, k))
+
+    def blocks(self):
+        s = self.s
+        U = [self._blk(s.U[i]) for i in range(s.n_node)]
+        B = [self._blk(s.c_B_blks[i]) for i in range(s.n_B)]
+        D = [self._blk(s.c_D_blks[i]) for i in range(s.n_D)]
+        return U, B, D

@@ -1,0 +1,320 @@
+/*
+ * h2eri.h -- C-ABI of the MI355X-native H2 ERI library (libh2eri_amd.so).
+ *
+ * Drop-in boundary for H2P-ERI's J/K apply path.  Every entry point below
+ * keeps the reference's name, argument meaning and `void` return; the
+ * reference declaration it replaces is cited beside it (paths relative to the
+ * H2P-ERI source tree).  Entry points without a counterpart in the reference
+ * are marked "additive".
+ *
+ * Construction (screening, partition, proxy-point ID, B/D blocks) runs on the
+ * host in C++; the U, B and D blocks are then uploaded once into flat device
+ * arrays and every matvec runs as hand-written gfx950 HIP kernels.
+ *
+ * Errors: the reference aborts through assert().  This library keeps the void
+ * signatures; a failing call records an error code and message, returns
+ * without touching its outputs beyond what it already wrote, and the caller
+ * may query h2eri_last_error()/h2eri_last_error_msg().  There is no CPU
+ * fallback: a device error is reported, never silently replaced.
+ */
+#ifndef H2ERI_AMD_H
+#define H2ERI_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Number of Cartesian components of a shell with angular momentum am
+ * (reference: src/CMS.h:25). */
+#define NCART(am) ((am >= 0) ? ((((am) + 2) * ((am) + 1)) >> 1) : 0)
+
+/* Contracted Gaussian shell.  Field names and meaning follow the fields of
+ * the SIMINT shell struct that H2P-ERI reads (src/CMS.c:91-108,
+ * src/H2ERI_shell_operations.c:159-171): after normalisation `coef[k]`
+ * already contains the primitive and contraction normalisation, so the x^am
+ * Cartesian component is  sum_k coef[k] * x^am * exp(-alpha[k] r^2). */
+typedef struct h2eri_shell {
+    int     am;
+    int     nprim;
+    double  x, y, z;
+    double *alpha;
+    double *coef;
+} shell_t;
+
+/* Dense matrix as used by the reference (src/H2ERI_aux_structs.h:150-157).
+ * ld > 0: dense row-major nrow x ncol.  ld < 0: low-rank block M = U*V of
+ * rank k = -ld stored as [U (nrow x k) | V^T (ncol x k)], both row-major
+ * (src/H2ERI_build_H2.c:876-901). */
+typedef struct H2E_dense_mat {
+    int     nrow;
+    int     ncol;
+    int     ld;
+    int     size;
+    double *data;
+} *H2E_dense_mat_p;
+
+typedef struct H2E_int_vec {
+    int  capacity;
+    int  length;
+    int *data;
+} *H2E_int_vec_p;
+
+/* Library state.  The leading fields keep the names, order and meaning of
+ * src/H2ERI_typedef.h:14-130 for everything a caller of the reference reads
+ * or writes (natom, nshell, shells, num_bf, n_matvec, mat_size, timers; see
+ * tests/test_J_input_mat.c:22,34,47-48).  SIMINT work buffers of the
+ * reference are replaced by opaque pointers; device state is appended at the
+ * end (the library allocates the struct, so appending is ABI-safe). */
+struct H2ERI {
+    int    natom;
+    int    nshell;
+    int    max_am;
+    int    max_shell_nbf;
+    int    num_bf;
+    int    num_sp;
+    int    num_sp_bfp;
+    int    pp_npts_layer;
+    int    pp_nlayer_ext;
+    int   *shell_bf_sidx;
+    int   *sp_nbfp;
+    int   *sp_bfp_sidx;
+    int   *sp_shell_idx;
+    int   *index_seq;
+    int   *node_adm_pairs;
+    int   *node_adm_pairs_sidx;
+    int   *node_inadm_pairs;
+    int   *node_inadm_pairs_sidx;
+    int   *plist;
+    int   *plist_idx;
+    int   *plist_sidx;
+    int   *dlist;
+    int   *dlist_sidx;
+    void **thread_Kmat_workbuf;
+    double scr_tol;
+    double ext_tol;
+    double *sp_center;
+    double *sp_extent;
+    double *box_extent;
+    double *unc_denmat_x;
+    double *H2_matvec_y;
+    shell_t *shells;
+    shell_t *sp_shells;
+    void    *sp;                    /* reference: SIMINT shell pairs; unused */
+    H2E_int_vec_p *J_pair;
+    H2E_int_vec_p *J_row;
+    H2E_int_vec_p *ovlp_ff_idx;
+    void    *simint_buffs;          /* reference: SIMINT buffers; unused */
+    void    *eri_batch_buffs;       /* reference: ERI batch buffers; unused */
+    H2E_dense_mat_p *c_B_blks;
+    H2E_dense_mat_p *c_D_blks;
+    void    *thread_buffs;
+
+    int    n_thread;
+    int    pt_dim;
+    int    xpt_dim;
+    int    krnl_dim;
+    int    QR_stop_type;
+    int    QR_stop_rank;
+    int    n_point;
+    int    krnl_mat_size;
+    int    max_leaf_points;
+    int    n_node;
+    int    root_idx;
+    int    n_leaf_node;
+    int    max_child;
+    int    max_neighbor;
+    int    max_level;
+    int    min_adm_level;
+    int    n_r_inadm_pair;
+    int    n_r_adm_pair;
+    int    n_UJ;
+    int    n_B;
+    int    n_D;
+    int    BD_JIT;
+    int    print_timers;
+    int    print_dbginfo;
+    int   *parent;
+    int   *children;
+    int   *pt_cluster;
+    int   *mat_cluster;
+    int   *n_child;
+    int   *node_level;
+    int   *node_height;
+    int   *level_n_node;
+    int   *level_nodes;
+    int   *height_n_node;
+    int   *height_nodes;
+    int   *r_inadm_pairs;
+    int   *r_adm_pairs;
+    int   *node_inadm_lists;
+    int   *node_n_r_inadm;
+    int   *node_n_r_adm;
+    int   *coord_idx;
+    int   *B_p2i_rowptr;
+    int   *B_p2i_colidx;
+    int   *B_p2i_val;
+    int   *D_p2i_rowptr;
+    int   *D_p2i_colidx;
+    int   *D_p2i_val;
+    int   *B_nrow;
+    int   *B_ncol;
+    int   *D_nrow;
+    int   *D_ncol;
+    size_t *B_ptr;
+    size_t *D_ptr;
+    double max_leaf_size;
+    double QR_stop_tol;
+    double *coord;
+    double *coord0;
+    double *enbox;
+    double *root_enbox;
+    double *B_data;
+    double *D_data;
+    void   *B_blk;
+    void   *D_blk0;
+    void   *D_blk1;
+    H2E_dense_mat_p *U;
+    H2E_dense_mat_p *y0;
+    H2E_dense_mat_p *y1;
+    int    n_matvec;
+    size_t mat_size[11];
+    double timers[11];
+
+    /* ---- appended: MI355X device state (opaque) ---- */
+    void  *dev;
+};
+typedef struct H2ERI *H2ERI_p;
+
+/* mat_size[] / timers[] indices (src/H2ERI_typedef.h:134-158). */
+enum { U_SIZE_IDX = 0, B_SIZE_IDX, D_SIZE_IDX, MV_FWD_SIZE_IDX, MV_MID_SIZE_IDX,
+       MV_BWD_SIZE_IDX, MV_DEN_SIZE_IDX, MV_VOP_SIZE_IDX };
+enum { PT_TIMER_IDX = 0, U_BUILD_TIMER_IDX, B_BUILD_TIMER_IDX, D_BUILD_TIMER_IDX,
+       MV_FWD_TIMER_IDX, MV_MID_TIMER_IDX, MV_BWD_TIMER_IDX, MV_DEN_TIMER_IDX,
+       MV_VOP_TIMER_IDX };
+
+/* ------------------------------------------------------------------ */
+/* Lifecycle (replaces src/H2ERI_typedef.h:169-179, src/H2ERI_typedef.c:303) */
+void H2ERI_init(H2ERI_p *h2eri_, const double scr_tol, const double ext_tol, const double QR_tol);
+void H2ERI_destroy(H2ERI_p h2eri);
+void H2ERI_print_statistic(H2ERI_p h2eri);
+void H2E_reset_timers(H2ERI_p h2eri);
+
+/* Molecule input (replaces src/CMS.h:49): reads the .mol format, normalises
+ * the shells. */
+void CMS_read_mol_file(const char *mol_fname, int *natom_, int *nshell_, shell_t **shells_);
+void CMS_destroy_shells(const int nshell, shell_t *shells);
+
+/* Host construction (replace src/H2ERI_shell_operations.h:25,
+ * src/H2ERI_partition.h:28, src/H2ERI_build_H2.h:16).  BD_JIT must be 0:
+ * this library stores B and D blocks (the JIT mode of the reference is not
+ * provided and is rejected with an error). */
+void H2ERI_process_shells(H2ERI_p h2eri);
+void H2ERI_partition(H2ERI_p h2eri);
+void H2ERI_build_H2(H2ERI_p h2eri, const int BD_JIT);
+
+/* ------------------------------------------------------------------ */
+/* Hot path.  Each call runs on the device; host pointers in, host pointers
+ * out, exactly as the reference.                                       */
+
+/* y := A_H2 x, x and y of length krnl_mat_size
+ * (replaces src/H2ERI_matvec.h:16). */
+void H2ERI_matvec(H2ERI_p h2eri, const double *x, double *y);
+
+/* J := contract(A_H2 uncontract(D)), D and J num_bf x num_bf row-major
+ * (replaces src/H2ERI_build_Coulomb.h:16). */
+void H2ERI_build_Coulomb(H2ERI_p h2eri, const double *den_mat, double *J_mat);
+
+/* K_mq := sum_{n,p} (mn|pq) D_np through the H2 representation
+ * (replaces src/H2ERI_build_exchange.h:16). */
+void H2ERI_build_exchange(H2ERI_p h2eri, const double *den_mat, double *K_mat);
+/* Kept for ABI compatibility (src/H2ERI_build_exchange.h:19-22); the
+ * library manages its exchange work space itself. */
+void H2ERI_exchange_workbuf_init(H2ERI_p h2eri);
+void H2ERI_exchange_workbuf_free(H2ERI_p h2eri);
+
+/* ------------------------------------------------------------------ */
+/* Additive entry points (no counterpart in the reference).             */
+
+/* Last error code (0 = none) and message of the calling thread. */
+int         h2eri_last_error(void);
+const char *h2eri_last_error_msg(void);
+void        h2eri_clear_error(void);
+
+/* Storage precision of the far-field U and B blocks on the device:
+ * 64 (default, the drop-in) or 32 (FP32-stored U/B, FP64 D and FP64
+ * accumulation).  Re-uploads U/B if the representation is resident. */
+void H2ERI_set_precision(H2ERI_p h2eri, const int ub_bits);
+
+/* Y := A_H2 X for nvec vectors stored column after column (X[k*N + i],
+ * N = krnl_mat_size).  Matrix blocks are streamed once per call. */
+void H2ERI_matvec_multi(H2ERI_p h2eri, const int nvec, const double *X, double *Y);
+
+/* J_k := contract(A_H2 uncontract(D_k)) for ndens densities stored one
+ * after another (each num_bf^2). */
+void H2ERI_build_Coulomb_multi(H2ERI_p h2eri, const int ndens, const double *den_mats, double *J_mats);
+
+/* Device-resident forms: x_dev / y_dev (and D/J) are device pointers on the
+ * library's current device; the call enqueues on `stream` (a hipStream_t,
+ * NULL = default stream) and does not synchronise. */
+void H2ERI_matvec_dev(H2ERI_p h2eri, const double *x_dev, double *y_dev, void *stream);
+void H2ERI_matvec_multi_dev(H2ERI_p h2eri, const int nvec, const double *X_dev, double *Y_dev, void *stream);
+void H2ERI_build_Coulomb_dev(H2ERI_p h2eri, const double *den_dev, double *J_dev, void *stream);
+
+/* Upload the stored representation to the device now (otherwise done by the
+ * first hot-path call).  Returns bytes resident on the device. */
+size_t H2ERI_upload(H2ERI_p h2eri);
+
+/* Versioned binary serialisation of a built representation (the
+ * reference has none).  Load replaces any construction. */
+int H2ERI_save_rep(H2ERI_p h2eri, const char *fname);
+int H2ERI_load_rep(H2ERI_p h2eri, const char *fname);
+
+/* Fill a partitioned handle with seeded synthetic (chemistry-free) U, B and
+ * D blocks instead of ERIs: for kernel tests and benchmarks of tree shapes
+ * independent of integrals.  lr_frac in [0,1] is the fraction of B and
+ * off-diagonal D blocks stored low-rank, diag_lr_frac that of the diagonal
+ * D blocks (dense diagonal blocks are symmetric). */
+void H2ERI_build_H2_synthetic(H2ERI_p h2eri, const unsigned long long seed,
+                              const int max_rank, const double lr_frac,
+                              const double diag_lr_frac);
+/* Partition a point cloud directly (synthetic trees): npts shell pairs at
+ * coord (3 x npts, column-major as the reference's sp_center), with nbfp[i]
+ * basis-function pairs each. */
+void H2ERI_partition_points_synthetic(H2ERI_p h2eri, const int npts, const double *coord,
+                                      const int *nbfp, const int max_leaf_points);
+
+/* Per-phase device timing of the last hot-path call, milliseconds:
+ * [0] upward U^T sweep, [1] B coupling, [2] y1 reduce, [3] downward U sweep,
+ * [4] near-field D, [5] output gather, [6] total. Collected only when
+ * H2ERI_set_phase_timing(h2eri, 1). */
+void H2ERI_set_phase_timing(H2ERI_p h2eri, const int on);
+void H2ERI_get_phase_ms(H2ERI_p h2eri, double *ms7);
+
+/* Algorithmic bytes moved by one single-vector matvec on the device with
+ * the current precision (U twice, B once, D once with packed diagonal
+ * blocks, vectors), and the reference-accounting bytes
+ * 8*(MV_FWD+MV_MID+MV_BWD+MV_DEN) of src/H2ERI_build_H2.c:801-805,1129-1130,
+ * 1369-1381.  out[0]=alg, out[1]=ref-accounting, out[2..5]=U,B,D,vector
+ * bytes of the algorithmic count. */
+void H2ERI_matvec_bytes(H2ERI_p h2eri, double *out6);
+
+/* Tile size of the device bi-GEMV tasks in bytes (tuning; re-plans the
+ * device layout on the next call). */
+void H2ERI_set_tile_bytes(H2ERI_p h2eri, const double bytes);
+
+/* Builds the device execution plan on the host only (no HIP call) and
+ * checks every access of every launch against its buffer.  Returns the
+ * number of tile tasks, -1 on failure. */
+int H2ERI_plan_check(H2ERI_p h2eri, const int ub_bits);
+
+/* sizeof(struct H2ERI) as compiled into the library (ABI check for
+ * bindings that mirror the struct). */
+size_t h2eri_struct_size(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* H2ERI_AMD_H */

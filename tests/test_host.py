@@ -1,0 +1,75 @@
+"""CPU tests of the library boundary and host logic (no GPU calls)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, make_synthetic
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "h2eri.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:[A-Za-z_][\w ]*\*?\s*\**)\b([A-Za-z_]\w*)\s*\(", text, flags=re.M))
+                  - {"NCART"})
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    import h2eri
+    lib = h2eri.capi.load()
+    names = _declared_functions()
+    assert "H2ERI_matvec" in names and "H2ERI_build_Coulomb" in names and "H2ERI_build_exchange" in names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) <= set(h2eri.capi.SIGNATURES), set(names) - set(h2eri.capi.SIGNATURES)
+    assert lib.h2eri_struct_size() == ctypes.sizeof(h2eri.capi.H2ERIStruct)
+
+
+def test_struct_leading_fields_match_reference_order():
+    # callers of the reference write natom/nshell/shells and read num_bf, n_matvec, timers
+    import h2eri
+    S = h2eri.capi.H2ERIStruct
+    order = [f[0] for f in S._fields_]
+    assert order[:9] == ["natom", "nshell", "max_am", "max_shell_nbf", "num_bf", "num_sp", "num_sp_bfp",
+                         "pp_npts_layer", "pp_nlayer_ext"]
+    assert order[-4:] == ["n_matvec", "mat_size", "timers", "dev"]
+
+
+@pytest.mark.parametrize("seed,n,leaf,mr,shape", [(1, 900, 30, 12, "chain"), (2, 1200, 150, 40, "chain"),
+                                                  (3, 2500, 200, 700, "chain"), (4, 1500, 40, 20, "cube"),
+                                                  (5, 40, 300, 8, "chain")])
+def test_device_plan_bounds(tmp_path, seed, n, leaf, mr, shape):
+    h = make_synthetic(tmp_path / "r.rep", n, seed, leaf=leaf, max_rank=mr, lr_frac=0.4, diag_lr_frac=0.3,
+                       shape=shape)
+    for bits in (64, 32):
+        assert h.plan_check(bits) > 0
+
+
+def test_save_load_roundtrip(tmp_path):
+    import h2eri
+    h = make_synthetic(tmp_path / "a.rep", 700, 9, leaf=25, max_rank=10, lr_frac=0.5, diag_lr_frac=0.5)
+    h2 = h2eri.H2ERI.load(str(tmp_path / "a.rep"))
+    h2.save(str(tmp_path / "b.rep"))
+    assert open(tmp_path / "a.rep", "rb").read() == open(tmp_path / "b.rep", "rb").read()
+    t1, t2 = h.tree(), h2.tree()
+    for k in ("parent", "children", "mat_cluster", "r_adm_pairs", "r_inadm_pairs", "level_nodes"):
+        assert np.array_equal(t1[k], t2[k]), k
+
+
+def test_errors_are_reported_not_aborting(tmp_path):
+    import h2eri
+    h = h2eri.H2ERI()
+    with pytest.raises(h2eri.H2ERIError):
+        h.read_mol(str(tmp_path / "missing.mol"))
+    with pytest.raises(h2eri.H2ERIError):
+        h.partition()
+    p = tmp_path / "trunc.rep"
+    make_synthetic(tmp_path / "full.rep", 300, 1, leaf=20)
+    p.write_bytes(open(tmp_path / "full.rep", "rb").read()[:5000])
+    with pytest.raises(h2eri.H2ERIError):
+        h2eri.H2ERI.load(str(p))
+    h3 = make_synthetic(tmp_path / "x.rep", 300, 2, leaf=20)
+    with pytest.raises(h2eri.H2ERIError):
+        h3.build_h2(bd_jit=1)
