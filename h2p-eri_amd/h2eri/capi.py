@@ -112,6 +112,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    # One HIP runtime per process: when PyTorch is present its libamdhip64
+    # must be loaded first, so that the library's libamdhip64.so.7 dependency
+    # resolves to the same runtime (PyTorch's own libraries ask for
+    # "libamdhip64.so", which would otherwise map a second runtime).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(lib, name)
