@@ -48,11 +48,15 @@ static_assert(sizeof(BTask) == 48, "BTask layout");
 
 struct RTask {
     long long dst;   // offset into workspace, or into the external output
-    int len;
+    int len;         // <= 64 elements (one chunk of a destination segment)
     int src_beg, src_end;
     int flags;       // bit 0: destination is the external output vector;
                      // bit 1: accumulate (dst += sum) instead of storing
+    int soff;        // element offset of this chunk inside every source slot
+    int pad_;
 };
+static_assert(sizeof(RTask) == 32, "RTask layout");
+static const int RCHUNK = 64;       // destination elements per reduce workgroup
 
 static const int WG = 256;          // threads per workgroup
 static const int COLS_PER_LANE = 8; // columns a lane holds in registers
@@ -146,7 +150,9 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, double *__re
     }
 }
 
-// One launch runs n64 FP64 tasks followed by n32 FP32 tasks.
+// One launch runs n64 FP64 tasks followed by n32 FP32 tasks.  PH only tags
+// the phase, so that each phase is a separately named kernel in profiles.
+template <int PH>
 __global__ __launch_bounds__(WG) void bgemv_kernel(const BTask *__restrict__ t64, int n64, const BTask *__restrict__ t32,
                                                    int n32, const double *__restrict__ S64,
                                                    const float *__restrict__ S32, double *__restrict__ Wv) {
@@ -161,16 +167,35 @@ __global__ __launch_bounds__(WG) void bgemv_kernel(const BTask *__restrict__ t64
     }
 }
 
-// Segmented fixed-order sum of partial slots into each destination.
+// Segmented fixed-order sum of partial slots into each destination chunk.
+// Lane i owns element i of the chunk; wave w sums the sources w, w+4, ...
+// (four loads in flight per lane), then the four wave sums are added in
+// wave order, so the result is deterministic.
 __global__ __launch_bounds__(WG) void reduce_kernel(const RTask *__restrict__ rt, const int *__restrict__ src,
                                                     double *__restrict__ Wv, double *__restrict__ ext) {
+    __shared__ double red[4][RCHUNK];
     const RTask t = rt[blockIdx.x];
-    double *dst = (t.flags & 1) ? (ext + t.dst) : (Wv + t.dst);
-    for (int i = threadIdx.x; i < t.len; i += WG) {
-        double s = 0.0;
-        for (int j = t.src_beg; j < t.src_end; j++) s += Wv[src[j] + i];
-        if (t.flags & 2) dst[i] += s;
-        else dst[i] = s;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double s = 0.0;
+    if (lane < t.len) {
+        const double *base = Wv + t.soff + lane;
+        int j = t.src_beg + w;
+        for (; j + 12 < t.src_end; j += 16) {
+            const double a0 = base[src[j]], a1 = base[src[j + 4]], a2 = base[src[j + 8]], a3 = base[src[j + 12]];
+            s += a0;
+            s += a1;
+            s += a2;
+            s += a3;
+        }
+        for (; j < t.src_end; j += 4) s += base[src[j]];
+    }
+    red[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && lane < t.len) {
+        const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+        double *dst = ((t.flags & 1) ? ext : Wv) + t.dst + lane;
+        if (t.flags & 2) *dst += v;
+        else *dst = v;
     }
 }
 
@@ -228,6 +253,15 @@ struct Stage {
     int phase;
     int t64_beg, t64_cnt, t32_beg, t32_cnt;
     int r_beg, r_cnt;
+    double mat_bytes;   // matrix bytes streamed by this stage's tile launch
+};
+
+// One timed launch of the last matvec (phase timing mode).
+struct LaunchRec {
+    int phase, kind;    // kind 0: bi-GEMV tiles, 1: segmented reduce
+    int blocks;
+    double mat_bytes;
+    float ms;
 };
 
 struct Segment {
@@ -266,8 +300,9 @@ struct DevState {
     double *d_xio = nullptr;    // host-pointer path staging
     size_t xio_len = 0;
     bool phase_timing = false;
-    hipEvent_t ev[PH_N + 1][2];
-    bool ev_made = false;
+    std::vector<hipEvent_t> evs;
+    std::vector<LaunchRec> launches;
+    bool collected = true;   // launch times of the last timed call already read
     double phase_ms[7] = {0, 0, 0, 0, 0, 0, 0};
     size_t resident_bytes = 0;
 };
@@ -302,8 +337,7 @@ void device_release(H2ERI_p h) {
     DevState *d = dstate(h);
     if (!d) return;
     release_buffers(d);
-    if (d->ev_made)
-        for (int i = 0; i <= PH_N; i++) { (void)hipEventDestroy(d->ev[i][0]); (void)hipEventDestroy(d->ev[i][1]); }
+    for (hipEvent_t e : d->evs) (void)hipEventDestroy(e);
     delete d;
     h->dev = nullptr;
 }
@@ -415,6 +449,9 @@ struct Builder {
         std::stable_sort(a32.begin(), a32.end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
         Stage s;
         s.phase = phase;
+        s.mat_bytes = 0.0;
+        for (const BTask &t : a64) s.mat_bytes += 8.0 * t.nrow * t.ncol;
+        for (const BTask &t : a32) s.mat_bytes += 4.0 * t.nrow * t.ncol;
         s.t64_beg = (int)P.t64.size();
         s.t64_cnt = (int)a64.size();
         s.t32_beg = (int)P.t32.size();
@@ -425,14 +462,19 @@ struct Builder {
         for (int sid : rsegs) {
             Segment &g = segs[sid];
             if (g.len <= 0) continue;
-            RTask r;
-            r.dst = g.off;
-            r.len = g.len;
-            r.src_beg = (int)P.rsrc.size();
+            const int sb = (int)P.rsrc.size();
             P.rsrc.insert(P.rsrc.end(), g.srcs.begin(), g.srcs.end());
-            r.src_end = (int)P.rsrc.size();
-            r.flags = (g.ext ? 1 : 0) | (g.acc ? 2 : 0);
-            P.rt.push_back(r);
+            for (int e0 = 0; e0 < g.len; e0 += RCHUNK) {
+                RTask r;
+                r.dst = g.off + e0;
+                r.len = std::min(RCHUNK, g.len - e0);
+                r.src_beg = sb;
+                r.src_end = (int)P.rsrc.size();
+                r.flags = (g.ext ? 1 : 0) | (g.acc ? 2 : 0);
+                r.soff = e0;
+                r.pad_ = 0;
+                P.rt.push_back(r);
+            }
         }
         s.r_cnt = (int)P.rt.size() - s.r_beg;
         if (s.t64_cnt + s.t32_cnt + s.r_cnt > 0) P.stages.push_back(s);
@@ -639,9 +681,11 @@ struct Builder {
         if (!chk_tasks(P.t64, (long long)host64.size(), 2) || !chk_tasks(P.t32, (long long)host32.size(), 4))
             return false;
         for (const RTask &r : P.rt) {
-            bool ok = r.len > 0 && r.dst >= 0 && r.src_beg >= 0 && r.src_end <= (int)P.rsrc.size() &&
-                      r.src_beg <= r.src_end && r.dst + r.len <= ((r.flags & 1) ? N : P.nW);
-            for (int j = r.src_beg; ok && j < r.src_end; j++) ok = P.rsrc[j] >= 0 && P.rsrc[j] + r.len <= P.nW;
+            bool ok = r.len > 0 && r.len <= RCHUNK && r.dst >= 0 && r.soff >= 0 && r.src_beg >= 0 &&
+                      r.src_end <= (int)P.rsrc.size() && r.src_beg <= r.src_end &&
+                      r.dst + r.len <= ((r.flags & 1) ? N : P.nW);
+            for (int j = r.src_beg; ok && j < r.src_end; j++)
+                ok = P.rsrc[j] >= 0 && P.rsrc[j] + r.soff + r.len <= P.nW;
             if (!ok) { set_error(ERR_ARG, "device plan check failed: reduce dst=%lld len=%d", r.dst, r.len); return false; }
         }
         for (const Stage &s : P.stages) {
@@ -657,13 +701,12 @@ struct Builder {
 
 }  // namespace
 
-static bool ensure_events(DevState *d) {
-    if (d->ev_made) return true;
-    for (int i = 0; i <= PH_N; i++) {
-        HIP_OK(hipEventCreate(&d->ev[i][0]));
-        HIP_OK(hipEventCreate(&d->ev[i][1]));
+static bool ensure_events(DevState *d, size_t n) {
+    while (d->evs.size() < n) {
+        hipEvent_t e;
+        HIP_OK(hipEventCreate(&e));
+        d->evs.push_back(e);
     }
-    d->ev_made = true;
     return true;
 }
 
@@ -711,46 +754,80 @@ static bool upload(H2ERI_p h) {
     return true;
 }
 
-// Enqueue one matvec: x already at Wv[0..N), result written to y_dev.
+static void launch_bgemv(int phase, int nb, const Stage &st, DevState *d, hipStream_t s) {
+    const BTask *t64 = d->d_t64 + st.t64_beg, *t32 = d->d_t32 + st.t32_beg;
+    switch (phase) {
+    case PH_UP:
+        hipLaunchKernelGGL(bgemv_kernel<PH_UP>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
+        break;
+    case PH_B:
+        hipLaunchKernelGGL(bgemv_kernel<PH_B>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
+        break;
+    case PH_Y1:
+        hipLaunchKernelGGL(bgemv_kernel<PH_Y1>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
+        break;
+    case PH_DOWN:
+        hipLaunchKernelGGL(bgemv_kernel<PH_DOWN>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
+        break;
+    case PH_D:
+        hipLaunchKernelGGL(bgemv_kernel<PH_D>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
+        break;
+    default:
+        hipLaunchKernelGGL(bgemv_kernel<PH_GATHER>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
+        break;
+    }
+}
+
+// Enqueue one matvec: x already at Wv[0..N), result written to y_dev.  In
+// phase-timing mode an event is recorded around every launch.
 static bool run_matvec(H2ERI_p h, DevState *d, double *y_dev, hipStream_t s) {
+    (void)h;
     const Plan &P = d->plan;
-    bool tim = d->phase_timing;
-    if (tim && !ensure_events(d)) return false;
-    int cur_phase = -1;
+    const bool tim = d->phase_timing;
+    if (tim) {
+        if (!ensure_events(d, 2 * P.stages.size() + 1)) return false;
+        d->launches.clear();
+        d->collected = false;
+        HIP_OK(hipEventRecord(d->evs[0], s));
+    }
     for (size_t si = 0; si < P.stages.size(); si++) {
         const Stage &st = P.stages[si];
-        if (tim && st.phase != cur_phase) {
-            if (cur_phase >= 0) HIP_OK(hipEventRecord(d->ev[cur_phase][1], s));
-            HIP_OK(hipEventRecord(d->ev[st.phase][0], s));
-            cur_phase = st.phase;
-        }
-        int nb = st.t64_cnt + st.t32_cnt;
+        const int nb = st.t64_cnt + st.t32_cnt;
         if (nb > 0) {
-            hipLaunchKernelGGL(bgemv_kernel, dim3(nb), dim3(WG), 0, s, d->d_t64 + st.t64_beg, st.t64_cnt,
-                               d->d_t32 + st.t32_beg, st.t32_cnt, d->S64, d->S32, d->Wv);
+            launch_bgemv(st.phase, nb, st, d, s);
             HIP_OK(hipGetLastError());
+            if (tim) {
+                d->launches.push_back(LaunchRec{st.phase, 0, nb, st.mat_bytes, 0.f});
+                HIP_OK(hipEventRecord(d->evs[d->launches.size()], s));
+            }
         }
         if (st.r_cnt > 0) {
             hipLaunchKernelGGL(reduce_kernel, dim3(st.r_cnt), dim3(WG), 0, s, d->d_rt + st.r_beg, d->d_rsrc, d->Wv, y_dev);
             HIP_OK(hipGetLastError());
+            if (tim) {
+                d->launches.push_back(LaunchRec{st.phase, 1, st.r_cnt, 0.0, 0.f});
+                HIP_OK(hipEventRecord(d->evs[d->launches.size()], s));
+            }
         }
     }
-    if (tim && cur_phase >= 0) HIP_OK(hipEventRecord(d->ev[cur_phase][1], s));
     return true;
 }
 
+// Per-launch and per-phase milliseconds of the last timed call (the stream
+// must have been synchronised).
 static void collect_phase_ms(H2ERI_p h, DevState *d) {
-    if (!d->phase_timing || !d->ev_made) return;
-    double tot = 0;
-    for (int p = 0; p < PH_N; p++) {
+    if (!d->phase_timing || d->launches.empty() || d->collected) return;
+    d->collected = true;
+    for (int p = 0; p < 7; p++) d->phase_ms[p] = 0.0;
+    for (size_t i = 0; i < d->launches.size(); i++) {
         float ms = 0.f;
-        bool used = false;
-        for (const Stage &st : d->plan.stages) if (st.phase == p) used = true;
-        if (used && hipEventElapsedTime(&ms, d->ev[p][0], d->ev[p][1]) == hipSuccess) d->phase_ms[p] = ms;
-        else d->phase_ms[p] = 0.0;
-        tot += d->phase_ms[p];
+        if (hipEventSynchronize(d->evs[i + 1]) != hipSuccess ||
+            hipEventElapsedTime(&ms, d->evs[i], d->evs[i + 1]) != hipSuccess)
+            ms = 0.f;
+        d->launches[i].ms = ms;
+        d->phase_ms[d->launches[i].phase] += ms;
+        d->phase_ms[6] += ms;
     }
-    d->phase_ms[6] = tot;
     // reference timer slots (seconds): forward, intermediate, backward, dense, vector ops
     h->timers[MV_FWD_TIMER_IDX] += d->phase_ms[PH_UP] * 1e-3;
     h->timers[MV_MID_TIMER_IDX] += (d->phase_ms[PH_B] + d->phase_ms[PH_Y1]) * 1e-3;
@@ -886,7 +963,24 @@ void H2ERI_set_phase_timing(H2ERI_p h, const int on) {
 void H2ERI_get_phase_ms(H2ERI_p h, double *ms7) {
     if (!h || !ms7) return;
     DevState *d = dstate_get(h);
+    collect_phase_ms(h, d);
     for (int i = 0; i < 7; i++) ms7[i] = d->phase_ms[i];
+}
+
+int H2ERI_get_launch_stats(H2ERI_p h, const int max_launch, double *out5) {
+    if (!h) return 0;
+    DevState *d = dstate_get(h);
+    collect_phase_ms(h, d);
+    const int n = (int)d->launches.size();
+    for (int i = 0; i < n && i < max_launch && out5; i++) {
+        const LaunchRec &r = d->launches[i];
+        out5[5 * i + 0] = r.phase;
+        out5[5 * i + 1] = r.kind;
+        out5[5 * i + 2] = r.blocks;
+        out5[5 * i + 3] = r.mat_bytes;
+        out5[5 * i + 4] = r.ms;
+    }
+    return n;
 }
 
 void H2ERI_matvec_bytes(H2ERI_p h, double *out6) {

@@ -219,6 +219,15 @@ class H2ERI:
         self.lib.H2ERI_get_phase_ms(self.h, _dptr(out))
         return dict(zip(["up", "B", "y1", "down", "D", "gather", "total"], out.tolist()))
 
+    def launch_stats(self):
+        """Per-launch records of the last timed call (phase timing on)."""
+        n = self.lib.H2ERI_get_launch_stats(self.h, 0, None)
+        out = np.zeros(5 * max(n, 1))
+        n = self.lib.H2ERI_get_launch_stats(self.h, n, _dptr(out))
+        names = ["up", "B", "y1", "down", "D", "gather"]
+        return [dict(phase=names[int(out[5 * i])], kind=("tiles" if out[5 * i + 1] == 0 else "reduce"),
+                     blocks=int(out[5 * i + 2]), mat_bytes=out[5 * i + 3], ms=out[5 * i + 4]) for i in range(n)]
+
     def matvec_bytes(self):
         out = np.zeros(6)
         self.lib.H2ERI_matvec_bytes(self.h, _dptr(out))

@@ -96,6 +96,7 @@ SIGNATURES = {
     "H2ERI_partition_points_synthetic": (None, [H2ERI_p, C.c_int, c_dbl_p, c_int_p, C.c_int]),
     "H2ERI_set_phase_timing": (None, [H2ERI_p, C.c_int]),
     "H2ERI_get_phase_ms": (None, [H2ERI_p, c_dbl_p]),
+    "H2ERI_get_launch_stats": (C.c_int, [H2ERI_p, C.c_int, c_dbl_p]),
     "H2ERI_matvec_bytes": (None, [H2ERI_p, c_dbl_p]),
     "H2ERI_set_tile_bytes": (None, [H2ERI_p, C.c_double]),
     "h2eri_struct_size": (C.c_size_t, []),

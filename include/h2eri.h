@@ -291,6 +291,11 @@ void H2ERI_partition_points_synthetic(H2ERI_p h2eri, const int npts, const doubl
  * H2ERI_set_phase_timing(h2eri, 1). */
 void H2ERI_set_phase_timing(H2ERI_p h2eri, const int on);
 void H2ERI_get_phase_ms(H2ERI_p h2eri, double *ms7);
+/* Per-launch records of the last timed call: for launch i, out5[5i..5i+4] =
+ * phase (0 up, 1 B, 2 y1, 3 down, 4 near field, 5 gather), kind (0 tile
+ * bi-GEMV, 1 segmented reduce), workgroups, matrix bytes streamed, ms.
+ * Returns the number of launches (fills at most max_launch). */
+int H2ERI_get_launch_stats(H2ERI_p h2eri, const int max_launch, double *out5);
 
 /* Algorithmic bytes moved by one single-vector matvec on the device with
  * the current precision (U twice, B once, D once with packed diagonal
