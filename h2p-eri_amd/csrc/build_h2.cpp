@@ -108,7 +108,15 @@ H2E_dense_mat_p compress_blk(int nrow, int ncol, std::vector<double> &blk, doubl
     std::vector<double> U;
     std::vector<int> J;
     if (nrow > 0 && ncol > 0) id_compress_rows(nrow, ncol, A.data(), tol, U, J);
-    const int r = (int)J.size();
+    int r = (int)J.size();
+    if (nrow > 0 && ncol > 0 && r == 0) {
+        // numerically zero block: store an exact rank-1 zero factorisation
+        // (a rank-0 block would read as ld = -0 = 0, i.e. dense)
+        H2E_dense_mat_p z = dmat_new(1, nrow + ncol);
+        z->nrow = nrow; z->ncol = ncol; z->ld = -1;
+        memset(z->data, 0, sizeof(double) * (size_t)(nrow + ncol));
+        return z;
+    }
     const long long old_size = (long long)nrow * ncol;
     const long long new_size = (long long)(nrow + ncol) * r;
     H2E_dense_mat_p res;

@@ -150,9 +150,7 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, double *__re
     }
 }
 
-// One launch runs n64 FP64 tasks followed by n32 FP32 tasks.  PH only tags
-// the phase, so that each phase is a separately named kernel in profiles.
-template <int PH>
+// One launch runs n64 FP64 tasks followed by n32 FP32 tasks.
 __global__ __launch_bounds__(WG) void bgemv_kernel(const BTask *__restrict__ t64, int n64, const BTask *__restrict__ t32,
                                                    int n32, const double *__restrict__ S64,
                                                    const float *__restrict__ S32, double *__restrict__ Wv) {
@@ -362,7 +360,6 @@ struct Builder {
     std::vector<double> host64;
     std::vector<float> host32;
     std::vector<Segment> segs;
-    std::vector<int> stage_tasks64, stage_tasks32;   // scratch per stage
 
     Builder(H2ERI_p h_, Plan &p_, int bits, double tb) : h(h_), P(p_), ub_bits(bits), tile_bytes(tb) {}
 
@@ -442,45 +439,107 @@ struct Builder {
         }
     }
 
-    void flush_stage(int phase, std::vector<BTask> &a64, std::vector<BTask> &a32, std::vector<int> &rsegs) {
-        // largest tiles first for load balance
-        auto key = [](const BTask &t) { return (long long)t.nrow * t.ncol; };
-        std::stable_sort(a64.begin(), a64.end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
-        std::stable_sort(a32.begin(), a32.end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
-        Stage s;
-        s.phase = phase;
-        s.mat_bytes = 0.0;
-        for (const BTask &t : a64) s.mat_bytes += 8.0 * t.nrow * t.ncol;
-        for (const BTask &t : a32) s.mat_bytes += 4.0 * t.nrow * t.ncol;
-        s.t64_beg = (int)P.t64.size();
-        s.t64_cnt = (int)a64.size();
-        s.t32_beg = (int)P.t32.size();
-        s.t32_cnt = (int)a32.size();
-        P.t64.insert(P.t64.end(), a64.begin(), a64.end());
-        P.t32.insert(P.t32.end(), a32.begin(), a32.end());
-        s.r_beg = (int)P.rt.size();
-        for (int sid : rsegs) {
-            Segment &g = segs[sid];
-            if (g.len <= 0) continue;
-            const int sb = (int)P.rsrc.size();
-            P.rsrc.insert(P.rsrc.end(), g.srcs.begin(), g.srcs.end());
-            for (int e0 = 0; e0 < g.len; e0 += RCHUNK) {
-                RTask r;
-                r.dst = g.off + e0;
-                r.len = std::min(RCHUNK, g.len - e0);
-                r.src_beg = sb;
-                r.src_end = (int)P.rsrc.size();
-                r.flags = (g.ext ? 1 : 0) | (g.acc ? 2 : 0);
-                r.soff = e0;
-                r.pad_ = 0;
-                P.rt.push_back(r);
-            }
+    // ---- stages: tiles of stage s run in one launch, then the reduces of s ----
+    std::vector<std::vector<BTask>> S64, S32;
+    std::vector<std::vector<int>> SR;
+    std::vector<int> sphase;
+    std::vector<double> sload;
+
+    std::vector<BTask> &T(int st, bool fp32) { return fp32 ? S32[st] : S64[st]; }
+
+    static double tasks_bytes(const std::vector<BTask> &tl, size_t from, double esz) {
+        double b = 0;
+        for (size_t i = from; i < tl.size(); i++) b += esz * tl[i].nrow * tl[i].ncol;
+        return b;
+    }
+
+    // add_matrix into stage st, accounting its bytes to the stage load
+    void add_at(int st, long long mat, int nrow, int ncol, int ld, bool fp32, long long in0, long long in1, int dst0,
+                int dst1, long long acc0 = -1) {
+        std::vector<BTask> &tl = T(st, fp32);
+        size_t n0 = tl.size();
+        add_matrix(tl, mat, nrow, ncol, ld, fp32, (int)in0, (int)in1, dst0, dst1, acc0);
+        sload[st] += tasks_bytes(tl, n0, fp32 ? 4.0 : 8.0);
+    }
+
+    // Dense block: out0 = M in0 -> dst0, out1 = M^T in1 -> dst1, one pass.
+    void add_dense(H2E_dense_mat_p blk, bool fp32, long long in0, long long in1, int dst0, int dst1, int st,
+                   double &bytes) {
+        int ld;
+        long long off = store(blk->data, blk->nrow, blk->ncol, fp32, ld);
+        add_at(st, off, blk->nrow, blk->ncol, ld, fp32, in0, in1, dst0, dst1);
+        bytes += (fp32 ? 4.0 : 8.0) * blk->nrow * blk->ncol;
+    }
+
+    // Low-rank block M = U V (U: nrow x k, V^T: ncol x k), out0 = U (V in0),
+    // out1 = V^T (U^T in1), in stages a < b < c.  With both outputs the
+    // smaller half is streamed in a and c and the other half once in b with a
+    // fused bi-GEMV, so 1.5 passes instead of 2; with out0 only, V^T then U.
+    void add_lowrank(H2E_dense_mat_p blk, bool fp32, long long in0, long long in1, int dst0, int dst1, int a, int b,
+                     int c, double &bytes) {
+        const int nr = blk->nrow, nc = blk->ncol, k = -blk->ld;
+        int Uld, Vld;
+        const long long Uo = store(blk->data, nr, k, fp32, Uld);
+        const long long Vo = store(blk->data + (size_t)nr * k, nc, k, fp32, Vld);
+        bytes += (fp32 ? 4.0 : 8.0) * (double)(nr + nc) * k;
+        const long long t_off = walloc(k), s_off = walloc(k);
+        const int st_t = new_seg(t_off, k, false), st_s = new_seg(s_off, k, false);
+        if (dst1 < 0) {
+            add_at(a, Vo, nc, k, Vld, fp32, -1, in0, -1, st_t);          // t = V in0
+            SR[a].push_back(st_t);
+            add_at(b, Uo, nr, k, Uld, fp32, t_off, -1, dst0, -1);        // out0 = U t
+            return;
         }
-        s.r_cnt = (int)P.rt.size() - s.r_beg;
-        if (s.t64_cnt + s.t32_cnt + s.r_cnt > 0) P.stages.push_back(s);
-        a64.clear();
-        a32.clear();
-        rsegs.clear();
+        if (nr <= nc) {   // U twice
+            add_at(a, Uo, nr, k, Uld, fp32, -1, in1, -1, st_s);          // s = U^T in1
+            SR[a].push_back(st_s);
+            add_at(b, Vo, nc, k, Vld, fp32, s_off, in0, dst1, st_t);     // out1 = V^T s, t = V in0
+            SR[b].push_back(st_t);
+            add_at(c, Uo, nr, k, Uld, fp32, t_off, -1, dst0, -1);        // out0 = U t
+        } else {          // V^T twice
+            add_at(a, Vo, nc, k, Vld, fp32, -1, in0, -1, st_t);          // t = V in0
+            SR[a].push_back(st_t);
+            add_at(b, Uo, nr, k, Uld, fp32, t_off, in1, dst0, st_s);     // out0 = U t, s = U^T in1
+            SR[b].push_back(st_s);
+            add_at(c, Vo, nc, k, Vld, fp32, s_off, -1, dst1, -1);        // out1 = V^T s
+        }
+    }
+
+    void finish_stages() {
+        auto key = [](const BTask &t) { return (long long)t.nrow * t.ncol; };
+        for (size_t st = 0; st < S64.size(); st++) {
+            std::stable_sort(S64[st].begin(), S64[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
+            std::stable_sort(S32[st].begin(), S32[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
+            Stage s;
+            s.phase = sphase[st];
+            s.mat_bytes = tasks_bytes(S64[st], 0, 8.0) + tasks_bytes(S32[st], 0, 4.0);
+            s.t64_beg = (int)P.t64.size();
+            s.t64_cnt = (int)S64[st].size();
+            s.t32_beg = (int)P.t32.size();
+            s.t32_cnt = (int)S32[st].size();
+            P.t64.insert(P.t64.end(), S64[st].begin(), S64[st].end());
+            P.t32.insert(P.t32.end(), S32[st].begin(), S32[st].end());
+            s.r_beg = (int)P.rt.size();
+            for (int sid : SR[st]) {
+                Segment &g = segs[sid];
+                if (g.len <= 0) continue;
+                const int sb = (int)P.rsrc.size();
+                P.rsrc.insert(P.rsrc.end(), g.srcs.begin(), g.srcs.end());
+                for (int e0 = 0; e0 < g.len; e0 += RCHUNK) {
+                    RTask r;
+                    r.dst = g.off + e0;
+                    r.len = std::min(RCHUNK, g.len - e0);
+                    r.src_beg = sb;
+                    r.src_end = (int)P.rsrc.size();
+                    r.flags = (g.ext ? 1 : 0) | (g.acc ? 2 : 0);
+                    r.soff = e0;
+                    r.pad_ = 0;
+                    P.rt.push_back(r);
+                }
+            }
+            s.r_cnt = (int)P.rt.size() - s.r_beg;
+            if (s.t64_cnt + s.t32_cnt + s.r_cnt > 0) P.stages.push_back(s);
+        }
     }
 
     bool build() {
@@ -512,7 +571,6 @@ struct Builder {
         }
         auto xseg = [&](int node) { return x_off + h->mat_cluster[2 * node]; };
         auto nlen = [&](int node) { return h->mat_cluster[2 * node + 1] - h->mat_cluster[2 * node] + 1; };
-        // segments
         std::vector<int> seg_y0(n_node, -1), seg_y1(n_node, -1), seg_y(n_node, -1);
         for (int i = 0; i < n_node; i++) {
             if (h->node_level[i] >= h->min_adm_level) {
@@ -521,6 +579,20 @@ struct Builder {
             }
             if (h->n_child[i] == 0) seg_y[i] = new_seg(h->mat_cluster[2 * i], nlen(i), true);
         }
+        // ---- stage layout along the critical path
+        const int minL = h->min_adm_level, maxL = h->max_level;
+        const int nU = (minL <= maxL) ? (maxL - minL + 1) : 0;
+        const int sBA = nU, sBB = nU + 1, sBC = nU + 2;
+        const int sD0 = nU + 3;
+        const int sG = sD0 + nU;
+        const int nst = sG + 1;
+        S64.assign(nst, {});
+        S32.assign(nst, {});
+        SR.assign(nst, {});
+        sload.assign(nst, 0.0);
+        sphase.assign(nst, PH_GATHER);
+        for (int s2 = 0; s2 < nU; s2++) { sphase[s2] = PH_UP; sphase[sD0 + s2] = PH_DOWN; }
+        sphase[sBA] = PH_B; sphase[sBB] = PH_B; sphase[sBC] = PH_Y1;
         // storage of U
         std::vector<long long> Uoff(n_node, -1);
         std::vector<int> Uld(n_node, 0);
@@ -530,71 +602,22 @@ struct Builder {
             Uoff[i] = store(U->data, U->nrow, U->ncol, ub32, Uld[i]);
             P.bytes_U += 2.0 * (ub32 ? 4.0 : 8.0) * U->nrow * U->ncol;
         }
-        std::vector<BTask> a64, a32;
-        std::vector<int> rsegs;
-        auto &tl_ub = [&]() -> std::vector<BTask> & { return ub32 ? a32 : a64; }();
-        // ---- upward sweep, one stage per level
-        for (int lv = h->max_level; lv >= h->min_adm_level; lv--) {
+        // ---- upward sweep: level maxL - s at stage s
+        for (int lv = maxL; lv >= minL && nU > 0; lv--) {
+            const int st = maxL - lv;
             for (int j = 0; j < h->level_n_node[lv]; j++) {
                 int node = h->level_nodes[(size_t)lv * nleaf + j];
-                rsegs.push_back(seg_y0[node]);
+                SR[st].push_back(seg_y0[node]);
                 if (Uoff[node] < 0) continue;
                 H2E_dense_mat_p U = h->U[node];
-                long long in1 = (h->n_child[node] == 0) ? xseg(node)
-                                                         : y0o[h->children[(size_t)node * 8 + 0]];
-                add_matrix(tl_ub, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, -1, (int)in1, -1, seg_y0[node]);
+                long long in1 = (h->n_child[node] == 0) ? xseg(node) : y0o[h->children[(size_t)node * 8 + 0]];
+                add_at(st, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, -1, in1, -1, seg_y0[node]);
             }
-            flush_stage(PH_UP, a64, a32, rsegs);
         }
-        // ---- B coupling blocks
-        std::vector<int> lrseg_t, lrseg_s;
-        struct LRJob { long long Uo, Vo; int nrow, ncol, k, Uld, Vld; bool fp32; int dst0, dst1; int st, ss; long long t_off, s_off; };
-        std::vector<LRJob> lr_jobs;
-        auto add_block = [&](H2E_dense_mat_p blk, bool fp32, long long in0, long long in1, int dst0, int dst1,
-                             std::vector<BTask> &tl, double &bytes) {
-            const int nr = blk->nrow, nc = blk->ncol;
-            if (nr == 0 || nc == 0) return;
-            const double esz = fp32 ? 4.0 : 8.0;
-            if (blk->ld >= 0) {
-                int ld;
-                long long off = store(blk->data, nr, nc, fp32, ld);
-                add_matrix(tl, off, nr, nc, ld, fp32, (int)in0, (int)in1, dst0, dst1);
-                bytes += esz * nr * nc;
-            } else {
-                const int k = -blk->ld;
-                if (k == 0) return;
-                LRJob j;
-                j.nrow = nr; j.ncol = nc; j.k = k; j.fp32 = fp32; j.dst0 = dst0; j.dst1 = dst1;
-                j.Uo = store(blk->data, nr, k, fp32, j.Uld);
-                j.Vo = store(blk->data + (size_t)nr * k, nc, k, fp32, j.Vld);
-                // phase A: t = V in0 = VT^T in0 (out1 of VT), s = U^T in1 (out1 of U)
-                j.t_off = j.s_off = -1;
-                j.st = j.ss = -1;
-                if (dst0 >= 0) {
-                    j.t_off = walloc(k);
-                    j.st = new_seg(j.t_off, k, false);
-                    add_matrix(tl, j.Vo, nc, k, j.Vld, fp32, -1, (int)in0, -1, j.st);
-                    rsegs.push_back(j.st);
-                }
-                if (dst1 >= 0) {
-                    j.s_off = walloc(k);
-                    j.ss = new_seg(j.s_off, k, false);
-                    add_matrix(tl, j.Uo, nr, k, j.Uld, fp32, -1, (int)in1, -1, j.ss);
-                    rsegs.push_back(j.ss);
-                }
-                lr_jobs.push_back(j);
-                bytes += esz * (double)(nr + nc) * k;
-            }
-        };
-        auto flush_lr_phaseB = [&](std::vector<BTask> &tl64, std::vector<BTask> &tl32) {
-            for (auto &j : lr_jobs) {
-                std::vector<BTask> &tl = j.fp32 ? tl32 : tl64;
-                if (j.dst0 >= 0) add_matrix(tl, j.Uo, j.nrow, j.k, j.Uld, j.fp32, (int)j.t_off, -1, j.dst0, -1);
-                if (j.dst1 >= 0) add_matrix(tl, j.Vo, j.ncol, j.k, j.Vld, j.fp32, (int)j.s_off, -1, j.dst1, -1);
-            }
-            lr_jobs.clear();
-        };
+        // ---- coupling blocks in stages sBA..sBC, y1 reduced after sBC
         for (int i = 0; i < h->n_B; i++) {
+            H2E_dense_mat_p blk = h->c_B_blks[i];
+            if (blk->nrow == 0 || blk->ncol == 0) continue;
             int n0 = h->r_adm_pairs[2 * i], n1 = h->r_adm_pairs[2 * i + 1];
             int l0 = h->node_level[n0], l1 = h->node_level[n1];
             long long in0, in1;
@@ -602,51 +625,70 @@ struct Builder {
             if (l0 == l1) { in0 = y0o[n1]; in1 = y0o[n0]; d0 = seg_y1[n0]; d1 = seg_y1[n1]; }
             else if (l0 > l1) { in0 = xseg(n1); in1 = y0o[n0]; d0 = seg_y1[n0]; d1 = seg_y[n1]; }
             else { in0 = y0o[n1]; in1 = xseg(n0); d0 = seg_y[n0]; d1 = seg_y1[n1]; }
-            add_block(h->c_B_blks[i], ub32, in0, in1, d0, d1, tl_ub, P.bytes_B);
+            if (blk->ld >= 0) add_dense(blk, ub32, in0, in1, d0, d1, sBA, P.bytes_B);
+            else add_lowrank(blk, ub32, in0, in1, d0, d1, sBA, sBB, sBC, P.bytes_B);
         }
-        flush_stage(PH_B, a64, a32, rsegs);    // dense B + low-rank phase A, reduce t/s
-        flush_lr_phaseB(a64, a32);
         for (int i = 0; i < n_node; i++)
-            if (seg_y1[i] >= 0) rsegs.push_back(seg_y1[i]);
-        flush_stage(PH_Y1, a64, a32, rsegs);   // low-rank phase B, reduce y1
-        // ---- downward sweep, one stage per level
-        for (int lv = h->min_adm_level; lv <= h->max_level; lv++) {
+            if (seg_y1[i] >= 0) SR[sBC].push_back(seg_y1[i]);
+        // ---- downward sweep: level minL + s at stage sD0 + s
+        for (int lv = minL; lv <= maxL && nU > 0; lv++) {
+            const int st = sD0 + (lv - minL);
             for (int j = 0; j < h->level_n_node[lv]; j++) {
                 int node = h->level_nodes[(size_t)lv * nleaf + j];
                 if (Uoff[node] < 0) continue;
                 H2E_dense_mat_p U = h->U[node];
                 if (h->n_child[node] == 0) {
-                    add_matrix(tl_ub, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, (int)y1o[node], -1, seg_y[node], -1);
+                    add_at(st, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, y1o[node], -1, seg_y[node], -1);
                 } else if (U->ncol <= MAX_TILE_COLS) {
                     // one column tile: each row has a unique writer, add in place
                     long long acc = y1o[h->children[(size_t)node * 8 + 0]];
-                    add_matrix(tl_ub, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, (int)y1o[node], -1, -1, -1, acc);
+                    add_at(st, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, y1o[node], -1, -1, -1, acc);
                 } else {
                     // several column tiles: partial slots, reduce adds into y1 of the children
                     int sg = new_seg(y1o[h->children[(size_t)node * 8 + 0]], U->nrow, false);
                     segs[sg].acc = true;
-                    add_matrix(tl_ub, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, (int)y1o[node], -1, sg, -1);
-                    rsegs.push_back(sg);
+                    add_at(st, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, y1o[node], -1, sg, -1);
+                    SR[st].push_back(sg);
                 }
             }
-            flush_stage(PH_DOWN, a64, a32, rsegs);
         }
-        // ---- near field
-        for (int i = 0; i < h->n_D; i++) {
+        // ---- near field: depends on x only, balanced over all stages
+        std::vector<int> order(h->n_D);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return blk_elems(h->c_D_blks[a]) > blk_elems(h->c_D_blks[b]); });
+        for (int i : order) {
             H2E_dense_mat_p blk = h->c_D_blks[i];
+            if (blk->nrow == 0 || blk->ncol == 0) continue;
+            long long in0, in1;
+            int d0, d1;
             if (i < nleaf) {
                 int node = h->height_nodes[i];
-                add_block(blk, false, xseg(node), -1, seg_y[node], -1, a64, P.bytes_D);
+                in0 = xseg(node); in1 = -1; d0 = seg_y[node]; d1 = -1;
             } else {
                 int n0 = h->r_inadm_pairs[2 * (i - nleaf)], n1 = h->r_inadm_pairs[2 * (i - nleaf) + 1];
-                add_block(blk, false, xseg(n1), xseg(n0), seg_y[n0], seg_y[n1], a64, P.bytes_D);
+                in0 = xseg(n1); in1 = xseg(n0); d0 = seg_y[n0]; d1 = seg_y[n1];
+            }
+            const double bytes = 8.0 * (double)blk_elems(blk);
+            if (blk->ld >= 0) {
+                int best = 0;
+                for (int st = 1; st < nst; st++) if (sload[st] < sload[best]) best = st;
+                add_dense(blk, false, in0, in1, d0, d1, best, P.bytes_D);
+            } else if (-blk->ld > 0) {
+                const int span = (d1 < 0) ? 2 : 3;
+                int best = 0;
+                double bestv = 1e300;
+                for (int a = 0; a + span - 1 < nst; a++) {
+                    double v = 0;
+                    for (int q = 0; q < span; q++) v = std::max(v, sload[a + q] + bytes / span);
+                    if (v < bestv) { bestv = v; best = a; }
+                }
+                add_lowrank(blk, false, in0, in1, d0, d1, best, best + 1, best + 2, P.bytes_D);
             }
         }
-        flush_stage(PH_D, a64, a32, rsegs);
-        flush_lr_phaseB(a64, a32);
         for (int i = 0; i < n_node; i++)
-            if (seg_y[i] >= 0) rsegs.push_back(seg_y[i]);
-        flush_stage(PH_GATHER, a64, a32, rsegs);
+            if (seg_y[i] >= 0) SR[sG].push_back(seg_y[i]);
+        finish_stages();
         P.n64 = (long long)host64.size();
         P.n32 = (long long)host32.size();
         P.bytes_vec = 8.0 * (2.0 * N + 4.0 * (double)tot_r);
@@ -754,28 +796,9 @@ static bool upload(H2ERI_p h) {
     return true;
 }
 
-static void launch_bgemv(int phase, int nb, const Stage &st, DevState *d, hipStream_t s) {
-    const BTask *t64 = d->d_t64 + st.t64_beg, *t32 = d->d_t32 + st.t32_beg;
-    switch (phase) {
-    case PH_UP:
-        hipLaunchKernelGGL(bgemv_kernel<PH_UP>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
-        break;
-    case PH_B:
-        hipLaunchKernelGGL(bgemv_kernel<PH_B>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
-        break;
-    case PH_Y1:
-        hipLaunchKernelGGL(bgemv_kernel<PH_Y1>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
-        break;
-    case PH_DOWN:
-        hipLaunchKernelGGL(bgemv_kernel<PH_DOWN>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
-        break;
-    case PH_D:
-        hipLaunchKernelGGL(bgemv_kernel<PH_D>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
-        break;
-    default:
-        hipLaunchKernelGGL(bgemv_kernel<PH_GATHER>, dim3(nb), dim3(WG), 0, s, t64, st.t64_cnt, t32, st.t32_cnt, d->S64, d->S32, d->Wv);
-        break;
-    }
+static void launch_bgemv(int nb, const Stage &st, DevState *d, hipStream_t s) {
+    hipLaunchKernelGGL(bgemv_kernel, dim3(nb), dim3(WG), 0, s, d->d_t64 + st.t64_beg, st.t64_cnt,
+                       d->d_t32 + st.t32_beg, st.t32_cnt, d->S64, d->S32, d->Wv);
 }
 
 // Enqueue one matvec: x already at Wv[0..N), result written to y_dev.  In
@@ -794,7 +817,7 @@ static bool run_matvec(H2ERI_p h, DevState *d, double *y_dev, hipStream_t s) {
         const Stage &st = P.stages[si];
         const int nb = st.t64_cnt + st.t32_cnt;
         if (nb > 0) {
-            launch_bgemv(st.phase, nb, st, d, s);
+            launch_bgemv(nb, st, d, s);
             HIP_OK(hipGetLastError());
             if (tim) {
                 d->launches.push_back(LaunchRec{st.phase, 0, nb, st.mat_bytes, 0.f});
@@ -1011,6 +1034,91 @@ size_t h2eri_struct_size(void) { return sizeof(struct H2ERI); }
 void H2ERI_build_exchange(H2ERI_p h, const double *den_mat, double *K_mat) {
     (void)h; (void)den_mat; (void)K_mat;
     set_error(ERR_UNSUPPORTED, "H2ERI_build_exchange: not available in this build yet");
+}
+
+// Verification of a plan on the host (no HIP call): executes every tile and
+// reduce task with the kernels' semantics in launch order, and checks that
+// within one launch no task reads an element another task of that launch
+// writes and no element is stored by two tasks.  For tests of the planner;
+// the device entry points never call it.
+int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const double *x, double *y) {
+    if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !x || !y) { set_error(ERR_ARG, "plan_emulate: bad args"); return -1; }
+    Plan P;
+    Builder b(h, P, ub_bits == 32 ? 32 : 64, dstate_get(h)->tile_bytes);
+    if (!b.build()) return -1;
+    const long long N = h->krnl_mat_size;
+    std::vector<double> Wv((size_t)P.nW, 0.0);
+    std::memcpy(Wv.data(), x, sizeof(double) * (size_t)N);
+    std::vector<int> wmark((size_t)P.nW, -1), rmark((size_t)P.nW, -1), ymark((size_t)N, -1);
+    int races = 0;
+    auto write_el = [&](long long e, int task, bool acc) {
+        if (wmark[e] >= 0 && wmark[e] != task && !acc) races++;
+        if (rmark[e] >= 0 && rmark[e] != task) races++;
+        wmark[e] = task;
+    };
+    auto read_el = [&](long long e, int task) {
+        if (wmark[e] >= 0 && wmark[e] != task) races++;
+        rmark[e] = task;
+    };
+    int launch = 0;
+    for (const Stage &st : P.stages) {
+        // tile launch
+        std::fill(wmark.begin(), wmark.end(), -1);
+        std::fill(rmark.begin(), rmark.end(), -1);
+        for (int q = 0; q < st.t64_cnt + st.t32_cnt; q++) {
+            const bool f32 = q >= st.t64_cnt;
+            const BTask &t = f32 ? P.t32[st.t32_beg + q - st.t64_cnt] : P.t64[st.t64_beg + q];
+            const int id = q;
+            auto M = [&](int r, int c) -> double {
+                long long e = t.mat + (long long)r * t.ld + c;
+                return f32 ? (double)b.host32[e] : b.host64[e];
+            };
+            if (t.out0 >= 0) {
+                for (int c = 0; c < t.ncol; c++) read_el(t.in0 + c, id);
+                for (int r = 0; r < t.nrow; r++) {
+                    double d = 0.0;
+                    for (int c = 0; c < t.ncol; c++) d += M(r, c) * Wv[t.in0 + c];
+                    write_el(t.out0 + r, id, (t.flags & 1) != 0);
+                    if (t.flags & 1) Wv[t.out0 + r] += d;
+                    else Wv[t.out0 + r] = d;
+                }
+            }
+            if (t.out1 >= 0) {
+                for (int r = 0; r < t.nrow; r++) read_el(t.in1 + r, id);
+                for (int c = 0; c < t.ncol; c++) {
+                    double s = 0.0;
+                    for (int r = 0; r < t.nrow; r++) s += M(r, c) * Wv[t.in1 + r];
+                    write_el(t.out1 + c, id, false);
+                    Wv[t.out1 + c] = s;
+                }
+            }
+        }
+        launch++;
+        // reduce launch
+        std::fill(wmark.begin(), wmark.end(), -1);
+        std::fill(rmark.begin(), rmark.end(), -1);
+        for (int q = 0; q < st.r_cnt; q++) {
+            const RTask &r = P.rt[st.r_beg + q];
+            for (int i = 0; i < r.len; i++) {
+                double s = 0.0;
+                for (int j = r.src_beg; j < r.src_end; j++) {
+                    read_el(P.rsrc[j] + r.soff + i, q);
+                    s += Wv[P.rsrc[j] + r.soff + i];
+                }
+                if (r.flags & 1) {
+                    long long e = r.dst + i;
+                    if (ymark[e] == launch) races++;
+                    ymark[e] = launch;
+                    y[e] = (r.flags & 2) ? y[e] + s : s;
+                } else {
+                    write_el(r.dst + i, q, false);
+                    Wv[r.dst + i] = (r.flags & 2) ? Wv[r.dst + i] + s : s;
+                }
+            }
+        }
+        launch++;
+    }
+    return races;
 }
 
 // Builds the device plan on the host only and runs its bounds check; no HIP

@@ -162,6 +162,14 @@ class H2ERI:
         _check(self.lib)
         return r
 
+    def plan_emulate(self, x, bits=64):
+        """Host execution of the device plan (tests only): (y, conflicts)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        r = self.lib.H2ERI_plan_emulate(self.h, bits, _dptr(x), _dptr(y))
+        _check(self.lib)
+        return y, r
+
     def upload(self):
         r = self.lib.H2ERI_upload(self.h)
         _check(self.lib)

@@ -101,6 +101,7 @@ SIGNATURES = {
     "H2ERI_set_tile_bytes": (None, [H2ERI_p, C.c_double]),
     "h2eri_struct_size": (C.c_size_t, []),
     "H2ERI_plan_check": (C.c_int, [H2ERI_p, C.c_int]),
+    "H2ERI_plan_emulate": (C.c_int, [H2ERI_p, C.c_int, c_dbl_p, c_dbl_p]),
 }
 
 _lib = None

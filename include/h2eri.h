@@ -314,6 +314,12 @@ void H2ERI_set_tile_bytes(H2ERI_p h2eri, const double bytes);
  * number of tile tasks, -1 on failure. */
 int H2ERI_plan_check(H2ERI_p h2eri, const int ub_bits);
 
+/* Host verification of the device plan (no HIP call, tests only): executes
+ * every launch of the plan with the kernels' semantics into y and returns
+ * the number of intra-launch read/write conflicts found (0 expected), -1 on
+ * failure.  The hot-path entry points never call it. */
+int H2ERI_plan_emulate(H2ERI_p h2eri, const int ub_bits, const double *x, double *y);
+
 /* sizeof(struct H2ERI) as compiled into the library (ABI check for
  * bindings that mirror the struct). */
 size_t h2eri_struct_size(void);

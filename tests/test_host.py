@@ -73,3 +73,24 @@ def test_errors_are_reported_not_aborting(tmp_path):
     h3 = make_synthetic(tmp_path / "x.rep", 300, 2, leaf=20)
     with pytest.raises(h2eri.H2ERIError):
         h3.build_h2(bd_jit=1)
+
+
+@pytest.mark.parametrize("seed,n,leaf,mr,shape", [(1, 900, 30, 12, "chain"), (2, 1200, 150, 40, "chain"),
+                                                  (4, 1500, 40, 20, "cube"), (5, 40, 300, 8, "chain"),
+                                                  (6, 2000, 120, 600, "chain")])
+def test_device_plan_semantics_match_oracle(tmp_path, seed, n, leaf, mr, shape):
+    """The launch plan executed on the host with the kernels' semantics equals
+    the oracle, and no launch has a read/write conflict between its tasks."""
+    import oracle_bind
+    from conftest import relerr
+    h = make_synthetic(tmp_path / "r.rep", n, seed, leaf=leaf, max_rank=mr, lr_frac=0.5, diag_lr_frac=0.4,
+                       shape=shape)
+    o = oracle_bind.Oracle(str(tmp_path / "r.rep"))
+    x = np.random.default_rng(seed).standard_normal(h.n)
+    yo = o.matvec(x)
+    y, conflicts = h.plan_emulate(x, 64)
+    assert conflicts == 0
+    assert relerr(y, yo) < 1e-13
+    y32, conflicts = h.plan_emulate(x, 32)
+    assert conflicts == 0
+    assert relerr(y32, yo) < 2e-6
