@@ -98,32 +98,56 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, double *__re
             a1[k][w] = 0.0;
         }
     const T *base = S + t.mat;
-    for (int r = grp; r < R; r += G) {
-        const T *row = base + (long long)r * t.ld;
-        V v[KP];
+    // RU rows per iteration: all RU*KP vector loads are issued before any is
+    // consumed, so every lane keeps RU*KP requests in flight.
+    constexpr int RU = 4;
+    for (int r0 = grp; r0 < R; r0 += RU * G) {
+        V v[RU][KP];
         #pragma unroll
-        for (int k = 0; k < KP; k++) {
-            const int c0 = (k * lpr + lane) * W;
-            if (c0 < C) v[k] = *reinterpret_cast<const V *>(row + c0);
-            else v[k] = V{};
+        for (int u = 0; u < RU; u++) {
+            const int r = r0 + u * G;
+            const T *row = base + (long long)r * t.ld;
+            #pragma unroll
+            for (int k = 0; k < KP; k++) {
+                const int c0 = (k * lpr + lane) * W;
+                if (r < R && c0 < C) v[u][k] = *reinterpret_cast<const V *>(row + c0);
+                else v[u][k] = V{};
+            }
         }
         if (do1) {
-            const double y = Wv[t.in1 + r];
             #pragma unroll
-            for (int k = 0; k < KP; k++)
+            for (int u = 0; u < RU; u++) {
+                const int r = r0 + u * G;
+                const double y = (r < R) ? Wv[t.in1 + r] : 0.0;
                 #pragma unroll
-                for (int w = 0; w < W; w++) a1[k][w] = fma(vget(v[k], w), y, a1[k][w]);
+                for (int k = 0; k < KP; k++)
+                    #pragma unroll
+                    for (int w = 0; w < W; w++) a1[k][w] = fma(vget(v[u][k], w), y, a1[k][w]);
+            }
         }
         if (do0) {
-            double d = 0.0;
+            double d[RU];
             #pragma unroll
-            for (int k = 0; k < KP; k++)
+            for (int u = 0; u < RU; u++) {
+                d[u] = 0.0;
                 #pragma unroll
-                for (int w = 0; w < W; w++) d = fma(vget(v[k], w), x0[k][w], d);
-            for (int m = lpr >> 1; m > 0; m >>= 1) d += __shfl_xor(d, m, lpr);
+                for (int k = 0; k < KP; k++)
+                    #pragma unroll
+                    for (int w = 0; w < W; w++) d[u] = fma(vget(v[u][k], w), x0[k][w], d[u]);
+            }
+            for (int m = lpr >> 1; m > 0; m >>= 1) {
+                #pragma unroll
+                for (int u = 0; u < RU; u++) d[u] += __shfl_xor(d[u], m, lpr);
+            }
             if (lane == 0) {
-                if (t.flags & 1) Wv[t.out0 + r] += d;
-                else Wv[t.out0 + r] = d;
+                #pragma unroll
+                for (int u = 0; u < RU; u++) {
+                    const int r = r0 + u * G;
+                    if (r < R) {
+                        if (t.flags & 1) Wv[t.out0 + r] += d[u];
+                        else Wv[t.out0 + r] = d[u];
+                    }
+                }
             }
         }
     }
@@ -284,7 +308,8 @@ struct DevState {
     int dev = -1;
     int ub_bits = 64;
     bool ready = false;
-    double tile_bytes = 256.0 * 1024.0;
+    double tile_bytes = 128.0 * 1024.0;
+    int nf_schedule = 0;      // 0: near field balanced over the sweep stages, 1: own stages first
     Plan plan;
     double *S64 = nullptr;
     float *S32 = nullptr;
@@ -361,7 +386,9 @@ struct Builder {
     std::vector<float> host32;
     std::vector<Segment> segs;
 
-    Builder(H2ERI_p h_, Plan &p_, int bits, double tb) : h(h_), P(p_), ub_bits(bits), tile_bytes(tb) {}
+    int nf_schedule = 0;
+    Builder(H2ERI_p h_, Plan &p_, int bits, double tb, int nfs = 0)
+        : h(h_), P(p_), ub_bits(bits), tile_bytes(tb), nf_schedule(nfs) {}
 
     long long walloc(long long n) {
         long long o = P.nW;
@@ -582,8 +609,9 @@ struct Builder {
         // ---- stage layout along the critical path
         const int minL = h->min_adm_level, maxL = h->max_level;
         const int nU = (minL <= maxL) ? (maxL - minL + 1) : 0;
-        const int sBA = nU, sBB = nU + 1, sBC = nU + 2;
-        const int sD0 = nU + 3;
+        const int s0 = nf_schedule == 1 ? 3 : 0;   // near field first: three stages of its own
+        const int sBA = s0 + nU, sBB = sBA + 1, sBC = sBA + 2;
+        const int sD0 = sBA + 3;
         const int sG = sD0 + nU;
         const int nst = sG + 1;
         S64.assign(nst, {});
@@ -591,7 +619,8 @@ struct Builder {
         SR.assign(nst, {});
         sload.assign(nst, 0.0);
         sphase.assign(nst, PH_GATHER);
-        for (int s2 = 0; s2 < nU; s2++) { sphase[s2] = PH_UP; sphase[sD0 + s2] = PH_DOWN; }
+        for (int q = 0; q < s0; q++) sphase[q] = PH_D;
+        for (int s2 = 0; s2 < nU; s2++) { sphase[s0 + s2] = PH_UP; sphase[sD0 + s2] = PH_DOWN; }
         sphase[sBA] = PH_B; sphase[sBB] = PH_B; sphase[sBC] = PH_Y1;
         // storage of U
         std::vector<long long> Uoff(n_node, -1);
@@ -604,7 +633,7 @@ struct Builder {
         }
         // ---- upward sweep: level maxL - s at stage s
         for (int lv = maxL; lv >= minL && nU > 0; lv--) {
-            const int st = maxL - lv;
+            const int st = s0 + maxL - lv;
             for (int j = 0; j < h->level_n_node[lv]; j++) {
                 int node = h->level_nodes[(size_t)lv * nleaf + j];
                 SR[st].push_back(seg_y0[node]);
@@ -670,7 +699,10 @@ struct Builder {
                 in0 = xseg(n1); in1 = xseg(n0); d0 = seg_y[n0]; d1 = seg_y[n1];
             }
             const double bytes = 8.0 * (double)blk_elems(blk);
-            if (blk->ld >= 0) {
+            if (nf_schedule == 1) {
+                if (blk->ld >= 0) add_dense(blk, false, in0, in1, d0, d1, 0, P.bytes_D);
+                else add_lowrank(blk, false, in0, in1, d0, d1, 0, 1, 2, P.bytes_D);
+            } else if (blk->ld >= 0) {
                 int best = 0;
                 for (int st = 1; st < nst; st++) if (sload[st] < sload[best]) best = st;
                 add_dense(blk, false, in0, in1, d0, d1, best, P.bytes_D);
@@ -759,7 +791,7 @@ static bool upload(H2ERI_p h) {
     if (h->BD_JIT) { set_error(ERR_UNSUPPORTED, "BD_JIT representations are not supported"); return false; }
     HIP_OK(hipGetDevice(&d->dev));
     release_buffers(d);
-    Builder b(h, d->plan, d->ub_bits, d->tile_bytes);
+    Builder b(h, d->plan, d->ub_bits, d->tile_bytes, d->nf_schedule);
     if (!b.build()) return false;
     Plan &P = d->plan;
     auto alloc_copy = [&](void **dst, const void *src, size_t bytes) -> bool {
@@ -1021,6 +1053,14 @@ void H2ERI_matvec_bytes(H2ERI_p h, double *out6) {
     out6[5] = P.bytes_vec;
 }
 
+// Near-field schedule (tuning knob; re-plans): 0 balances the near-field
+// tiles over the sweep launches, 1 runs them in three launches of their own.
+void H2ERI_set_schedule(H2ERI_p h, const int mode) {
+    if (!h || (mode != 0 && mode != 1)) { set_error(ERR_ARG, "H2ERI_set_schedule: 0 or 1"); return; }
+    DevState *d = dstate_get(h);
+    if (d->nf_schedule != mode) { d->nf_schedule = mode; release_buffers(d); }
+}
+
 // Tile size of the bi-GEMV tasks in bytes (tuning knob; re-plans).
 void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
     if (!h || bytes < 1024) return;
@@ -1044,7 +1084,7 @@ void H2ERI_build_exchange(H2ERI_p h, const double *den_mat, double *K_mat) {
 int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const double *x, double *y) {
     if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !x || !y) { set_error(ERR_ARG, "plan_emulate: bad args"); return -1; }
     Plan P;
-    Builder b(h, P, ub_bits == 32 ? 32 : 64, dstate_get(h)->tile_bytes);
+    Builder b(h, P, ub_bits == 32 ? 32 : 64, dstate_get(h)->tile_bytes, dstate_get(h)->nf_schedule);
     if (!b.build()) return -1;
     const long long N = h->krnl_mat_size;
     std::vector<double> Wv((size_t)P.nW, 0.0);
@@ -1127,7 +1167,7 @@ int H2ERI_plan_check(H2ERI_p h, const int ub_bits) {
     if (!h || !h->U || !h->c_B_blks || !h->c_D_blks) { set_error(ERR_ARG, "plan_check: nothing built"); return -1; }
     Plan P;
     double tb = dstate_get(h)->tile_bytes;
-    Builder b(h, P, ub_bits == 32 ? 32 : 64, tb);
+    Builder b(h, P, ub_bits == 32 ? 32 : 64, tb, dstate_get(h)->nf_schedule);
     if (!b.build()) return -1;
     return (int)(P.t64.size() + P.t32.size());
 }

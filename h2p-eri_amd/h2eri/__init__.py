@@ -170,6 +170,10 @@ class H2ERI:
         _check(self.lib)
         return y, r
 
+    def set_schedule(self, mode):
+        self.lib.H2ERI_set_schedule(self.h, mode)
+        _check(self.lib)
+
     def upload(self):
         r = self.lib.H2ERI_upload(self.h)
         _check(self.lib)

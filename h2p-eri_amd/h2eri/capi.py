@@ -99,6 +99,7 @@ SIGNATURES = {
     "H2ERI_get_launch_stats": (C.c_int, [H2ERI_p, C.c_int, c_dbl_p]),
     "H2ERI_matvec_bytes": (None, [H2ERI_p, c_dbl_p]),
     "H2ERI_set_tile_bytes": (None, [H2ERI_p, C.c_double]),
+    "H2ERI_set_schedule": (None, [H2ERI_p, C.c_int]),
     "h2eri_struct_size": (C.c_size_t, []),
     "H2ERI_plan_check": (C.c_int, [H2ERI_p, C.c_int]),
     "H2ERI_plan_emulate": (C.c_int, [H2ERI_p, C.c_int, c_dbl_p, c_dbl_p]),

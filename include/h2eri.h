@@ -309,6 +309,11 @@ void H2ERI_matvec_bytes(H2ERI_p h2eri, double *out6);
  * device layout on the next call). */
 void H2ERI_set_tile_bytes(H2ERI_p h2eri, const double bytes);
 
+/* Near-field schedule (tuning; re-plans): 0 balances the near-field tiles
+ * over the launches of the sweeps, 1 gives them three launches of their own
+ * ahead of the sweeps. */
+void H2ERI_set_schedule(H2ERI_p h2eri, const int mode);
+
 /* Builds the device execution plan on the host only (no HIP call) and
  * checks every access of every launch against its buffer.  Returns the
  * number of tile tasks, -1 on failure. */

@@ -94,3 +94,7 @@ def test_device_plan_semantics_match_oracle(tmp_path, seed, n, leaf, mr, shape):
     y32, conflicts = h.plan_emulate(x, 32)
     assert conflicts == 0
     assert relerr(y32, yo) < 2e-6
+    h.set_schedule(1)
+    y1, conflicts = h.plan_emulate(x, 64)
+    assert conflicts == 0
+    assert relerr(y1, yo) < 1e-13
