@@ -13,19 +13,30 @@
 //
 // Device design (independent of the reference's OpenMP structure):
 //   * every matrix block (U, dense B/D, both halves of low-rank B/D) lives in
-//     one flat storage array with its row stride padded to the vector width;
-//   * each phase is a list of tiles, "bi-GEMV tasks": tile x in0 -> out0
-//     (row dots), tile^T x in1 -> out1 (column sums), executed by one
-//     workgroup each in one launch;
-//   * every tile output goes to a private slot; a segmented-reduce launch
-//     sums the slots of each destination in a fixed order, so results are
-//     bit-reproducible and no atomics are needed;
+//     one flat storage array per precision, row stride padded to the vector
+//     width; the storage is uploaded once;
+//   * work is a list of tiles ("bi-GEMV tasks"): tile x in0 -> out0 (row
+//     dots), tile^T x in1 -> out1 (column sums), one workgroup each; the tiles
+//     of one stage run in one launch;
+//   * every tile output goes to a private slot; every intermediate vector
+//     (y0, y1, the k-vectors of low-rank blocks, the output segments) is
+//     "virtual": a fixed list of slots that the consuming tile sums once into
+//     LDS.  Where a vector has many early slots and several consumers, a fold
+//     task (a fixed-order sum into one slot) runs in the launch before its
+//     first consumer, beside that launch's tiles; the output y is written by
+//     one final gather launch.  All sums run in a fixed order: results are
+//     bit-reproducible and no atomics are used;
+//   * the near-field tiles depend on x only and are balanced over the launches
+//     of the sweeps' critical path;
+//   * low-rank blocks take 1.5 passes: the smaller half twice, the other half
+//     once with a fused bi-GEMV;
+//   * NV vectors (1, 2, 4, 8) are processed per pass over the matrix data;
 //   * U and B may be stored in FP32 (accumulation stays FP64), D stays FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
-#include <map>
 #include <numeric>
 
 #include "h2eri_internal.h"
@@ -33,52 +44,106 @@
 namespace h2e {
 
 // ---------------------------------------------------------------------------
-// Task descriptors
+// Descriptors
 // ---------------------------------------------------------------------------
+// Vector sources (vsrc entries): e >= 0 is an internal vector at element
+// offset e of the workspace (NV-interleaved: element i of vector v lives at
+// Wv[(e + i) * NV + v]); e < 0 is the external input at element offset
+// -e - 1 (vector v at X[v * ldx + ...]); an external source is always the
+// only source of its vector.
 struct BTask {
     long long mat;   // element offset into the storage array of its precision
     int nrow, ncol, ld;
     int lpr;         // lanes per row: 4, 8, 16, 32 or 64
-    int in0, in1;    // workspace offsets (-1: unused)
-    int out0, out1;  // workspace offsets (-1: unused)
-    int flags;       // bit 0: out0 accumulates (+=) instead of storing
-    int pad_;
+    int v0b, v0e, v0off;   // in0 (length ncol): sum of vsrc[v0b..v0e) at element offset v0off
+    int v1b, v1e, v1off;   // in1 (length nrow)
+    int out0, out1;  // slot offsets (-1: unused)
+    int pad_[2];
 };
-static_assert(sizeof(BTask) == 48, "BTask layout");
+static_assert(sizeof(BTask) == 64, "BTask layout");
 
+// Reduce task: dst[i] = sum over rsrc[src_beg..src_end) of Wv[src + soff + i].
 struct RTask {
-    long long dst;   // offset into workspace, or into the external output
-    int len;         // <= 64 elements (one chunk of a destination segment)
+    long long dst;   // element offset: external output (flags & 1) or workspace
+    int len;         // <= RCHUNK elements
     int src_beg, src_end;
-    int flags;       // bit 0: destination is the external output vector;
-                     // bit 1: accumulate (dst += sum) instead of storing
     int soff;        // element offset of this chunk inside every source slot
+    int flags;       // 1: external output
     int pad_;
 };
 static_assert(sizeof(RTask) == 32, "RTask layout");
-static const int RCHUNK = 64;       // destination elements per reduce workgroup
 
-static const int WG = 256;          // threads per workgroup
-static const int COLS_PER_LANE = 8; // columns a lane holds in registers
-static const int MAX_TILE_COLS = 64 * COLS_PER_LANE;
+static const int WG = 256;       // threads per workgroup (4 waves)
+static const int RCHUNK = 64;    // output elements per reduce task
+static const int LDS_DOUBLES = 4096;   // 32 KB of LDS per workgroup
+static const int XIN_DOUBLES = 1024;   // in0 staging (ncol * NV <= 1024)
+static const int YIN_DOUBLES = LDS_DOUBLES - XIN_DOUBLES;   // in1 staging (nrow * NV)
 
-// ---------------------------------------------------------------------------
-// Kernels
-// ---------------------------------------------------------------------------
-template <typename T> struct VecT;
-template <> struct VecT<double> { typedef double2 type; static const int W = 2; };
-template <> struct VecT<float> { typedef float4 type; static const int W = 4; };
+// Per (storage type, NV) tile configuration: W elements per vector load, KP
+// loads per row and lane (columns per lane CPL = W * KP), RU rows per loop
+// iteration.  CPL * NV <= 16 keeps the cross-wave reduction in 32 KB of LDS.
+template <typename T, int NV> struct Cfg;
+template <> struct Cfg<double, 1> { typedef double2 V; static const int W = 2, KP = 4, RU = 4; };
+template <> struct Cfg<double, 2> { typedef double2 V; static const int W = 2, KP = 4, RU = 2; };
+template <> struct Cfg<double, 4> { typedef double2 V; static const int W = 2, KP = 2, RU = 2; };
+template <> struct Cfg<double, 8> { typedef double2 V; static const int W = 2, KP = 1, RU = 2; };
+template <> struct Cfg<float, 1> { typedef float4 V; static const int W = 4, KP = 2, RU = 4; };
+template <> struct Cfg<float, 2> { typedef float4 V; static const int W = 4, KP = 2, RU = 2; };
+template <> struct Cfg<float, 4> { typedef float4 V; static const int W = 4, KP = 1, RU = 2; };
+template <> struct Cfg<float, 8> { typedef float2 V; static const int W = 2, KP = 1, RU = 2; };
+
+// columns per lane for a batch width (same for both storage types)
+static inline int cpl_of(int nv) { return nv <= 2 ? 8 : (nv == 4 ? 4 : 2); }
 
 __device__ __forceinline__ double vget(const double2 &v, int i) { return i == 0 ? v.x : v.y; }
+__device__ __forceinline__ double vget(const float2 &v, int i) { return (double)(i == 0 ? v.x : v.y); }
 __device__ __forceinline__ double vget(const float4 &v, int i) {
     return (double)(i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)));
 }
 
-template <typename T>
-__device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, double *__restrict__ Wv, double *red) {
-    typedef typename VecT<T>::type V;
-    constexpr int W = VecT<T>::W;
-    constexpr int KP = COLS_PER_LANE / W;
+// Element idx of the virtual vector vsrc[b..e): NV values into out.
+template <int NV>
+__device__ __forceinline__ void vsum(const int *__restrict__ vsrc, int b, int e, int idx, const double *__restrict__ Wv,
+                                     const double *__restrict__ X, long long ldx, double (&out)[NV]) {
+    #pragma unroll
+    for (int v = 0; v < NV; v++) out[v] = 0.0;
+    const int s0 = vsrc[b];
+    if (s0 < 0) {
+        const double *p = X + (long long)(-s0 - 1 + idx);
+        #pragma unroll
+        for (int v = 0; v < NV; v++) out[v] = p[(long long)v * ldx];
+        return;
+    }
+    int j = b;
+    for (; j + 3 < e; j += 4) {
+        const double *p0 = Wv + (long long)(vsrc[j] + idx) * NV;
+        const double *p1 = Wv + (long long)(vsrc[j + 1] + idx) * NV;
+        const double *p2 = Wv + (long long)(vsrc[j + 2] + idx) * NV;
+        const double *p3 = Wv + (long long)(vsrc[j + 3] + idx) * NV;
+        double a0[NV], a1[NV], a2[NV], a3[NV];
+        #pragma unroll
+        for (int v = 0; v < NV; v++) { a0[v] = p0[v]; a1[v] = p1[v]; a2[v] = p2[v]; a3[v] = p3[v]; }
+        #pragma unroll
+        for (int v = 0; v < NV; v++) {
+            out[v] += a0[v];
+            out[v] += a1[v];
+            out[v] += a2[v];
+            out[v] += a3[v];
+        }
+    }
+    for (; j < e; j++) {
+        const double *p = Wv + (long long)(vsrc[j] + idx) * NV;
+        #pragma unroll
+        for (int v = 0; v < NV; v++) out[v] += p[v];
+    }
+}
+
+template <typename T, int NV>
+__device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *__restrict__ vsrc,
+                           double *__restrict__ Wv, const double *__restrict__ X, long long ldx, double *lds) {
+    typedef Cfg<T, NV> C_;
+    typedef typename C_::V V;
+    constexpr int W = C_::W, KP = C_::KP, RU = C_::RU, CPL = W * KP;
     const int lpr = t.lpr;
     const int tid = threadIdx.x;
     const int lane = tid & (lpr - 1);
@@ -86,23 +151,44 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, double *__re
     const int G = WG / lpr;
     const int R = t.nrow, C = t.ncol;
     const bool do0 = t.out0 >= 0, do1 = t.out1 >= 0;
+    double *xin = lds;
+    double *yin = lds + XIN_DOUBLES;
 
-    double x0[KP][W];
-    double a1[KP][W];
+    // stage the input vectors once per tile (sums of their slots)
+    if (do0)
+        for (int q = tid; q < C; q += WG) {
+            double tmp[NV];
+            vsum<NV>(vsrc, t.v0b, t.v0e, t.v0off + q, Wv, X, ldx, tmp);
+            #pragma unroll
+            for (int v = 0; v < NV; v++) xin[q * NV + v] = tmp[v];
+        }
+    if (do1)
+        for (int q = tid; q < R; q += WG) {
+            double tmp[NV];
+            vsum<NV>(vsrc, t.v1b, t.v1e, t.v1off + q, Wv, X, ldx, tmp);
+            #pragma unroll
+            for (int v = 0; v < NV; v++) yin[q * NV + v] = tmp[v];
+        }
+    __syncthreads();
+
+    double x0[KP][W][NV];
+    double a1[KP][W][NV];
     #pragma unroll
     for (int k = 0; k < KP; k++)
         #pragma unroll
         for (int w = 0; w < W; w++) {
             const int c = (k * lpr + lane) * W + w;
-            x0[k][w] = (do0 && c < C) ? Wv[t.in0 + c] : 0.0;
-            a1[k][w] = 0.0;
+            #pragma unroll
+            for (int v = 0; v < NV; v++) {
+                x0[k][w][v] = (do0 && c < C) ? xin[c * NV + v] : 0.0;
+                a1[k][w][v] = 0.0;
+            }
         }
     const T *base = S + t.mat;
     // RU rows per iteration: all RU*KP vector loads are issued before any is
     // consumed, so every lane keeps RU*KP requests in flight.
-    constexpr int RU = 4;
     for (int r0 = grp; r0 < R; r0 += RU * G) {
-        V v[RU][KP];
+        V m[RU][KP];
         #pragma unroll
         for (int u = 0; u < RU; u++) {
             const int r = r0 + u * G;
@@ -110,114 +196,148 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, double *__re
             #pragma unroll
             for (int k = 0; k < KP; k++) {
                 const int c0 = (k * lpr + lane) * W;
-                if (r < R && c0 < C) v[u][k] = *reinterpret_cast<const V *>(row + c0);
-                else v[u][k] = V{};
+                if (r < R && c0 < C) m[u][k] = *reinterpret_cast<const V *>(row + c0);
+                else m[u][k] = V{};
             }
         }
         if (do1) {
             #pragma unroll
             for (int u = 0; u < RU; u++) {
                 const int r = r0 + u * G;
-                const double y = (r < R) ? Wv[t.in1 + r] : 0.0;
+                double y[NV];
+                #pragma unroll
+                for (int v = 0; v < NV; v++) y[v] = r < R ? yin[r * NV + v] : 0.0;
                 #pragma unroll
                 for (int k = 0; k < KP; k++)
                     #pragma unroll
-                    for (int w = 0; w < W; w++) a1[k][w] = fma(vget(v[u][k], w), y, a1[k][w]);
+                    for (int w = 0; w < W; w++) {
+                        const double mv = vget(m[u][k], w);
+                        #pragma unroll
+                        for (int v = 0; v < NV; v++) a1[k][w][v] = fma(mv, y[v], a1[k][w][v]);
+                    }
             }
         }
         if (do0) {
-            double d[RU];
+            double d[RU][NV];
             #pragma unroll
             for (int u = 0; u < RU; u++) {
-                d[u] = 0.0;
+                #pragma unroll
+                for (int v = 0; v < NV; v++) d[u][v] = 0.0;
                 #pragma unroll
                 for (int k = 0; k < KP; k++)
                     #pragma unroll
-                    for (int w = 0; w < W; w++) d[u] = fma(vget(v[u][k], w), x0[k][w], d[u]);
+                    for (int w = 0; w < W; w++) {
+                        const double mv = vget(m[u][k], w);
+                        #pragma unroll
+                        for (int v = 0; v < NV; v++) d[u][v] = fma(mv, x0[k][w][v], d[u][v]);
+                    }
             }
-            for (int m = lpr >> 1; m > 0; m >>= 1) {
+            for (int s = lpr >> 1; s > 0; s >>= 1) {
                 #pragma unroll
-                for (int u = 0; u < RU; u++) d[u] += __shfl_xor(d[u], m, lpr);
+                for (int u = 0; u < RU; u++)
+                    #pragma unroll
+                    for (int v = 0; v < NV; v++) d[u][v] += __shfl_xor(d[u][v], s, lpr);
             }
             if (lane == 0) {
                 #pragma unroll
                 for (int u = 0; u < RU; u++) {
                     const int r = r0 + u * G;
                     if (r < R) {
-                        if (t.flags & 1) Wv[t.out0 + r] += d[u];
-                        else Wv[t.out0 + r] = d[u];
+                        double *o = Wv + (long long)(t.out0 + r) * NV;
+                        #pragma unroll
+                        for (int v = 0; v < NV; v++) o[v] = d[u][v];
                     }
                 }
             }
         }
     }
     if (do1) {
-        // reduce across row groups: first inside the wave, then across waves
+        // reduce across the row groups of a wave, then across the 4 waves
         #pragma unroll
         for (int k = 0; k < KP; k++)
             #pragma unroll
             for (int w = 0; w < W; w++)
-                for (int m = lpr; m < 64; m <<= 1) a1[k][w] += __shfl_xor(a1[k][w], m, 64);
+                #pragma unroll
+                for (int v = 0; v < NV; v++)
+                    for (int s = lpr; s < 64; s <<= 1) a1[k][w][v] += __shfl_xor(a1[k][w][v], s, 64);
+        __syncthreads();   // the staged in1 is dead from here on
+        double *red = lds;
         const int wave = tid >> 6;
         const int lw = tid & 63;
+        constexpr int STRIDE = 64 * CPL * NV;   // doubles per wave
         if (lw < lpr) {
             #pragma unroll
             for (int k = 0; k < KP; k++)
                 #pragma unroll
-                for (int w = 0; w < W; w++) red[wave * MAX_TILE_COLS + (k * lpr + lw) * W + w] = a1[k][w];
+                for (int w = 0; w < W; w++)
+                    #pragma unroll
+                    for (int v = 0; v < NV; v++) red[wave * STRIDE + ((k * lpr + lw) * W + w) * NV + v] = a1[k][w][v];
         }
         __syncthreads();
-        for (int c = tid; c < C; c += WG) {
-            double s = red[c] + red[MAX_TILE_COLS + c] + red[2 * MAX_TILE_COLS + c] + red[3 * MAX_TILE_COLS + c];
-            Wv[t.out1 + c] = s;
+        for (int q = tid; q < C * NV; q += WG) {
+            const double s = (red[q] + red[STRIDE + q]) + (red[2 * STRIDE + q] + red[3 * STRIDE + q]);
+            Wv[(long long)t.out1 * NV + q] = s;
         }
     }
 }
 
-// One launch runs n64 FP64 tasks followed by n32 FP32 tasks.
-__global__ __launch_bounds__(WG) void bgemv_kernel(const BTask *__restrict__ t64, int n64, const BTask *__restrict__ t32,
-                                                   int n32, const double *__restrict__ S64,
-                                                   const float *__restrict__ S32, double *__restrict__ Wv) {
-    __shared__ double red[4 * MAX_TILE_COLS];
+// Fixed-order sum of slots into one chunk.  Lane i owns element i of the
+// chunk; wave w sums the sources w, w+4, ... in order; the four wave sums are
+// added as (w0 + w1) + (w2 + w3).
+template <int NV>
+__device__ void reduce_task(const RTask &t, const int *__restrict__ src, double *__restrict__ Wv,
+                            double *__restrict__ Y, long long ldy, double *lds) {
+    double *red = lds;   // 4 x RCHUNK * NV
+    const int w = threadIdx.x >> 6;
+    for (int q = threadIdx.x & 63; q < RCHUNK * NV; q += 64) {
+        double s = 0.0;
+        if (q < t.len * NV) {
+            const double *base = Wv + (long long)t.soff * NV + q;
+            int j = t.src_beg + w;
+            for (; j + 12 < t.src_end; j += 16) {
+                const double a0 = base[(long long)src[j] * NV], a1 = base[(long long)src[j + 4] * NV];
+                const double a2 = base[(long long)src[j + 8] * NV], a3 = base[(long long)src[j + 12] * NV];
+                s += a0;
+                s += a1;
+                s += a2;
+                s += a3;
+            }
+            for (; j < t.src_end; j += 4) s += base[(long long)src[j] * NV];
+        }
+        red[w * RCHUNK * NV + q] = s;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < t.len * NV; q += WG) {
+        const double s = (red[q] + red[RCHUNK * NV + q]) + (red[2 * RCHUNK * NV + q] + red[3 * RCHUNK * NV + q]);
+        if (t.flags & 1) {
+            const int i = q / NV, v = q - i * NV;
+            Y[(long long)v * ldy + t.dst + i] = s;
+        } else {
+            Wv[t.dst * NV + q] = s;
+        }
+    }
+}
+
+// One launch per stage: n64 FP64 tiles, then n32 FP32 tiles, then nr reduce
+// tasks (all independent of each other within the launch).
+template <int NV>
+__global__ __launch_bounds__(WG) void stage_kernel(const BTask *__restrict__ t64, int n64, const BTask *__restrict__ t32,
+                                                   int n32, const RTask *__restrict__ rt, int nr,
+                                                   const int *__restrict__ vsrc, const int *__restrict__ rsrc,
+                                                   const double *__restrict__ S64, const float *__restrict__ S32,
+                                                   double *__restrict__ Wv, const double *__restrict__ X,
+                                                   double *__restrict__ Y, long long ldx) {
+    __shared__ double lds[LDS_DOUBLES];
     const int b = blockIdx.x;
     if (b < n64) {
         const BTask t = t64[b];
-        bgemv_tile<double>(t, S64, Wv, red);
+        bgemv_tile<double, NV>(t, S64, vsrc, Wv, X, ldx, lds);
     } else if (b - n64 < n32) {
         const BTask t = t32[b - n64];
-        bgemv_tile<float>(t, S32, Wv, red);
-    }
-}
-
-// Segmented fixed-order sum of partial slots into each destination chunk.
-// Lane i owns element i of the chunk; wave w sums the sources w, w+4, ...
-// (four loads in flight per lane), then the four wave sums are added in
-// wave order, so the result is deterministic.
-__global__ __launch_bounds__(WG) void reduce_kernel(const RTask *__restrict__ rt, const int *__restrict__ src,
-                                                    double *__restrict__ Wv, double *__restrict__ ext) {
-    __shared__ double red[4][RCHUNK];
-    const RTask t = rt[blockIdx.x];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    double s = 0.0;
-    if (lane < t.len) {
-        const double *base = Wv + t.soff + lane;
-        int j = t.src_beg + w;
-        for (; j + 12 < t.src_end; j += 16) {
-            const double a0 = base[src[j]], a1 = base[src[j + 4]], a2 = base[src[j + 8]], a3 = base[src[j + 12]];
-            s += a0;
-            s += a1;
-            s += a2;
-            s += a3;
-        }
-        for (; j < t.src_end; j += 4) s += base[src[j]];
-    }
-    red[w][lane] = s;
-    __syncthreads();
-    if (w == 0 && lane < t.len) {
-        const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-        double *dst = ((t.flags & 1) ? ext : Wv) + t.dst + lane;
-        if (t.flags & 2) *dst += v;
-        else *dst = v;
+        bgemv_tile<float, NV>(t, S32, vsrc, Wv, X, ldx, lds);
+    } else if (b - n64 - n32 < nr) {
+        const RTask t = rt[b - n64 - n32];
+        reduce_task<NV>(t, rsrc, Wv, Y, ldx, lds);
     }
 }
 
@@ -267,65 +387,132 @@ __global__ void zero_kernel(double *__restrict__ p, size_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// Plan
+// Storage (uploaded once per precision) and plans (one per batch width)
 // ---------------------------------------------------------------------------
 enum Phase { PH_UP = 0, PH_B = 1, PH_Y1 = 2, PH_DOWN = 3, PH_D = 4, PH_GATHER = 5, PH_N = 6 };
+
+struct MatRef {
+    long long off = -1;
+    int nrow = 0, ncol = 0, ld = 0;
+    bool f32 = false;
+};
+struct BlkRef {
+    bool lr = false;
+    MatRef M;      // dense block
+    MatRef U, V;   // low-rank halves: U (nrow x k), V^T (ncol x k)
+};
+
+struct Storage {
+    std::vector<double> h64;
+    std::vector<float> h32;
+    std::vector<MatRef> U;
+    std::vector<BlkRef> B, D;
+    double bytes_U = 0, bytes_B = 0, bytes_D = 0;   // algorithmic (U counted twice)
+
+    MatRef store(const double *src, int nrow, int ncol, bool fp32) {
+        MatRef m;
+        const int W = fp32 ? 4 : 2;
+        m.nrow = nrow; m.ncol = ncol; m.f32 = fp32;
+        m.ld = (ncol + W - 1) / W * W;
+        const size_t n = (size_t)nrow * m.ld;
+        if (fp32) {
+            m.off = (long long)h32.size();
+            h32.resize(h32.size() + ((n + 63) & ~(size_t)63), 0.0f);
+            for (int r = 0; r < nrow; r++)
+                for (int c = 0; c < ncol; c++) h32[m.off + (size_t)r * m.ld + c] = (float)src[(size_t)r * ncol + c];
+        } else {
+            m.off = (long long)h64.size();
+            h64.resize(h64.size() + ((n + 31) & ~(size_t)31), 0.0);
+            for (int r = 0; r < nrow; r++) memcpy(&h64[m.off + (size_t)r * m.ld], src + (size_t)r * ncol, sizeof(double) * ncol);
+        }
+        return m;
+    }
+    BlkRef store_blk(H2E_dense_mat_p b, bool fp32, double &bytes) {
+        BlkRef r;
+        if (!b || b->nrow == 0 || b->ncol == 0) return r;
+        const double esz = fp32 ? 4.0 : 8.0;
+        if (b->ld >= 0) {
+            r.M = store(b->data, b->nrow, b->ncol, fp32);
+            bytes += esz * b->nrow * b->ncol;
+        } else {
+            const int k = -b->ld;
+            r.lr = true;
+            r.U = store(b->data, b->nrow, k, fp32);
+            r.V = store(b->data + (size_t)b->nrow * k, b->ncol, k, fp32);
+            bytes += esz * (double)(b->nrow + b->ncol) * k;
+        }
+        return r;
+    }
+    void build(H2ERI_p h, bool ub32) {
+        U.assign(h->n_node, MatRef());
+        for (int i = 0; i < h->n_node; i++) {
+            H2E_dense_mat_p u = h->U[i];
+            if (!u || u->nrow == 0 || u->ncol == 0) continue;
+            U[i] = store(u->data, u->nrow, u->ncol, ub32);
+            bytes_U += 2.0 * (ub32 ? 4.0 : 8.0) * u->nrow * u->ncol;
+        }
+        B.resize(h->n_B);
+        for (int i = 0; i < h->n_B; i++) B[i] = store_blk(h->c_B_blks[i], ub32, bytes_B);
+        D.resize(h->n_D);
+        for (int i = 0; i < h->n_D; i++) D[i] = store_blk(h->c_D_blks[i], false, bytes_D);
+    }
+};
 
 struct Stage {
     int phase;
     int t64_beg, t64_cnt, t32_beg, t32_cnt;
     int r_beg, r_cnt;
-    double mat_bytes;   // matrix bytes streamed by this stage's tile launch
+    double mat_bytes;   // matrix bytes streamed by this launch
 };
 
-// One timed launch of the last matvec (phase timing mode).
+struct Plan {
+    int nv = 1;
+    std::vector<BTask> t64, t32;
+    std::vector<RTask> rt;
+    std::vector<int> rsrc, vsrc;
+    std::vector<Stage> stages;
+    long long nW = 0;   // workspace elements per vector
+    double bytes_vec = 0;
+    int n_fold = 0;     // fold reduce tasks
+};
+
+// One timed launch of the last call (phase timing mode).
 struct LaunchRec {
-    int phase, kind;    // kind 0: bi-GEMV tiles, 1: segmented reduce
-    int blocks;
+    int phase, blocks;
     double mat_bytes;
     float ms;
 };
 
-struct Segment {
-    long long off;   // workspace offset, or output offset when ext
-    int len;
-    bool ext;
-    std::vector<int> srcs;
-    bool acc = false;   // reduce adds to the destination
+struct DevPlan {
+    Plan P;
+    bool ready = false;
+    double *Wv = nullptr;
+    BTask *d_t64 = nullptr, *d_t32 = nullptr;
+    RTask *d_rt = nullptr;
+    int *d_rsrc = nullptr, *d_vsrc = nullptr;
 };
 
-struct Plan {
-    std::vector<BTask> t64, t32;
-    std::vector<RTask> rt;
-    std::vector<int> rsrc;
-    std::vector<Stage> stages;
-    long long nW = 0;
-    long long n64 = 0, n32 = 0;
-    double bytes_U = 0, bytes_B = 0, bytes_D = 0, bytes_vec = 0;
-};
+static const int NV_VARIANTS[4] = {1, 2, 4, 8};
+static inline int nv_index(int nv) { return nv == 1 ? 0 : (nv == 2 ? 1 : (nv == 4 ? 2 : 3)); }
 
 struct DevState {
     int dev = -1;
     int ub_bits = 64;
-    bool ready = false;
     double tile_bytes = 128.0 * 1024.0;
     int nf_schedule = 0;      // 0: near field balanced over the sweep stages, 1: own stages first
-    Plan plan;
+    Storage st;
+    bool st_ready = false;
     double *S64 = nullptr;
     float *S32 = nullptr;
-    double *Wv = nullptr;
-    BTask *d_t64 = nullptr, *d_t32 = nullptr;
-    RTask *d_rt = nullptr;
-    int *d_rsrc = nullptr;
-    // Coulomb
+    DevPlan plans[4];
+    // Coulomb and host-pointer staging
     int *d_gidx = nullptr;
-    double *d_D = nullptr, *d_J = nullptr, *d_Jh = nullptr, *d_y = nullptr;
-    double *d_xio = nullptr;    // host-pointer path staging
-    size_t xio_len = 0;
+    double *d_D = nullptr, *d_J = nullptr, *d_Jh = nullptr, *d_y = nullptr, *d_x = nullptr;
+    size_t d_io_len = 0;
     bool phase_timing = false;
     std::vector<hipEvent_t> evs;
     std::vector<LaunchRec> launches;
-    bool collected = true;   // launch times of the last timed call already read
+    bool collected = true;
     double phase_ms[7] = {0, 0, 0, 0, 0, 0, 0};
     size_t resident_bytes = 0;
 };
@@ -346,13 +533,24 @@ static void free_dev(void *&p) {
     if (p) { (void)hipFree(p); p = nullptr; }
 }
 
+static void release_plans(DevState *d) {
+    for (DevPlan &p : d->plans) {
+        void **ps[] = {(void **)&p.Wv, (void **)&p.d_t64, (void **)&p.d_t32, (void **)&p.d_rt, (void **)&p.d_rsrc,
+                       (void **)&p.d_vsrc};
+        for (void **q : ps) free_dev(*q);
+        p.ready = false;
+        p.P = Plan();
+    }
+}
+
 static void release_buffers(DevState *d) {
-    void **ps[] = {(void **)&d->S64, (void **)&d->S32, (void **)&d->Wv, (void **)&d->d_t64, (void **)&d->d_t32,
-                   (void **)&d->d_rt, (void **)&d->d_rsrc, (void **)&d->d_gidx, (void **)&d->d_D,
-                   (void **)&d->d_J, (void **)&d->d_Jh, (void **)&d->d_y, (void **)&d->d_xio};
+    release_plans(d);
+    void **ps[] = {(void **)&d->S64, (void **)&d->S32, (void **)&d->d_gidx, (void **)&d->d_D, (void **)&d->d_J,
+                   (void **)&d->d_Jh, (void **)&d->d_y, (void **)&d->d_x};
     for (void **p : ps) free_dev(*p);
-    d->xio_len = 0;
-    d->ready = false;
+    d->d_io_len = 0;
+    d->st = Storage();
+    d->st_ready = false;
     d->resident_bytes = 0;
 }
 
@@ -377,169 +575,346 @@ static DevState *dstate_get(H2ERI_p h) {
 
 namespace {
 
-struct Builder {
-    H2ERI_p h;
-    Plan &P;
-    int ub_bits;
-    double tile_bytes;
-    std::vector<double> host64;
-    std::vector<float> host32;
-    std::vector<Segment> segs;
+struct Src {
+    int off;     // workspace element offset (or the encoded external input)
+    int stage;   // launch that writes it (-1: external input)
+};
 
-    int nf_schedule = 0;
-    Builder(H2ERI_p h_, Plan &p_, int bits, double tb, int nfs = 0)
-        : h(h_), P(p_), ub_bits(bits), tile_bytes(tb), nf_schedule(nfs) {}
+// Virtual vector: element i = sum over srcs (in order) of the source at i.
+struct VVec {
+    int len = 0;
+    std::vector<Src> srcs;
+    int fc = INT_MAX;     // first consuming stage
+    double reads = 0;     // elements read by its consumer tiles
+    int lb = -1, le = -1; // range in vsrc once emitted
+};
+
+// Tile before emission: inputs are vector ids.
+struct PTile {
+    BTask t;
+    int st;
+    bool f32;
+    int iv0, iv1;
+};
+
+struct Planner {
+    H2ERI_p h;
+    const Storage &S;
+    Plan &P;
+    int nv;
+    double tile_bytes;
+    int nf_schedule;
+    int maxc;   // max tile columns for this batch width
+    std::vector<VVec> vv;
+    std::vector<PTile> tiles;
+    std::vector<std::vector<RTask>> SR;   // reduce tasks of each stage
+    std::vector<int> sphase;
+    std::vector<double> sload;
+    int nst = 0, sG = 0;
+
+    Planner(H2ERI_p h_, const Storage &s_, Plan &p_, int nv_, double tb, int nfs)
+        : h(h_), S(s_), P(p_), nv(nv_), tile_bytes(tb), nf_schedule(nfs), maxc(64 * cpl_of(nv_)) {}
 
     long long walloc(long long n) {
         long long o = P.nW;
-        P.nW += (n + 7) & ~7LL;   // keep 64-byte alignment of every region
+        P.nW += (n + 7) & ~7LL;
         return o;
     }
-
-    // store a row-major nrow x ncol matrix (source ld = ncol) with padded ld
-    long long store(const double *src, int nrow, int ncol, bool fp32, int &ld_out) {
-        const int W = fp32 ? 4 : 2;
-        int ld = (ncol + W - 1) / W * W;
-        ld_out = ld;
-        if (fp32) {
-            long long off = (long long)host32.size();
-            size_t n = (size_t)nrow * ld;
-            host32.resize(host32.size() + ((n + 63) & ~(size_t)63), 0.0f);
-            for (int r = 0; r < nrow; r++)
-                for (int c = 0; c < ncol; c++) host32[off + (size_t)r * ld + c] = (float)src[(size_t)r * ncol + c];
-            return off;
-        }
-        long long off = (long long)host64.size();
-        size_t n = (size_t)nrow * ld;
-        host64.resize(host64.size() + ((n + 31) & ~(size_t)31), 0.0);
-        for (int r = 0; r < nrow; r++) memcpy(&host64[off + (size_t)r * ld], src + (size_t)r * ncol, sizeof(double) * ncol);
-        return off;
+    int new_vec(int len) {
+        VVec v;
+        v.len = len;
+        vv.push_back(v);
+        return (int)vv.size() - 1;
+    }
+    void use(int vec, int st, int len) {
+        VVec &v = vv[vec];
+        v.fc = std::min(v.fc, st);
+        v.reads += len;
     }
 
-    int new_seg(long long off, int len, bool ext) {
-        segs.push_back(Segment{off, len, ext, {}, false});
-        return (int)segs.size() - 1;
-    }
-
-    // Tile an nrow x ncol matrix into tasks.  dst0/dst1 are segment ids (-1 none).
-    // out0 of the matrix goes to dst0, out1 to dst1.  When acc0_off >= 0 the
-    // row dots are accumulated straight into the workspace at acc0_off (one
-    // column tile only; unique writer).
-    void add_matrix(std::vector<BTask> &tl, long long mat, int nrow, int ncol, int ld, bool fp32, int in0, int in1,
-                    int dst0, int dst1, long long acc0_off = -1) {
-        if (nrow <= 0 || ncol <= 0) return;
-        const double esz = fp32 ? 4.0 : 8.0;
-        const int nct = (ncol + MAX_TILE_COLS - 1) / MAX_TILE_COLS;
-        // balanced column tile width, a multiple of 8 so every tile starts on a
-        // vector-aligned column (<= MAX_TILE_COLS since that is a multiple of 8)
-        const int ctw = ((ncol + nct - 1) / nct + 7) / 8 * 8;
+    // Tiles of rows [r0, r0+nrow) of matrix m in stage st.  in0 (length ncol)
+    // is vector v0 at element offset o0, in1 (length nrow) vector v1 at o1;
+    // out0 feeds vector d0 (full length nrow), out1 vector d1 (full length ncol).
+    void add_matrix(int st, const MatRef &m, int r0_, int nrow, int v0, int o0, int v1, int o1, int d0, int d1) {
+        const int ncol = m.ncol;
+        if (nrow <= 0 || ncol <= 0 || (d0 < 0 && d1 < 0)) return;
+        const double esz = m.f32 ? 4.0 : 8.0;
+        const int nct = (ncol + maxc - 1) / maxc;
+        const int ctw = ((ncol + nct - 1) / nct + 7) / 8 * 8;   // multiple of 8 (vector alignment), <= maxc
         int rows = (int)std::max(1.0, tile_bytes / (esz * ctw));
+        if (d1 >= 0) rows = std::min(rows, YIN_DOUBLES / nv);
         const int nrt = (nrow + rows - 1) / rows;
-        rows = (nrow + nrt - 1) / nrt;                 // balanced row tiles
-        // partial slots: out0 per column tile (full length), out1 per row tile
+        rows = (nrow + nrt - 1) / nrt;
         std::vector<long long> s0(nct, -1), s1(nrt, -1);
-        if (dst0 >= 0 && acc0_off < 0)
-            for (int c = 0; c < nct; c++) { s0[c] = walloc(nrow); segs[dst0].srcs.push_back((int)s0[c]); }
-        if (dst1 >= 0)
-            for (int r = 0; r < nrt; r++) { s1[r] = walloc(ncol); segs[dst1].srcs.push_back((int)s1[r]); }
+        if (d0 >= 0) for (int c = 0; c < nct; c++) { s0[c] = walloc(nrow); vv[d0].srcs.push_back(Src{(int)s0[c], st}); }
+        if (d1 >= 0) for (int r = 0; r < nrt; r++) { s1[r] = walloc(ncol); vv[d1].srcs.push_back(Src{(int)s1[r], st}); }
         for (int rt = 0; rt < nrt; rt++) {
-            const int r0 = rt * rows, nr = std::min(rows, nrow - r0);
+            const int ra = rt * rows, nr = std::min(rows, nrow - ra);
             for (int ct = 0; ct < nct; ct++) {
                 const int c0 = ct * ctw, nc = std::min(ctw, ncol - c0);
-                BTask t;
-                t.mat = mat + (long long)r0 * ld + c0;
+                PTile p;
+                memset(&p.t, 0, sizeof(p.t));
+                BTask &t = p.t;
+                t.mat = m.off + (long long)(r0_ + ra) * m.ld + c0;
                 t.nrow = nr;
                 t.ncol = nc;
-                t.ld = ld;
+                t.ld = m.ld;
                 int lpr = 4;
-                while (lpr * COLS_PER_LANE < nc) lpr <<= 1;
+                while (lpr * cpl_of(nv) < nc) lpr <<= 1;
                 t.lpr = lpr;
-                t.in0 = (dst0 >= 0 || acc0_off >= 0) ? in0 + c0 : -1;
-                t.in1 = dst1 >= 0 ? in1 + r0 : -1;
-                if (acc0_off >= 0) { t.out0 = (int)(acc0_off + r0); t.flags = 1; }
-                else { t.out0 = dst0 >= 0 ? (int)(s0[ct] + r0) : -1; t.flags = 0; }
-                t.out1 = dst1 >= 0 ? (int)(s1[rt] + c0) : -1;
-                t.pad_ = 0;
-                if (t.out0 < 0 && t.out1 < 0) continue;
-                tl.push_back(t);
+                t.v0off = o0 + c0;
+                t.v1off = o1 + ra;
+                t.out0 = d0 >= 0 ? (int)(s0[ct] + ra) : -1;
+                t.out1 = d1 >= 0 ? (int)(s1[rt] + c0) : -1;
+                p.st = st;
+                p.f32 = m.f32;
+                p.iv0 = d0 >= 0 ? v0 : -1;
+                p.iv1 = d1 >= 0 ? v1 : -1;
+                if (p.iv0 >= 0) use(v0, st, nc);
+                if (p.iv1 >= 0) use(v1, st, nr);
+                tiles.push_back(p);
+                sload[st] += esz * nr * nc;
             }
         }
     }
 
-    // ---- stages: tiles of stage s run in one launch, then the reduces of s ----
-    std::vector<std::vector<BTask>> S64, S32;
-    std::vector<std::vector<int>> SR;
-    std::vector<int> sphase;
-    std::vector<double> sload;
-
-    std::vector<BTask> &T(int st, bool fp32) { return fp32 ? S32[st] : S64[st]; }
-
-    static double tasks_bytes(const std::vector<BTask> &tl, size_t from, double esz) {
-        double b = 0;
-        for (size_t i = from; i < tl.size(); i++) b += esz * tl[i].nrow * tl[i].ncol;
-        return b;
-    }
-
-    // add_matrix into stage st, accounting its bytes to the stage load
-    void add_at(int st, long long mat, int nrow, int ncol, int ld, bool fp32, long long in0, long long in1, int dst0,
-                int dst1, long long acc0 = -1) {
-        std::vector<BTask> &tl = T(st, fp32);
-        size_t n0 = tl.size();
-        add_matrix(tl, mat, nrow, ncol, ld, fp32, (int)in0, (int)in1, dst0, dst1, acc0);
-        sload[st] += tasks_bytes(tl, n0, fp32 ? 4.0 : 8.0);
-    }
-
-    // Dense block: out0 = M in0 -> dst0, out1 = M^T in1 -> dst1, one pass.
-    void add_dense(H2E_dense_mat_p blk, bool fp32, long long in0, long long in1, int dst0, int dst1, int st,
-                   double &bytes) {
-        int ld;
-        long long off = store(blk->data, blk->nrow, blk->ncol, fp32, ld);
-        add_at(st, off, blk->nrow, blk->ncol, ld, fp32, in0, in1, dst0, dst1);
-        bytes += (fp32 ? 4.0 : 8.0) * blk->nrow * blk->ncol;
-    }
-
-    // Low-rank block M = U V (U: nrow x k, V^T: ncol x k), out0 = U (V in0),
-    // out1 = V^T (U^T in1), in stages a < b < c.  With both outputs the
-    // smaller half is streamed in a and c and the other half once in b with a
-    // fused bi-GEMV, so 1.5 passes instead of 2; with out0 only, V^T then U.
-    void add_lowrank(H2E_dense_mat_p blk, bool fp32, long long in0, long long in1, int dst0, int dst1, int a, int b,
-                     int c, double &bytes) {
-        const int nr = blk->nrow, nc = blk->ncol, k = -blk->ld;
-        int Uld, Vld;
-        const long long Uo = store(blk->data, nr, k, fp32, Uld);
-        const long long Vo = store(blk->data + (size_t)nr * k, nc, k, fp32, Vld);
-        bytes += (fp32 ? 4.0 : 8.0) * (double)(nr + nc) * k;
-        const long long t_off = walloc(k), s_off = walloc(k);
-        const int st_t = new_seg(t_off, k, false), st_s = new_seg(s_off, k, false);
-        if (dst1 < 0) {
-            add_at(a, Vo, nc, k, Vld, fp32, -1, in0, -1, st_t);          // t = V in0
-            SR[a].push_back(st_t);
-            add_at(b, Uo, nr, k, Uld, fp32, t_off, -1, dst0, -1);        // out0 = U t
+    // Dense or low-rank block with inputs (v0,o0) / (v1,o1) and outputs d0 / d1
+    // in stages a < b < c (low-rank: smaller half in a and c, the other in b).
+    void add_block(const BlkRef &blk, int v0, int o0, int v1, int o1, int d0, int d1, int a, int b, int c) {
+        if (!blk.lr) {
+            if (blk.M.off < 0) return;
+            add_matrix(a, blk.M, 0, blk.M.nrow, v0, o0, v1, o1, d0, d1);
+            return;
+        }
+        const int nr = blk.U.nrow, nc = blk.V.nrow, k = blk.U.ncol;
+        const int vt = new_vec(k), vs = new_vec(k);
+        if (d1 < 0) {
+            add_matrix(a, blk.V, 0, nc, -1, 0, v0, o0, -1, vt);   // t = V in0
+            add_matrix(b, blk.U, 0, nr, vt, 0, -1, 0, d0, -1);    // out0 = U t
             return;
         }
         if (nr <= nc) {   // U twice
-            add_at(a, Uo, nr, k, Uld, fp32, -1, in1, -1, st_s);          // s = U^T in1
-            SR[a].push_back(st_s);
-            add_at(b, Vo, nc, k, Vld, fp32, s_off, in0, dst1, st_t);     // out1 = V^T s, t = V in0
-            SR[b].push_back(st_t);
-            add_at(c, Uo, nr, k, Uld, fp32, t_off, -1, dst0, -1);        // out0 = U t
+            add_matrix(a, blk.U, 0, nr, -1, 0, v1, o1, -1, vs);   // s = U^T in1
+            add_matrix(b, blk.V, 0, nc, vs, 0, v0, o0, d1, vt);   // out1 = V^T s, t = V in0
+            add_matrix(c, blk.U, 0, nr, vt, 0, -1, 0, d0, -1);    // out0 = U t
         } else {          // V^T twice
-            add_at(a, Vo, nc, k, Vld, fp32, -1, in0, -1, st_t);          // t = V in0
-            SR[a].push_back(st_t);
-            add_at(b, Uo, nr, k, Uld, fp32, t_off, in1, dst0, st_s);     // out0 = U t, s = U^T in1
-            SR[b].push_back(st_s);
-            add_at(c, Vo, nc, k, Vld, fp32, s_off, -1, dst1, -1);        // out1 = V^T s
+            add_matrix(a, blk.V, 0, nc, -1, 0, v0, o0, -1, vt);   // t = V in0
+            add_matrix(b, blk.U, 0, nr, vt, 0, v1, o1, d0, vs);   // out0 = U t, s = U^T in1
+            add_matrix(c, blk.V, 0, nc, vs, 0, -1, 0, d1, -1);    // out1 = V^T s
         }
     }
 
-    void finish_stages() {
+    // Reduce tasks at stage st: slot dst (or external output) = sum of srcs.
+    void emit_reduce(int st, long long dst, bool external, int len, const std::vector<Src> &srcs) {
+        const int sb = (int)P.rsrc.size();
+        for (const Src &s : srcs) P.rsrc.push_back(s.off);
+        for (int e0 = 0; e0 < len; e0 += RCHUNK) {
+            RTask r;
+            memset(&r, 0, sizeof(r));
+            r.dst = dst + e0;
+            r.len = std::min(RCHUNK, len - e0);
+            r.src_beg = sb;
+            r.src_end = (int)P.rsrc.size();
+            r.soff = e0;
+            r.flags = external ? 1 : 0;
+            SR[st].push_back(r);
+        }
+    }
+
+    // Replaces the sources written before stage c - 1 by one fold slot written
+    // in stage c - 1 when that saves reads (consumer tiles: `reads` elements
+    // in total) or, for reduce consumers, when there are many of them.
+    std::vector<Src> fold(const std::vector<Src> &L, int c, double reads, int len, bool reducer, int depth) {
+        if (c - 1 < 0 || depth > 8) return L;
+        std::vector<Src> early, late;
+        for (const Src &s : L) {
+            if (s.stage >= 0 && s.stage <= c - 2) early.push_back(s);
+            else late.push_back(s);
+        }
+        const double m = (double)early.size();
+        if (m < 2) return L;
+        const bool worth = reducer ? m >= 4 : (m - 1) * reads > (m + 1) * (double)len;
+        if (!worth) return L;
+        std::vector<Src> sub = fold(early, c - 1, len, len, true, depth + 1);
+        const long long slot = walloc(len);
+        emit_reduce(c - 1, slot, false, len, sub);
+        P.n_fold++;
+        std::vector<Src> out;
+        out.push_back(Src{(int)slot, c - 1});
+        out.insert(out.end(), late.begin(), late.end());
+        return out;
+    }
+
+    std::pair<int, int> emit_vec(int vec) {
+        VVec &v = vv[vec];
+        if (v.lb < 0) {
+            v.lb = (int)P.vsrc.size();
+            for (const Src &s : v.srcs) P.vsrc.push_back(s.off);
+            v.le = (int)P.vsrc.size();
+        }
+        return {v.lb, v.le};
+    }
+
+    bool build() {
+        const int n_node = h->n_node, nleaf = h->n_leaf_node, N = h->krnl_mat_size;
+        P = Plan();
+        P.nv = nv;
+        // the external input: one vector whose only source is the encoded "-1"
+        const int X = new_vec(N);
+        vv[X].srcs.push_back(Src{-1, -1});
+        std::vector<int> y0(n_node, -1), y1(n_node, -1), yv(n_node, -1);
+        long long tot_r = 0;
+        for (int i = 0; i < n_node; i++) {
+            const int rk = S.U[i].off >= 0 ? S.U[i].ncol : 0;
+            tot_r += rk;
+            if (rk > 0) { y0[i] = new_vec(rk); y1[i] = new_vec(rk); }
+            if (h->n_child[i] == 0) yv[i] = new_vec(h->mat_cluster[2 * i + 1] - h->mat_cluster[2 * i] + 1);
+        }
+        auto xo = [&](int node) { return h->mat_cluster[2 * node]; };
+        // stages along the critical path
+        const int minL = h->min_adm_level, maxL = h->max_level;
+        const int nU = (minL >= 0 && minL <= maxL) ? (maxL - minL + 1) : 0;
+        const int s0 = nf_schedule == 1 ? 3 : 0;
+        const int sBA = s0 + nU, sBB = sBA + 1, sBC = sBA + 2;
+        const int sD0 = sBA + 3;
+        sG = sD0 + nU;
+        nst = sG + 1;
+        SR.assign(nst, {});
+        sload.assign(nst, 0.0);
+        sphase.assign(nst, PH_GATHER);
+        for (int q = 0; q < s0; q++) sphase[q] = PH_D;
+        for (int q = 0; q < nU; q++) { sphase[s0 + q] = PH_UP; sphase[sD0 + q] = PH_DOWN; }
+        sphase[sBA] = PH_B; sphase[sBB] = PH_B; sphase[sBC] = PH_Y1;
+        // y0 of a node is needed when a coupling block reads it or its parent's is needed
+        std::vector<char> need0(n_node, 0);
+        for (int i = 0; i < h->n_B; i++) {
+            const int n0 = h->r_adm_pairs[2 * i], n1 = h->r_adm_pairs[2 * i + 1];
+            const int l0 = h->node_level[n0], l1 = h->node_level[n1];
+            if (l0 >= l1) need0[n0] = 1;
+            if (l1 >= l0) need0[n1] = 1;
+        }
+        for (int lv = 1; lv <= maxL; lv++)
+            for (int j = 0; j < h->level_n_node[lv]; j++) {
+                const int node = h->level_nodes[(size_t)lv * nleaf + j];
+                if (h->parent[node] >= 0 && need0[h->parent[node]]) need0[node] = 1;
+            }
+        // ---- upward sweep: level maxL - q at stage s0 + q; inner nodes tile by child
+        for (int lv = maxL; nU > 0 && lv >= minL; lv--) {
+            const int st = s0 + maxL - lv;
+            for (int j = 0; j < h->level_n_node[lv]; j++) {
+                const int node = h->level_nodes[(size_t)lv * nleaf + j];
+                if (y0[node] < 0 || !need0[node]) continue;
+                const MatRef &U = S.U[node];
+                if (h->n_child[node] == 0) {
+                    add_matrix(st, U, 0, U.nrow, -1, 0, X, xo(node), -1, y0[node]);
+                } else {
+                    int row = 0;
+                    for (int k = 0; k < h->n_child[node]; k++) {
+                        const int ch = h->children[(size_t)node * h->max_child + k];
+                        const int rc = S.U[ch].off >= 0 ? S.U[ch].ncol : 0;
+                        if (rc > 0) add_matrix(st, U, row, rc, -1, 0, y0[ch], 0, -1, y0[node]);
+                        row += rc;
+                    }
+                    if (row != U.nrow) { set_error(ERR_ARG, "device plan: U row count of node %d", node); return false; }
+                }
+            }
+        }
+        // ---- coupling blocks in stages sBA..sBC
+        for (int i = 0; i < h->n_B; i++) {
+            const int n0 = h->r_adm_pairs[2 * i], n1 = h->r_adm_pairs[2 * i + 1];
+            const int l0 = h->node_level[n0], l1 = h->node_level[n1];
+            int v0, o0, v1, o1, d0, d1;
+            if (l0 == l1) { v0 = y0[n1]; o0 = 0; v1 = y0[n0]; o1 = 0; d0 = y1[n0]; d1 = y1[n1]; }
+            else if (l0 > l1) { v0 = X; o0 = xo(n1); v1 = y0[n0]; o1 = 0; d0 = y1[n0]; d1 = yv[n1]; }
+            else { v0 = y0[n1]; o0 = 0; v1 = X; o1 = xo(n0); d0 = yv[n0]; d1 = y1[n1]; }
+            if (v0 < 0 || v1 < 0 || d0 < 0 || d1 < 0) continue;   // zero-rank side: no contribution
+            add_block(S.B[i], v0, o0, v1, o1, d0, d1, sBA, sBB, sBC);
+        }
+        // ---- downward sweep: level minL + q at stage sD0 + q; inner nodes tile by child
+        for (int lv = minL; nU > 0 && lv <= maxL; lv++) {
+            const int st = sD0 + (lv - minL);
+            for (int j = 0; j < h->level_n_node[lv]; j++) {
+                const int node = h->level_nodes[(size_t)lv * nleaf + j];
+                if (y1[node] < 0 || vv[y1[node]].srcs.empty()) continue;   // nothing to push down
+                const MatRef &U = S.U[node];
+                if (h->n_child[node] == 0) {
+                    add_matrix(st, U, 0, U.nrow, y1[node], 0, -1, 0, yv[node], -1);
+                } else {
+                    int row = 0;
+                    for (int k = 0; k < h->n_child[node]; k++) {
+                        const int ch = h->children[(size_t)node * h->max_child + k];
+                        const int rc = S.U[ch].off >= 0 ? S.U[ch].ncol : 0;
+                        if (rc > 0) add_matrix(st, U, row, rc, y1[node], 0, -1, 0, y1[ch], -1);
+                        row += rc;
+                    }
+                }
+            }
+        }
+        // ---- near field: depends on x only, balanced over the tile launches
+        std::vector<int> order(h->n_D);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return blk_elems(h->c_D_blks[a]) > blk_elems(h->c_D_blks[b]); });
+        for (int i : order) {
+            const BlkRef &blk = S.D[i];
+            if (!blk.lr && blk.M.off < 0) continue;
+            int v0 = X, o0, v1 = X, o1, d0, d1;
+            if (i < nleaf) {
+                const int node = h->height_nodes[i];
+                o0 = xo(node); o1 = 0; d0 = yv[node]; d1 = -1; v1 = -1;
+            } else {
+                const int n0 = h->r_inadm_pairs[2 * (i - nleaf)], n1 = h->r_inadm_pairs[2 * (i - nleaf) + 1];
+                o0 = xo(n1); o1 = xo(n0); d0 = yv[n0]; d1 = yv[n1];
+            }
+            if (d0 < 0 || (d1 < 0 && i >= nleaf)) { set_error(ERR_ARG, "device plan: near-field block %d on a non-leaf", i); return false; }
+            const double bytes = 8.0 * (double)blk_elems(h->c_D_blks[i]);
+            const int span = !blk.lr ? 1 : (d1 < 0 ? 2 : 3);
+            int best = 0;
+            if (nf_schedule == 0) {
+                double bestv = 1e300;
+                for (int a = 0; a + span - 1 < sG; a++) {
+                    double v = 0;
+                    for (int q = 0; q < span; q++) v = std::max(v, sload[a + q] + bytes / span);
+                    if (v < bestv) { bestv = v; best = a; }
+                }
+            }
+            add_block(blk, v0, o0, v1, o1, d0, d1, best, best + 1, best + 2);
+        }
+        if (has_error()) return false;
+        // ---- folds of the tile inputs, then the final gather of every leaf
+        for (size_t q = 0; q < vv.size(); q++) {
+            VVec &v = vv[q];
+            if ((int)q == X || v.fc == INT_MAX) continue;
+            v.srcs = fold(v.srcs, v.fc, v.reads, v.len, false, 0);
+        }
+        for (int i = 0; i < n_node; i++) {
+            if (yv[i] < 0) continue;
+            const VVec &v = vv[yv[i]];
+            if (v.srcs.empty()) { set_error(ERR_ARG, "device plan: leaf %d receives no contribution", i); return false; }
+            emit_reduce(sG, xo(i), true, v.len, fold(v.srcs, sG, v.len, v.len, true, 0));
+        }
+        finish();
+        P.bytes_vec = 8.0 * nv * (2.0 * N + 4.0 * (double)tot_r);
+        if (P.nW * nv >= (1LL << 31)) { set_error(ERR_UNSUPPORTED, "workspace exceeds 2^31 doubles"); return false; }
+        return validate();
+    }
+
+    void finish() {
+        std::vector<std::vector<BTask>> S64(nst), S32(nst);
+        for (PTile &p : tiles) {
+            if (p.iv0 >= 0) { auto r = emit_vec(p.iv0); p.t.v0b = r.first; p.t.v0e = r.second; }
+            if (p.iv1 >= 0) { auto r = emit_vec(p.iv1); p.t.v1b = r.first; p.t.v1e = r.second; }
+            (p.f32 ? S32 : S64)[p.st].push_back(p.t);
+        }
         auto key = [](const BTask &t) { return (long long)t.nrow * t.ncol; };
-        for (size_t st = 0; st < S64.size(); st++) {
+        for (int st = 0; st < nst; st++) {
             std::stable_sort(S64[st].begin(), S64[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
             std::stable_sort(S32[st].begin(), S32[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
             Stage s;
             s.phase = sphase[st];
-            s.mat_bytes = tasks_bytes(S64[st], 0, 8.0) + tasks_bytes(S32[st], 0, 4.0);
+            s.mat_bytes = 0;
+            for (const BTask &t : S64[st]) s.mat_bytes += 8.0 * t.nrow * t.ncol;
+            for (const BTask &t : S32[st]) s.mat_bytes += 4.0 * t.nrow * t.ncol;
             s.t64_beg = (int)P.t64.size();
             s.t64_cnt = (int)S64[st].size();
             s.t32_beg = (int)P.t32.size();
@@ -547,233 +922,72 @@ struct Builder {
             P.t64.insert(P.t64.end(), S64[st].begin(), S64[st].end());
             P.t32.insert(P.t32.end(), S32[st].begin(), S32[st].end());
             s.r_beg = (int)P.rt.size();
-            for (int sid : SR[st]) {
-                Segment &g = segs[sid];
-                if (g.len <= 0) continue;
-                const int sb = (int)P.rsrc.size();
-                P.rsrc.insert(P.rsrc.end(), g.srcs.begin(), g.srcs.end());
-                for (int e0 = 0; e0 < g.len; e0 += RCHUNK) {
-                    RTask r;
-                    r.dst = g.off + e0;
-                    r.len = std::min(RCHUNK, g.len - e0);
-                    r.src_beg = sb;
-                    r.src_end = (int)P.rsrc.size();
-                    r.flags = (g.ext ? 1 : 0) | (g.acc ? 2 : 0);
-                    r.soff = e0;
-                    r.pad_ = 0;
-                    P.rt.push_back(r);
-                }
-            }
-            s.r_cnt = (int)P.rt.size() - s.r_beg;
+            s.r_cnt = (int)SR[st].size();
+            P.rt.insert(P.rt.end(), SR[st].begin(), SR[st].end());
             if (s.t64_cnt + s.t32_cnt + s.r_cnt > 0) P.stages.push_back(s);
         }
     }
 
-    bool build() {
-        const int n_node = h->n_node, nleaf = h->n_leaf_node, N = h->krnl_mat_size;
-        const bool ub32 = ub_bits == 32;
-        P = Plan();
-        const long long x_off = walloc(N);
-        // y0 / y1 with each node's children contiguous (children order)
-        std::vector<long long> y0o(n_node, -1), y1o(n_node, -1);
-        std::vector<int> rank(n_node, 0);
-        for (int i = 0; i < n_node; i++) rank[i] = h->U[i] ? h->U[i]->ncol : 0;
-        long long tot_r = 0;
-        for (int i = 0; i < n_node; i++) tot_r += rank[i];
-        const long long y0base = walloc(tot_r), y1base = walloc(tot_r);
-        {
-            long long pos = 0;
-            y0o[h->root_idx] = pos;
-            pos += rank[h->root_idx];
-            for (int lv = 0; lv <= h->max_level; lv++)
-                for (int j = 0; j < h->level_n_node[lv]; j++) {
-                    int node = h->level_nodes[(size_t)lv * nleaf + j];
-                    for (int k = 0; k < h->n_child[node]; k++) {
-                        int ch = h->children[(size_t)node * 8 + k];
-                        y0o[ch] = pos;
-                        pos += rank[ch];
-                    }
-                }
-            for (int i = 0; i < n_node; i++) { y1o[i] = y1base + y0o[i]; y0o[i] = y0base + y0o[i]; }
-        }
-        auto xseg = [&](int node) { return x_off + h->mat_cluster[2 * node]; };
-        auto nlen = [&](int node) { return h->mat_cluster[2 * node + 1] - h->mat_cluster[2 * node] + 1; };
-        std::vector<int> seg_y0(n_node, -1), seg_y1(n_node, -1), seg_y(n_node, -1);
-        for (int i = 0; i < n_node; i++) {
-            if (h->node_level[i] >= h->min_adm_level) {
-                seg_y0[i] = new_seg(y0o[i], rank[i], false);
-                seg_y1[i] = new_seg(y1o[i], rank[i], false);
-            }
-            if (h->n_child[i] == 0) seg_y[i] = new_seg(h->mat_cluster[2 * i], nlen(i), true);
-        }
-        // ---- stage layout along the critical path
-        const int minL = h->min_adm_level, maxL = h->max_level;
-        const int nU = (minL <= maxL) ? (maxL - minL + 1) : 0;
-        const int s0 = nf_schedule == 1 ? 3 : 0;   // near field first: three stages of its own
-        const int sBA = s0 + nU, sBB = sBA + 1, sBC = sBA + 2;
-        const int sD0 = sBA + 3;
-        const int sG = sD0 + nU;
-        const int nst = sG + 1;
-        S64.assign(nst, {});
-        S32.assign(nst, {});
-        SR.assign(nst, {});
-        sload.assign(nst, 0.0);
-        sphase.assign(nst, PH_GATHER);
-        for (int q = 0; q < s0; q++) sphase[q] = PH_D;
-        for (int s2 = 0; s2 < nU; s2++) { sphase[s0 + s2] = PH_UP; sphase[sD0 + s2] = PH_DOWN; }
-        sphase[sBA] = PH_B; sphase[sBB] = PH_B; sphase[sBC] = PH_Y1;
-        // storage of U
-        std::vector<long long> Uoff(n_node, -1);
-        std::vector<int> Uld(n_node, 0);
-        for (int i = 0; i < n_node; i++) {
-            H2E_dense_mat_p U = h->U[i];
-            if (!U || U->nrow == 0 || U->ncol == 0) continue;
-            Uoff[i] = store(U->data, U->nrow, U->ncol, ub32, Uld[i]);
-            P.bytes_U += 2.0 * (ub32 ? 4.0 : 8.0) * U->nrow * U->ncol;
-        }
-        // ---- upward sweep: level maxL - s at stage s
-        for (int lv = maxL; lv >= minL && nU > 0; lv--) {
-            const int st = s0 + maxL - lv;
-            for (int j = 0; j < h->level_n_node[lv]; j++) {
-                int node = h->level_nodes[(size_t)lv * nleaf + j];
-                SR[st].push_back(seg_y0[node]);
-                if (Uoff[node] < 0) continue;
-                H2E_dense_mat_p U = h->U[node];
-                long long in1 = (h->n_child[node] == 0) ? xseg(node) : y0o[h->children[(size_t)node * 8 + 0]];
-                add_at(st, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, -1, in1, -1, seg_y0[node]);
-            }
-        }
-        // ---- coupling blocks in stages sBA..sBC, y1 reduced after sBC
-        for (int i = 0; i < h->n_B; i++) {
-            H2E_dense_mat_p blk = h->c_B_blks[i];
-            if (blk->nrow == 0 || blk->ncol == 0) continue;
-            int n0 = h->r_adm_pairs[2 * i], n1 = h->r_adm_pairs[2 * i + 1];
-            int l0 = h->node_level[n0], l1 = h->node_level[n1];
-            long long in0, in1;
-            int d0, d1;
-            if (l0 == l1) { in0 = y0o[n1]; in1 = y0o[n0]; d0 = seg_y1[n0]; d1 = seg_y1[n1]; }
-            else if (l0 > l1) { in0 = xseg(n1); in1 = y0o[n0]; d0 = seg_y1[n0]; d1 = seg_y[n1]; }
-            else { in0 = y0o[n1]; in1 = xseg(n0); d0 = seg_y[n0]; d1 = seg_y1[n1]; }
-            if (blk->ld >= 0) add_dense(blk, ub32, in0, in1, d0, d1, sBA, P.bytes_B);
-            else add_lowrank(blk, ub32, in0, in1, d0, d1, sBA, sBB, sBC, P.bytes_B);
-        }
-        for (int i = 0; i < n_node; i++)
-            if (seg_y1[i] >= 0) SR[sBC].push_back(seg_y1[i]);
-        // ---- downward sweep: level minL + s at stage sD0 + s
-        for (int lv = minL; lv <= maxL && nU > 0; lv++) {
-            const int st = sD0 + (lv - minL);
-            for (int j = 0; j < h->level_n_node[lv]; j++) {
-                int node = h->level_nodes[(size_t)lv * nleaf + j];
-                if (Uoff[node] < 0) continue;
-                H2E_dense_mat_p U = h->U[node];
-                if (h->n_child[node] == 0) {
-                    add_at(st, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, y1o[node], -1, seg_y[node], -1);
-                } else if (U->ncol <= MAX_TILE_COLS) {
-                    // one column tile: each row has a unique writer, add in place
-                    long long acc = y1o[h->children[(size_t)node * 8 + 0]];
-                    add_at(st, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, y1o[node], -1, -1, -1, acc);
-                } else {
-                    // several column tiles: partial slots, reduce adds into y1 of the children
-                    int sg = new_seg(y1o[h->children[(size_t)node * 8 + 0]], U->nrow, false);
-                    segs[sg].acc = true;
-                    add_at(st, Uoff[node], U->nrow, U->ncol, Uld[node], ub32, y1o[node], -1, sg, -1);
-                    SR[st].push_back(sg);
-                }
-            }
-        }
-        // ---- near field: depends on x only, balanced over all stages
-        std::vector<int> order(h->n_D);
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int a, int b) { return blk_elems(h->c_D_blks[a]) > blk_elems(h->c_D_blks[b]); });
-        for (int i : order) {
-            H2E_dense_mat_p blk = h->c_D_blks[i];
-            if (blk->nrow == 0 || blk->ncol == 0) continue;
-            long long in0, in1;
-            int d0, d1;
-            if (i < nleaf) {
-                int node = h->height_nodes[i];
-                in0 = xseg(node); in1 = -1; d0 = seg_y[node]; d1 = -1;
-            } else {
-                int n0 = h->r_inadm_pairs[2 * (i - nleaf)], n1 = h->r_inadm_pairs[2 * (i - nleaf) + 1];
-                in0 = xseg(n1); in1 = xseg(n0); d0 = seg_y[n0]; d1 = seg_y[n1];
-            }
-            const double bytes = 8.0 * (double)blk_elems(blk);
-            if (nf_schedule == 1) {
-                if (blk->ld >= 0) add_dense(blk, false, in0, in1, d0, d1, 0, P.bytes_D);
-                else add_lowrank(blk, false, in0, in1, d0, d1, 0, 1, 2, P.bytes_D);
-            } else if (blk->ld >= 0) {
-                int best = 0;
-                for (int st = 1; st < nst; st++) if (sload[st] < sload[best]) best = st;
-                add_dense(blk, false, in0, in1, d0, d1, best, P.bytes_D);
-            } else if (-blk->ld > 0) {
-                const int span = (d1 < 0) ? 2 : 3;
-                int best = 0;
-                double bestv = 1e300;
-                for (int a = 0; a + span - 1 < nst; a++) {
-                    double v = 0;
-                    for (int q = 0; q < span; q++) v = std::max(v, sload[a + q] + bytes / span);
-                    if (v < bestv) { bestv = v; best = a; }
-                }
-                add_lowrank(blk, false, in0, in1, d0, d1, best, best + 1, best + 2, P.bytes_D);
-            }
-        }
-        for (int i = 0; i < n_node; i++)
-            if (seg_y[i] >= 0) SR[sG].push_back(seg_y[i]);
-        finish_stages();
-        P.n64 = (long long)host64.size();
-        P.n32 = (long long)host32.size();
-        P.bytes_vec = 8.0 * (2.0 * N + 4.0 * (double)tot_r);
-        if (P.nW >= (1LL << 31)) { set_error(ERR_UNSUPPORTED, "workspace exceeds 2^31 doubles"); return false; }
-        return validate();
-    }
-
-    // Every access of every launch checked on the host before any launch:
-    // tile reads inside storage and vector-aligned, vector reads/writes inside
-    // the workspace, reduce sources/destinations inside their arrays.
+    // Every access of every launch checked on the host before any launch.
     bool validate() const {
         const long long N = h->krnl_mat_size;
-        auto chk_tasks = [&](const std::vector<BTask> &tl, long long nstore, int W) -> bool {
+        auto src_ok = [&](int b, int e, long long off, long long len) {
+            if (b < 0 || e > (int)P.vsrc.size() || b >= e) return false;
+            if (P.vsrc[b] < 0) return e == b + 1 && -(long long)P.vsrc[b] - 1 + off >= 0 && -(long long)P.vsrc[b] - 1 + off + len <= N;
+            for (int j = b; j < e; j++) {
+                const long long s = P.vsrc[j];
+                if (s < 0 || s + off < 0 || s + off + len > P.nW) return false;
+            }
+            return true;
+        };
+        auto chk = [&](const std::vector<BTask> &tl, bool f32) {
+            const long long nstore = f32 ? (long long)S.h32.size() : (long long)S.h64.size();
+            const int W = !f32 ? 2 : (nv == 8 ? 2 : 4);
             for (const BTask &t : tl) {
-                bool ok = t.nrow > 0 && t.ncol > 0 && t.ncol <= MAX_TILE_COLS && t.ld >= t.ncol && t.ld % W == 0 &&
-                          t.mat % W == 0 && t.mat >= 0 && (t.lpr & (t.lpr - 1)) == 0 && t.lpr >= 4 && t.lpr <= 64 &&
-                          t.lpr * COLS_PER_LANE >= t.ncol;
-                // last vector of the last row
-                long long last = t.mat + (long long)(t.nrow - 1) * t.ld + ((t.ncol - 1) / W) * W + W;
+                bool ok = t.nrow > 0 && t.ncol > 0 && t.ncol <= maxc && t.ncol * nv <= XIN_DOUBLES && t.ld >= t.ncol &&
+                          t.ld % W == 0 && t.mat % W == 0 && t.mat >= 0 && (t.lpr & (t.lpr - 1)) == 0 && t.lpr >= 4 &&
+                          t.lpr <= 64 && t.lpr * cpl_of(nv) >= t.ncol && (t.out0 >= 0 || t.out1 >= 0);
+                const long long last = t.mat + (long long)(t.nrow - 1) * t.ld + ((t.ncol - 1) / W) * W + W;
                 ok = ok && last <= nstore;
-                if (t.out0 >= 0) ok = ok && t.in0 >= 0 && t.in0 + t.ncol <= P.nW && t.out0 + t.nrow <= P.nW;
-                if (t.out1 >= 0) ok = ok && t.in1 >= 0 && t.in1 + t.nrow <= P.nW && t.out1 + t.ncol <= P.nW;
-                ok = ok && (t.out0 >= 0 || t.out1 >= 0);
+                if (t.out0 >= 0) ok = ok && src_ok(t.v0b, t.v0e, t.v0off, t.ncol) && t.out0 + (long long)t.nrow <= P.nW;
+                if (t.out1 >= 0)
+                    ok = ok && t.nrow * nv <= YIN_DOUBLES && src_ok(t.v1b, t.v1e, t.v1off, t.nrow) &&
+                         t.out1 + (long long)t.ncol <= P.nW;
                 if (!ok) {
-                    set_error(ERR_ARG, "device plan check failed: task mat=%lld %dx%d ld=%d lpr=%d", t.mat, t.nrow,
-                              t.ncol, t.ld, t.lpr);
+                    set_error(ERR_ARG,
+                              "device plan check failed (nv %d): tile mat=%lld %dx%d ld=%d lpr=%d in0 [%d,%d)+%d "
+                              "in1 [%d,%d)+%d out %d %d",
+                              nv, t.mat, t.nrow, t.ncol, t.ld, t.lpr, t.v0b, t.v0e, t.v0off, t.v1b, t.v1e, t.v1off,
+                              t.out0, t.out1);
                     return false;
                 }
             }
             return true;
         };
-        if (!chk_tasks(P.t64, (long long)host64.size(), 2) || !chk_tasks(P.t32, (long long)host32.size(), 4))
-            return false;
+        if (!chk(P.t64, false) || !chk(P.t32, true)) return false;
         for (const RTask &r : P.rt) {
             bool ok = r.len > 0 && r.len <= RCHUNK && r.dst >= 0 && r.soff >= 0 && r.src_beg >= 0 &&
-                      r.src_end <= (int)P.rsrc.size() && r.src_beg <= r.src_end &&
-                      r.dst + r.len <= ((r.flags & 1) ? N : P.nW);
-            for (int j = r.src_beg; ok && j < r.src_end; j++)
-                ok = P.rsrc[j] >= 0 && P.rsrc[j] + r.soff + r.len <= P.nW;
+                      r.src_end <= (int)P.rsrc.size() && r.src_beg < r.src_end;
+            ok = ok && ((r.flags & 1) ? r.dst + r.len <= N : r.dst + r.len <= P.nW);
+            for (int j = r.src_beg; ok && j < r.src_end; j++) ok = P.rsrc[j] >= 0 && (long long)P.rsrc[j] + r.soff + r.len <= P.nW;
             if (!ok) { set_error(ERR_ARG, "device plan check failed: reduce dst=%lld len=%d", r.dst, r.len); return false; }
         }
-        for (const Stage &s : P.stages) {
+        for (const Stage &s : P.stages)
             if (s.t64_beg + s.t64_cnt > (int)P.t64.size() || s.t32_beg + s.t32_cnt > (int)P.t32.size() ||
                 s.r_beg + s.r_cnt > (int)P.rt.size()) {
                 set_error(ERR_ARG, "device plan check failed: stage ranges");
                 return false;
             }
-        }
         return true;
     }
 };
 
 }  // namespace
+
+static bool build_plan(H2ERI_p h, const Storage &S, Plan &P, int nv, double tb, int nfs) {
+    Planner pl(h, S, P, nv, tb, nfs);
+    return pl.build();
+}
 
 static bool ensure_events(DevState *d, size_t n) {
     while (d->evs.size() < n) {
@@ -784,16 +998,14 @@ static bool ensure_events(DevState *d, size_t n) {
     return true;
 }
 
-static bool upload(H2ERI_p h) {
+static bool upload_storage(H2ERI_p h) {
     DevState *d = dstate_get(h);
-    if (d->ready) return true;
+    if (d->st_ready) return true;
     if (!h->U || !h->c_B_blks || !h->c_D_blks) { set_error(ERR_ARG, "no H2 representation built or loaded"); return false; }
     if (h->BD_JIT) { set_error(ERR_UNSUPPORTED, "BD_JIT representations are not supported"); return false; }
     HIP_OK(hipGetDevice(&d->dev));
     release_buffers(d);
-    Builder b(h, d->plan, d->ub_bits, d->tile_bytes, d->nf_schedule);
-    if (!b.build()) return false;
-    Plan &P = d->plan;
+    d->st.build(h, d->ub_bits == 32);
     auto alloc_copy = [&](void **dst, const void *src, size_t bytes) -> bool {
         if (bytes == 0) bytes = 64;
         HIP_OK(hipMalloc(dst, bytes));
@@ -801,17 +1013,12 @@ static bool upload(H2ERI_p h) {
         d->resident_bytes += bytes;
         return true;
     };
-    if (!alloc_copy((void **)&d->S64, b.host64.empty() ? nullptr : b.host64.data(), b.host64.size() * 8)) return false;
-    if (!alloc_copy((void **)&d->S32, b.host32.empty() ? nullptr : b.host32.data(), b.host32.size() * 4)) return false;
-    if (!alloc_copy((void **)&d->Wv, nullptr, (size_t)P.nW * 8)) return false;
-    HIP_OK(hipMemset(d->Wv, 0, (size_t)P.nW * 8));
-    if (!alloc_copy((void **)&d->d_t64, P.t64.empty() ? nullptr : P.t64.data(), P.t64.size() * sizeof(BTask))) return false;
-    if (!alloc_copy((void **)&d->d_t32, P.t32.empty() ? nullptr : P.t32.data(), P.t32.size() * sizeof(BTask))) return false;
-    if (!alloc_copy((void **)&d->d_rt, P.rt.empty() ? nullptr : P.rt.data(), P.rt.size() * sizeof(RTask))) return false;
-    if (!alloc_copy((void **)&d->d_rsrc, P.rsrc.empty() ? nullptr : P.rsrc.data(), P.rsrc.size() * sizeof(int))) return false;
+    if (!alloc_copy((void **)&d->S64, d->st.h64.empty() ? nullptr : d->st.h64.data(), d->st.h64.size() * 8)) return false;
+    if (!alloc_copy((void **)&d->S32, d->st.h32.empty() ? nullptr : d->st.h32.data(), d->st.h32.size() * 4)) return false;
     // Coulomb index map: x index -> (D element, 2x flag)
     if (h->num_bf > 0 && h->sp_shell_idx && h->shell_bf_sidx) {
         const int np = h->num_sp, nbf = h->num_bf, N = h->krnl_mat_size;
+        if ((long long)nbf * nbf >= (1LL << 30)) { set_error(ERR_UNSUPPORTED, "num_bf too large for the index map"); return false; }
         std::vector<int> g(N);
         for (int i = 0; i < np; i++) {
             int s0 = h->sp_shell_idx[i], s1 = h->sp_shell_idx[i + np];
@@ -824,60 +1031,80 @@ static bool upload(H2ERI_p h) {
         }
         if (!alloc_copy((void **)&d->d_gidx, g.data(), (size_t)N * sizeof(int))) return false;
     }
-    d->ready = true;
+    d->st_ready = true;
     return true;
 }
 
-static void launch_bgemv(int nb, const Stage &st, DevState *d, hipStream_t s) {
-    hipLaunchKernelGGL(bgemv_kernel, dim3(nb), dim3(WG), 0, s, d->d_t64 + st.t64_beg, st.t64_cnt,
-                       d->d_t32 + st.t32_beg, st.t32_cnt, d->S64, d->S32, d->Wv);
+static DevPlan *get_plan(H2ERI_p h, int nv) {
+    if (!upload_storage(h)) return nullptr;
+    DevState *d = dstate(h);
+    DevPlan &dp = d->plans[nv_index(nv)];
+    if (dp.ready) return &dp;
+    if (!build_plan(h, d->st, dp.P, nv, d->tile_bytes, d->nf_schedule)) return nullptr;
+    const Plan &P = dp.P;
+    auto alloc_copy = [&](void **dst, const void *src, size_t bytes) -> bool {
+        if (bytes == 0) bytes = 64;
+        hipError_t e = hipMalloc(dst, bytes);
+        if (e == hipSuccess && src) e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { set_error(ERR_DEVICE, "plan upload: %s", hipGetErrorString(e)); return false; }
+        return true;
+    };
+    if (!alloc_copy((void **)&dp.Wv, nullptr, (size_t)P.nW * nv * 8)) return nullptr;
+    if (hipMemset(dp.Wv, 0, (size_t)P.nW * nv * 8) != hipSuccess) { set_error(ERR_DEVICE, "workspace memset"); return nullptr; }
+    if (!alloc_copy((void **)&dp.d_t64, P.t64.empty() ? nullptr : P.t64.data(), P.t64.size() * sizeof(BTask))) return nullptr;
+    if (!alloc_copy((void **)&dp.d_t32, P.t32.empty() ? nullptr : P.t32.data(), P.t32.size() * sizeof(BTask))) return nullptr;
+    if (!alloc_copy((void **)&dp.d_rt, P.rt.empty() ? nullptr : P.rt.data(), P.rt.size() * sizeof(RTask))) return nullptr;
+    if (!alloc_copy((void **)&dp.d_rsrc, P.rsrc.empty() ? nullptr : P.rsrc.data(), P.rsrc.size() * sizeof(int))) return nullptr;
+    if (!alloc_copy((void **)&dp.d_vsrc, P.vsrc.empty() ? nullptr : P.vsrc.data(), P.vsrc.size() * sizeof(int))) return nullptr;
+    dp.ready = true;
+    return &dp;
 }
 
-// Enqueue one matvec: x already at Wv[0..N), result written to y_dev.  In
-// phase-timing mode an event is recorded around every launch.
-static bool run_matvec(H2ERI_p h, DevState *d, double *y_dev, hipStream_t s) {
-    (void)h;
-    const Plan &P = d->plan;
+template <int NV>
+static void launch_stage(const Stage &st, const DevPlan &dp, DevState *d, const double *X, double *Y, long long ld,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(stage_kernel<NV>, dim3(st.t64_cnt + st.t32_cnt + st.r_cnt), dim3(WG), 0, s,
+                       dp.d_t64 + st.t64_beg, st.t64_cnt, dp.d_t32 + st.t32_beg, st.t32_cnt, dp.d_rt + st.r_beg,
+                       st.r_cnt, dp.d_vsrc, dp.d_rsrc, d->S64, d->S32, dp.Wv, X, Y, ld);
+}
+
+// Enqueue one batched matvec of nv vectors: X (nv x N, vector-major) -> Y.
+// In phase-timing mode an event pair brackets every launch.
+static bool run_matvec(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, double *Y, hipStream_t s) {
+    const Plan &P = dp.P;
+    const long long ld = h->krnl_mat_size;
     const bool tim = d->phase_timing;
     if (tim) {
-        if (!ensure_events(d, 2 * P.stages.size() + 1)) return false;
+        if (!ensure_events(d, 2 * P.stages.size() + 2)) return false;
         d->launches.clear();
         d->collected = false;
-        HIP_OK(hipEventRecord(d->evs[0], s));
     }
-    for (size_t si = 0; si < P.stages.size(); si++) {
-        const Stage &st = P.stages[si];
-        const int nb = st.t64_cnt + st.t32_cnt;
-        if (nb > 0) {
-            launch_bgemv(nb, st, d, s);
-            HIP_OK(hipGetLastError());
-            if (tim) {
-                d->launches.push_back(LaunchRec{st.phase, 0, nb, st.mat_bytes, 0.f});
-                HIP_OK(hipEventRecord(d->evs[d->launches.size()], s));
-            }
+    for (const Stage &st : P.stages) {
+        if (tim) HIP_OK(hipEventRecord(d->evs[2 * d->launches.size()], s));
+        switch (P.nv) {
+        case 1: launch_stage<1>(st, dp, d, X, Y, ld, s); break;
+        case 2: launch_stage<2>(st, dp, d, X, Y, ld, s); break;
+        case 4: launch_stage<4>(st, dp, d, X, Y, ld, s); break;
+        default: launch_stage<8>(st, dp, d, X, Y, ld, s); break;
         }
-        if (st.r_cnt > 0) {
-            hipLaunchKernelGGL(reduce_kernel, dim3(st.r_cnt), dim3(WG), 0, s, d->d_rt + st.r_beg, d->d_rsrc, d->Wv, y_dev);
-            HIP_OK(hipGetLastError());
-            if (tim) {
-                d->launches.push_back(LaunchRec{st.phase, 1, st.r_cnt, 0.0, 0.f});
-                HIP_OK(hipEventRecord(d->evs[d->launches.size()], s));
-            }
+        HIP_OK(hipGetLastError());
+        if (tim) {
+            HIP_OK(hipEventRecord(d->evs[2 * d->launches.size() + 1], s));
+            d->launches.push_back(LaunchRec{st.phase, st.t64_cnt + st.t32_cnt + st.r_cnt, st.mat_bytes, 0.f});
         }
     }
     return true;
 }
 
-// Per-launch and per-phase milliseconds of the last timed call (the stream
-// must have been synchronised).
+// Per-launch and per-phase milliseconds of the last timed call.
 static void collect_phase_ms(H2ERI_p h, DevState *d) {
     if (!d->phase_timing || d->launches.empty() || d->collected) return;
     d->collected = true;
     for (int p = 0; p < 7; p++) d->phase_ms[p] = 0.0;
     for (size_t i = 0; i < d->launches.size(); i++) {
         float ms = 0.f;
-        if (hipEventSynchronize(d->evs[i + 1]) != hipSuccess ||
-            hipEventElapsedTime(&ms, d->evs[i], d->evs[i + 1]) != hipSuccess)
+        if (hipEventSynchronize(d->evs[2 * i + 1]) != hipSuccess ||
+            hipEventElapsedTime(&ms, d->evs[2 * i], d->evs[2 * i + 1]) != hipSuccess)
             ms = 0.f;
         d->launches[i].ms = ms;
         d->phase_ms[d->launches[i].phase] += ms;
@@ -891,53 +1118,71 @@ static void collect_phase_ms(H2ERI_p h, DevState *d) {
     h->timers[MV_VOP_TIMER_IDX] += d->phase_ms[PH_GATHER] * 1e-3;
 }
 
-static bool matvec_dev_impl(H2ERI_p h, const double *x_dev, double *y_dev, hipStream_t s) {
-    if (!upload(h)) return false;
-    DevState *d = dstate(h);
-    const int N = h->krnl_mat_size;
-    HIP_OK(hipMemcpyAsync(d->Wv, x_dev, sizeof(double) * (size_t)N, hipMemcpyDeviceToDevice, s));
-    return run_matvec(h, d, y_dev, s);
-}
-
-static bool ensure_xio(DevState *d, size_t n) {
-    if (d->xio_len >= n) return true;
-    free_dev((void *&)d->d_xio);
-    HIP_OK(hipMalloc(&d->d_xio, sizeof(double) * n));
-    d->xio_len = n;
+// Batched device matvec: nvec vectors, processed in passes of 8, 4, 2, 1.
+static bool matvec_multi_dev_impl(H2ERI_p h, int nvec, const double *X, double *Y, hipStream_t s) {
+    const size_t N = (size_t)h->krnl_mat_size;
+    int done = 0;
+    while (done < nvec) {
+        const int rem = nvec - done;
+        const int nv = rem >= 8 ? 8 : (rem >= 4 ? 4 : (rem >= 2 ? 2 : 1));
+        DevPlan *dp = get_plan(h, nv);
+        if (!dp) return false;
+        if (!run_matvec(h, dstate(h), *dp, X + done * N, Y + done * N, s)) return false;
+        done += nv;
+    }
     return true;
 }
 
-static bool matvec_host_impl(H2ERI_p h, const double *x, double *y) {
-    if (!upload(h)) return false;
+static bool ensure_io(DevState *d, size_t n) {
+    if (d->d_io_len >= n) return true;
+    free_dev((void *&)d->d_x);
+    free_dev((void *&)d->d_y);
+    d->d_io_len = 0;
+    HIP_OK(hipMalloc(&d->d_x, sizeof(double) * n));
+    HIP_OK(hipMalloc(&d->d_y, sizeof(double) * n));
+    d->d_io_len = n;
+    return true;
+}
+
+static bool matvec_host_impl(H2ERI_p h, int nvec, const double *x, double *y) {
+    if (!upload_storage(h)) return false;
     DevState *d = dstate(h);
     const size_t N = (size_t)h->krnl_mat_size;
-    if (!ensure_xio(d, N)) return false;
-    HIP_OK(hipMemcpy(d->Wv, x, sizeof(double) * N, hipMemcpyHostToDevice));
-    if (!run_matvec(h, d, d->d_xio, nullptr)) return false;
-    HIP_OK(hipMemcpy(y, d->d_xio, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (!ensure_io(d, N * nvec)) return false;
+    HIP_OK(hipMemcpy(d->d_x, x, sizeof(double) * N * nvec, hipMemcpyHostToDevice));
+    if (!matvec_multi_dev_impl(h, nvec, d->d_x, d->d_y, nullptr)) return false;
+    HIP_OK(hipMemcpy(y, d->d_y, sizeof(double) * N * nvec, hipMemcpyDeviceToHost));
     collect_phase_ms(h, d);
-    h->n_matvec++;
+    h->n_matvec += nvec;
     return true;
 }
 
-static bool coulomb_dev_impl(H2ERI_p h, const double *D_dev, double *J_dev, hipStream_t s) {
-    if (!upload(h)) return false;
+// J_k = contract(A uncontract(D_k)) for ndens densities on the device.
+static bool coulomb_dev_impl(H2ERI_p h, int ndens, const double *D_dev, double *J_dev, hipStream_t s) {
+    if (!upload_storage(h)) return false;
     DevState *d = dstate(h);
     if (!d->d_gidx) { set_error(ERR_ARG, "Coulomb build needs shell-pair metadata"); return false; }
     const int N = h->krnl_mat_size, nbf = h->num_bf;
-    if (!d->d_Jh) HIP_OK(hipMalloc(&d->d_Jh, sizeof(double) * (size_t)nbf * nbf));
-    if (!d->d_y) HIP_OK(hipMalloc(&d->d_y, sizeof(double) * (size_t)N));
-    hipLaunchKernelGGL(uncontract_kernel, dim3((N + 255) / 256), dim3(256), 0, s, d->d_gidx, D_dev, d->Wv, N);
-    HIP_OK(hipGetLastError());
-    if (!run_matvec(h, d, d->d_y, s)) return false;
-    size_t nn = (size_t)nbf * nbf;
-    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<size_t>((nn + 255) / 256, 4096)), dim3(256), 0, s, d->d_Jh, nn);
-    HIP_OK(hipGetLastError());
-    hipLaunchKernelGGL(contract_scatter_kernel, dim3((N + 255) / 256), dim3(256), 0, s, d->d_gidx, d->d_y, d->d_Jh, N);
-    HIP_OK(hipGetLastError());
-    dim3 grid((nbf + 31) / 32, (nbf + 31) / 32);
-    hipLaunchKernelGGL(symmetrize_kernel, grid, dim3(256), 0, s, d->d_Jh, J_dev, nbf);
-    HIP_OK(hipGetLastError());
+    const size_t nn = (size_t)nbf * nbf;
+    if (!ensure_io(d, (size_t)N * ndens)) return false;
+    if (!d->d_Jh) HIP_OK(hipMalloc(&d->d_Jh, sizeof(double) * nn));
+    for (int k = 0; k < ndens; k++) {
+        hipLaunchKernelGGL(uncontract_kernel, dim3((N + 255) / 256), dim3(256), 0, s, d->d_gidx, D_dev + k * nn,
+                           d->d_x + (size_t)k * N, N);
+        HIP_OK(hipGetLastError());
+    }
+    if (!matvec_multi_dev_impl(h, ndens, d->d_x, d->d_y, s)) return false;
+    for (int k = 0; k < ndens; k++) {
+        hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<size_t>((nn + 255) / 256, 4096)), dim3(256), 0, s,
+                           d->d_Jh, nn);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(contract_scatter_kernel, dim3((N + 255) / 256), dim3(256), 0, s, d->d_gidx,
+                           d->d_y + (size_t)k * N, d->d_Jh, N);
+        HIP_OK(hipGetLastError());
+        dim3 grid((nbf + 31) / 32, (nbf + 31) / 32);
+        hipLaunchKernelGGL(symmetrize_kernel, grid, dim3(256), 0, s, d->d_Jh, J_dev + k * nn, nbf);
+        HIP_OK(hipGetLastError());
+    }
     return true;
 }
 
@@ -949,54 +1194,55 @@ extern "C" {
 
 void H2ERI_matvec(H2ERI_p h, const double *x, double *y) {
     if (!h || !x || !y) { set_error(ERR_ARG, "H2ERI_matvec: null argument"); return; }
-    matvec_host_impl(h, x, y);
+    matvec_host_impl(h, 1, x, y);
 }
 
 void H2ERI_matvec_dev(H2ERI_p h, const double *x_dev, double *y_dev, void *stream) {
     if (!h || !x_dev || !y_dev) { set_error(ERR_ARG, "H2ERI_matvec_dev: null argument"); return; }
-    matvec_dev_impl(h, x_dev, y_dev, (hipStream_t)stream);
+    matvec_multi_dev_impl(h, 1, x_dev, y_dev, (hipStream_t)stream);
 }
 
 void H2ERI_matvec_multi(H2ERI_p h, const int nvec, const double *X, double *Y) {
     if (!h || nvec < 0 || (nvec > 0 && (!X || !Y))) { set_error(ERR_ARG, "H2ERI_matvec_multi: bad argument"); return; }
-    const size_t N = (size_t)h->krnl_mat_size;
-    for (int k = 0; k < nvec; k++) {
-        if (!matvec_host_impl(h, X + k * N, Y + k * N)) return;
-    }
+    if (nvec > 0) matvec_host_impl(h, nvec, X, Y);
 }
 
 void H2ERI_matvec_multi_dev(H2ERI_p h, const int nvec, const double *X_dev, double *Y_dev, void *stream) {
-    if (!h || nvec < 0) { set_error(ERR_ARG, "H2ERI_matvec_multi_dev: bad argument"); return; }
-    const size_t N = (size_t)h->krnl_mat_size;
-    for (int k = 0; k < nvec; k++)
-        if (!matvec_dev_impl(h, X_dev + k * N, Y_dev + k * N, (hipStream_t)stream)) return;
+    if (!h || nvec < 0 || (nvec > 0 && (!X_dev || !Y_dev))) { set_error(ERR_ARG, "H2ERI_matvec_multi_dev: bad argument"); return; }
+    if (nvec > 0) matvec_multi_dev_impl(h, nvec, X_dev, Y_dev, (hipStream_t)stream);
+}
+
+void H2ERI_build_Coulomb_multi(H2ERI_p h, const int ndens, const double *den_mats, double *J_mats) {
+    if (!h || ndens < 0 || (ndens > 0 && (!den_mats || !J_mats))) { set_error(ERR_ARG, "H2ERI_build_Coulomb_multi: bad argument"); return; }
+    if (ndens == 0 || !upload_storage(h)) return;
+    DevState *d = dstate(h);
+    const size_t nn = (size_t)h->num_bf * h->num_bf;
+    free_dev((void *&)d->d_D);
+    free_dev((void *&)d->d_J);
+    if (hipMalloc(&d->d_D, nn * 8 * ndens) != hipSuccess || hipMalloc(&d->d_J, nn * 8 * ndens) != hipSuccess) {
+        set_error(ERR_DEVICE, "H2ERI_build_Coulomb: device allocation of D/J failed");
+        return;
+    }
+    if (hipMemcpy(d->d_D, den_mats, nn * 8 * ndens, hipMemcpyHostToDevice) != hipSuccess) { set_error(ERR_DEVICE, "copy D"); return; }
+    if (!coulomb_dev_impl(h, ndens, d->d_D, d->d_J, nullptr)) return;
+    if (hipMemcpy(J_mats, d->d_J, nn * 8 * ndens, hipMemcpyDeviceToHost) != hipSuccess) { set_error(ERR_DEVICE, "copy J"); return; }
+    collect_phase_ms(h, d);
+    h->n_matvec += ndens;
 }
 
 void H2ERI_build_Coulomb(H2ERI_p h, const double *den_mat, double *J_mat) {
     if (!h || !den_mat || !J_mat) { set_error(ERR_ARG, "H2ERI_build_Coulomb: null argument"); return; }
-    if (!upload(h)) return;
-    DevState *d = dstate(h);
-    const size_t nn = (size_t)h->num_bf * h->num_bf;
-    if (!d->d_D) { if (hipMalloc(&d->d_D, nn * 8) != hipSuccess) { set_error(ERR_DEVICE, "alloc D"); return; } }
-    if (!d->d_J) { if (hipMalloc(&d->d_J, nn * 8) != hipSuccess) { set_error(ERR_DEVICE, "alloc J"); return; } }
-    if (hipMemcpy(d->d_D, den_mat, nn * 8, hipMemcpyHostToDevice) != hipSuccess) { set_error(ERR_DEVICE, "copy D"); return; }
-    if (!coulomb_dev_impl(h, d->d_D, d->d_J, nullptr)) return;
-    if (hipMemcpy(J_mat, d->d_J, nn * 8, hipMemcpyDeviceToHost) != hipSuccess) { set_error(ERR_DEVICE, "copy J"); return; }
-    collect_phase_ms(h, d);
-    h->n_matvec++;
-}
-
-void H2ERI_build_Coulomb_multi(H2ERI_p h, const int ndens, const double *den_mats, double *J_mats) {
-    const size_t nn = h ? (size_t)h->num_bf * h->num_bf : 0;
-    for (int k = 0; k < ndens; k++) {
-        H2ERI_build_Coulomb(h, den_mats + k * nn, J_mats + k * nn);
-        if (h2eri_last_error()) return;
-    }
+    H2ERI_build_Coulomb_multi(h, 1, den_mat, J_mat);
 }
 
 void H2ERI_build_Coulomb_dev(H2ERI_p h, const double *den_dev, double *J_dev, void *stream) {
     if (!h || !den_dev || !J_dev) { set_error(ERR_ARG, "H2ERI_build_Coulomb_dev: null argument"); return; }
-    coulomb_dev_impl(h, den_dev, J_dev, (hipStream_t)stream);
+    coulomb_dev_impl(h, 1, den_dev, J_dev, (hipStream_t)stream);
+}
+
+void H2ERI_build_Coulomb_multi_dev(H2ERI_p h, const int ndens, const double *den_dev, double *J_dev, void *stream) {
+    if (!h || ndens < 0 || (ndens > 0 && (!den_dev || !J_dev))) { set_error(ERR_ARG, "H2ERI_build_Coulomb_multi_dev: bad argument"); return; }
+    if (ndens > 0) coulomb_dev_impl(h, ndens, den_dev, J_dev, (hipStream_t)stream);
 }
 
 void H2ERI_set_precision(H2ERI_p h, const int ub_bits) {
@@ -1006,7 +1252,7 @@ void H2ERI_set_precision(H2ERI_p h, const int ub_bits) {
 }
 
 size_t H2ERI_upload(H2ERI_p h) {
-    if (!h || !upload(h)) return 0;
+    if (!h || !get_plan(h, 1)) return 0;
     return dstate(h)->resident_bytes;
 }
 
@@ -1030,7 +1276,7 @@ int H2ERI_get_launch_stats(H2ERI_p h, const int max_launch, double *out5) {
     for (int i = 0; i < n && i < max_launch && out5; i++) {
         const LaunchRec &r = d->launches[i];
         out5[5 * i + 0] = r.phase;
-        out5[5 * i + 1] = r.kind;
+        out5[5 * i + 1] = 0;
         out5[5 * i + 2] = r.blocks;
         out5[5 * i + 3] = r.mat_bytes;
         out5[5 * i + 4] = r.ms;
@@ -1040,17 +1286,18 @@ int H2ERI_get_launch_stats(H2ERI_p h, const int max_launch, double *out5) {
 
 void H2ERI_matvec_bytes(H2ERI_p h, double *out6) {
     if (!h || !out6) return;
-    if (!upload(h)) return;
-    const Plan &P = dstate(h)->plan;
-    double alg = P.bytes_U + P.bytes_B + P.bytes_D + P.bytes_vec;
+    DevPlan *dp = get_plan(h, 1);
+    if (!dp) return;
+    const Storage &S = dstate(h)->st;
+    const double alg = S.bytes_U + S.bytes_B + S.bytes_D + dp->P.bytes_vec;
     const size_t *ms = h->mat_size;
-    double ref = 8.0 * (double)(ms[MV_FWD_SIZE_IDX] + ms[MV_MID_SIZE_IDX] + ms[MV_BWD_SIZE_IDX] + ms[MV_DEN_SIZE_IDX]);
+    const double ref = 8.0 * (double)(ms[MV_FWD_SIZE_IDX] + ms[MV_MID_SIZE_IDX] + ms[MV_BWD_SIZE_IDX] + ms[MV_DEN_SIZE_IDX]);
     out6[0] = alg;
     out6[1] = ref;
-    out6[2] = P.bytes_U;
-    out6[3] = P.bytes_B;
-    out6[4] = P.bytes_D;
-    out6[5] = P.bytes_vec;
+    out6[2] = S.bytes_U;
+    out6[3] = S.bytes_B;
+    out6[4] = S.bytes_D;
+    out6[5] = dp->P.bytes_vec;
 }
 
 // Near-field schedule (tuning knob; re-plans): 0 balances the near-field
@@ -1058,7 +1305,7 @@ void H2ERI_matvec_bytes(H2ERI_p h, double *out6) {
 void H2ERI_set_schedule(H2ERI_p h, const int mode) {
     if (!h || (mode != 0 && mode != 1)) { set_error(ERR_ARG, "H2ERI_set_schedule: 0 or 1"); return; }
     DevState *d = dstate_get(h);
-    if (d->nf_schedule != mode) { d->nf_schedule = mode; release_buffers(d); }
+    if (d->nf_schedule != mode) { d->nf_schedule = mode; release_plans(d); }
 }
 
 // Tile size of the bi-GEMV tasks in bytes (tuning knob; re-plans).
@@ -1066,7 +1313,7 @@ void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
     if (!h || bytes < 1024) return;
     DevState *d = dstate_get(h);
     d->tile_bytes = bytes;
-    release_buffers(d);
+    release_plans(d);
 }
 
 size_t h2eri_struct_size(void) { return sizeof(struct H2ERI); }
@@ -1077,99 +1324,118 @@ void H2ERI_build_exchange(H2ERI_p h, const double *den_mat, double *K_mat) {
 }
 
 // Verification of a plan on the host (no HIP call): executes every tile and
-// reduce task with the kernels' semantics in launch order, and checks that
-// within one launch no task reads an element another task of that launch
-// writes and no element is stored by two tasks.  For tests of the planner;
-// the device entry points never call it.
-int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const double *x, double *y) {
-    if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !x || !y) { set_error(ERR_ARG, "plan_emulate: bad args"); return -1; }
+// reduce task with the kernels' semantics in launch order for nv vectors
+// (X, Y vector-major, nv x N), and checks that within one launch no task
+// reads a workspace element another task of that launch writes and no
+// element is stored twice.  For tests of the planner; the device entry points
+// never call it.  Returns the number of conflicts, -1 on failure.
+int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double *X, double *Y) {
+    if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !X || !Y || (nv != 1 && nv != 2 && nv != 4 && nv != 8)) {
+        set_error(ERR_ARG, "plan_emulate: bad args");
+        return -1;
+    }
+    Storage S;
+    S.build(h, ub_bits == 32);
     Plan P;
-    Builder b(h, P, ub_bits == 32 ? 32 : 64, dstate_get(h)->tile_bytes, dstate_get(h)->nf_schedule);
-    if (!b.build()) return -1;
+    DevState *ds = dstate_get(h);
+    if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule)) return -1;
     const long long N = h->krnl_mat_size;
-    std::vector<double> Wv((size_t)P.nW, 0.0);
-    std::memcpy(Wv.data(), x, sizeof(double) * (size_t)N);
-    std::vector<int> wmark((size_t)P.nW, -1), rmark((size_t)P.nW, -1), ymark((size_t)N, -1);
-    int races = 0;
-    auto write_el = [&](long long e, int task, bool acc) {
-        if (wmark[e] >= 0 && wmark[e] != task && !acc) races++;
-        if (rmark[e] >= 0 && rmark[e] != task) races++;
-        wmark[e] = task;
-    };
-    auto read_el = [&](long long e, int task) {
-        if (wmark[e] >= 0 && wmark[e] != task) races++;
+    std::vector<double> Wv((size_t)P.nW * nv, std::nan(""));
+    std::vector<int> wmark((size_t)P.nW, -1), rmark((size_t)P.nW, -1);
+    std::vector<int> ymark((size_t)N, -1);
+    int conflicts = 0;
+    auto rdw = [&](long long e, int task) {
+        if (wmark[e] >= 0 && wmark[e] != task) conflicts++;
         rmark[e] = task;
     };
-    int launch = 0;
+    auto wr = [&](long long e, int task) {
+        if ((wmark[e] >= 0 && wmark[e] != task) || (rmark[e] >= 0 && rmark[e] != task)) conflicts++;
+        wmark[e] = task;
+    };
+    auto vin = [&](int b, int e, long long idx, int v, int task) -> double {
+        double s = 0.0;
+        for (int j = b; j < e; j++) {
+            const int src = P.vsrc[j];
+            if (src < 0) return X[(long long)v * N + (-src - 1) + idx];
+            rdw(src + idx, task);
+            s += Wv[(size_t)(src + idx) * nv + v];
+        }
+        return s;
+    };
     for (const Stage &st : P.stages) {
-        // tile launch
         std::fill(wmark.begin(), wmark.end(), -1);
         std::fill(rmark.begin(), rmark.end(), -1);
-        for (int q = 0; q < st.t64_cnt + st.t32_cnt; q++) {
+        const int ntile = st.t64_cnt + st.t32_cnt;
+        for (int q = 0; q < ntile; q++) {
             const bool f32 = q >= st.t64_cnt;
             const BTask &t = f32 ? P.t32[st.t32_beg + q - st.t64_cnt] : P.t64[st.t64_beg + q];
-            const int id = q;
             auto M = [&](int r, int c) -> double {
-                long long e = t.mat + (long long)r * t.ld + c;
-                return f32 ? (double)b.host32[e] : b.host64[e];
+                const long long e = t.mat + (long long)r * t.ld + c;
+                return f32 ? (double)S.h32[e] : S.h64[e];
             };
-            if (t.out0 >= 0) {
-                for (int c = 0; c < t.ncol; c++) read_el(t.in0 + c, id);
-                for (int r = 0; r < t.nrow; r++) {
-                    double d = 0.0;
-                    for (int c = 0; c < t.ncol; c++) d += M(r, c) * Wv[t.in0 + c];
-                    write_el(t.out0 + r, id, (t.flags & 1) != 0);
-                    if (t.flags & 1) Wv[t.out0 + r] += d;
-                    else Wv[t.out0 + r] = d;
-                }
-            }
-            if (t.out1 >= 0) {
-                for (int r = 0; r < t.nrow; r++) read_el(t.in1 + r, id);
-                for (int c = 0; c < t.ncol; c++) {
-                    double s = 0.0;
-                    for (int r = 0; r < t.nrow; r++) s += M(r, c) * Wv[t.in1 + r];
-                    write_el(t.out1 + c, id, false);
-                    Wv[t.out1 + c] = s;
-                }
+            for (int v = 0; v < nv; v++) {
+                std::vector<double> x0(t.ncol, 0.0), y1(t.nrow, 0.0);
+                if (t.out0 >= 0)
+                    for (int c = 0; c < t.ncol; c++) x0[c] = vin(t.v0b, t.v0e, t.v0off + c, v, q);
+                if (t.out1 >= 0)
+                    for (int r = 0; r < t.nrow; r++) y1[r] = vin(t.v1b, t.v1e, t.v1off + r, v, q);
+                if (t.out0 >= 0)
+                    for (int r = 0; r < t.nrow; r++) {
+                        double d = 0.0;
+                        for (int c = 0; c < t.ncol; c++) d += M(r, c) * x0[c];
+                        wr(t.out0 + r, q);
+                        Wv[(size_t)(t.out0 + r) * nv + v] = d;
+                    }
+                if (t.out1 >= 0)
+                    for (int c = 0; c < t.ncol; c++) {
+                        double s = 0.0;
+                        for (int r = 0; r < t.nrow; r++) s += M(r, c) * y1[r];
+                        wr(t.out1 + c, q);
+                        Wv[(size_t)(t.out1 + c) * nv + v] = s;
+                    }
             }
         }
-        launch++;
-        // reduce launch
-        std::fill(wmark.begin(), wmark.end(), -1);
-        std::fill(rmark.begin(), rmark.end(), -1);
         for (int q = 0; q < st.r_cnt; q++) {
             const RTask &r = P.rt[st.r_beg + q];
+            const int task = ntile + q;
             for (int i = 0; i < r.len; i++) {
-                double s = 0.0;
+                std::vector<double> s(nv, 0.0);
                 for (int j = r.src_beg; j < r.src_end; j++) {
-                    read_el(P.rsrc[j] + r.soff + i, q);
-                    s += Wv[P.rsrc[j] + r.soff + i];
+                    const long long e = P.rsrc[j] + r.soff + i;
+                    rdw(e, task);
+                    for (int v = 0; v < nv; v++) s[v] += Wv[(size_t)e * nv + v];
                 }
                 if (r.flags & 1) {
-                    long long e = r.dst + i;
-                    if (ymark[e] == launch) races++;
-                    ymark[e] = launch;
-                    y[e] = (r.flags & 2) ? y[e] + s : s;
+                    if (ymark[r.dst + i] >= 0) conflicts++;
+                    ymark[r.dst + i] = task;
+                    for (int v = 0; v < nv; v++) Y[(size_t)v * N + r.dst + i] = s[v];
                 } else {
-                    write_el(r.dst + i, q, false);
-                    Wv[r.dst + i] = (r.flags & 2) ? Wv[r.dst + i] + s : s;
+                    wr(r.dst + i, task);
+                    for (int v = 0; v < nv; v++) Wv[(size_t)(r.dst + i) * nv + v] = s[v];
                 }
             }
         }
-        launch++;
     }
-    return races;
+    for (long long i = 0; i < N; i++)
+        if (ymark[i] < 0) conflicts++;   // every output element written exactly once
+    return conflicts;
 }
 
-// Builds the device plan on the host only and runs its bounds check; no HIP
-// call is made.  Returns the number of tile tasks, or -1 on failure.
+// Builds the device plans of every batch width on the host only and runs
+// their bounds checks; no HIP call is made.  Returns the number of tile
+// tasks of the single-vector plan, or -1 on failure.
 int H2ERI_plan_check(H2ERI_p h, const int ub_bits) {
     if (!h || !h->U || !h->c_B_blks || !h->c_D_blks) { set_error(ERR_ARG, "plan_check: nothing built"); return -1; }
-    Plan P;
-    double tb = dstate_get(h)->tile_bytes;
-    Builder b(h, P, ub_bits == 32 ? 32 : 64, tb, dstate_get(h)->nf_schedule);
-    if (!b.build()) return -1;
-    return (int)(P.t64.size() + P.t32.size());
+    Storage S;
+    S.build(h, ub_bits == 32);
+    DevState *ds = dstate_get(h);
+    int tasks = 0;
+    for (int nv : NV_VARIANTS) {
+        Plan P;
+        if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule)) return -1;
+        if (nv == 1) tasks = (int)(P.t64.size() + P.t32.size());
+    }
+    return tasks;
 }
 
 void H2ERI_exchange_workbuf_init(H2ERI_p) {}

@@ -163,10 +163,12 @@ class H2ERI:
         return r
 
     def plan_emulate(self, x, bits=64):
-        """Host execution of the device plan (tests only): (y, conflicts)."""
+        """Host execution of the device plan (tests only): (y, conflicts).
+        x is one vector (N,) or a batch (nv, N) with nv in {1, 2, 4, 8}."""
         x = np.ascontiguousarray(x, dtype=np.float64)
+        nv = 1 if x.ndim == 1 else x.shape[0]
         y = np.zeros_like(x)
-        r = self.lib.H2ERI_plan_emulate(self.h, bits, _dptr(x), _dptr(y))
+        r = self.lib.H2ERI_plan_emulate(self.h, bits, nv, _dptr(x), _dptr(y))
         _check(self.lib)
         return y, r
 
@@ -215,6 +217,19 @@ class H2ERI:
         self.lib.H2ERI_build_Coulomb_dev(self.h, C.c_void_p(d_ptr), C.c_void_p(j_ptr), C.c_void_p(stream or 0))
         _check(self.lib)
 
+    def build_coulomb_multi(self, Ds):
+        Ds = np.ascontiguousarray(Ds, dtype=np.float64)
+        assert Ds.ndim == 3 and Ds.shape[1:] == (self.nbf, self.nbf)
+        Js = np.empty_like(Ds)
+        self.lib.H2ERI_build_Coulomb_multi(self.h, Ds.shape[0], _dptr(Ds), _dptr(Js))
+        _check(self.lib)
+        return Js
+
+    def build_coulomb_multi_dev(self, ndens, d_ptr, j_ptr, stream=None):
+        self.lib.H2ERI_build_Coulomb_multi_dev(self.h, ndens, C.c_void_p(d_ptr), C.c_void_p(j_ptr),
+                                               C.c_void_p(stream or 0))
+        _check(self.lib)
+
     def build_exchange(self, D):
         D = np.ascontiguousarray(D, dtype=np.float64)
         assert D.shape == (self.nbf, self.nbf)
@@ -237,8 +252,8 @@ class H2ERI:
         out = np.zeros(5 * max(n, 1))
         n = self.lib.H2ERI_get_launch_stats(self.h, n, _dptr(out))
         names = ["up", "B", "y1", "down", "D", "gather"]
-        return [dict(phase=names[int(out[5 * i])], kind=("tiles" if out[5 * i + 1] == 0 else "reduce"),
-                     blocks=int(out[5 * i + 2]), mat_bytes=out[5 * i + 3], ms=out[5 * i + 4]) for i in range(n)]
+        return [dict(phase=names[int(out[5 * i])], blocks=int(out[5 * i + 2]), mat_bytes=out[5 * i + 3],
+                     ms=out[5 * i + 4]) for i in range(n)]
 
     def matvec_bytes(self):
         out = np.zeros(6)

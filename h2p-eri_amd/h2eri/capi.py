@@ -89,6 +89,7 @@ SIGNATURES = {
     "H2ERI_matvec_dev": (None, [H2ERI_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "H2ERI_matvec_multi_dev": (None, [H2ERI_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "H2ERI_build_Coulomb_dev": (None, [H2ERI_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "H2ERI_build_Coulomb_multi_dev": (None, [H2ERI_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "H2ERI_upload": (C.c_size_t, [H2ERI_p]),
     "H2ERI_save_rep": (C.c_int, [H2ERI_p, C.c_char_p]),
     "H2ERI_load_rep": (C.c_int, [H2ERI_p, C.c_char_p]),
@@ -102,7 +103,7 @@ SIGNATURES = {
     "H2ERI_set_schedule": (None, [H2ERI_p, C.c_int]),
     "h2eri_struct_size": (C.c_size_t, []),
     "H2ERI_plan_check": (C.c_int, [H2ERI_p, C.c_int]),
-    "H2ERI_plan_emulate": (C.c_int, [H2ERI_p, C.c_int, c_dbl_p, c_dbl_p]),
+    "H2ERI_plan_emulate": (C.c_int, [H2ERI_p, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
 }
 
 _lib = None

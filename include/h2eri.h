@@ -261,6 +261,8 @@ void H2ERI_build_Coulomb_multi(H2ERI_p h2eri, const int ndens, const double *den
 void H2ERI_matvec_dev(H2ERI_p h2eri, const double *x_dev, double *y_dev, void *stream);
 void H2ERI_matvec_multi_dev(H2ERI_p h2eri, const int nvec, const double *X_dev, double *Y_dev, void *stream);
 void H2ERI_build_Coulomb_dev(H2ERI_p h2eri, const double *den_dev, double *J_dev, void *stream);
+void H2ERI_build_Coulomb_multi_dev(H2ERI_p h2eri, const int ndens, const double *den_dev, double *J_dev,
+                                   void *stream);
 
 /* Upload the stored representation to the device now (otherwise done by the
  * first hot-path call).  Returns bytes resident on the device. */
@@ -285,16 +287,17 @@ void H2ERI_build_H2_synthetic(H2ERI_p h2eri, const unsigned long long seed,
 void H2ERI_partition_points_synthetic(H2ERI_p h2eri, const int npts, const double *coord,
                                       const int *nbfp, const int max_leaf_points);
 
-/* Per-phase device timing of the last hot-path call, milliseconds:
- * [0] upward U^T sweep, [1] B coupling, [2] y1 reduce, [3] downward U sweep,
- * [4] near-field D, [5] output gather, [6] total. Collected only when
+/* Per-phase device timing of the last hot-path call, milliseconds, by the
+ * phase each launch belongs to: [0] upward U^T sweep, [1] B coupling (first
+ * two launches), [2] B coupling (last launch), [3] downward U sweep, [4]
+ * near-field-only launches, [5] output gather, [6] total.  Near-field tiles
+ * share the sweep launches and count in their phase.  Collected only when
  * H2ERI_set_phase_timing(h2eri, 1). */
 void H2ERI_set_phase_timing(H2ERI_p h2eri, const int on);
 void H2ERI_get_phase_ms(H2ERI_p h2eri, double *ms7);
 /* Per-launch records of the last timed call: for launch i, out5[5i..5i+4] =
- * phase (0 up, 1 B, 2 y1, 3 down, 4 near field, 5 gather), kind (0 tile
- * bi-GEMV, 1 segmented reduce), workgroups, matrix bytes streamed, ms.
- * Returns the number of launches (fills at most max_launch). */
+ * phase (numbered as above), 0 (reserved), workgroups, matrix bytes
+ * streamed, ms.  Returns the number of launches (fills at most max_launch). */
 int H2ERI_get_launch_stats(H2ERI_p h2eri, const int max_launch, double *out5);
 
 /* Algorithmic bytes moved by one single-vector matvec on the device with
@@ -320,10 +323,11 @@ void H2ERI_set_schedule(H2ERI_p h2eri, const int mode);
 int H2ERI_plan_check(H2ERI_p h2eri, const int ub_bits);
 
 /* Host verification of the device plan (no HIP call, tests only): executes
- * every launch of the plan with the kernels' semantics into y and returns
- * the number of intra-launch read/write conflicts found (0 expected), -1 on
- * failure.  The hot-path entry points never call it. */
-int H2ERI_plan_emulate(H2ERI_p h2eri, const int ub_bits, const double *x, double *y);
+ * every launch of the plan for nv (1, 2, 4 or 8) vectors X (nv x N,
+ * vector-major) with the kernels' semantics into Y and returns the number of
+ * intra-launch read/write conflicts plus output elements not written exactly
+ * once (0 expected), -1 on failure.  The hot-path entry points never call it. */
+int H2ERI_plan_emulate(H2ERI_p h2eri, const int ub_bits, const int nv, const double *X, double *Y);
 
 /* sizeof(struct H2ERI) as compiled into the library (ABI check for
  * bindings that mirror the struct). */

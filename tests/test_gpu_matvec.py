@@ -87,10 +87,44 @@ def test_matvec_device_pointers(reps):
     assert relerr(yd.cpu().numpy(), o.matvec(x)) < 1e-13
 
 
-def test_matvec_multi(reps):
+@pytest.mark.parametrize("nvec", [2, 3, 4, 8, 15])
+@pytest.mark.parametrize("name", ["cube", "chain_lrdiag", "big_rank"])
+def test_matvec_multi(reps, name, nvec):
+    """Batched passes (8, 4, 2, 1 vectors per pass over the matrix data)."""
+    h, path = reps[name]
+    o = oracle_bind.Oracle(path)
+    X = np.random.default_rng(6 + nvec).standard_normal((nvec, h.n))
+    Y = h.matvec_multi(X)
+    for k in range(nvec):
+        assert relerr(Y[k], o.matvec(X[k])) < 1e-13
+
+
+@pytest.mark.parametrize("nvec", [1, 8])
+def test_matvec_multi_fp32_storage(reps, nvec):
     h, path = reps["cube"]
     o = oracle_bind.Oracle(path)
-    X = np.random.default_rng(6).standard_normal((3, h.n))
-    Y = h.matvec_multi(X)
-    for k in range(3):
+    X = np.random.default_rng(7).standard_normal((nvec, h.n))
+    h.set_precision(32)
+    try:
+        Y = h.matvec_multi(X)
+        Y2 = h.matvec_multi(X)
+    finally:
+        h.set_precision(64)
+    assert np.array_equal(Y, Y2)
+    for k in range(nvec):
+        assert 0.0 < relerr(Y[k], o.matvec(X[k])) < 2e-6
+
+
+def test_matvec_multi_device_pointers(reps):
+    import torch
+    h, path = reps["chain"]
+    o = oracle_bind.Oracle(path)
+    X = np.random.default_rng(8).standard_normal((4, h.n))
+    Xd = torch.from_numpy(X).to("cuda:0")
+    Yd = torch.empty_like(Xd)
+    torch.cuda.synchronize()
+    h.matvec_multi_dev(4, Xd.data_ptr(), Yd.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    Y = Yd.cpu().numpy()
+    for k in range(4):
         assert relerr(Y[k], o.matvec(X[k])) < 1e-13

@@ -98,3 +98,21 @@ def test_device_plan_semantics_match_oracle(tmp_path, seed, n, leaf, mr, shape):
     y1, conflicts = h.plan_emulate(x, 64)
     assert conflicts == 0
     assert relerr(y1, yo) < 1e-13
+
+
+@pytest.mark.parametrize("nv", [2, 4, 8])
+@pytest.mark.parametrize("seed,n,leaf,mr,shape", [(2, 1200, 150, 40, "chain"), (4, 1500, 40, 20, "cube"),
+                                                  (6, 2000, 120, 600, "chain")])
+def test_device_plan_semantics_batched(tmp_path, nv, seed, n, leaf, mr, shape):
+    """Batched plans (nv vectors per pass over the matrix data) reproduce the
+    oracle for every vector, with no intra-launch conflicts."""
+    import oracle_bind
+    from conftest import relerr
+    h = make_synthetic(tmp_path / "r.rep", n, seed, leaf=leaf, max_rank=mr, lr_frac=0.5, diag_lr_frac=0.4,
+                       shape=shape)
+    o = oracle_bind.Oracle(str(tmp_path / "r.rep"))
+    X = np.random.default_rng(seed + 10 * nv).standard_normal((nv, h.n))
+    Y, conflicts = h.plan_emulate(X, 64)
+    assert conflicts == 0
+    for v in range(nv):
+        assert relerr(Y[v], o.matvec(X[v])) < 1e-13
