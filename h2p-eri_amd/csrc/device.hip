@@ -80,11 +80,13 @@ static const int XIN_DOUBLES = 1024;   // in0 staging (ncol * NV <= 1024)
 static const int YIN_DOUBLES = LDS_DOUBLES - XIN_DOUBLES;   // in1 staging (nrow * NV)
 
 // Per (storage type, NV) tile configuration: W elements per vector load, KP
-// loads per row and lane (columns per lane CPL = W * KP), RU rows per loop
-// iteration.  CPL * NV <= 16 keeps the cross-wave reduction in 32 KB of LDS.
+// loads per row and lane (columns per lane CPL = W * KP), RU rows per block
+// (two blocks of RU*KP loads are in flight).  CPL * NV <= 16 keeps the
+// cross-wave reduction in 32 KB of LDS; the sizes keep the kernel within 128
+// VGPRs (four waves per SIMD).
 template <typename T, int NV> struct Cfg;
-template <> struct Cfg<double, 1> { typedef double2 V; static const int W = 2, KP = 4, RU = 4; };
-template <> struct Cfg<double, 2> { typedef double2 V; static const int W = 2, KP = 4, RU = 2; };
+template <> struct Cfg<double, 1> { typedef double2 V; static const int W = 2, KP = 4, RU = 2; };
+template <> struct Cfg<double, 2> { typedef double2 V; static const int W = 2, KP = 4, RU = 1; };
 template <> struct Cfg<double, 4> { typedef double2 V; static const int W = 2, KP = 2, RU = 2; };
 template <> struct Cfg<double, 8> { typedef double2 V; static const int W = 2, KP = 1, RU = 2; };
 template <> struct Cfg<float, 1> { typedef float4 V; static const int W = 4, KP = 2, RU = 4; };
@@ -153,6 +155,28 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
     const bool do0 = t.out0 >= 0, do1 = t.out1 >= 0;
     double *xin = lds;
     double *yin = lds + XIN_DOUBLES;
+    const T *base = S + t.mat;
+    // Row blocks of RU rows; every lane issues RU*KP vector loads per block.
+    // Loads are unconditional: rows past the tile are clamped to its last row
+    // and column positions past it to its last vector position.  Such
+    // entries meet a zero input (x0 = 0 for c >= C, y = 0 for r >= R) and
+    // their outputs are not stored (the stored blocks hold finite values).
+    const int cmax = ((C - 1) / W) * W;
+    int coff[KP];
+    #pragma unroll
+    for (int k = 0; k < KP; k++) coff[k] = min((k * lpr + lane) * W, cmax);
+    auto load_rows = [&](V (&mm)[RU][KP], int r0) {
+        #pragma unroll
+        for (int u = 0; u < RU; u++) {
+            const T *row = base + (long long)min(r0 + u * G, R - 1) * t.ld;
+            #pragma unroll
+            for (int k = 0; k < KP; k++) mm[u][k] = *reinterpret_cast<const V *>(row + coff[k]);
+        }
+    };
+    // Software pipeline: the next row block is always in flight while the
+    // current one is consumed; the first is in flight during the staging.
+    V nxt[RU][KP];
+    load_rows(nxt, grp);
 
     // stage the input vectors once per tile (sums of their slots)
     if (do0)
@@ -184,22 +208,13 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
                 a1[k][w][v] = 0.0;
             }
         }
-    const T *base = S + t.mat;
-    // RU rows per iteration: all RU*KP vector loads are issued before any is
-    // consumed, so every lane keeps RU*KP requests in flight.
     for (int r0 = grp; r0 < R; r0 += RU * G) {
         V m[RU][KP];
         #pragma unroll
-        for (int u = 0; u < RU; u++) {
-            const int r = r0 + u * G;
-            const T *row = base + (long long)r * t.ld;
+        for (int u = 0; u < RU; u++)
             #pragma unroll
-            for (int k = 0; k < KP; k++) {
-                const int c0 = (k * lpr + lane) * W;
-                if (r < R && c0 < C) m[u][k] = *reinterpret_cast<const V *>(row + c0);
-                else m[u][k] = V{};
-            }
-        }
+            for (int k = 0; k < KP; k++) m[u][k] = nxt[u][k];
+        load_rows(nxt, r0 + RU * G);
         if (do1) {
             #pragma unroll
             for (int u = 0; u < RU; u++) {
@@ -232,20 +247,44 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
                         for (int v = 0; v < NV; v++) d[u][v] = fma(mv, x0[k][w][v], d[u][v]);
                     }
             }
-            for (int s = lpr >> 1; s > 0; s >>= 1) {
+            // Reduce-scatter over the lpr lanes of the row group: each step
+            // halves the values a lane holds (it keeps one half, sends the
+            // other to its partner), so RU*NV sums cost about RU*NV shuffles
+            // instead of RU*NV*log2(lpr); a plain butterfly finishes when
+            // lanes outnumber values.
+            constexpr int NVAL = RU * NV;
+            double vals[NVAL];
+            #pragma unroll
+            for (int u = 0; u < RU; u++)
                 #pragma unroll
-                for (int u = 0; u < RU; u++)
+                for (int v = 0; v < NV; v++) vals[u * NV + v] = d[u][v];
+            int s = lpr >> 1, base = 0, held = NVAL;
+            #pragma unroll
+            for (int cnt = NVAL; cnt > 1; cnt >>= 1) {
+                if (s > 0) {
+                    const bool up = (lane & s) != 0;
+                    const int half = cnt >> 1;
                     #pragma unroll
-                    for (int v = 0; v < NV; v++) d[u][v] += __shfl_xor(d[u][v], s, lpr);
+                    for (int i = 0; i < half; i++) {
+                        const double send = up ? vals[i] : vals[i + half];
+                        const double keep = up ? vals[i + half] : vals[i];
+                        vals[i] = keep + __shfl_xor(send, s, lpr);
+                    }
+                    if (up) base += half;
+                    held = half;
+                    s >>= 1;
+                }
             }
-            if (lane == 0) {
+            const int wmask = s > 0 ? (s << 1) - 1 : 0;   // lanes that differ below this hold equal sums
+            for (; s > 0; s >>= 1) vals[0] += __shfl_xor(vals[0], s, lpr);
+            if ((lane & wmask) == 0) {
                 #pragma unroll
-                for (int u = 0; u < RU; u++) {
-                    const int r = r0 + u * G;
-                    if (r < R) {
-                        double *o = Wv + (long long)(t.out0 + r) * NV;
-                        #pragma unroll
-                        for (int v = 0; v < NV; v++) o[v] = d[u][v];
+                for (int i = 0; i < NVAL; i++) {
+                    if (i < held) {
+                        const int q = base + i;
+                        const int u = q / NV, v = q - u * NV;
+                        const int r = r0 + u * G;
+                        if (r < R) Wv[(long long)(t.out0 + r) * NV + v] = vals[i];
                     }
                 }
             }
@@ -460,6 +499,7 @@ struct Storage {
 
 struct Stage {
     int phase;
+    int stream;         // 0: main stream, 1: near-field stream (joins before the gather)
     int t64_beg, t64_cnt, t32_beg, t32_cnt;
     int r_beg, r_cnt;
     double mat_bytes;   // matrix bytes streamed by this launch
@@ -499,7 +539,7 @@ struct DevState {
     int dev = -1;
     int ub_bits = 64;
     double tile_bytes = 128.0 * 1024.0;
-    int nf_schedule = 0;      // 0: near field balanced over the sweep stages, 1: own stages first
+    int nf_schedule = 2;      // near-field placement, see H2ERI_set_schedule
     Storage st;
     bool st_ready = false;
     double *S64 = nullptr;
@@ -509,6 +549,8 @@ struct DevState {
     int *d_gidx = nullptr;
     double *d_D = nullptr, *d_J = nullptr, *d_Jh = nullptr, *d_y = nullptr, *d_x = nullptr;
     size_t d_io_len = 0;
+    hipStream_t side = nullptr;         // near-field stream (schedule 2)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool phase_timing = false;
     std::vector<hipEvent_t> evs;
     std::vector<LaunchRec> launches;
@@ -559,6 +601,9 @@ void device_release(H2ERI_p h) {
     if (!d) return;
     release_buffers(d);
     for (hipEvent_t e : d->evs) (void)hipEventDestroy(e);
+    if (d->ev_fork) (void)hipEventDestroy(d->ev_fork);
+    if (d->ev_join) (void)hipEventDestroy(d->ev_join);
+    if (d->side) (void)hipStreamDestroy(d->side);
     delete d;
     h->dev = nullptr;
 }
@@ -610,7 +655,7 @@ struct Planner {
     std::vector<std::vector<RTask>> SR;   // reduce tasks of each stage
     std::vector<int> sphase;
     std::vector<double> sload;
-    int nst = 0, sG = 0;
+    int nst = 0, sG = 0, sS = 0;   // stages: main 0..sG, near-field stream sS..nst-1 (schedule 2)
 
     Planner(H2ERI_p h_, const Storage &s_, Plan &p_, int nv_, double tb, int nfs)
         : h(h_), S(s_), P(p_), nv(nv_), tile_bytes(tb), nf_schedule(nfs), maxc(64 * cpl_of(nv_)) {}
@@ -724,11 +769,13 @@ struct Planner {
     // Replaces the sources written before stage c - 1 by one fold slot written
     // in stage c - 1 when that saves reads (consumer tiles: `reads` elements
     // in total) or, for reduce consumers, when there are many of them.
+    // Stages of one stream are [lo, c); a fold runs in stage c - 1 of that stream.
     std::vector<Src> fold(const std::vector<Src> &L, int c, double reads, int len, bool reducer, int depth) {
-        if (c - 1 < 0 || depth > 8) return L;
+        const int lo = c > sG ? sS : 0;
+        if (c - 1 < lo || depth > 8) return L;
         std::vector<Src> early, late;
         for (const Src &s : L) {
-            if (s.stage >= 0 && s.stage <= c - 2) early.push_back(s);
+            if (s.stage >= lo && s.stage <= c - 2) early.push_back(s);
             else late.push_back(s);
         }
         const double m = (double)early.size();
@@ -778,13 +825,15 @@ struct Planner {
         const int sBA = s0 + nU, sBB = sBA + 1, sBC = sBA + 2;
         const int sD0 = sBA + 3;
         sG = sD0 + nU;
-        nst = sG + 1;
+        sS = sG + 1;
+        nst = nf_schedule == 2 ? sS + 3 : sG + 1;
         SR.assign(nst, {});
         sload.assign(nst, 0.0);
         sphase.assign(nst, PH_GATHER);
         for (int q = 0; q < s0; q++) sphase[q] = PH_D;
         for (int q = 0; q < nU; q++) { sphase[s0 + q] = PH_UP; sphase[sD0 + q] = PH_DOWN; }
         sphase[sBA] = PH_B; sphase[sBB] = PH_B; sphase[sBC] = PH_Y1;
+        for (int q = sS; q < nst; q++) sphase[q] = PH_D;
         // y0 of a node is needed when a coupling block reads it or its parent's is needed
         std::vector<char> need0(n_node, 0);
         for (int i = 0; i < h->n_B; i++) {
@@ -869,7 +918,7 @@ struct Planner {
             if (d0 < 0 || (d1 < 0 && i >= nleaf)) { set_error(ERR_ARG, "device plan: near-field block %d on a non-leaf", i); return false; }
             const double bytes = 8.0 * (double)blk_elems(h->c_D_blks[i]);
             const int span = !blk.lr ? 1 : (d1 < 0 ? 2 : 3);
-            int best = 0;
+            int best = nf_schedule == 2 ? sS : 0;
             if (nf_schedule == 0) {
                 double bestv = 1e300;
                 for (int a = 0; a + span - 1 < sG; a++) {
@@ -891,7 +940,14 @@ struct Planner {
             if (yv[i] < 0) continue;
             const VVec &v = vv[yv[i]];
             if (v.srcs.empty()) { set_error(ERR_ARG, "device plan: leaf %d receives no contribution", i); return false; }
-            emit_reduce(sG, xo(i), true, v.len, fold(v.srcs, sG, v.len, v.len, true, 0));
+            std::vector<Src> mainS, sideS;
+            for (const Src &q : v.srcs) (q.stage > sG ? sideS : mainS).push_back(q);
+            std::vector<Src> all = fold(mainS, sG, v.len, v.len, true, 0);
+            if (!sideS.empty()) {
+                std::vector<Src> f = fold(sideS, nst, v.len, v.len, true, 0);
+                all.insert(all.end(), f.begin(), f.end());
+            }
+            emit_reduce(sG, xo(i), true, v.len, all);
         }
         finish();
         P.bytes_vec = 8.0 * nv * (2.0 * N + 4.0 * (double)tot_r);
@@ -912,6 +968,7 @@ struct Planner {
             std::stable_sort(S32[st].begin(), S32[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
             Stage s;
             s.phase = sphase[st];
+            s.stream = st > sG ? 1 : 0;
             s.mat_bytes = 0;
             for (const BTask &t : S64[st]) s.mat_bytes += 8.0 * t.nrow * t.ncol;
             for (const BTask &t : S32[st]) s.mat_bytes += 4.0 * t.nrow * t.ncol;
@@ -1074,24 +1131,44 @@ static bool run_matvec(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, dou
     const Plan &P = dp.P;
     const long long ld = h->krnl_mat_size;
     const bool tim = d->phase_timing;
+    bool has_side = false;
+    for (const Stage &st : P.stages) has_side |= st.stream == 1;
     if (tim) {
         if (!ensure_events(d, 2 * P.stages.size() + 2)) return false;
         d->launches.clear();
         d->collected = false;
     }
-    for (const Stage &st : P.stages) {
-        if (tim) HIP_OK(hipEventRecord(d->evs[2 * d->launches.size()], s));
+    auto launch = [&](const Stage &st, hipStream_t q) -> bool {
+        if (tim) HIP_OK(hipEventRecord(d->evs[2 * d->launches.size()], q));
         switch (P.nv) {
-        case 1: launch_stage<1>(st, dp, d, X, Y, ld, s); break;
-        case 2: launch_stage<2>(st, dp, d, X, Y, ld, s); break;
-        case 4: launch_stage<4>(st, dp, d, X, Y, ld, s); break;
-        default: launch_stage<8>(st, dp, d, X, Y, ld, s); break;
+        case 1: launch_stage<1>(st, dp, d, X, Y, ld, q); break;
+        case 2: launch_stage<2>(st, dp, d, X, Y, ld, q); break;
+        case 4: launch_stage<4>(st, dp, d, X, Y, ld, q); break;
+        default: launch_stage<8>(st, dp, d, X, Y, ld, q); break;
         }
         HIP_OK(hipGetLastError());
         if (tim) {
-            HIP_OK(hipEventRecord(d->evs[2 * d->launches.size() + 1], s));
+            HIP_OK(hipEventRecord(d->evs[2 * d->launches.size() + 1], q));
             d->launches.push_back(LaunchRec{st.phase, st.t64_cnt + st.t32_cnt + st.r_cnt, st.mat_bytes, 0.f});
         }
+        return true;
+    };
+    if (has_side) {
+        // near-field launches on their own stream, forked after everything
+        // already enqueued on s and joined before the gather
+        if (!d->side) HIP_OK(hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking));
+        if (!d->ev_fork) HIP_OK(hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming));
+        if (!d->ev_join) HIP_OK(hipEventCreateWithFlags(&d->ev_join, hipEventDisableTiming));
+        HIP_OK(hipEventRecord(d->ev_fork, s));
+        HIP_OK(hipStreamWaitEvent(d->side, d->ev_fork, 0));
+        for (const Stage &st : P.stages)
+            if (st.stream == 1 && !launch(st, d->side)) return false;
+        HIP_OK(hipEventRecord(d->ev_join, d->side));
+    }
+    for (const Stage &st : P.stages) {
+        if (st.stream != 0) continue;
+        if (has_side && st.phase == PH_GATHER) HIP_OK(hipStreamWaitEvent(s, d->ev_join, 0));
+        if (!launch(st, s)) return false;
     }
     return true;
 }
@@ -1301,9 +1378,11 @@ void H2ERI_matvec_bytes(H2ERI_p h, double *out6) {
 }
 
 // Near-field schedule (tuning knob; re-plans): 0 balances the near-field
-// tiles over the sweep launches, 1 runs them in three launches of their own.
+// tiles over the sweep launches, 1 runs them in three launches of their own
+// ahead of the sweeps, 2 (default) runs those three launches on a second
+// stream concurrently with the sweeps, joined before the gather.
 void H2ERI_set_schedule(H2ERI_p h, const int mode) {
-    if (!h || (mode != 0 && mode != 1)) { set_error(ERR_ARG, "H2ERI_set_schedule: 0 or 1"); return; }
+    if (!h || mode < 0 || mode > 2) { set_error(ERR_ARG, "H2ERI_set_schedule: 0, 1 or 2"); return; }
     DevState *d = dstate_get(h);
     if (d->nf_schedule != mode) { d->nf_schedule = mode; release_plans(d); }
 }
@@ -1343,15 +1422,26 @@ int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double 
     std::vector<double> Wv((size_t)P.nW * nv, std::nan(""));
     std::vector<int> wmark((size_t)P.nW, -1), rmark((size_t)P.nW, -1);
     std::vector<int> ymark((size_t)N, -1);
+    // elements written on the near-field stream: the main stream may touch
+    // them only in the gather (after the join)
+    std::vector<char> side_w((size_t)P.nW, 0);
+    bool in_side = false, before_join = true;
     int conflicts = 0;
     auto rdw = [&](long long e, int task) {
         if (wmark[e] >= 0 && wmark[e] != task) conflicts++;
+        if (!in_side && before_join && side_w[e]) conflicts++;
         rmark[e] = task;
     };
     auto wr = [&](long long e, int task) {
         if ((wmark[e] >= 0 && wmark[e] != task) || (rmark[e] >= 0 && rmark[e] != task)) conflicts++;
+        if (in_side) side_w[e] = 1;
+        else if (side_w[e]) conflicts++;
         wmark[e] = task;
     };
+    // launch order of the emulation: near-field stream first, then the main stream
+    std::vector<Stage> order;
+    for (const Stage &st : P.stages) if (st.stream == 1) order.push_back(st);
+    for (const Stage &st : P.stages) if (st.stream == 0) order.push_back(st);
     auto vin = [&](int b, int e, long long idx, int v, int task) -> double {
         double s = 0.0;
         for (int j = b; j < e; j++) {
@@ -1362,9 +1452,11 @@ int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double 
         }
         return s;
     };
-    for (const Stage &st : P.stages) {
+    for (const Stage &st : order) {
         std::fill(wmark.begin(), wmark.end(), -1);
         std::fill(rmark.begin(), rmark.end(), -1);
+        in_side = st.stream == 1;
+        before_join = st.phase != PH_GATHER;
         const int ntile = st.t64_cnt + st.t32_cnt;
         for (int q = 0; q < ntile; q++) {
             const bool f32 = q >= st.t64_cnt;
