@@ -2,12 +2,14 @@
 """Benchmark of the H2 J-build matvec on one MI355X.
 
 Workload (default): linear alkane C30H62, cc-pVDZ (760 Cartesian basis
-functions), H2 relative tolerance 1e-6, one seeded damped density with 121
-occupied columns.  The H2 representation is constructed on the host, uploaded
-once, and each step is one H2 matvec y = A_H2 x on the uncontracted density,
-with x and y resident in HBM.  Both the FP64 form (the drop-in) and the
-FP32-stored U/B form (FP64 near field and accumulation) are timed; `value` is
-the FP32-stored form's algorithmic HBM bandwidth.
+functions), H2 relative tolerance 1e-6, seeded damped density with 121
+occupied columns.  The H2 representation is constructed on the host and
+uploaded once.  A step is one H2 matvec y = A_H2 x on the uncontracted
+density, replayed from a captured HIP graph, with x and y resident in HBM.
+Timed forms: FP64 storage (the drop-in) and FP32-stored U/B (FP64 near field
+and accumulation), each for 1 density and for batches of 2, 4 and 8 densities
+per pass over the matrix data.  `value` is the FP32-stored single-density
+form's algorithmic HBM throughput; the batched forms are reported beside it.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -30,25 +32,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_steps(h, x, y, stream, steps, warmup, torch):
+def time_graph(h, torch, stream, steps, warmup):
+    """ms per replay of the captured graph, HIP events on the replay stream."""
     for _ in range(warmup):
-        h.matvec_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+        h.graph_replay(stream.cuda_stream)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(steps):
-        h.matvec_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+        h.graph_replay(stream.cuda_stream)
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
 
 
-def launch_profile(h, x, y, stream, steps, torch):
-    """Per-launch HIP-event times averaged over `steps` timed matvecs."""
+def launch_profile(h, torch, x, y, stream, reps):
+    """Per-launch durations (HIP events around each launch, on the stream the
+    launch runs on), averaged over `reps` eager matvecs."""
     h.set_phase_timing(True)
     acc = None
-    for _ in range(steps):
+    for _ in range(reps):
         h.matvec_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
         torch.cuda.synchronize()
         st = h.launch_stats()
@@ -59,11 +63,13 @@ def launch_profile(h, x, y, stream, steps, torch):
                 a["ms"] += r["ms"]
     h.set_phase_timing(False)
     for a in acc:
-        a["ms"] /= steps
+        a["ms"] /= reps
     return acc
 
 
 def cpu_baseline(rep_path, x, seconds):
+    """The oracle (plain C + OpenMP restatement of the reference matvec) on
+    the host cores, on the same representation."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind
     nthr = os.cpu_count() or 1
@@ -75,14 +81,29 @@ def cpu_baseline(rep_path, x, seconds):
     lib = oracle_bind.load()
     lib.orc_set_threads(nthr)
     o = oracle_bind.Oracle(rep_path)
-    y = o.matvec(x)   # warm-up (first touch)
+    o.matvec(x)   # first touch
     t0 = time.perf_counter()
     n = 0
-    while n < 10 or (time.perf_counter() - t0 < seconds and n < 200):
-        y = o.matvec(x)
+    while n < 5 or (time.perf_counter() - t0 < seconds and n < 200):
+        o.matvec(x)
         n += 1
     ms = (time.perf_counter() - t0) / n * 1e3
-    return ms, nthr, n, y, o
+    return ms, nthr, n, o
+
+
+def uncontract(tr, D, n):
+    """x of src/H2ERI_build_Coulomb.c:26-62 (host copy for the timed input)."""
+    import numpy as np
+    x = np.empty(n)
+    sp, sb, bf = tr["sp_shell_idx"], tr["sp_bfp_sidx"], tr["shell_bf_sidx"]
+    npair = tr["num_sp"]
+    for i in range(npair):
+        s0, s1 = sp[i], sp[i + npair]
+        r0, nr = bf[s0], bf[s0 + 1] - bf[s0]
+        c0, nc = bf[s1], bf[s1 + 1] - bf[s1]
+        blk = D[c0:c0 + nc, r0:r0 + nr] * (1.0 if s0 == s1 else 2.0)
+        x[sb[i]:sb[i + 1]] = blk.reshape(-1)
+    return x
 
 
 def main():
@@ -93,13 +114,16 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batches", default="1,2,4,8", help="densities per pass to time")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MB first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (got --gpus {args.gpus}); the single-GPU "
-              "kernel is the unit of work and no multi-GPU path is provided.", file=sys.stderr)
+              "matvec is the unit of work and no multi-GPU path is provided.", file=sys.stderr)
         sys.exit(2)
+    batches = [int(b) for b in args.batches.split(",")]
+    assert batches[0] == 1 and all(b in (1, 2, 4, 8) for b in batches)
 
     import numpy as np
     import torch
@@ -117,27 +141,19 @@ def main():
         f"{h.s.num_sp} shell pairs, N={h.n}; building H2 ...")
     h.build_h2()
     t_build = time.perf_counter() - t0
-    sizes = {k: h.s.mat_size[i] * 8 / 1e6 for i, k in enumerate(["U_MB", "B_MB", "D_MB"])}
-    log(f"[bench] H2 built in {t_build:.1f} s: U {sizes['U_MB']:.1f} MB, B {sizes['B_MB']:.1f} MB, "
-        f"D {sizes['D_MB']:.1f} MB")
+    sizes = {k: round(h.s.mat_size[i] * 8 / 1e6, 2) for i, k in enumerate(["U_MB", "B_MB", "D_MB"])}
+    log(f"[bench] H2 built in {t_build:.1f} s: {sizes}")
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     D_dev = torch.from_numpy(D).to(dev)
     J_dev = torch.empty_like(D_dev)
-    tr = h.tree()
-    # uncontracted density x (src/H2ERI_build_Coulomb.c:26-62 layout)
-    x = np.empty(h.n)
-    sp, sb, bf = tr["sp_shell_idx"], tr["sp_bfp_sidx"], tr["shell_bf_sidx"]
-    npair = tr["num_sp"]
-    for i in range(npair):
-        s0, s1 = sp[i], sp[i + npair]
-        r0, nr = bf[s0], bf[s0 + 1] - bf[s0]
-        c0, nc = bf[s1], bf[s1 + 1] - bf[s1]
-        blk = D[c0:c0 + nc, r0:r0 + nr] * (1.0 if s0 == s1 else 2.0)
-        x[sb[i]:sb[i + 1]] = blk.reshape(-1)
-    x_dev = torch.from_numpy(x).to(dev)
-    y_dev = torch.empty_like(x_dev)
+    x = uncontract(h.tree(), D, h.n)
+    nb_max = max(batches)
+    # batch of densities: scaled copies of x (the matvec is linear; inputs are synthetic)
+    X = np.stack([x * (1.0 + 0.05 * k) for k in range(nb_max)])
+    X_dev = torch.from_numpy(X).to(dev)
+    Y_dev = torch.empty_like(X_dev)
 
     if args.calibrate_copy_mb > 0:
         n = args.calibrate_copy_mb * (1 << 20) // 8
@@ -154,66 +170,101 @@ def main():
         h.set_precision(bits)
         resident = h.upload()
         nb = h.matvec_bytes()
-        ms = time_steps(h, x_dev, y_dev, stream, args.steps, args.warmup, torch)
-        prof = launch_profile(h, x_dev, y_dev, stream, max(3, min(args.steps, 20)), torch)
+        mat = nb["U"] + nb["B"] + nb["D"]
+        r = dict(resident=resident, nb=nb, batch={})
+        for nv in batches:
+            h.graph_capture("matvec", nv, X_dev.data_ptr(), Y_dev.data_ptr())
+            steps = args.steps if nv == 1 else max(10, args.steps // 2)
+            ms = time_graph(h, torch, stream, steps, args.warmup)
+            alg = mat + nv * nb["vec"]
+            r["batch"][nv] = dict(ms=ms, ms_per_density=ms / nv, alg=alg, GBs=alg / ms / 1e6)
+            if nv == 1:
+                Y1 = Y_dev[0].cpu().numpy()
+            else:   # every density of the batch equals the single-density product (linearity)
+                Yb = Y_dev[:nv].cpu().numpy()
+                r["batch"][nv]["max_rel_dev_vs_single"] = max(
+                    float(np.linalg.norm(Yb[k] - (1.0 + 0.05 * k) * Y1) / np.linalg.norm((1.0 + 0.05 * k) * Y1))
+                    for k in range(nv))
+            log(f"[bench] FP{bits}-stored, {nv} densities/pass: {ms:.4f} ms/pass, {ms / nv:.4f} ms/density, "
+                f"{alg / ms / 1e6:.0f} GB/s algorithmic")
+        r["y"] = Y1
+        r["prof"] = launch_profile(h, torch, X_dev[0], Y_dev[0], stream, 10)
         h.build_coulomb_dev(D_dev.data_ptr(), J_dev.data_ptr(), stream.cuda_stream)
         torch.cuda.synchronize()
-        res[bits] = dict(ms=ms, alg=nb["alg"], ref=nb["ref_accounting"], resident=resident, nb=nb, prof=prof,
-                         J=J_dev.cpu().numpy(), y=y_dev.cpu().numpy())
-        log(f"[bench] FP{bits}-stored U/B: {ms:.4f} ms/matvec, {nb['alg'] / ms / 1e6:.0f} GB/s algorithmic "
-            f"({nb['alg'] / 1e6:.1f} MB/matvec)")
+        r["J"] = J_dev.cpu().numpy()
+        res[bits] = r
     r64, r32 = res[64], res[32]
     eps_prec = float(np.linalg.norm(r32["J"] - r64["J"]) / np.linalg.norm(r64["J"]))
 
-    # dominant kernel: the tile kernel (bgemv_kernel), all its launches of the
-    # FP32-stored form; algorithmic matrix bytes (U twice, B and D once)
-    tiles = [p for p in r32["prof"] if p["kind"] == "tiles"]
-    tile_ms = sum(p["ms"] for p in tiles)
-    alg_mat = r32["nb"]["U"] + r32["nb"]["B"] + r32["nb"]["D"]
-    dom_gbs = alg_mat / tile_ms / 1e6
+    # dominant kernel: stage_kernel (every launch of the matvec is one), FP32-stored single density
+    prof = r32["prof"]
+    kern_ms = sum(p["ms"] for p in prof)
+    alg1 = r32["batch"][1]["alg"]
+    dom_gbs = alg1 / kern_ms / 1e6
     cpu = None
     eps_parity = None
+    eps_parity32 = None
     if not args.no_cpu:
         with tempfile.TemporaryDirectory() as d:
             rp = os.path.join(d, "rep.h2")
             h.save(rp)
-            cpu_ms, nthr, nrep, y_cpu, o = cpu_baseline(rp, x, args.cpu_seconds)
+            cpu_ms, nthr, nrep, o = cpu_baseline(rp, x, args.cpu_seconds)
             J_cpu = o.coulomb(D)
             o.close()
         eps_parity = float(np.linalg.norm(r64["J"] - J_cpu) / np.linalg.norm(J_cpu))
-        cpu = {"value": round(r64["alg"] / cpu_ms / 1e6, 2), "unit": "GB/s", "cores": nthr, "kind": "port",
-               "sample": f"{nrep} oracle FP64 matvecs on the same C30H62 representation, "
-                         f"{cpu_ms:.2f} ms each"}
+        eps_parity32 = float(np.linalg.norm(r32["J"] - J_cpu) / np.linalg.norm(J_cpu))
+        cpu = {"value": round(r64["batch"][1]["alg"] / cpu_ms / 1e6, 2), "unit": "GB/s", "cores": nthr,
+               "kind": "port",
+               "sample": f"{nrep} oracle FP64 matvecs of the same C30H62 representation, {cpu_ms:.2f} ms each"}
+        log(f"[bench] CPU oracle: {cpu_ms:.2f} ms/matvec on {nthr} threads")
+    # HBM traffic per launch of the same command, from the rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs recorded next to the kernel-trace summary
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_c30h62_j_matvec.json")
+    if args.config == 1 and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_launch"]
+    best32 = min(r32["batch"].values(), key=lambda b: b["ms_per_density"])
+    best32_nv = [k for k, b in r32["batch"].items() if b is best32][0]
+    fp64_plain = r64["batch"][1]["ms"]
     out = {
         "metric": METRIC,
-        "value": round(r32["alg"] / r32["ms"] / 1e6, 2),
+        "value": round(r32["batch"][1]["GBs"], 2),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r32["ms"], 5),
+        "ms_per_step": round(r32["batch"][1]["ms"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-stored U/B, f64 D and accumulation",
         "data": "synthetic (seeded C30H62 geometry and density)",
-        "config": {"workload": f"C30H62 cc-pVDZ J matvec, reltol {reltol:g}, 1 density",
+        "config": {"workload": f"C30H62 cc-pVDZ J matvec, reltol {reltol:g}, 1 density per step",
                    "num_bf": h.nbf, "krnl_mat_size": h.n, "H2_MB": sizes,
-                   "alg_MB_fp32": round(r32["alg"] / 1e6, 2), "alg_MB_fp64": round(r64["alg"] / 1e6, 2),
-                   "cache_resident": r32["alg"] < MALL_BYTES},
-        "fp64": {"ms": round(r64["ms"], 5), "GBs": round(r64["alg"] / r64["ms"] / 1e6, 2)},
-        "fp32_speedup_vs_fp64": round(r64["ms"] / r32["ms"], 4),
+                   "alg_MB_fp32": round(r32["batch"][1]["alg"] / 1e6, 2),
+                   "alg_MB_fp64": round(r64["batch"][1]["alg"] / 1e6, 2),
+                   "cache_resident": r32["batch"][1]["alg"] < MALL_BYTES, "timing": "hipGraph replays"},
+        "fp64": {"ms": round(fp64_plain, 5), "GBs": round(r64["batch"][1]["GBs"], 2)},
+        "fp32_speedup_vs_fp64": round(fp64_plain / r32["batch"][1]["ms"], 4),
+        "batched": {f"fp{bits}": {str(nv): {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
+                                            for k, v in b.items() if k != "alg"}
+                                  for nv, b in res[bits]["batch"].items()} for bits in (64, 32)},
+        "fp32_batched_best": {"densities_per_pass": best32_nv, "ms_per_density": round(best32["ms_per_density"], 5),
+                              "speedup_vs_fp64_plain": round(fp64_plain / best32["ms_per_density"], 3),
+                              "GBs_algorithmic": round(best32["GBs"], 2),
+                              "roofline_frac": round(best32["GBs"] / HBM_PEAK_GBS, 4)},
         "rel_err_J_fp32_vs_fp64": eps_prec,
         "rel_err_J_fp64_vs_oracle": eps_parity,
+        "rel_err_J_fp32_vs_oracle": eps_parity32,
         "build_s": round(t_build, 2),
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "bgemv_kernel", "launches": len(tiles),
-                     "kernel_ms_per_matvec": round(tile_ms, 5),
-                     "alg_bytes_per_launch": round(alg_mat / len(tiles), 1),
-                     "avg_launch_ms": round(tile_ms / len(tiles), 6),
-                     "streamed_bytes_per_matvec": sum(p["mat_bytes"] for p in tiles)},
-        "phases_ms_fp32": {p: round(sum(q["ms"] for q in r32["prof"] if q["phase"] == p), 5)
+                     "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "stage_kernel<1>", "launches_per_matvec": len(prof),
+                     "alg_bytes_per_launch": round(alg1 / len(prof), 1),
+                     "avg_launch_ms": round(kern_ms / len(prof), 6),
+                     "streamed_matrix_bytes_per_matvec": sum(p["mat_bytes"] for p in prof)},
+        "phases_ms_fp32": {p: round(sum(q["ms"] for q in prof if q["phase"] == p), 5)
                            for p in ("up", "B", "y1", "down", "D", "gather")},
         "cpu_baseline": cpu,
     }

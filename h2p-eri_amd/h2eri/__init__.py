@@ -230,6 +230,17 @@ class H2ERI:
                                                C.c_void_p(stream or 0))
         _check(self.lib)
 
+    def graph_capture(self, kind, n, in_ptr, out_ptr):
+        """Capture the device matvec (kind "matvec") or Coulomb build (kind
+        "coulomb") on fixed device buffers into a HIP graph."""
+        k = {"matvec": 0, "coulomb": 1}[kind]
+        self.lib.H2ERI_graph_capture(self.h, k, n, C.c_void_p(in_ptr), C.c_void_p(out_ptr))
+        _check(self.lib)
+
+    def graph_replay(self, stream=None):
+        self.lib.H2ERI_graph_replay(self.h, C.c_void_p(stream or 0))
+        _check(self.lib)
+
     def build_exchange(self, D):
         D = np.ascontiguousarray(D, dtype=np.float64)
         assert D.shape == (self.nbf, self.nbf)

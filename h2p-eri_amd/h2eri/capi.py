@@ -91,6 +91,8 @@ SIGNATURES = {
     "H2ERI_build_Coulomb_dev": (None, [H2ERI_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "H2ERI_build_Coulomb_multi_dev": (None, [H2ERI_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "H2ERI_upload": (C.c_size_t, [H2ERI_p]),
+    "H2ERI_graph_capture": (C.c_int, [H2ERI_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "H2ERI_graph_replay": (C.c_int, [H2ERI_p, C.c_void_p]),
     "H2ERI_save_rep": (C.c_int, [H2ERI_p, C.c_char_p]),
     "H2ERI_load_rep": (C.c_int, [H2ERI_p, C.c_char_p]),
     "H2ERI_build_H2_synthetic": (None, [H2ERI_p, C.c_ulonglong, C.c_int, C.c_double, C.c_double]),

@@ -264,6 +264,16 @@ void H2ERI_build_Coulomb_dev(H2ERI_p h2eri, const double *den_dev, double *J_dev
 void H2ERI_build_Coulomb_multi_dev(H2ERI_p h2eri, const int ndens, const double *den_dev, double *J_dev,
                                    void *stream);
 
+/* HIP-graph form of the device hot path, for loops that repeat it on the
+ * same buffers (SCF iterations).  kind 0 captures H2ERI_matvec_multi_dev
+ * with n vectors, kind 1 H2ERI_build_Coulomb_multi_dev with n densities, on
+ * the device buffers in_dev/out_dev; nothing runs during the capture.
+ * H2ERI_graph_replay enqueues the captured sequence on `stream`.  One graph
+ * per handle; changing precision, tile size or schedule releases it.
+ * Return 0, or -1 with the error set. */
+int H2ERI_graph_capture(H2ERI_p h2eri, const int kind, const int n, const double *in_dev, double *out_dev);
+int H2ERI_graph_replay(H2ERI_p h2eri, void *stream);
+
 /* Upload the stored representation to the device now (otherwise done by the
  * first hot-path call).  Returns bytes resident on the device. */
 size_t H2ERI_upload(H2ERI_p h2eri);

@@ -87,3 +87,20 @@ def test_coulomb_alk32_fixture():
     assert relerr(h.build_coulomb(D), Jref) < 5e-7
     h.set_precision(32)
     assert relerr(h.build_coulomb(D), Jref) < 5e-7
+
+
+def test_coulomb_graph_replay(butane):
+    import torch
+    h, path, D = butane
+    o = oracle_bind.Oracle(path)
+    Dd = torch.from_numpy(D).to("cuda:0")
+    Jd = torch.empty_like(Dd)
+    s = torch.cuda.current_stream().cuda_stream
+    h.graph_capture("coulomb", 1, Dd.data_ptr(), Jd.data_ptr())
+    for k in range(3):
+        Dk = D * (1.0 + 0.1 * k)
+        Dd.copy_(torch.from_numpy(Dk))
+        Jd.zero_()
+        h.graph_replay(s)
+        torch.cuda.synchronize()
+        assert relerr(Jd.cpu().numpy(), o.coulomb(Dk)) < 1e-13

@@ -128,3 +128,24 @@ def test_matvec_multi_device_pointers(reps):
     Y = Yd.cpu().numpy()
     for k in range(4):
         assert relerr(Y[k], o.matvec(X[k])) < 1e-13
+
+
+@pytest.mark.parametrize("nvec", [1, 4])
+def test_graph_replay_matches_eager(reps, nvec):
+    """A captured graph replayed several times, with eager work and new input
+    values between the replays, equals the eager call every time."""
+    import torch
+    h, path = reps["chain_lrdiag"]
+    rng = np.random.default_rng(20 + nvec)
+    Xd = torch.empty((nvec, h.n), dtype=torch.float64, device="cuda:0")
+    Yd = torch.empty_like(Xd)
+    Ye = torch.empty_like(Xd)
+    s = torch.cuda.current_stream().cuda_stream
+    h.graph_capture("matvec", nvec, Xd.data_ptr(), Yd.data_ptr())
+    for _ in range(3):
+        Xd.copy_(torch.from_numpy(rng.standard_normal((nvec, h.n))))
+        Yd.zero_()
+        h.graph_replay(s)
+        h.matvec_multi_dev(nvec, Xd.data_ptr(), Ye.data_ptr(), s)
+        torch.cuda.synchronize()
+        assert torch.equal(Yd, Ye)
