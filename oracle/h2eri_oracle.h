@@ -9,7 +9,7 @@
  * version 1) with its own reader, and follows:
  *   orc_matvec          src/H2ERI_matvec.c:24-63, 93-407, 572-757, 882-957
  *   orc_build_coulomb   src/H2ERI_build_Coulomb.c:26-152
- *   orc_build_exchange  src/H2ERI_build_exchange.c:193-1450
+ *   orc_build_exchange  src/H2ERI_build_exchange.c:1356-1450 (same quantity, block by block)
  *   orc_dense_matrix    (brute force) A_H2 assembled block by block from the
  *                       expanded nested bases, independent of the sweeps
  * Parity pins: see the header comment of each source file.
