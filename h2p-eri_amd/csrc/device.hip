@@ -606,6 +606,7 @@ static void release_buffers(DevState *d) {
 }
 
 void device_release(H2ERI_p h) {
+    exchange_release(h);
     DevState *d = dstate(h);
     if (!d) return;
     release_buffers(d);
@@ -620,6 +621,7 @@ void device_release(H2ERI_p h) {
 void device_invalidate(H2ERI_p h) {
     DevState *d = dstate(h);
     if (d) release_buffers(d);
+    exchange_release(h);
 }
 
 static DevState *dstate_get(H2ERI_p h) {
@@ -1485,10 +1487,6 @@ void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
 
 size_t h2eri_struct_size(void) { return sizeof(struct H2ERI); }
 
-void H2ERI_build_exchange(H2ERI_p h, const double *den_mat, double *K_mat) {
-    (void)h; (void)den_mat; (void)K_mat;
-    set_error(ERR_UNSUPPORTED, "H2ERI_build_exchange: not available in this build yet");
-}
 
 // Verification of a plan on the host (no HIP call): executes every tile and
 // reduce task with the kernels' semantics in launch order for nv vectors
@@ -1618,7 +1616,5 @@ int H2ERI_plan_check(H2ERI_p h, const int ub_bits) {
     return tasks;
 }
 
-void H2ERI_exchange_workbuf_init(H2ERI_p) {}
-void H2ERI_exchange_workbuf_free(H2ERI_p) {}
 
 }  // extern "C"

@@ -1,0 +1,672 @@
+// Exchange matrix through the H2 representation on the MI355X:
+//
+//   K_mq = sum_{n,p} (mn|pq) D_np            (src/H2ERI_build_exchange.c:1356-1450)
+//
+// The reference evaluates K with partial multi-column H2 products, one per
+// (N,Q) shell pair (:1021-1353), with gathered inputs (:609-688) and an atomic
+// scatter of the outputs into K (:691-771).  This device version evaluates
+// the same sum block by block of A_H2 instead:
+//
+//   A_H2 = sum_i D_ii + sum_{(i,j)} [F_ij + F_ij^T],   F_ij = L_ij R_ij^T
+//
+// - diagonal near-field blocks are applied once;
+// - every reduced admissible (B) or inadmissible (D) pair is applied with its
+//   transpose (src/H2ERI_matvec.c:299-407,572-660).
+// L and R come from the expanded nested bases (P_n = U of a leaf, or
+// blockdiag(P of children) U of an inner node) and the block's core (dense C,
+// or U V^T for a low-rank block), as listed at build_terms.
+//
+// Row x of A_H2 is the basis-function pair (a_x, b_x) of a screened shell
+// pair, stored with the smaller shell first.  When the two shells differ,
+// the row stands for both (ab| and (ba|; a same-shell pair has one row per
+// ordered (a,b).  For a term with rows in node i and columns in node j, the
+// contraction factorises through the basis-function sets B(i), B(j) of the
+// two nodes:
+//
+//   G[t][n][q] = sum_{c in j, (p,q) orientation of c} R[c][t] D[n][p]   (ket, n in B(i), q in B(j))
+//   K[m][q]   += sum_{r in i, (m,n) orientation of r} sum_t L[r][t] G[t][n][q]   (bra)
+//
+// For big nodes this costs kk (|B(i)| m_j + m_i |B(j)|) rather than the
+// m_i m_j kk of forming F.  Dense near-field blocks are contracted directly.
+//
+// Accumulation and symmetry:
+// - For symmetric D the transpose terms give the transpose of the F terms:
+//   K = K1 + K1^T + K_diag.
+// - Contributions go into K1 and K_diag with FP64 atomic adds, as the
+//   reference's scatter does (:767), so the last bits of K depend on the
+//   order of the adds.
+// - L and R products of the low-rank and coupling cores are FP64 MFMA GEMMs
+//   (v_mfma_f64_16x16x4f64).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "h2eri_internal.h"
+
+namespace h2e {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+enum KSrc { KS_PB = 0, KS_CB = 1, KS_SC = 2 };
+
+struct KTerm {
+    int direct;           // 1: dense core contracted directly (identity bases on both sides)
+    int diag;             // 1: diagonal near-field block (into K_diag), 0: into K1
+    int ni, nj;           // node-list ids of the row and column ranges
+    int m0, m1, kk;
+    int Lsrc, Lld;        // L[r][t] = src[Loff + r*Lld + t]   (direct: C[r][c] = src[Loff + r*Lld + c])
+    long long Loff;
+    int Rsrc, Rrs, Rcs;   // R[c][t] = src[Roff + c*Rrs + t*Rcs]
+    long long Roff;
+    long long Goff;       // G (kk x |B(i)| x |B(j)|) in scratch
+    int pad_[2];
+};
+
+struct KGemm {           // out (m x n, ld ldo) = A (m x k, PB, ld lda) * X (k x n, CB, ld ldx)
+    long long Aoff, Xoff, Ooff;
+    int m, n, k, lda, ldx, ldo;
+};
+
+struct KTask {           // one workgroup of a phase: term, first (n,q) or (m,q) index, t chunk
+    int term, e0, t0, pad_;
+};
+
+struct KGemmTask {
+    int g, row0, col0, pad_;
+};
+
+struct KNode {           // per row/column range: basis-function set and the two orientation lists
+    int bf_off, nbf;      // B(node) = bfidx[bf_off .. bf_off + nbf)
+    int q_ptr;            // byQ CSR rows: qptr[q_ptr .. q_ptr + nbf]
+    int m_ptr;            // byM CSR rows: mptr[m_ptr .. m_ptr + nbf]
+};
+
+static const int KWG = 256;
+static const int KTC = 8;   // t values per ket thread
+
+// G[t][n][q] for t in [t0, t0+KTC), one (n, q) per thread.
+__global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
+                                               const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
+                                               const int *__restrict__ qptr, const int2 *__restrict__ qent,
+                                               const double *__restrict__ PB, const double *__restrict__ CB,
+                                               double *__restrict__ SC, const double *__restrict__ D, int nbf) {
+    const KTask tk = tasks[blockIdx.x];
+    const KTerm t = terms[tk.term];
+    const KNode ni = nodes[t.ni], nj = nodes[t.nj];
+    const int e = tk.e0 + threadIdx.x;
+    if (e >= ni.nbf * nj.nbf) return;
+    const int nl = e / nj.nbf, ql = e - nl * nj.nbf;
+    const int n = bfidx[ni.bf_off + nl];
+    const double *R = (t.Rsrc == KS_PB ? PB : (t.Rsrc == KS_CB ? CB : SC)) + t.Roff;
+    const double *Dn = D + (size_t)n * nbf;
+    double acc[KTC];
+    #pragma unroll
+    for (int u = 0; u < KTC; u++) acc[u] = 0.0;
+    const int tl = min(KTC, t.kk - tk.t0);
+    for (int j = qptr[nj.q_ptr + ql]; j < qptr[nj.q_ptr + ql + 1]; j++) {
+        const int2 cp = qent[j];   // (column, p)
+        const double d = Dn[cp.y];
+        const double *Rc = R + (long long)cp.x * t.Rrs + (long long)tk.t0 * t.Rcs;
+        #pragma unroll
+        for (int u = 0; u < KTC; u++)
+            if (u < tl) acc[u] = fma(Rc[(long long)u * t.Rcs], d, acc[u]);
+    }
+    double *G = SC + t.Goff + (size_t)nl * nj.nbf + ql;
+    const size_t tstride = (size_t)ni.nbf * nj.nbf;
+    #pragma unroll
+    for (int u = 0; u < KTC; u++)
+        if (u < tl) G[(size_t)(tk.t0 + u) * tstride] = acc[u];
+}
+
+// K[m][q] += sum_{(r,n)} sum_t L[r][t] G[t][n][q], one (m, q) per thread.
+__global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
+                                               const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
+                                               const int *__restrict__ mptr, const int2 *__restrict__ ment,
+                                               const double *__restrict__ PB, const double *__restrict__ CB,
+                                               const double *__restrict__ SC, double *__restrict__ K1,
+                                               double *__restrict__ Kd, int nbf) {
+    const KTask tk = tasks[blockIdx.x];
+    const KTerm t = terms[tk.term];
+    const KNode ni = nodes[t.ni], nj = nodes[t.nj];
+    const int e = tk.e0 + threadIdx.x;
+    if (e >= ni.nbf * nj.nbf) return;
+    const int ml = e / nj.nbf, ql = e - ml * nj.nbf;
+    const double *L = (t.Lsrc == KS_PB ? PB : (t.Lsrc == KS_CB ? CB : SC)) + t.Loff;
+    const double *G = SC + t.Goff + ql;
+    const size_t tstride = (size_t)ni.nbf * nj.nbf;
+    double acc = 0.0;
+    for (int j = mptr[ni.m_ptr + ml]; j < mptr[ni.m_ptr + ml + 1]; j++) {
+        const int2 rn = ment[j];   // (row, local n)
+        const double *Lr = L + (long long)rn.x * t.Lld;
+        const double *Gn = G + (size_t)rn.y * nj.nbf;
+        for (int u = 0; u < t.kk; u++) acc = fma(Lr[u], Gn[(size_t)u * tstride], acc);
+    }
+    const int m = bfidx[ni.bf_off + ml], q = bfidx[nj.bf_off + ql];
+    unsafeAtomicAdd((t.diag ? Kd : K1) + (size_t)m * nbf + q, acc);
+}
+
+// Dense core with identity bases: K[m][q] += sum_{(r,n)} sum_{(c,p)} C[r][c] D[n][p].
+__global__ __launch_bounds__(KWG) void kx_direct(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
+                                                  const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
+                                                  const int *__restrict__ qptr, const int2 *__restrict__ qent,
+                                                  const int *__restrict__ mptr, const int2 *__restrict__ ment,
+                                                  const double *__restrict__ CB, const double *__restrict__ D,
+                                                  double *__restrict__ K1, double *__restrict__ Kd, int nbf) {
+    const KTask tk = tasks[blockIdx.x];
+    const KTerm t = terms[tk.term];
+    const KNode ni = nodes[t.ni], nj = nodes[t.nj];
+    const int e = tk.e0 + threadIdx.x;
+    if (e >= ni.nbf * nj.nbf) return;
+    const int ml = e / nj.nbf, ql = e - ml * nj.nbf;
+    const double *C = CB + t.Loff;
+    double acc = 0.0;
+    const int q0 = qptr[nj.q_ptr + ql], q1 = qptr[nj.q_ptr + ql + 1];
+    for (int j = mptr[ni.m_ptr + ml]; j < mptr[ni.m_ptr + ml + 1]; j++) {
+        const int2 rn = ment[j];
+        const double *Cr = C + (long long)rn.x * t.Lld;
+        const double *Dn = D + (size_t)bfidx[ni.bf_off + rn.y] * nbf;
+        double s = 0.0;
+        for (int i = q0; i < q1; i++) {
+            const int2 cp = qent[i];
+            s = fma(Cr[cp.x], Dn[cp.y], s);
+        }
+        acc += s;
+    }
+    const int m = bfidx[ni.bf_off + ml], q = bfidx[nj.bf_off + ql];
+    unsafeAtomicAdd((t.diag ? Kd : K1) + (size_t)m * nbf + q, acc);
+}
+
+// Grouped FP64 GEMM on MFMA: a 64 x 64 output tile per workgroup, wave w owns
+// rows [16w, 16w+16) and four 16 x 16 column tiles.  Operand layout of
+// v_mfma_f64_16x16x4f64 (checked on gfx950): lane l holds A[l%16][l/16] and
+// B[l/16][l%16]; output register r of lane l is C[l/16 + 4r][l%16].
+__global__ __launch_bounds__(KWG) void kx_gemm(const KGemmTask *__restrict__ tasks, const KGemm *__restrict__ gemms,
+                                                const double *__restrict__ PB, const double *__restrict__ CB,
+                                                double *__restrict__ SC) {
+    const KGemmTask tk = tasks[blockIdx.x];
+    const KGemm g = gemms[tk.g];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ar = tk.row0 + 16 * w + (lane & 15);
+    const int kl = lane >> 4;
+    const double *A = PB + g.Aoff;
+    const double *X = CB + g.Xoff;
+    d4 acc[4];
+    #pragma unroll
+    for (int c = 0; c < 4; c++) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < g.k; k0 += 4) {
+        const int kk = k0 + kl;
+        const double a = (ar < g.m && kk < g.k) ? A[(long long)ar * g.lda + kk] : 0.0;
+        #pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int col = tk.col0 + 16 * c + (lane & 15);
+            const double b = (kk < g.k && col < g.n) ? X[(long long)kk * g.ldx + col] : 0.0;
+            acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+        }
+    }
+    #pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int col = tk.col0 + 16 * c + (lane & 15);
+        #pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = tk.row0 + 16 * w + (lane >> 4) + 4 * r;
+            if (row < g.m && col < g.n) SC[g.Ooff + (long long)row * g.ldo + col] = acc[c][r];
+        }
+    }
+}
+
+__global__ void kx_zero(double *__restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t s = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += s) p[i] = 0.0;
+}
+
+// K = K1 + K1^T + K_diag
+__global__ void kx_final(const double *__restrict__ K1, const double *__restrict__ Kd, double *__restrict__ K, int nbf) {
+    const size_t n2 = (size_t)nbf * nbf;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t s = (size_t)gridDim.x * blockDim.x;
+    for (; i < n2; i += s) {
+        const size_t r = i / nbf, c = i - r * nbf;
+        K[i] = K1[i] + K1[c * nbf + r] + Kd[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host plan
+// ---------------------------------------------------------------------------
+struct KPlan {
+    std::vector<double> PB, CB;
+    std::vector<KTerm> terms;
+    std::vector<KGemm> gemms;
+    std::vector<KNode> nodes;
+    std::vector<int> bfidx, qptr, mptr;
+    std::vector<int2> qent, ment;
+    std::vector<KTask> ket, bra, direct;
+    std::vector<KGemmTask> gtasks;
+    long long nSC = 0;
+    double flops = 0;
+};
+
+struct KState {
+    KPlan P;
+    bool ready = false;
+    double *PB = nullptr, *CB = nullptr, *SC = nullptr, *K1 = nullptr, *Kd = nullptr, *D = nullptr, *K = nullptr;
+    KTerm *terms = nullptr;
+    KGemm *gemms = nullptr;
+    KNode *nodes = nullptr;
+    int *bfidx = nullptr, *qptr = nullptr, *mptr = nullptr;
+    int2 *qent = nullptr, *ment = nullptr;
+    KTask *ket = nullptr, *bra = nullptr, *direct = nullptr;
+    KGemmTask *gtasks = nullptr;
+    size_t resident = 0;
+};
+
+static void kfree(void *&p) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+}
+
+static void kstate_release(KState *k) {
+    void **ps[] = {(void **)&k->PB, (void **)&k->CB, (void **)&k->SC, (void **)&k->K1, (void **)&k->Kd, (void **)&k->D,
+                   (void **)&k->K, (void **)&k->terms, (void **)&k->gemms, (void **)&k->nodes, (void **)&k->bfidx,
+                   (void **)&k->qptr, (void **)&k->mptr, (void **)&k->qent, (void **)&k->ment, (void **)&k->ket,
+                   (void **)&k->bra, (void **)&k->direct, (void **)&k->gtasks};
+    for (void **p : ps) kfree(*p);
+    k->ready = false;
+    k->P = KPlan();
+}
+
+struct KBuilder {
+    H2ERI_p h;
+    KPlan &P;
+    std::vector<int> ra, rb;
+    std::vector<char> sw;
+    std::vector<long long> pb_off;   // expanded basis of each tree node in PB (-1: none)
+    std::vector<int> node_list;      // tree node -> KNode id (-1: not built)
+
+    KBuilder(H2ERI_p h_, KPlan &p_) : h(h_), P(p_) {}
+
+    int rank_of(int n) const { return h->U[n] ? h->U[n]->ncol : 0; }
+    int rows_of(int n) const { return h->mat_cluster[2 * n + 1] - h->mat_cluster[2 * n] + 1; }
+
+    bool pair_rows() {
+        const int N = h->krnl_mat_size, np = h->num_sp;
+        ra.assign(N, -1); rb.assign(N, -1); sw.assign(N, 0);
+        for (int i = 0; i < np; i++) {
+            const int s0 = h->sp_shell_idx[i], s1 = h->sp_shell_idx[i + np];
+            const int f0 = h->shell_bf_sidx[s0], n0 = h->shell_bf_sidx[s0 + 1] - f0;
+            const int f1 = h->shell_bf_sidx[s1], n1 = h->shell_bf_sidx[s1 + 1] - f1;
+            for (int j = 0; j < n1; j++)
+                for (int k = 0; k < n0; k++) {
+                    const int x = h->sp_bfp_sidx[i] + j * n0 + k;
+                    if (x < 0 || x >= N) { set_error(ERR_ARG, "exchange: shell-pair layout out of range"); return false; }
+                    ra[x] = f0 + k; rb[x] = f1 + j; sw[x] = s0 != s1;
+                }
+        }
+        for (int x = 0; x < N; x++)
+            if (ra[x] < 0) { set_error(ERR_ARG, "exchange: row %d belongs to no shell pair", x); return false; }
+        return true;
+    }
+
+    // expanded nested bases of every node with a basis, leaves first
+    void expand_bases() {
+        const int nn = h->n_node, nl = h->n_leaf_node;
+        pb_off.assign(nn, -1);
+        for (int lv = h->max_level; lv >= 0; lv--)
+            for (int j = 0; j < h->level_n_node[lv]; j++) {
+                const int node = h->level_nodes[(size_t)lv * nl + j];
+                const int k = rank_of(node);
+                if (k == 0 || h->U[node]->nrow == 0) continue;
+                const int s = h->mat_cluster[2 * node], m = rows_of(node);
+                const long long off = (long long)P.PB.size();
+                P.PB.resize(P.PB.size() + (size_t)m * k, 0.0);
+                double *Pn = P.PB.data() + off;
+                const H2E_dense_mat_p U = h->U[node];
+                if (h->n_child[node] == 0) {
+                    if (U->nrow != m) continue;
+                    for (int r = 0; r < m; r++) memcpy(Pn + (size_t)r * k, U->data + (size_t)r * U->ld, sizeof(double) * k);
+                } else {
+                    int uoff = 0;
+                    for (int c = 0; c < h->n_child[node]; c++) {
+                        const int ch = h->children[(size_t)node * h->max_child + c];
+                        const int kc = rank_of(ch);
+                        if (kc > 0 && pb_off[ch] >= 0) {
+                            const double *Pc = P.PB.data() + pb_off[ch];
+                            const int cs = h->mat_cluster[2 * ch] - s, cm = rows_of(ch);
+                            for (int row = 0; row < cm; row++)
+                                for (int l = 0; l < kc; l++) {
+                                    const double a = Pc[(size_t)row * kc + l];
+                                    const double *urow = U->data + (size_t)(uoff + l) * U->ld;
+                                    double *o = Pn + (size_t)(cs + row) * k;
+                                    for (int t = 0; t < k; t++) o[t] += a * urow[t];
+                                }
+                        }
+                        uoff += kc;
+                    }
+                }
+                pb_off[node] = off;
+            }
+    }
+
+    // basis-function set and orientation lists of a tree node's row range
+    int list_of(int node) {
+        if (node_list[node] >= 0) return node_list[node];
+        const int s = h->mat_cluster[2 * node], m = rows_of(node);
+        std::vector<int> bfs;
+        bfs.reserve(2 * m);
+        for (int r = 0; r < m; r++) { bfs.push_back(ra[s + r]); bfs.push_back(rb[s + r]); }
+        std::sort(bfs.begin(), bfs.end());
+        bfs.erase(std::unique(bfs.begin(), bfs.end()), bfs.end());
+        const int nb = (int)bfs.size();
+        std::vector<int> loc(h->num_bf, -1);
+        for (int i = 0; i < nb; i++) loc[bfs[i]] = i;
+        KNode kn;
+        kn.bf_off = (int)P.bfidx.size();
+        kn.nbf = nb;
+        P.bfidx.insert(P.bfidx.end(), bfs.begin(), bfs.end());
+        // byQ: ket orientations (p,q) of each row c; byM: bra orientations (m,n) of each row r
+        std::vector<std::vector<int2>> byq(nb), bym(nb);
+        for (int r = 0; r < m; r++) {
+            const int x = s + r, a = ra[x], b = rb[x];
+            byq[loc[b]].push_back(make_int2(r, a));
+            bym[loc[a]].push_back(make_int2(r, loc[b]));
+            if (sw[x]) {
+                byq[loc[a]].push_back(make_int2(r, b));
+                bym[loc[b]].push_back(make_int2(r, loc[a]));
+            }
+        }
+        kn.q_ptr = (int)P.qptr.size();
+        for (int i = 0; i < nb; i++) { P.qptr.push_back((int)P.qent.size()); P.qent.insert(P.qent.end(), byq[i].begin(), byq[i].end()); }
+        P.qptr.push_back((int)P.qent.size());
+        kn.m_ptr = (int)P.mptr.size();
+        for (int i = 0; i < nb; i++) { P.mptr.push_back((int)P.ment.size()); P.ment.insert(P.ment.end(), bym[i].begin(), bym[i].end()); }
+        P.mptr.push_back((int)P.ment.size());
+        P.nodes.push_back(kn);
+        node_list[node] = (int)P.nodes.size() - 1;
+        return node_list[node];
+    }
+
+    struct Core { bool lr; long long C, U, VT; int nrow, ncol, k; };
+    Core store_core(H2E_dense_mat_p b) {
+        Core c{false, -1, -1, -1, b->nrow, b->ncol, 0};
+        const long long off = (long long)P.CB.size();
+        if (b->ld >= 0) {
+            P.CB.insert(P.CB.end(), b->data, b->data + (size_t)b->nrow * b->ncol);
+            c.C = off;
+        } else {
+            c.lr = true;
+            c.k = -b->ld;
+            P.CB.insert(P.CB.end(), b->data, b->data + (size_t)(b->nrow + b->ncol) * c.k);
+            c.U = off;
+            c.VT = off + (long long)b->nrow * c.k;
+        }
+        return c;
+    }
+    long long salloc(long long n) {
+        const long long o = P.nSC;
+        P.nSC += (n + 31) & ~31LL;
+        return o;
+    }
+    long long gemm(long long Aoff, int lda, int m, int k, long long Xoff, int ldx, int n) {
+        KGemm g;
+        g.Aoff = Aoff; g.lda = lda; g.m = m; g.k = k; g.Xoff = Xoff; g.ldx = ldx; g.n = n;
+        g.Ooff = salloc((long long)m * n);
+        g.ldo = n;
+        P.gemms.push_back(g);
+        P.flops += 2.0 * m * n * k;
+        return g.Ooff;
+    }
+
+    // one term: rows of node n0 (basis side if bas0) x columns of node n1 (bas1), core b
+    void add_term(int n0, bool bas0, int n1, bool bas1, H2E_dense_mat_p b, int diag) {
+        if (!b || b->nrow == 0 || b->ncol == 0) return;
+        const int m0 = rows_of(n0), m1 = rows_of(n1);
+        const int k0 = bas0 ? rank_of(n0) : m0, k1 = bas1 ? rank_of(n1) : m1;
+        if (b->nrow != k0 || b->ncol != k1) return;   // not part of A_H2
+        if ((bas0 && pb_off[n0] < 0) || (bas1 && pb_off[n1] < 0)) return;
+        const Core c = store_core(b);
+        KTerm t;
+        memset(&t, 0, sizeof(t));
+        t.diag = diag;
+        t.ni = list_of(n0);
+        t.nj = list_of(n1);
+        t.m0 = m0; t.m1 = m1;
+        if (!bas0 && !bas1 && !c.lr) {            // dense near-field block: direct contraction
+            t.direct = 1;
+            t.kk = m1;
+            t.Lsrc = KS_CB; t.Loff = c.C; t.Lld = m1;
+            P.terms.push_back(t);
+            return;
+        }
+        if (!c.lr) {
+            if (bas0 && bas1) {        // L = P0 C, R = P1
+                t.kk = k1;
+                t.Lsrc = KS_SC; t.Loff = gemm(pb_off[n0], k0, m0, k0, c.C, k1, k1); t.Lld = k1;
+                t.Rsrc = KS_PB; t.Roff = pb_off[n1]; t.Rrs = k1; t.Rcs = 1;
+            } else if (bas0) {         // L = P0, R = C^T
+                t.kk = k0;
+                t.Lsrc = KS_PB; t.Loff = pb_off[n0]; t.Lld = k0;
+                t.Rsrc = KS_CB; t.Roff = c.C; t.Rrs = 1; t.Rcs = m1;
+            } else {                   // L = C, R = P1
+                t.kk = k1;
+                t.Lsrc = KS_CB; t.Loff = c.C; t.Lld = k1;
+                t.Rsrc = KS_PB; t.Roff = pb_off[n1]; t.Rrs = k1; t.Rcs = 1;
+            }
+        } else {
+            const int k = c.k;
+            t.kk = k;
+            if (bas0) { t.Lsrc = KS_SC; t.Loff = gemm(pb_off[n0], k0, m0, k0, c.U, k, k); t.Lld = k; }
+            else { t.Lsrc = KS_CB; t.Loff = c.U; t.Lld = k; }
+            if (bas1) { t.Rsrc = KS_SC; t.Roff = gemm(pb_off[n1], k1, m1, k1, c.VT, k, k); t.Rrs = k; t.Rcs = 1; }
+            else { t.Rsrc = KS_CB; t.Roff = c.VT; t.Rrs = k; t.Rcs = 1; }
+        }
+        t.Goff = salloc((long long)t.kk * P.nodes[t.ni].nbf * P.nodes[t.nj].nbf);
+        P.terms.push_back(t);
+    }
+
+    bool build() {
+        if (h->num_bf <= 0 || !h->sp_shell_idx || !h->shell_bf_sidx) { set_error(ERR_ARG, "exchange needs shell-pair metadata"); return false; }
+        if (h->BD_JIT) { set_error(ERR_UNSUPPORTED, "H2ERI_build_exchange does not support BD_JIT representations"); return false; }
+        if (!pair_rows()) return false;
+        expand_bases();
+        node_list.assign(h->n_node, -1);
+        const int nl = h->n_leaf_node;
+        for (int i = 0; i < nl; i++) add_term(h->height_nodes[i], false, h->height_nodes[i], false, h->c_D_blks[i], 1);
+        for (int i = 0; i < h->n_r_inadm_pair; i++)
+            add_term(h->r_inadm_pairs[2 * i], false, h->r_inadm_pairs[2 * i + 1], false, h->c_D_blks[nl + i], 0);
+        for (int i = 0; i < h->n_r_adm_pair; i++) {
+            const int n0 = h->r_adm_pairs[2 * i], n1 = h->r_adm_pairs[2 * i + 1];
+            const int l0 = h->node_level[n0], l1 = h->node_level[n1];
+            add_term(n0, l0 >= l1, n1, l1 >= l0, h->c_B_blks[i], 0);
+        }
+        // launch task lists
+        for (int g = 0; g < (int)P.gemms.size(); g++)
+            for (int r0 = 0; r0 < P.gemms[g].m; r0 += 64)
+                for (int c0 = 0; c0 < P.gemms[g].n; c0 += 64) P.gtasks.push_back(KGemmTask{g, r0, c0, 0});
+        for (int ti = 0; ti < (int)P.terms.size(); ti++) {
+            const KTerm &t = P.terms[ti];
+            const long long ne = (long long)P.nodes[t.ni].nbf * P.nodes[t.nj].nbf;
+            for (long long e0 = 0; e0 < ne; e0 += KWG) {
+                if (t.direct) { P.direct.push_back(KTask{ti, (int)e0, 0, 0}); continue; }
+                for (int t0 = 0; t0 < t.kk; t0 += KTC) P.ket.push_back(KTask{ti, (int)e0, t0, 0});
+                P.bra.push_back(KTask{ti, (int)e0, 0, 0});
+            }
+        }
+        return validate();
+    }
+
+    // every access of every kernel checked on the host before any launch
+    bool validate() const {
+        const long long nPB = (long long)P.PB.size(), nCB = (long long)P.CB.size(), nSC = P.nSC;
+        const int nbf = h->num_bf;
+        auto size_of = [&](int src) { return src == KS_PB ? nPB : (src == KS_CB ? nCB : nSC); };
+        for (const KGemm &g : P.gemms) {
+            const bool ok = g.m > 0 && g.n > 0 && g.k > 0 && g.Aoff >= 0 && g.Aoff + (long long)(g.m - 1) * g.lda + g.k <= nPB &&
+                            g.Xoff >= 0 && g.Xoff + (long long)(g.k - 1) * g.ldx + g.n <= nCB && g.Ooff >= 0 &&
+                            g.Ooff + (long long)(g.m - 1) * g.ldo + g.n <= nSC && g.lda >= g.k && g.ldx >= g.n && g.ldo >= g.n;
+            if (!ok) { set_error(ERR_ARG, "exchange plan check failed: gemm"); return false; }
+        }
+        for (const KNode &n : P.nodes) {
+            bool ok = n.bf_off >= 0 && n.bf_off + n.nbf <= (long long)P.bfidx.size() && n.q_ptr + n.nbf < (long long)P.qptr.size() &&
+                      n.m_ptr + n.nbf < (long long)P.mptr.size();
+            for (int i = 0; ok && i < n.nbf; i++) ok = P.bfidx[n.bf_off + i] >= 0 && P.bfidx[n.bf_off + i] < nbf;
+            if (!ok) { set_error(ERR_ARG, "exchange plan check failed: node lists"); return false; }
+        }
+        for (const KTerm &t : P.terms) {
+            const KNode &ni = P.nodes[t.ni], &nj = P.nodes[t.nj];
+            bool ok = t.kk > 0 && (long long)ni.nbf * nj.nbf < (1LL << 31);
+            // entries: columns < m1, p < nbf; rows < m0, local n < |B(i)|
+            for (int j = P.qptr[nj.q_ptr]; ok && j < P.qptr[nj.q_ptr + nj.nbf]; j++)
+                ok = P.qent[j].x >= 0 && P.qent[j].x < t.m1 && P.qent[j].y >= 0 && P.qent[j].y < nbf;
+            for (int j = P.mptr[ni.m_ptr]; ok && j < P.mptr[ni.m_ptr + ni.nbf]; j++)
+                ok = P.ment[j].x >= 0 && P.ment[j].x < t.m0 && P.ment[j].y >= 0 && P.ment[j].y < ni.nbf;
+            ok = ok && t.Loff >= 0 && t.Loff + (long long)(t.m0 - 1) * t.Lld + t.kk <= size_of(t.Lsrc);
+            if (!t.direct) {
+                const long long last = t.Roff + (long long)(t.m1 - 1) * t.Rrs + (long long)(t.kk - 1) * t.Rcs;
+                ok = ok && t.Roff >= 0 && last < size_of(t.Rsrc) && t.Goff >= 0 &&
+                     t.Goff + (long long)t.kk * ni.nbf * nj.nbf <= nSC;
+            }
+            if (!ok) { set_error(ERR_ARG, "exchange plan check failed: term"); return false; }
+        }
+        return true;
+    }
+};
+
+}  // namespace
+
+static KState *kstate(H2ERI_p h) { return (KState *)h->thread_Kmat_workbuf; }
+
+void exchange_release(H2ERI_p h) {
+    KState *k = kstate(h);
+    if (!k) return;
+    kstate_release(k);
+    delete k;
+    h->thread_Kmat_workbuf = nullptr;
+}
+
+#define KHIP(expr)                                                                                           \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess) {                                                                              \
+            set_error(ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return false;                                                                                    \
+        }                                                                                                    \
+    } while (0)
+
+static bool k_upload(H2ERI_p h) {
+    if (!h->U || !h->c_B_blks || !h->c_D_blks) { set_error(ERR_ARG, "no H2 representation built or loaded"); return false; }
+    if (!h->thread_Kmat_workbuf) h->thread_Kmat_workbuf = (void **)new KState();
+    KState *k = kstate(h);
+    if (k->ready) return true;
+    kstate_release(k);
+    KBuilder b(h, k->P);
+    if (!b.build()) return false;
+    const KPlan &P = k->P;
+    const int nbf = h->num_bf;
+    auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
+        if (bytes == 0) bytes = 64;
+        KHIP(hipMalloc(dst, bytes));
+        if (src) KHIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        k->resident += bytes;
+        return true;
+    };
+    const size_t nn2 = (size_t)nbf * nbf;
+    if (!up((void **)&k->PB, P.PB.empty() ? nullptr : P.PB.data(), P.PB.size() * 8) ||
+        !up((void **)&k->CB, P.CB.empty() ? nullptr : P.CB.data(), P.CB.size() * 8) ||
+        !up((void **)&k->SC, nullptr, (size_t)P.nSC * 8) || !up((void **)&k->K1, nullptr, nn2 * 8) ||
+        !up((void **)&k->Kd, nullptr, nn2 * 8) || !up((void **)&k->D, nullptr, nn2 * 8) ||
+        !up((void **)&k->K, nullptr, nn2 * 8) ||
+        !up((void **)&k->terms, P.terms.empty() ? nullptr : P.terms.data(), P.terms.size() * sizeof(KTerm)) ||
+        !up((void **)&k->gemms, P.gemms.empty() ? nullptr : P.gemms.data(), P.gemms.size() * sizeof(KGemm)) ||
+        !up((void **)&k->nodes, P.nodes.empty() ? nullptr : P.nodes.data(), P.nodes.size() * sizeof(KNode)) ||
+        !up((void **)&k->bfidx, P.bfidx.data(), P.bfidx.size() * 4) || !up((void **)&k->qptr, P.qptr.data(), P.qptr.size() * 4) ||
+        !up((void **)&k->mptr, P.mptr.data(), P.mptr.size() * 4) ||
+        !up((void **)&k->qent, P.qent.empty() ? nullptr : P.qent.data(), P.qent.size() * sizeof(int2)) ||
+        !up((void **)&k->ment, P.ment.empty() ? nullptr : P.ment.data(), P.ment.size() * sizeof(int2)) ||
+        !up((void **)&k->ket, P.ket.empty() ? nullptr : P.ket.data(), P.ket.size() * sizeof(KTask)) ||
+        !up((void **)&k->bra, P.bra.empty() ? nullptr : P.bra.data(), P.bra.size() * sizeof(KTask)) ||
+        !up((void **)&k->direct, P.direct.empty() ? nullptr : P.direct.data(), P.direct.size() * sizeof(KTask)) ||
+        !up((void **)&k->gtasks, P.gtasks.empty() ? nullptr : P.gtasks.data(), P.gtasks.size() * sizeof(KGemmTask)))
+        return false;
+    // L / R products do not depend on D: computed once at upload
+    if (!P.gtasks.empty()) {
+        hipLaunchKernelGGL(kx_gemm, dim3((unsigned)P.gtasks.size()), dim3(KWG), 0, 0, k->gtasks, k->gemms, k->PB, k->CB, k->SC);
+        KHIP(hipGetLastError());
+        KHIP(hipDeviceSynchronize());
+    }
+    k->ready = true;
+    return true;
+}
+
+static bool k_run(H2ERI_p h, const double *D_dev, double *K_dev, hipStream_t s) {
+    if (!k_upload(h)) return false;
+    KState *k = kstate(h);
+    const KPlan &P = k->P;
+    const int nbf = h->num_bf;
+    const size_t nn2 = (size_t)nbf * nbf;
+    const unsigned zb = (unsigned)std::min<size_t>((nn2 + 255) / 256, 2048);
+    hipLaunchKernelGGL(kx_zero, dim3(zb), dim3(256), 0, s, k->K1, nn2);
+    hipLaunchKernelGGL(kx_zero, dim3(zb), dim3(256), 0, s, k->Kd, nn2);
+    KHIP(hipGetLastError());
+    if (!P.ket.empty())
+        hipLaunchKernelGGL(kx_ket, dim3((unsigned)P.ket.size()), dim3(KWG), 0, s, k->ket, k->terms, k->nodes, k->bfidx,
+                           k->qptr, k->qent, k->PB, k->CB, k->SC, D_dev, nbf);
+    if (!P.direct.empty())
+        hipLaunchKernelGGL(kx_direct, dim3((unsigned)P.direct.size()), dim3(KWG), 0, s, k->direct, k->terms, k->nodes,
+                           k->bfidx, k->qptr, k->qent, k->mptr, k->ment, k->CB, D_dev, k->K1, k->Kd, nbf);
+    if (!P.bra.empty())
+        hipLaunchKernelGGL(kx_bra, dim3((unsigned)P.bra.size()), dim3(KWG), 0, s, k->bra, k->terms, k->nodes, k->bfidx,
+                           k->mptr, k->ment, k->PB, k->CB, k->SC, k->K1, k->Kd, nbf);
+    KHIP(hipGetLastError());
+    hipLaunchKernelGGL(kx_final, dim3(zb), dim3(256), 0, s, k->K1, k->Kd, K_dev, nbf);
+    KHIP(hipGetLastError());
+    return true;
+}
+
+}  // namespace h2e
+
+using namespace h2e;
+
+extern "C" {
+
+void H2ERI_build_exchange(H2ERI_p h, const double *den_mat, double *K_mat) {
+    if (!h || !den_mat || !K_mat) { set_error(ERR_ARG, "H2ERI_build_exchange: null argument"); return; }
+    if (!k_upload(h)) return;
+    KState *k = kstate(h);
+    const size_t nn2 = (size_t)h->num_bf * h->num_bf;
+    if (hipMemcpy(k->D, den_mat, nn2 * 8, hipMemcpyHostToDevice) != hipSuccess) { set_error(ERR_DEVICE, "copy D"); return; }
+    if (!k_run(h, k->D, k->K, nullptr)) return;
+    if (hipMemcpy(K_mat, k->K, nn2 * 8, hipMemcpyDeviceToHost) != hipSuccess) { set_error(ERR_DEVICE, "copy K"); return; }
+    h->n_matvec++;
+}
+
+void H2ERI_build_exchange_dev(H2ERI_p h, const double *den_dev, double *K_dev, void *stream) {
+    if (!h || !den_dev || !K_dev) { set_error(ERR_ARG, "H2ERI_build_exchange_dev: null argument"); return; }
+    k_run(h, den_dev, K_dev, (hipStream_t)stream);
+}
+
+// Builds the exchange plan on the device now (otherwise done by the first
+// call); the reference allocates its per-thread K work buffers here.
+void H2ERI_exchange_workbuf_init(H2ERI_p h) {
+    if (h) k_upload(h);
+}
+
+void H2ERI_exchange_workbuf_free(H2ERI_p h) {
+    if (h) exchange_release(h);
+}
+
+// Work figures of the exchange plan: out4 = {GEMM flops at upload, terms,
+// device bytes resident, scratch doubles}.
+void H2ERI_exchange_stats(H2ERI_p h, double *out4) {
+    if (!h || !out4 || !k_upload(h)) return;
+    KState *k = kstate(h);
+    out4[0] = k->P.flops;
+    out4[1] = (double)k->P.terms.size();
+    out4[2] = (double)k->resident;
+    out4[3] = (double)k->P.nSC;
+}
+
+}  // extern "C"

@@ -95,5 +95,7 @@ void update_mat_sizes(H2ERI_p h);
 // device layer (device.hip)
 void device_release(H2ERI_p h);
 void device_invalidate(H2ERI_p h);
+// exchange layer (exchange.hip): device state held in h->thread_Kmat_workbuf
+void exchange_release(H2ERI_p h);
 
 }  // namespace h2e

@@ -249,6 +249,16 @@ class H2ERI:
         _check(self.lib)
         return K
 
+    def build_exchange_dev(self, d_ptr, k_ptr, stream=None):
+        self.lib.H2ERI_build_exchange_dev(self.h, C.c_void_p(d_ptr), C.c_void_p(k_ptr), C.c_void_p(stream or 0))
+        _check(self.lib)
+
+    def exchange_stats(self):
+        out = np.zeros(4)
+        self.lib.H2ERI_exchange_stats(self.h, _dptr(out))
+        _check(self.lib)
+        return dict(zip(["gemm_flops", "terms", "resident_bytes", "scratch_doubles"], out.tolist()))
+
     def set_phase_timing(self, on=True):
         self.lib.H2ERI_set_phase_timing(self.h, 1 if on else 0)
 

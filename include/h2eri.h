@@ -226,13 +226,22 @@ void H2ERI_matvec(H2ERI_p h2eri, const double *x, double *y);
  * (replaces src/H2ERI_build_Coulomb.h:16). */
 void H2ERI_build_Coulomb(H2ERI_p h2eri, const double *den_mat, double *J_mat);
 
-/* K_mq := sum_{n,p} (mn|pq) D_np through the H2 representation
- * (replaces src/H2ERI_build_exchange.h:16). */
+/* K_mq := sum_{n,p} (mn|pq) D_np through the H2 representation, D symmetric,
+ * D and K num_bf x num_bf row-major (replaces src/H2ERI_build_exchange.h:16).
+ * Accumulates with FP64 atomics (as the reference's scatter does), so the
+ * last bits of K may differ between calls. */
 void H2ERI_build_exchange(H2ERI_p h2eri, const double *den_mat, double *K_mat);
-/* Kept for ABI compatibility (src/H2ERI_build_exchange.h:19-22); the
- * library manages its exchange work space itself. */
+/* src/H2ERI_build_exchange.h:19-22: init builds and uploads the exchange plan
+ * now (otherwise the first K build does); free releases its device memory. */
 void H2ERI_exchange_workbuf_init(H2ERI_p h2eri);
 void H2ERI_exchange_workbuf_free(H2ERI_p h2eri);
+
+/* Additive: device-resident exchange build (den_dev, K_dev: num_bf^2 doubles
+ * on the library's device; enqueued on `stream`, no synchronisation), and the
+ * work figures of the exchange plan: out4 = {GEMM flops done once at upload,
+ * number of block terms, device bytes resident, scratch doubles}. */
+void H2ERI_build_exchange_dev(H2ERI_p h2eri, const double *den_dev, double *K_dev, void *stream);
+void H2ERI_exchange_stats(H2ERI_p h2eri, double *out4);
 
 /* ------------------------------------------------------------------ */
 /* Additive entry points (no counterpart in the reference).             */

@@ -117,3 +117,14 @@ def test_device_plan_semantics_batched(tmp_path, nv, seed, n, leaf, mr, shape):
     assert conflicts == 0
     for v in range(nv):
         assert relerr(Y[v], o.matvec(X[v])) < 1e-13
+
+
+def test_exchange_needs_molecule(tmp_path):
+    """A synthetic (chemistry-free) representation has no shell pairs: the
+    exchange build reports an error before touching the device."""
+    import h2eri
+    h = make_synthetic(tmp_path / "s.rep", 300, 3, leaf=20)
+    buf = np.zeros(16)
+    h.lib.H2ERI_build_exchange(h.h, buf.ctypes.data_as(h2eri.capi.c_dbl_p), buf.ctypes.data_as(h2eri.capi.c_dbl_p))
+    with pytest.raises(h2eri.H2ERIError, match="shell-pair metadata"):
+        h2eri._check(h.lib)
