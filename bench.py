@@ -133,13 +133,16 @@ def main():
 
     t0 = time.perf_counter()
     atoms, mol, D, reltol = M.workload(args.config)
+    wname = M.workload_name(args.config)
     h = h2eri.H2ERI(1e-10, 1e-10, reltol)
     h.read_mol_text(mol)
     h.process_shells()
     h.partition()
     log(f"[bench] config {args.config}: {len(atoms)} atoms, {h.nbf} basis functions, "
         f"{h.s.num_sp} shell pairs, N={h.n}; building H2 ...")
+    h.s.print_dbginfo = 1   # construction prints a line per U level and after B and D
     h.build_h2()
+    h.s.print_dbginfo = 0
     t_build = time.perf_counter() - t0
     sizes = {k: round(h.s.mat_size[i] * 8 / 1e6, 2) for i, k in enumerate(["U_MB", "B_MB", "D_MB"])}
     log(f"[bench] H2 built in {t_build:.1f} s: {sizes}")
@@ -237,7 +240,7 @@ def main():
         eps_parity32 = float(np.linalg.norm(r32["J"] - J_cpu) / np.linalg.norm(J_cpu))
         cpu = {"value": round(r64["batch"][1]["alg"] / cpu_ms / 1e6, 2), "unit": "GB/s", "cores": nthr,
                "kind": "port",
-               "sample": f"{nrep} oracle FP64 matvecs of the same C30H62 representation, {cpu_ms:.2f} ms each"}
+               "sample": f"{nrep} oracle FP64 matvecs of the same {wname} representation, {cpu_ms:.2f} ms each"}
         log(f"[bench] CPU oracle: {cpu_ms:.2f} ms/matvec on {nthr} threads")
     # HBM traffic per launch of the same command, from the rocprofv3 FETCH_SIZE /
     # WRITE_SIZE runs recorded next to the kernel-trace summary
@@ -261,8 +264,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-stored U/B, f64 D and accumulation",
-        "data": "synthetic (seeded C30H62 geometry and density)",
-        "config": {"workload": f"C30H62 cc-pVDZ J matvec, reltol {reltol:g}, 1 density per step",
+        "data": f"synthetic (seeded {wname} geometry and density)",
+        "config": {"workload": f"{wname} cc-pVDZ J matvec, reltol {reltol:g}, 1 density per step",
                    "num_bf": h.nbf, "krnl_mat_size": h.n, "H2_MB": sizes,
                    "alg_MB_fp32": round(r32["batch"][1]["alg"] / 1e6, 2),
                    "alg_MB_fp64": round(r64["batch"][1]["alg"] / 1e6, 2),

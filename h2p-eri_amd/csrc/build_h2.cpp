@@ -413,6 +413,10 @@ void build_U(H2ERI_p h, uint64_t seed) {
         }
         for (int lf : lvl_leaf[lv])
             for (int k = h->pt_cluster[2 * lf]; k <= h->pt_cluster[2 * lf + 1]; k++) skel[k] = 1;
+        if (h->print_dbginfo) {
+            fprintf(stderr, "[H2ERI_build_H2] U level %d: %d nodes done\n", lv, nn);
+            fflush(stderr);
+        }
     }
     for (int i = 0; i < n_node; i++) {
         if (!h->U[i]) { h->U[i] = dmat_new(0, 0); }
@@ -567,6 +571,10 @@ void H2ERI_build_H2(H2ERI_p h, const int BD_JIT) {
         eri_work_free(w);
     }
     double t2 = wtime();
+    if (h->print_dbginfo) {
+        fprintf(stderr, "[H2ERI_build_H2] %d B blocks done (%.1f s)\n", nB, t2 - t0);
+        fflush(stderr);
+    }
     const int nleaf = h->n_leaf_node, nD = h->n_D;
     h->c_D_blks = (H2E_dense_mat_p *)calloc(nD, sizeof(H2E_dense_mat_p));
     #pragma omp parallel
@@ -617,6 +625,10 @@ void H2ERI_build_H2(H2ERI_p h, const int BD_JIT) {
         eri_work_free(w);
     }
     double t3 = wtime();
+    if (h->print_dbginfo) {
+        fprintf(stderr, "[H2ERI_build_H2] %d D blocks done (%.1f s)\n", nD, t3 - t0);
+        fflush(stderr);
+    }
     update_mat_sizes(h);
     h->timers[U_BUILD_TIMER_IDX] = t1 - t0;
     h->timers[B_BUILD_TIMER_IDX] = t2 - t1;

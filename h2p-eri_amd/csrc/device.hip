@@ -539,7 +539,7 @@ struct DevState {
     int dev = -1;
     int ub_bits = 64;
     double tile_bytes = 128.0 * 1024.0;
-    int nf_schedule = 2;      // near-field placement, see H2ERI_set_schedule
+    int nf_schedule = 3;      // near-field placement, see H2ERI_set_schedule (two groups on a second stream)
     Storage st;
     bool st_ready = false;
     double *S64 = nullptr;
@@ -837,7 +837,12 @@ struct Planner {
         const int sD0 = sBA + 3;
         sG = sD0 + nU;
         sS = sG + 1;
-        nst = nf_schedule == 2 ? sS + 3 : sG + 1;
+        // schedule >= 2: near field on its own stream in (schedule - 1) groups
+        // of three launches each, balanced by bytes, so that a low-rank block's
+        // re-read half is read again shortly after its first pass
+        const int ngroup = nf_schedule >= 2 ? nf_schedule - 1 : 0;
+        std::vector<double> gload(std::max(ngroup, 1), 0.0);
+        nst = ngroup > 0 ? sS + 3 * ngroup : sG + 1;
         SR.assign(nst, {});
         sload.assign(nst, 0.0);
         sphase.assign(nst, PH_GATHER);
@@ -929,8 +934,12 @@ struct Planner {
             if (d0 < 0 || (d1 < 0 && i >= nleaf)) { set_error(ERR_ARG, "device plan: near-field block %d on a non-leaf", i); return false; }
             const double bytes = 8.0 * (double)blk_elems(h->c_D_blks[i]);
             const int span = !blk.lr ? 1 : (d1 < 0 ? 2 : 3);
-            int best = nf_schedule == 2 ? sS : 0;
-            if (nf_schedule == 0) {
+            int best = 0;
+            if (ngroup > 0) {
+                const int g = (int)(std::min_element(gload.begin(), gload.end()) - gload.begin());
+                gload[g] += bytes;
+                best = sS + 3 * g;
+            } else if (nf_schedule == 0) {
                 double bestv = 1e300;
                 for (int a = 0; a + span - 1 < sG; a++) {
                     double v = 0;
@@ -1469,10 +1478,11 @@ void H2ERI_matvec_bytes(H2ERI_p h, double *out6) {
 
 // Near-field schedule (tuning knob; re-plans): 0 balances the near-field
 // tiles over the sweep launches, 1 runs them in three launches of their own
-// ahead of the sweeps, 2 (default) runs those three launches on a second
-// stream concurrently with the sweeps, joined before the gather.
+// ahead of the sweeps, 2 + g (default 3) runs them on a second stream, concurrently with
+// the sweeps and joined before the gather, in g + 1 byte-balanced groups of
+// three launches.
 void H2ERI_set_schedule(H2ERI_p h, const int mode) {
-    if (!h || mode < 0 || mode > 2) { set_error(ERR_ARG, "H2ERI_set_schedule: 0, 1 or 2"); return; }
+    if (!h || mode < 0 || mode > 17) { set_error(ERR_ARG, "H2ERI_set_schedule: 0 to 17"); return; }
     DevState *d = dstate_get(h);
     if (d->nf_schedule != mode) { d->nf_schedule = mode; release_plans(d); }
 }

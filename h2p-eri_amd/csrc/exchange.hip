@@ -658,6 +658,25 @@ void H2ERI_exchange_workbuf_free(H2ERI_p h) {
     if (h) exchange_release(h);
 }
 
+// Builds the exchange plan on the host only (no HIP call) and checks every
+// access of its kernels; out4 as H2ERI_exchange_stats with out4[2] the bytes
+// the plan would make resident.  Returns the number of terms, -1 on failure.
+int H2ERI_exchange_plan_check(H2ERI_p h, double *out4) {
+    if (!h || !h->U || !h->c_B_blks || !h->c_D_blks) { set_error(ERR_ARG, "exchange_plan_check: nothing built"); return -1; }
+    KPlan P;
+    KBuilder b(h, P);
+    if (!b.build()) return -1;
+    if (out4) {
+        const double nn2 = (double)h->num_bf * h->num_bf;
+        out4[0] = P.flops;
+        out4[1] = (double)P.terms.size();
+        out4[2] = 8.0 * ((double)P.PB.size() + (double)P.CB.size() + (double)P.nSC + 4.0 * nn2) +
+                  4.0 * ((double)P.bfidx.size() + P.qptr.size() + P.mptr.size()) + 8.0 * ((double)P.qent.size() + P.ment.size());
+        out4[3] = (double)P.nSC;
+    }
+    return (int)P.terms.size();
+}
+
 // Work figures of the exchange plan: out4 = {GEMM flops at upload, terms,
 // device bytes resident, scratch doubles}.
 void H2ERI_exchange_stats(H2ERI_p h, double *out4) {

@@ -259,6 +259,13 @@ class H2ERI:
         _check(self.lib)
         return dict(zip(["gemm_flops", "terms", "resident_bytes", "scratch_doubles"], out.tolist()))
 
+    def exchange_plan_check(self):
+        """Host-only build and bounds check of the exchange plan (no GPU needed)."""
+        out = np.zeros(4)
+        n = self.lib.H2ERI_exchange_plan_check(self.h, _dptr(out))
+        _check(self.lib)
+        return dict(terms=n, **dict(zip(["gemm_flops", "terms_f", "resident_bytes", "scratch_doubles"], out.tolist())))
+
     def set_phase_timing(self, on=True):
         self.lib.H2ERI_set_phase_timing(self.h, 1 if on else 0)
 

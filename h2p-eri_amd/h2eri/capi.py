@@ -82,6 +82,7 @@ SIGNATURES = {
     "H2ERI_exchange_workbuf_free": (None, [H2ERI_p]),
     "H2ERI_build_exchange_dev": (None, [H2ERI_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "H2ERI_exchange_stats": (None, [H2ERI_p, c_dbl_p]),
+    "H2ERI_exchange_plan_check": (C.c_int, [H2ERI_p, c_dbl_p]),
     "h2eri_last_error": (C.c_int, []),
     "h2eri_last_error_msg": (C.c_char_p, []),
     "h2eri_clear_error": (None, []),

@@ -166,6 +166,14 @@ CONFIGS = {
 }
 
 
+def workload_name(config):
+    """Short name of a configuration's molecule (C30H62, (H2O)64, ...)."""
+    cfg = CONFIGS[config]
+    if cfg["kind"] == "alkane":
+        return f"C{cfg['n']}H{2 * cfg['n'] + 2}"
+    return f"(H2O){cfg['n']}"
+
+
 def workload(config):
     """(atoms, mol text, density, reltol) of a configuration, seeded."""
     cfg = CONFIGS[config]

@@ -242,6 +242,10 @@ void H2ERI_exchange_workbuf_free(H2ERI_p h2eri);
  * number of block terms, device bytes resident, scratch doubles}. */
 void H2ERI_build_exchange_dev(H2ERI_p h2eri, const double *den_dev, double *K_dev, void *stream);
 void H2ERI_exchange_stats(H2ERI_p h2eri, double *out4);
+/* Additive: the exchange plan built and bounds-checked on the host only (no
+ * HIP call); out4 as above with out4[2] the bytes it would make resident.
+ * Returns the number of block terms, -1 on failure. */
+int H2ERI_exchange_plan_check(H2ERI_p h2eri, double *out4);
 
 /* ------------------------------------------------------------------ */
 /* Additive entry points (no counterpart in the reference).             */
@@ -333,7 +337,9 @@ void H2ERI_set_tile_bytes(H2ERI_p h2eri, const double bytes);
 
 /* Near-field schedule (tuning; re-plans): 0 balances the near-field tiles
  * over the launches of the sweeps, 1 gives them three launches of their own
- * ahead of the sweeps. */
+ * ahead of the sweeps, 2 + g (g = 0..15) runs them on a second stream
+ * concurrently with the sweeps in g + 1 byte-balanced groups of three
+ * launches. */
 void H2ERI_set_schedule(H2ERI_p h2eri, const int mode);
 
 /* Builds the device execution plan on the host only (no HIP call) and

@@ -1,0 +1,93 @@
+"""GPU parity on small analogues of the benchmark configurations.
+
+* 3D globular case with exchange: a cluster of 8 water molecules (O and H
+  cc-pVDZ blocks): J in FP64 and FP32-stored form, and K, against the oracle.
+* tight tolerance: a C8H18 chain at reltol 1e-8 (deeper ranks): J in FP64
+  against the oracle and the FP32-stored J error.
+* many densities: 16 damped densities on one representation in one batched
+  call, each against the oracle.
+* uneven molecules: alkanes of 3, 5 and 9 carbons, each handle with its own
+  density, J against the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle_bind
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def build(tmp_path_factory, name, atoms, reltol):
+    import h2eri
+    from h2eri import molecules as M
+    h = h2eri.H2ERI(1e-10, 1e-10, reltol)
+    h.build_from_mol(text=M.mol_text(atoms))
+    p = str(tmp_path_factory.mktemp(name) / f"{name}.rep")
+    h.save(p)
+    return h, p
+
+
+@pytest.fixture(scope="module")
+def water8(tmp_path_factory):
+    from h2eri import molecules as M
+    rng = M.rng_for(400)
+    atoms = M.water_cluster(8, rng)
+    D = M.density(atoms, 40, rng)
+    h, p = build(tmp_path_factory, "w8", atoms, 1e-6)
+    return h, p, D
+
+
+def test_water_cluster_j_and_k(water8):
+    h, p, D = water8
+    o = oracle_bind.Oracle(p)
+    assert h.s.n_B > 0
+    J = h.build_coulomb(D)
+    assert relerr(J, o.coulomb(D)) < 1e-13
+    h.set_precision(32)
+    try:
+        J32 = h.build_coulomb(D)
+    finally:
+        h.set_precision(64)
+    assert relerr(J32, J) < 1e-6
+    K = h.build_exchange(D)
+    assert relerr(K, o.exchange(D)) < 1e-12
+
+
+def test_tight_tolerance_chain(tmp_path_factory):
+    from h2eri import molecules as M
+    rng = M.rng_for(401)
+    atoms = M.alkane(8, rng)
+    D = M.density(atoms, 33, rng)
+    h, p = build(tmp_path_factory, "c8t", atoms, 1e-8)
+    o = oracle_bind.Oracle(p)
+    J = h.build_coulomb(D)
+    Jo = o.coulomb(D)
+    assert relerr(J, Jo) < 1e-13
+    h.set_precision(32)
+    try:
+        assert relerr(h.build_coulomb(D), Jo) < 1e-6
+    finally:
+        h.set_precision(64)
+
+
+def test_sixteen_densities(water8):
+    from h2eri import molecules as M
+    h, p, D0 = water8
+    o = oracle_bind.Oracle(p)
+    rng = np.random.default_rng(5)
+    atoms = M.water_cluster(8, M.rng_for(400))
+    Ds = np.array([M.density(atoms, 40, rng) for _ in range(16)])
+    Js = h.build_coulomb_multi(Ds)
+    for k in range(16):
+        assert relerr(Js[k], o.coulomb(Ds[k])) < 1e-13
+
+
+def test_uneven_molecules(tmp_path_factory):
+    from h2eri import molecules as M
+    for n in (3, 5, 9):
+        rng = M.rng_for(500 + n)
+        atoms = M.alkane(n, rng)
+        D = M.density(atoms, 4 * n + 1, rng)
+        h, p = build(tmp_path_factory, f"c{n}", atoms, 1e-6)
+        assert relerr(h.build_coulomb(D), oracle_bind.Oracle(p).coulomb(D)) < 1e-13

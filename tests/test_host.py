@@ -94,7 +94,7 @@ def test_device_plan_semantics_match_oracle(tmp_path, seed, n, leaf, mr, shape):
     y32, conflicts = h.plan_emulate(x, 32)
     assert conflicts == 0
     assert relerr(y32, yo) < 2e-6
-    for sched in (0, 1):   # near field in the sweep launches / in launches of its own
+    for sched in (0, 1, 4):   # near field in the sweep launches / own launches / 3 groups on a second stream
         h.set_schedule(sched)
         y1, conflicts = h.plan_emulate(x, 64)
         assert conflicts == 0
