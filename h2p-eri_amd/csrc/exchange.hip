@@ -23,8 +23,8 @@
 // contraction factorises through the basis-function sets B(i), B(j) of the
 // two nodes:
 //
-//   G[t][n][q] = sum_{c in j, (p,q) orientation of c} R[c][t] D[n][p]   (ket, n in B(i), q in B(j))
-//   K[m][q]   += sum_{r in i, (m,n) orientation of r} sum_t L[r][t] G[t][n][q]   (bra)
+//   G[n][t][q] = sum_{c in j, (p,q) orientation of c} R[c][t] D[n][p]   (ket, n in B(i), q in B(j))
+//   K[m][q]   += sum_{r in i, (m,n) orientation of r} sum_t L[r][t] G[n][t][q]   (bra)
 //
 // For big nodes this costs kk (|B(i)| m_j + m_i |B(j)|) rather than the
 // m_i m_j kk of forming F.  Dense near-field blocks are contracted directly.
@@ -35,8 +35,8 @@
 // - Contributions go into K1 and K_diag with FP64 atomic adds, as the
 //   reference's scatter does (:767), so the last bits of K depend on the
 //   order of the adds.
-// - L and R products of the low-rank and coupling cores are FP64 MFMA GEMMs
-//   (v_mfma_f64_16x16x4f64).
+// - The L and R products of the cores, the ket and the bra are FP64 MFMA
+//   tiles (v_mfma_f64_16x16x4f64) with FP64 accumulation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -62,7 +62,7 @@ struct KTerm {
     long long Loff;
     int Rsrc, Rrs, Rcs;   // R[c][t] = src[Roff + c*Rrs + t*Rcs]
     long long Roff;
-    long long Goff;       // G (kk x |B(i)| x |B(j)|) in scratch
+    long long Goff;       // G (|B(i)| x kk x |B(j)|) in scratch
     int pad_[2];
 };
 
@@ -71,82 +71,129 @@ struct KGemm {           // out (m x n, ld ldo) = A (m x k, PB, ld lda) * X (k x
     int m, n, k, lda, ldx, ldo;
 };
 
-struct KTask {           // one workgroup of a phase: term, first (n,q) or (m,q) index, t chunk
-    int term, e0, t0, pad_;
+struct KTask {           // one workgroup: term and tile origin
+    int term;
+    int e0;               // ket: first local n; bra: first local q; direct: first (m,q) index
+    int t0;               // ket: first t; bra: split of local m
+    int pad_;             // ket: first local q
 };
 
 struct KGemmTask {
     int g, row0, col0, pad_;
 };
 
-struct KNode {           // per row/column range: basis-function set and the two orientation lists
+struct KNode {           // per row/column range: basis-function set and the orientation lists
     int bf_off, nbf;      // B(node) = bfidx[bf_off .. bf_off + nbf)
-    int q_ptr;            // byQ CSR rows: qptr[q_ptr .. q_ptr + nbf]
-    int m_ptr;            // byM CSR rows: mptr[m_ptr .. m_ptr + nbf]
+    int q_ptr;            // byQ CSR rows: qptr[q_ptr .. q_ptr + nbf]   entries (column c, p)
+    int m_ptr;            // byM CSR rows: mptr[m_ptr .. m_ptr + nbf]   entries (row r, local n)
+    int s_ptr;            // byN split table: nspl[s_ptr + n * (nsplit + 1) + s] = first entry (row r, local m)
+    int nsplit;           //   of local n with local m >= s * KSPLIT (entries of each n sorted by m)
 };
 
 static const int KWG = 256;
-static const int KTC = 8;   // t values per ket thread
+static const int KQC = 16;      // column functions q per ket task
+static const int KSPLIT = 256;  // local m rows per bra task (LDS block KSPLIT x 16)
 
-// G[t][n][q] for t in [t0, t0+KTC), one (n, q) per thread.
+// Ket on MFMA: G[n][t][q] = sum over the entries (c, p) of q of R[c][t] D[n][p].
+// Per q this is a GEMM (rows t, columns n, inner dimension the entries of q).
+// Task = (term, t tile of 16, 64 functions n (16 per wave), KQC functions q).
 __global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
                                                const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
                                                const int *__restrict__ qptr, const int2 *__restrict__ qent,
                                                const double *__restrict__ PB, const double *__restrict__ CB,
                                                double *__restrict__ SC, const double *__restrict__ D, int nbf) {
-    const KTask tk = tasks[blockIdx.x];
+    const KTask tk = tasks[blockIdx.x];   // term, nl0, t0, ql0
     const KTerm t = terms[tk.term];
     const KNode ni = nodes[t.ni], nj = nodes[t.nj];
-    const int e = tk.e0 + threadIdx.x;
-    if (e >= ni.nbf * nj.nbf) return;
-    const int nl = e / nj.nbf, ql = e - nl * nj.nbf;
-    const int n = bfidx[ni.bf_off + nl];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nl0 = tk.e0 + 16 * w;
+    if (nl0 >= ni.nbf) return;   // whole wave
     const double *R = (t.Rsrc == KS_PB ? PB : (t.Rsrc == KS_CB ? CB : SC)) + t.Roff;
-    const double *Dn = D + (size_t)n * nbf;
-    double acc[KTC];
-    #pragma unroll
-    for (int u = 0; u < KTC; u++) acc[u] = 0.0;
-    const int tl = min(KTC, t.kk - tk.t0);
-    for (int j = qptr[nj.q_ptr + ql]; j < qptr[nj.q_ptr + ql + 1]; j++) {
-        const int2 cp = qent[j];   // (column, p)
-        const double d = Dn[cp.y];
-        const double *Rc = R + (long long)cp.x * t.Rrs + (long long)tk.t0 * t.Rcs;
+    const int nl = nl0 + (lane & 15);
+    const bool nv = nl < ni.nbf;
+    const double *Drow = D + (size_t)(nv ? bfidx[ni.bf_off + nl] : 0) * nbf;
+    const int ta = tk.t0 + (lane & 15);   // A operand row (t)
+    const bool tv = ta < t.kk;
+    const int kl = lane >> 4;
+    double *G = SC + t.Goff;
+    const int qend = min(tk.pad_ + KQC, nj.nbf);
+    for (int ql = tk.pad_; ql < qend; ql++) {
+        const int q0 = qptr[nj.q_ptr + ql], q1 = qptr[nj.q_ptr + ql + 1];
+        d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+        for (int e0 = q0; e0 < q1; e0 += 4) {
+            const int e = e0 + kl;
+            const bool ev = e < q1;
+            const int2 cp = ev ? qent[e] : make_int2(0, 0);
+            const double a = (ev && tv) ? R[(long long)cp.x * t.Rrs + (long long)ta * t.Rcs] : 0.0;
+            const double b = (ev && nv) ? Drow[cp.y] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
         #pragma unroll
-        for (int u = 0; u < KTC; u++)
-            if (u < tl) acc[u] = fma(Rc[(long long)u * t.Rcs], d, acc[u]);
+        for (int r = 0; r < 4; r++) {
+            const int tt = tk.t0 + kl + 4 * r;
+            if (tt < t.kk && nv) G[((size_t)nl * t.kk + tt) * nj.nbf + ql] = acc[r];
+        }
     }
-    double *G = SC + t.Goff + (size_t)nl * nj.nbf + ql;
-    const size_t tstride = (size_t)ni.nbf * nj.nbf;
-    #pragma unroll
-    for (int u = 0; u < KTC; u++)
-        if (u < tl) G[(size_t)(tk.t0 + u) * tstride] = acc[u];
 }
 
-// K[m][q] += sum_{(r,n)} sum_t L[r][t] G[t][n][q], one (m, q) per thread.
+// Bra on MFMA: K[m][q] += sum_n sum_{(r,m) of n} sum_t L[r][t] G[n][t][q].
+// For each n the rows r that pair with n form a GEMM (rows: those entries,
+// columns q, inner dimension t); an entry's local m is unique within one n,
+// so the 16 x 16 output tiles of one n add into an LDS block without
+// conflicts; a barrier separates successive n.  Task = (term, 16 functions q,
+// split of KSPLIT functions m); the block is added into K at the end.
 __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
                                                const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
-                                               const int *__restrict__ mptr, const int2 *__restrict__ ment,
+                                               const int *__restrict__ nspl, const int2 *__restrict__ nent,
                                                const double *__restrict__ PB, const double *__restrict__ CB,
                                                const double *__restrict__ SC, double *__restrict__ K1,
                                                double *__restrict__ Kd, int nbf) {
-    const KTask tk = tasks[blockIdx.x];
+    __shared__ double Kb[KSPLIT * 16];
+    const KTask tk = tasks[blockIdx.x];   // term, ql0, split
     const KTerm t = terms[tk.term];
     const KNode ni = nodes[t.ni], nj = nodes[t.nj];
-    const int e = tk.e0 + threadIdx.x;
-    if (e >= ni.nbf * nj.nbf) return;
-    const int ml = e / nj.nbf, ql = e - ml * nj.nbf;
+    const int split = tk.t0, ql0 = tk.e0;
+    const int ml0 = split * KSPLIT;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, kl = lane >> 4;
+    for (int i = threadIdx.x; i < KSPLIT * 16; i += KWG) Kb[i] = 0.0;
+    __syncthreads();
     const double *L = (t.Lsrc == KS_PB ? PB : (t.Lsrc == KS_CB ? CB : SC)) + t.Loff;
-    const double *G = SC + t.Goff + ql;
-    const size_t tstride = (size_t)ni.nbf * nj.nbf;
-    double acc = 0.0;
-    for (int j = mptr[ni.m_ptr + ml]; j < mptr[ni.m_ptr + ml + 1]; j++) {
-        const int2 rn = ment[j];   // (row, local n)
-        const double *Lr = L + (long long)rn.x * t.Lld;
-        const double *Gn = G + (size_t)rn.y * nj.nbf;
-        for (int u = 0; u < t.kk; u++) acc = fma(Lr[u], Gn[(size_t)u * tstride], acc);
+    const double *G = SC + t.Goff;
+    const int ql = ql0 + (lane & 15);
+    const bool qv = ql < nj.nbf;
+    const int ns1 = ni.nsplit + 1;
+    for (int nl = 0; nl < ni.nbf; nl++) {
+        const int eb = nspl[ni.s_ptr + nl * ns1 + split], ee = nspl[ni.s_ptr + nl * ns1 + split + 1];
+        for (int e0 = eb + 16 * w; e0 < ee; e0 += 64) {
+            const int e = e0 + (lane & 15);
+            const bool ev = e < ee;
+            const int row = ev ? nent[e].x : 0;
+            const double *Lr = L + (long long)row * t.Lld;
+            const double *Gn = G + (size_t)nl * t.kk * nj.nbf + ql;
+            d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+            for (int t0 = 0; t0 < t.kk; t0 += 4) {
+                const int tt = t0 + kl;
+                const bool tv = tt < t.kk;
+                const double a = (ev && tv) ? Lr[tt] : 0.0;
+                const double b = (qv && tv) ? Gn[(size_t)tt * nj.nbf] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            }
+            #pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int er = e0 + kl + 4 * r;
+                if (er < ee && qv) Kb[(nent[er].y - ml0) * 16 + (lane & 15)] += acc[r];
+            }
+        }
+        __syncthreads();
     }
-    const int m = bfidx[ni.bf_off + ml], q = bfidx[nj.bf_off + ql];
-    unsafeAtomicAdd((t.diag ? Kd : K1) + (size_t)m * nbf + q, acc);
+    double *Kout = t.diag ? Kd : K1;
+    const int mlN = min(KSPLIT, ni.nbf - ml0);
+    for (int i = threadIdx.x; i < mlN * 16; i += KWG) {
+        const int qq = ql0 + (i & 15);
+        const double v = Kb[i];
+        if (qq < nj.nbf && v != 0.0)
+            unsafeAtomicAdd(Kout + (size_t)bfidx[ni.bf_off + ml0 + (i >> 4)] * nbf + bfidx[nj.bf_off + qq], v);
+    }
 }
 
 // Dense core with identity bases: K[m][q] += sum_{(r,n)} sum_{(c,p)} C[r][c] D[n][p].
@@ -243,8 +290,8 @@ struct KPlan {
     std::vector<KTerm> terms;
     std::vector<KGemm> gemms;
     std::vector<KNode> nodes;
-    std::vector<int> bfidx, qptr, mptr;
-    std::vector<int2> qent, ment;
+    std::vector<int> bfidx, qptr, mptr, nspl;
+    std::vector<int2> qent, ment, nent;
     std::vector<KTask> ket, bra, direct;
     std::vector<KGemmTask> gtasks;
     long long nSC = 0;
@@ -258,8 +305,8 @@ struct KState {
     KTerm *terms = nullptr;
     KGemm *gemms = nullptr;
     KNode *nodes = nullptr;
-    int *bfidx = nullptr, *qptr = nullptr, *mptr = nullptr;
-    int2 *qent = nullptr, *ment = nullptr;
+    int *bfidx = nullptr, *qptr = nullptr, *mptr = nullptr, *nspl = nullptr;
+    int2 *qent = nullptr, *ment = nullptr, *nent = nullptr;
     KTask *ket = nullptr, *bra = nullptr, *direct = nullptr;
     KGemmTask *gtasks = nullptr;
     size_t resident = 0;
@@ -272,7 +319,8 @@ static void kfree(void *&p) {
 static void kstate_release(KState *k) {
     void **ps[] = {(void **)&k->PB, (void **)&k->CB, (void **)&k->SC, (void **)&k->K1, (void **)&k->Kd, (void **)&k->D,
                    (void **)&k->K, (void **)&k->terms, (void **)&k->gemms, (void **)&k->nodes, (void **)&k->bfidx,
-                   (void **)&k->qptr, (void **)&k->mptr, (void **)&k->qent, (void **)&k->ment, (void **)&k->ket,
+                   (void **)&k->qptr, (void **)&k->mptr, (void **)&k->qent, (void **)&k->ment, (void **)&k->nspl,
+                   (void **)&k->nent, (void **)&k->ket,
                    (void **)&k->bra, (void **)&k->direct, (void **)&k->gtasks};
     for (void **p : ps) kfree(*p);
     k->ready = false;
@@ -384,6 +432,22 @@ struct KBuilder {
         kn.m_ptr = (int)P.mptr.size();
         for (int i = 0; i < nb; i++) { P.mptr.push_back((int)P.ment.size()); P.ment.insert(P.ment.end(), bym[i].begin(), bym[i].end()); }
         P.mptr.push_back((int)P.ment.size());
+        // byN: for each local n the (row, local m) entries, sorted by m, with the
+        // first entry of every split of KSPLIT functions m
+        std::vector<std::vector<int2>> byn(nb);
+        for (int ml = 0; ml < nb; ml++)
+            for (const int2 &e : bym[ml]) byn[e.y].push_back(make_int2(e.x, ml));
+        kn.nsplit = (nb + KSPLIT - 1) / KSPLIT;
+        kn.s_ptr = (int)P.nspl.size();
+        for (int nl = 0; nl < nb; nl++) {
+            const int base = (int)P.nent.size();
+            int k = 0;
+            for (int sp = 0; sp <= kn.nsplit; sp++) {
+                while (k < (int)byn[nl].size() && byn[nl][k].y < sp * KSPLIT) k++;
+                P.nspl.push_back(base + k);
+            }
+            P.nent.insert(P.nent.end(), byn[nl].begin(), byn[nl].end());
+        }
         P.nodes.push_back(kn);
         node_list[node] = (int)P.nodes.size() - 1;
         return node_list[node];
@@ -488,12 +552,17 @@ struct KBuilder {
                 for (int c0 = 0; c0 < P.gemms[g].n; c0 += 64) P.gtasks.push_back(KGemmTask{g, r0, c0, 0});
         for (int ti = 0; ti < (int)P.terms.size(); ti++) {
             const KTerm &t = P.terms[ti];
-            const long long ne = (long long)P.nodes[t.ni].nbf * P.nodes[t.nj].nbf;
-            for (long long e0 = 0; e0 < ne; e0 += KWG) {
-                if (t.direct) { P.direct.push_back(KTask{ti, (int)e0, 0, 0}); continue; }
-                for (int t0 = 0; t0 < t.kk; t0 += KTC) P.ket.push_back(KTask{ti, (int)e0, t0, 0});
-                P.bra.push_back(KTask{ti, (int)e0, 0, 0});
+            const KNode &ni = P.nodes[t.ni], &nj = P.nodes[t.nj];
+            if (t.direct) {
+                const long long ne = (long long)ni.nbf * nj.nbf;
+                for (long long e0 = 0; e0 < ne; e0 += KWG) P.direct.push_back(KTask{ti, (int)e0, 0, 0});
+                continue;
             }
+            for (int t0 = 0; t0 < t.kk; t0 += 16)
+                for (int nl0 = 0; nl0 < ni.nbf; nl0 += 64)
+                    for (int ql0 = 0; ql0 < nj.nbf; ql0 += KQC) P.ket.push_back(KTask{ti, nl0, t0, ql0});
+            for (int ql0 = 0; ql0 < nj.nbf; ql0 += 16)
+                for (int sp = 0; sp < ni.nsplit; sp++) P.bra.push_back(KTask{ti, ql0, sp, 0});
         }
         return validate();
     }
@@ -523,6 +592,17 @@ struct KBuilder {
                 ok = P.qent[j].x >= 0 && P.qent[j].x < t.m1 && P.qent[j].y >= 0 && P.qent[j].y < nbf;
             for (int j = P.mptr[ni.m_ptr]; ok && j < P.mptr[ni.m_ptr + ni.nbf]; j++)
                 ok = P.ment[j].x >= 0 && P.ment[j].x < t.m0 && P.ment[j].y >= 0 && P.ment[j].y < ni.nbf;
+            // byN splits: increasing, inside nent, entries of a split within its m range
+            const int ns1 = ni.nsplit + 1;
+            ok = ok && ni.nsplit >= 1 && ni.s_ptr >= 0 && ni.s_ptr + (long long)ni.nbf * ns1 <= (long long)P.nspl.size();
+            for (int nl = 0; ok && nl < ni.nbf; nl++)
+                for (int sp = 0; ok && sp < ni.nsplit; sp++) {
+                    const int eb = P.nspl[ni.s_ptr + nl * ns1 + sp], ee = P.nspl[ni.s_ptr + nl * ns1 + sp + 1];
+                    ok = eb >= 0 && eb <= ee && ee <= (int)P.nent.size();
+                    for (int e = eb; ok && e < ee; e++)
+                        ok = P.nent[e].x >= 0 && P.nent[e].x < t.m0 && P.nent[e].y >= sp * KSPLIT &&
+                             P.nent[e].y < std::min((sp + 1) * KSPLIT, ni.nbf);
+                }
             ok = ok && t.Loff >= 0 && t.Loff + (long long)(t.m0 - 1) * t.Lld + t.kk <= size_of(t.Lsrc);
             if (!t.direct) {
                 const long long last = t.Roff + (long long)(t.m1 - 1) * t.Rrs + (long long)(t.kk - 1) * t.Rcs;
@@ -584,6 +664,8 @@ static bool k_upload(H2ERI_p h) {
         !up((void **)&k->nodes, P.nodes.empty() ? nullptr : P.nodes.data(), P.nodes.size() * sizeof(KNode)) ||
         !up((void **)&k->bfidx, P.bfidx.data(), P.bfidx.size() * 4) || !up((void **)&k->qptr, P.qptr.data(), P.qptr.size() * 4) ||
         !up((void **)&k->mptr, P.mptr.data(), P.mptr.size() * 4) ||
+        !up((void **)&k->nspl, P.nspl.data(), P.nspl.size() * 4) ||
+        !up((void **)&k->nent, P.nent.empty() ? nullptr : P.nent.data(), P.nent.size() * sizeof(int2)) ||
         !up((void **)&k->qent, P.qent.empty() ? nullptr : P.qent.data(), P.qent.size() * sizeof(int2)) ||
         !up((void **)&k->ment, P.ment.empty() ? nullptr : P.ment.data(), P.ment.size() * sizeof(int2)) ||
         !up((void **)&k->ket, P.ket.empty() ? nullptr : P.ket.data(), P.ket.size() * sizeof(KTask)) ||
@@ -619,7 +701,7 @@ static bool k_run(H2ERI_p h, const double *D_dev, double *K_dev, hipStream_t s) 
                            k->bfidx, k->qptr, k->qent, k->mptr, k->ment, k->CB, D_dev, k->K1, k->Kd, nbf);
     if (!P.bra.empty())
         hipLaunchKernelGGL(kx_bra, dim3((unsigned)P.bra.size()), dim3(KWG), 0, s, k->bra, k->terms, k->nodes, k->bfidx,
-                           k->mptr, k->ment, k->PB, k->CB, k->SC, k->K1, k->Kd, nbf);
+                           k->nspl, k->nent, k->PB, k->CB, k->SC, k->K1, k->Kd, nbf);
     KHIP(hipGetLastError());
     hipLaunchKernelGGL(kx_final, dim3(zb), dim3(256), 0, s, k->K1, k->Kd, K_dev, nbf);
     KHIP(hipGetLastError());
