@@ -380,6 +380,41 @@ __global__ __launch_bounds__(WG) void stage_kernel(const BTask *__restrict__ t64
     }
 }
 
+// Several molecules in one launch: every task names its molecule, whose
+// storage, plan arrays, workspace and vectors come from a pointer table.
+struct MolPtrs {
+    const double *S64;
+    const float *S32;
+    double *Wv;
+    const BTask *t64, *t32;
+    const RTask *rt;
+    const int *vsrc, *rsrc;
+    const double *X;
+    double *Y;
+    long long ldx;
+};
+struct GTask {
+    int mol, kind, idx, pad_;   // kind 0: FP64 tile, 1: FP32 tile, 2: reduce task
+};
+
+template <int NV>
+__global__ __launch_bounds__(WG) void stage_kernel_grouped(const GTask *__restrict__ tasks,
+                                                           const MolPtrs *__restrict__ mols) {
+    __shared__ double lds[LDS_DOUBLES];
+    const GTask g = tasks[blockIdx.x];
+    const MolPtrs m = mols[g.mol];
+    if (g.kind == 0) {
+        const BTask t = m.t64[g.idx];
+        bgemv_tile<double, NV>(t, m.S64, m.vsrc, m.Wv, m.X, m.ldx, lds);
+    } else if (g.kind == 1) {
+        const BTask t = m.t32[g.idx];
+        bgemv_tile<float, NV>(t, m.S32, m.vsrc, m.Wv, m.X, m.ldx, lds);
+    } else {
+        const RTask t = m.rt[g.idx];
+        reduce_task<NV>(t, m.rsrc, m.Wv, m.Y, m.ldx, lds);
+    }
+}
+
 // x[i] = c_i * D[g_i >> 1], c_i = 2 when the low bit is set
 // (src/H2ERI_build_Coulomb.c:26-62).
 __global__ void uncontract_kernel(const int *__restrict__ gidx, const double *__restrict__ D, double *__restrict__ x,
@@ -1493,6 +1528,164 @@ void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
     DevState *d = dstate_get(h);
     d->tile_bytes = bytes;
     release_plans(d);
+}
+
+// Autotuner: times the device matvec of nv vectors for every pair of tile
+// size and near-field schedule in the candidate lists (HIP events around
+// `reps` back-to-back calls after one warm-up) and keeps the fastest pair.
+// out3 = {best ms per call, best tile bytes, best schedule}.  Returns 0, or
+// -1 with the error set.
+int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out3) {
+    if (!h || (nv != 1 && nv != 2 && nv != 4 && nv != 8) || reps < 1) { set_error(ERR_ARG, "H2ERI_autotune: bad argument"); return -1; }
+    if (!upload_storage(h)) return -1;
+    DevState *d = dstate(h);
+    const size_t N = (size_t)h->krnl_mat_size;
+    if (!ensure_io(d, N * nv)) return -1;
+    // inputs: a fixed pattern (the time does not depend on the values)
+    hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(256), 0, 0, d->d_x, N * nv);
+    if (hipGetLastError() != hipSuccess) { set_error(ERR_DEVICE, "autotune: zero"); return -1; }
+    const double tiles[] = {65536.0, 131072.0, 262144.0};
+    const int scheds[] = {0, 2, 3, 4};
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { set_error(ERR_DEVICE, "autotune: events"); return -1; }
+    double best = 1e300, best_tile = d->tile_bytes;
+    int best_sched = d->nf_schedule;
+    const bool was_timing = d->phase_timing;
+    d->phase_timing = false;
+    bool ok = true;
+    for (double tb : tiles) {
+        for (int sc : scheds) {
+            release_plans(d);
+            d->tile_bytes = tb;
+            d->nf_schedule = sc;
+            ok = matvec_multi_dev_impl(h, nv, d->d_x, d->d_y, nullptr);   // plans, uploads, warms up
+            ok = ok && hipEventRecord(e0, nullptr) == hipSuccess;
+            for (int r = 0; ok && r < reps; r++) ok = matvec_multi_dev_impl(h, nv, d->d_x, d->d_y, nullptr);
+            ok = ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+            float ms = 0.f;
+            ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            if (!ok) break;
+            if (ms / reps < best) { best = ms / reps; best_tile = tb; best_sched = sc; }
+        }
+        if (!ok) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    d->phase_timing = was_timing;
+    release_plans(d);
+    d->tile_bytes = best_tile;
+    d->nf_schedule = best_sched;
+    if (!ok) {
+        if (!has_error()) set_error(ERR_DEVICE, "autotune: timing failed");
+        return -1;
+    }
+    if (out3) { out3[0] = best; out3[1] = best_tile; out3[2] = best_sched; }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Molecule groups: one launch per stage index for several representations
+// ---------------------------------------------------------------------------
+struct H2EGroup {
+    std::vector<H2ERI_p> hs;
+    std::vector<DevPlan> plans;          // schedule-0 plans (near field inside the sweep launches)
+    std::vector<int> launch_beg;          // GTask ranges per launch
+    GTask *d_tasks = nullptr;
+    MolPtrs *d_mols = nullptr;
+    int nlaunch = 0;
+    size_t ntask = 0;
+};
+
+static void group_free(H2EGroup *g) {
+    for (DevPlan &p : g->plans) {
+        void **ps[] = {(void **)&p.Wv, (void **)&p.d_t64, (void **)&p.d_t32, (void **)&p.d_rt, (void **)&p.d_rsrc,
+                       (void **)&p.d_vsrc};
+        for (void **q : ps) free_dev(*q);
+    }
+    free_dev((void *&)g->d_tasks);
+    free_dev((void *&)g->d_mols);
+    delete g;
+}
+
+// Groups nh representations (single vector each) for H2ERI_group_matvec: y_k
+// := A_k x_k for k < nh with fixed device buffers X_dev[k], Y_dev[k].  Stage
+// l of every molecule runs in launch l, so molecules of uneven size fill the
+// device together.  The handles must outlive the group and keep their
+// precision; returns NULL with the error set on failure.
+void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_dev, double *const *Y_dev) {
+    if (nh < 1 || !handles || !X_dev || !Y_dev) { set_error(ERR_ARG, "H2ERI_group_create: bad argument"); return nullptr; }
+    H2EGroup *g = new H2EGroup();
+    g->plans.resize(nh);
+    std::vector<MolPtrs> mols(nh);
+    std::vector<std::vector<GTask>> per_launch;
+    for (int k = 0; k < nh; k++) {
+        H2ERI_p h = handles[k];
+        if (!h || !X_dev[k] || !Y_dev[k]) { set_error(ERR_ARG, "H2ERI_group_create: null handle or buffer %d", k); group_free(g); return nullptr; }
+        if (!upload_storage(h)) { group_free(g); return nullptr; }
+        DevState *d = dstate(h);
+        DevPlan &dp = g->plans[k];
+        if (!build_plan(h, d->st, dp.P, 1, d->tile_bytes, 0)) { group_free(g); return nullptr; }
+        const Plan &P = dp.P;
+        auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
+            if (bytes == 0) bytes = 64;
+            hipError_t e = hipMalloc(dst, bytes);
+            if (e == hipSuccess && src) e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) { set_error(ERR_DEVICE, "group upload: %s", hipGetErrorString(e)); return false; }
+            return true;
+        };
+        bool ok = up((void **)&dp.Wv, nullptr, (size_t)P.nW * 8) &&
+                  up((void **)&dp.d_t64, P.t64.empty() ? nullptr : P.t64.data(), P.t64.size() * sizeof(BTask)) &&
+                  up((void **)&dp.d_t32, P.t32.empty() ? nullptr : P.t32.data(), P.t32.size() * sizeof(BTask)) &&
+                  up((void **)&dp.d_rt, P.rt.empty() ? nullptr : P.rt.data(), P.rt.size() * sizeof(RTask)) &&
+                  up((void **)&dp.d_rsrc, P.rsrc.empty() ? nullptr : P.rsrc.data(), P.rsrc.size() * sizeof(int)) &&
+                  up((void **)&dp.d_vsrc, P.vsrc.empty() ? nullptr : P.vsrc.data(), P.vsrc.size() * sizeof(int));
+        if (!ok) { group_free(g); return nullptr; }
+        mols[k] = MolPtrs{d->S64, d->S32, dp.Wv, dp.d_t64, dp.d_t32, dp.d_rt, dp.d_vsrc, dp.d_rsrc, X_dev[k], Y_dev[k],
+                          (long long)h->krnl_mat_size};
+        if (per_launch.size() < P.stages.size()) per_launch.resize(P.stages.size());
+        for (size_t l = 0; l < P.stages.size(); l++) {
+            const Stage &st = P.stages[l];
+            for (int i = 0; i < st.t64_cnt; i++) per_launch[l].push_back(GTask{k, 0, st.t64_beg + i, 0});
+            for (int i = 0; i < st.t32_cnt; i++) per_launch[l].push_back(GTask{k, 1, st.t32_beg + i, 0});
+            for (int i = 0; i < st.r_cnt; i++) per_launch[l].push_back(GTask{k, 2, st.r_beg + i, 0});
+        }
+        g->hs.push_back(h);
+    }
+    std::vector<GTask> all;
+    for (auto &v : per_launch) {
+        g->launch_beg.push_back((int)all.size());
+        all.insert(all.end(), v.begin(), v.end());
+    }
+    g->launch_beg.push_back((int)all.size());
+    g->nlaunch = (int)per_launch.size();
+    g->ntask = all.size();
+    if (hipMalloc(&g->d_tasks, sizeof(GTask) * std::max<size_t>(all.size(), 1)) != hipSuccess ||
+        hipMalloc(&g->d_mols, sizeof(MolPtrs) * nh) != hipSuccess ||
+        (!all.empty() && hipMemcpy(g->d_tasks, all.data(), sizeof(GTask) * all.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(g->d_mols, mols.data(), sizeof(MolPtrs) * nh, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error(ERR_DEVICE, "H2ERI_group_create: upload failed");
+        group_free(g);
+        return nullptr;
+    }
+    return g;
+}
+
+// Enqueues one matvec of every molecule of the group on `stream`.
+int H2ERI_group_matvec(void *group, void *stream) {
+    H2EGroup *g = (H2EGroup *)group;
+    if (!g) { set_error(ERR_ARG, "H2ERI_group_matvec: null group"); return -1; }
+    for (int l = 0; l < g->nlaunch; l++) {
+        const int nb = g->launch_beg[l + 1] - g->launch_beg[l];
+        if (nb == 0) continue;
+        hipLaunchKernelGGL(stage_kernel_grouped<1>, dim3(nb), dim3(WG), 0, (hipStream_t)stream, g->d_tasks + g->launch_beg[l],
+                           g->d_mols);
+        if (hipGetLastError() != hipSuccess) { set_error(ERR_DEVICE, "H2ERI_group_matvec: launch failed"); return -1; }
+    }
+    return 0;
+}
+
+void H2ERI_group_destroy(void *group) {
+    if (group) group_free((H2EGroup *)group);
 }
 
 size_t h2eri_struct_size(void) { return sizeof(struct H2ERI); }

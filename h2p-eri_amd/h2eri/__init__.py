@@ -14,7 +14,7 @@ import numpy as np
 
 from . import capi
 
-__all__ = ["H2ERI", "H2ERIError", "capi"]
+__all__ = ["H2ERI", "Group", "H2ERIError", "capi"]
 
 
 class H2ERIError(RuntimeError):
@@ -43,6 +43,36 @@ def _darr(p, n):
     if not p or n <= 0:
         return np.zeros(0, dtype=np.float64)
     return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+
+class Group:
+    """Several representations whose matvecs run together, one launch per
+    stage index (H2ERI_group_create / _matvec / _destroy)."""
+
+    def __init__(self, handles, x_ptrs, y_ptrs):
+        self.lib = capi.load()
+        n = len(handles)
+        hs = (capi.H2ERI_p * n)(*[h.h for h in handles])
+        xs = (C.c_void_p * n)(*x_ptrs)
+        ys = (C.c_void_p * n)(*y_ptrs)
+        self.handles = list(handles)   # keep the handles alive
+        self.g = self.lib.H2ERI_group_create(n, hs, xs, ys)
+        _check(self.lib)
+
+    def matvec(self, stream=None):
+        self.lib.H2ERI_group_matvec(self.g, C.c_void_p(stream or 0))
+        _check(self.lib)
+
+    def close(self):
+        if self.g:
+            self.lib.H2ERI_group_destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class H2ERI:
@@ -265,6 +295,13 @@ class H2ERI:
         n = self.lib.H2ERI_exchange_plan_check(self.h, _dptr(out))
         _check(self.lib)
         return dict(terms=n, **dict(zip(["gemm_flops", "terms_f", "resident_bytes", "scratch_doubles"], out.tolist())))
+
+    def autotune(self, nv=1, reps=10):
+        """Times tile sizes x near-field schedules on the device, keeps the best."""
+        out = np.zeros(3)
+        self.lib.H2ERI_autotune(self.h, nv, reps, _dptr(out))
+        _check(self.lib)
+        return dict(ms=out[0], tile_bytes=out[1], schedule=int(out[2]))
 
     def set_phase_timing(self, on=True):
         self.lib.H2ERI_set_phase_timing(self.h, 1 if on else 0)

@@ -107,6 +107,10 @@ SIGNATURES = {
     "H2ERI_set_tile_bytes": (None, [H2ERI_p, C.c_double]),
     "H2ERI_set_schedule": (None, [H2ERI_p, C.c_int]),
     "h2eri_struct_size": (C.c_size_t, []),
+    "H2ERI_autotune": (C.c_int, [H2ERI_p, C.c_int, C.c_int, c_dbl_p]),
+    "H2ERI_group_create": (C.c_void_p, [C.c_int, C.POINTER(H2ERI_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "H2ERI_group_matvec": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "H2ERI_group_destroy": (None, [C.c_void_p]),
     "H2ERI_plan_check": (C.c_int, [H2ERI_p, C.c_int]),
     "H2ERI_plan_emulate": (C.c_int, [H2ERI_p, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
 }

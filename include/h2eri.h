@@ -342,6 +342,22 @@ void H2ERI_set_tile_bytes(H2ERI_p h2eri, const double bytes);
  * launches. */
 void H2ERI_set_schedule(H2ERI_p h2eri, const int mode);
 
+/* Molecule groups (uneven molecules in one launch per stage index):
+ * H2ERI_group_create plans nh uploaded representations together for fixed
+ * device vectors (y_k := A_k x_k, one vector each); H2ERI_group_matvec
+ * enqueues one matvec of every molecule on `stream` (stage l of all molecules
+ * in launch l; graph-capturable); H2ERI_group_destroy frees the group.  The
+ * handles must outlive the group and keep their precision. */
+void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_dev, double *const *Y_dev);
+int   H2ERI_group_matvec(void *group, void *stream);
+void  H2ERI_group_destroy(void *group);
+
+/* Autotuner: times nv-vector device matvecs (reps calls after a warm-up)
+ * for each tile size in {64, 128, 256} KiB and near-field schedule in
+ * {0, 2, 3, 4}, and keeps the fastest pair for later calls.  out3 = {ms per
+ * call, tile bytes, schedule} of the pair kept.  Returns 0, -1 on error. */
+int H2ERI_autotune(H2ERI_p h2eri, const int nv, const int reps, double *out3);
+
 /* Builds the device execution plan on the host only (no HIP call) and
  * checks every access of every launch against its buffer.  Returns the
  * number of tile tasks, -1 on failure. */

@@ -91,3 +91,38 @@ def test_uneven_molecules(tmp_path_factory):
         D = M.density(atoms, 4 * n + 1, rng)
         h, p = build(tmp_path_factory, f"c{n}", atoms, 1e-6)
         assert relerr(h.build_coulomb(D), oracle_bind.Oracle(p).coulomb(D)) < 1e-13
+
+
+def test_molecule_group(tmp_path_factory):
+    """Uneven molecules in one launch per stage index (H2ERI_group_*)."""
+    import torch
+    import h2eri
+    from h2eri import molecules as M
+    hs, paths, xs = [], [], []
+    for n in (4, 7, 11):
+        rng = M.rng_for(600 + n)
+        atoms = M.alkane(n, rng)
+        h, p = build(tmp_path_factory, f"g{n}", atoms, 1e-6)
+        hs.append(h)
+        paths.append(p)
+        xs.append(np.random.default_rng(n).standard_normal(h.n))
+    X = [torch.from_numpy(x).to("cuda:0") for x in xs]
+    Y = [torch.zeros_like(x) for x in X]
+    g = h2eri.Group(hs, [x.data_ptr() for x in X], [y.data_ptr() for y in Y])
+    for rep in range(2):
+        for y in Y:
+            y.zero_()
+        g.matvec(torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        for k in range(3):
+            assert relerr(Y[k].cpu().numpy(), oracle_bind.Oracle(paths[k]).matvec(xs[k])) < 1e-13
+    g.close()
+
+
+def test_autotune_keeps_results(tmp_path):
+    from conftest import make_synthetic
+    h = make_synthetic(tmp_path / "a.rep", 2000, 21, leaf=60, max_rank=30, lr_frac=0.4, diag_lr_frac=0.3)
+    res = h.autotune(nv=1, reps=3)
+    assert res["ms"] > 0 and res["tile_bytes"] in (65536, 131072, 262144) and res["schedule"] in (0, 2, 3, 4)
+    x = np.random.default_rng(2).standard_normal(h.n)
+    assert relerr(h.matvec(x), oracle_bind.Oracle(str(tmp_path / "a.rep")).matvec(x)) < 1e-13
