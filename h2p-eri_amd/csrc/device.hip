@@ -1588,12 +1588,14 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out3) {
 // ---------------------------------------------------------------------------
 struct H2EGroup {
     std::vector<H2ERI_p> hs;
-    std::vector<DevPlan> plans;          // schedule-0 plans (near field inside the sweep launches)
-    std::vector<int> launch_beg;          // GTask ranges per launch
+    std::vector<DevPlan> plans;          // one plan per molecule, the handle's near-field schedule
+    std::vector<int> launch_beg;          // GTask ranges: main-stream launches, then near-field launches
     GTask *d_tasks = nullptr;
     MolPtrs *d_mols = nullptr;
-    int nlaunch = 0;
+    int nmain = 0, nside = 0;
     size_t ntask = 0;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 static void group_free(H2EGroup *g) {
@@ -1604,14 +1606,20 @@ static void group_free(H2EGroup *g) {
     }
     free_dev((void *&)g->d_tasks);
     free_dev((void *&)g->d_mols);
+    if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
+    if (g->ev_join) (void)hipEventDestroy(g->ev_join);
+    if (g->side) (void)hipStreamDestroy(g->side);
     delete g;
 }
 
 // Groups nh representations (single vector each) for H2ERI_group_matvec: y_k
-// := A_k x_k for k < nh with fixed device buffers X_dev[k], Y_dev[k].  Stage
-// l of every molecule runs in launch l, so molecules of uneven size fill the
-// device together.  The handles must outlive the group and keep their
-// precision; returns NULL with the error set on failure.
+// := A_k x_k for k < nh with fixed device buffers X_dev[k], Y_dev[k].  The
+// main-stream stages of all molecules are right-aligned, so that every
+// molecule's gather falls in the last main launch: launch l runs stage
+// l - (L - L_k) of molecule k.  Their near-field stages run left-aligned on a
+// second stream joined before that last launch.  Molecules of uneven size so
+// fill the device together.  The handles must outlive the group and keep
+// their precision; returns NULL with the error set on failure.
 void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_dev, double *const *Y_dev) {
     if (nh < 1 || !handles || !X_dev || !Y_dev) { set_error(ERR_ARG, "H2ERI_group_create: bad argument"); return nullptr; }
     H2EGroup *g = new H2EGroup();
@@ -1624,7 +1632,7 @@ void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_
         if (!upload_storage(h)) { group_free(g); return nullptr; }
         DevState *d = dstate(h);
         DevPlan &dp = g->plans[k];
-        if (!build_plan(h, d->st, dp.P, 1, d->tile_bytes, 0)) { group_free(g); return nullptr; }
+        if (!build_plan(h, d->st, dp.P, 1, d->tile_bytes, d->nf_schedule)) { group_free(g); return nullptr; }
         const Plan &P = dp.P;
         auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
             if (bytes == 0) bytes = 64;
@@ -1642,23 +1650,54 @@ void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_
         if (!ok) { group_free(g); return nullptr; }
         mols[k] = MolPtrs{d->S64, d->S32, dp.Wv, dp.d_t64, dp.d_t32, dp.d_rt, dp.d_vsrc, dp.d_rsrc, X_dev[k], Y_dev[k],
                           (long long)h->krnl_mat_size};
-        if (per_launch.size() < P.stages.size()) per_launch.resize(P.stages.size());
-        for (size_t l = 0; l < P.stages.size(); l++) {
-            const Stage &st = P.stages[l];
-            for (int i = 0; i < st.t64_cnt; i++) per_launch[l].push_back(GTask{k, 0, st.t64_beg + i, 0});
-            for (int i = 0; i < st.t32_cnt; i++) per_launch[l].push_back(GTask{k, 1, st.t32_beg + i, 0});
-            for (int i = 0; i < st.r_cnt; i++) per_launch[l].push_back(GTask{k, 2, st.r_beg + i, 0});
-        }
         g->hs.push_back(h);
     }
-    std::vector<GTask> all;
-    for (auto &v : per_launch) {
-        g->launch_beg.push_back((int)all.size());
-        all.insert(all.end(), v.begin(), v.end());
+    // main-stream stages right-aligned, near-field stages left-aligned
+    int L = 0, Ls = 0;
+    for (const DevPlan &dp : g->plans) {
+        int m = 0, sd = 0;
+        for (const Stage &st : dp.P.stages) (st.stream == 0 ? m : sd)++;
+        L = std::max(L, m);
+        Ls = std::max(Ls, sd);
     }
+    std::vector<std::vector<GTask>> mainL(L), sideL(Ls);
+    for (int k = 0; k < nh; k++) {
+        const Plan &P = g->plans[k].P;
+        int m = 0;
+        for (const Stage &st : P.stages) m += st.stream == 0;
+        int li = L - m, si = 0;
+        for (const Stage &st : P.stages) {
+            std::vector<GTask> &dst = st.stream == 0 ? mainL[li++] : sideL[si++];
+            for (int i = 0; i < st.t64_cnt; i++) dst.push_back(GTask{k, 0, st.t64_beg + i, 0});
+            for (int i = 0; i < st.t32_cnt; i++) dst.push_back(GTask{k, 1, st.t32_beg + i, 0});
+            for (int i = 0; i < st.r_cnt; i++) dst.push_back(GTask{k, 2, st.r_beg + i, 0});
+        }
+    }
+    // within a launch: largest tasks first across all molecules (reduce tasks last)
+    auto weight = [&](const GTask &t) -> long long {
+        const Plan &P = g->plans[t.mol].P;
+        if (t.kind == 2) return 0;
+        const BTask &b = t.kind == 0 ? P.t64[t.idx] : P.t32[t.idx];
+        return (long long)b.nrow * b.ncol * (t.kind == 0 ? 2 : 1);
+    };
+    std::vector<GTask> all;
+    for (auto *lists : {&mainL, &sideL})
+        for (auto &v : *lists) {
+            std::stable_sort(v.begin(), v.end(), [&](const GTask &a, const GTask &b) { return weight(a) > weight(b); });
+            g->launch_beg.push_back((int)all.size());
+            all.insert(all.end(), v.begin(), v.end());
+        }
     g->launch_beg.push_back((int)all.size());
-    g->nlaunch = (int)per_launch.size();
+    g->nmain = L;
+    g->nside = Ls;
     g->ntask = all.size();
+    if (Ls > 0 && (hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
+                   hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                   hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming) != hipSuccess)) {
+        set_error(ERR_DEVICE, "H2ERI_group_create: stream setup failed");
+        group_free(g);
+        return nullptr;
+    }
     if (hipMalloc(&g->d_tasks, sizeof(GTask) * std::max<size_t>(all.size(), 1)) != hipSuccess ||
         hipMalloc(&g->d_mols, sizeof(MolPtrs) * nh) != hipSuccess ||
         (!all.empty() && hipMemcpy(g->d_tasks, all.data(), sizeof(GTask) * all.size(), hipMemcpyHostToDevice) != hipSuccess) ||
@@ -1674,12 +1713,29 @@ void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_
 int H2ERI_group_matvec(void *group, void *stream) {
     H2EGroup *g = (H2EGroup *)group;
     if (!g) { set_error(ERR_ARG, "H2ERI_group_matvec: null group"); return -1; }
-    for (int l = 0; l < g->nlaunch; l++) {
+    hipStream_t s = (hipStream_t)stream;
+    auto launch = [&](int l, hipStream_t q) -> bool {
         const int nb = g->launch_beg[l + 1] - g->launch_beg[l];
-        if (nb == 0) continue;
-        hipLaunchKernelGGL(stage_kernel_grouped<1>, dim3(nb), dim3(WG), 0, (hipStream_t)stream, g->d_tasks + g->launch_beg[l],
-                           g->d_mols);
-        if (hipGetLastError() != hipSuccess) { set_error(ERR_DEVICE, "H2ERI_group_matvec: launch failed"); return -1; }
+        if (nb == 0) return true;
+        hipLaunchKernelGGL(stage_kernel_grouped<1>, dim3(nb), dim3(WG), 0, q, g->d_tasks + g->launch_beg[l], g->d_mols);
+        if (hipGetLastError() != hipSuccess) { set_error(ERR_DEVICE, "H2ERI_group_matvec: launch failed"); return false; }
+        return true;
+    };
+    if (g->nside > 0) {
+        if (hipEventRecord(g->ev_fork, s) != hipSuccess || hipStreamWaitEvent(g->side, g->ev_fork, 0) != hipSuccess) {
+            set_error(ERR_DEVICE, "H2ERI_group_matvec: fork failed");
+            return -1;
+        }
+        for (int l = 0; l < g->nside; l++)
+            if (!launch(g->nmain + l, g->side)) return -1;
+        if (hipEventRecord(g->ev_join, g->side) != hipSuccess) { set_error(ERR_DEVICE, "H2ERI_group_matvec: join failed"); return -1; }
+    }
+    for (int l = 0; l < g->nmain; l++) {
+        if (g->nside > 0 && l == g->nmain - 1 && hipStreamWaitEvent(s, g->ev_join, 0) != hipSuccess) {
+            set_error(ERR_DEVICE, "H2ERI_group_matvec: join failed");
+            return -1;
+        }
+        if (!launch(l, s)) return -1;
     }
     return 0;
 }
