@@ -365,7 +365,10 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic_c30h62_j_matvec.json")
     if args.config == 1 and os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f)["traffic_bytes_per_launch"]
+            rec = json.load(f)
+        # a recording of another launch structure does not describe this plan
+        if rec.get("launches_per_matvec") == len(r32["prof"]):
+            traffic = rec["traffic_bytes_per_launch"]
     best32 = min(r32["batch"].values(), key=lambda b: b["ms_per_density"])
     best32_nv = [k for k, b in r32["batch"].items() if b is best32][0]
     fp64_plain = r64["batch"][1]["ms"]
