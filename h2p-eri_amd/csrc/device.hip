@@ -38,6 +38,7 @@
 #include <climits>
 #include <cmath>
 #include <numeric>
+#include <queue>
 
 #include "h2eri_internal.h"
 
@@ -65,7 +66,7 @@ static_assert(sizeof(BTask) == 64, "BTask layout");
 // Reduce task: dst[i] = sum over rsrc[src_beg..src_end) of Wv[src + soff + i].
 struct RTask {
     long long dst;   // element offset: external output (flags & 1) or workspace
-    int len;         // <= RCHUNK elements
+    int len;         // elements: <= the plan's reduce chunk (len * NV <= 1024)
     int src_beg, src_end;
     int soff;        // element offset of this chunk inside every source slot
     int flags;       // 1: external output
@@ -74,7 +75,11 @@ struct RTask {
 static_assert(sizeof(RTask) == 32, "RTask layout");
 
 static const int WG = 256;       // threads per workgroup (4 waves)
-static const int RCHUNK = 64;    // output elements per reduce task
+// Output elements per reduce task: 64 for the launch-per-stage executor (many
+// small tasks spread a launch over the chip), 1024 / NV for the persistent
+// executor (fewer tasks, fewer counter operations).  Both sum every element
+// in the same order.
+static inline int rchunk_for(int executor, int nv) { return executor == 1 ? 1024 / nv : 64; }
 static const int LDS_DOUBLES = 4096;   // 32 KB of LDS per workgroup
 static const int XIN_DOUBLES = 1024;   // in0 staging (ncol * NV <= 1024)
 static const int YIN_DOUBLES = LDS_DOUBLES - XIN_DOUBLES;   // in1 staging (nrow * NV)
@@ -103,8 +108,24 @@ __device__ __forceinline__ double vget(const float4 &v, int i) {
     return (double)(i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)));
 }
 
+// Workspace accesses.  COH = true (persistent executor, where a producer and
+// its consumer may run on different XCDs within one launch) makes them
+// agent-scope relaxed atomics: coherent at device scope without any L2
+// writeback or invalidation; the ordering against the dependency counters
+// comes from waiting for the stores' completion before a counter moves.
+template <bool COH>
+__device__ __forceinline__ double wld(const double *p) {
+    if constexpr (COH) return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+template <bool COH>
+__device__ __forceinline__ void wst(double *p, double v) {
+    if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
 // Element idx of the virtual vector vsrc[b..e): NV values into out.
-template <int NV>
+template <int NV, bool COH>
 __device__ __forceinline__ void vsum(const int *__restrict__ vsrc, int b, int e, int idx, const double *__restrict__ Wv,
                                      const double *__restrict__ X, long long ldx, double (&out)[NV]) {
     #pragma unroll
@@ -124,7 +145,12 @@ __device__ __forceinline__ void vsum(const int *__restrict__ vsrc, int b, int e,
         const double *p3 = Wv + (long long)(vsrc[j + 3] + idx) * NV;
         double a0[NV], a1[NV], a2[NV], a3[NV];
         #pragma unroll
-        for (int v = 0; v < NV; v++) { a0[v] = p0[v]; a1[v] = p1[v]; a2[v] = p2[v]; a3[v] = p3[v]; }
+        for (int v = 0; v < NV; v++) {
+            a0[v] = wld<COH>(p0 + v);
+            a1[v] = wld<COH>(p1 + v);
+            a2[v] = wld<COH>(p2 + v);
+            a3[v] = wld<COH>(p3 + v);
+        }
         #pragma unroll
         for (int v = 0; v < NV; v++) {
             out[v] += a0[v];
@@ -136,11 +162,11 @@ __device__ __forceinline__ void vsum(const int *__restrict__ vsrc, int b, int e,
     for (; j < e; j++) {
         const double *p = Wv + (long long)(vsrc[j] + idx) * NV;
         #pragma unroll
-        for (int v = 0; v < NV; v++) out[v] += p[v];
+        for (int v = 0; v < NV; v++) out[v] += wld<COH>(p + v);
     }
 }
 
-template <typename T, int NV>
+template <typename T, int NV, bool COH>
 __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *__restrict__ vsrc,
                            double *__restrict__ Wv, const double *__restrict__ X, long long ldx, double *lds) {
     typedef Cfg<T, NV> C_;
@@ -182,14 +208,14 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
     if (do0)
         for (int q = tid; q < C; q += WG) {
             double tmp[NV];
-            vsum<NV>(vsrc, t.v0b, t.v0e, t.v0off + q, Wv, X, ldx, tmp);
+            vsum<NV, COH>(vsrc, t.v0b, t.v0e, t.v0off + q, Wv, X, ldx, tmp);
             #pragma unroll
             for (int v = 0; v < NV; v++) xin[q * NV + v] = tmp[v];
         }
     if (do1)
         for (int q = tid; q < R; q += WG) {
             double tmp[NV];
-            vsum<NV>(vsrc, t.v1b, t.v1e, t.v1off + q, Wv, X, ldx, tmp);
+            vsum<NV, COH>(vsrc, t.v1b, t.v1e, t.v1off + q, Wv, X, ldx, tmp);
             #pragma unroll
             for (int v = 0; v < NV; v++) yin[q * NV + v] = tmp[v];
         }
@@ -284,7 +310,7 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
                         const int q = base + i;
                         const int u = q / NV, v = q - u * NV;
                         const int r = r0 + u * G;
-                        if (r < R) Wv[(long long)(t.out0 + r) * NV + v] = vals[i];
+                        if (r < R) wst<COH>(Wv + (long long)(t.out0 + r) * NV + v, vals[i]);
                     }
                 }
             }
@@ -315,44 +341,43 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
         __syncthreads();
         for (int q = tid; q < C * NV; q += WG) {
             const double s = (red[q] + red[STRIDE + q]) + (red[2 * STRIDE + q] + red[3 * STRIDE + q]);
-            Wv[(long long)t.out1 * NV + q] = s;
+            wst<COH>(Wv + (long long)t.out1 * NV + q, s);
         }
     }
 }
 
-// Fixed-order sum of slots into one chunk.  Lane i owns element i of the
-// chunk; wave w sums the sources w, w+4, ... in order; the four wave sums are
-// added as (w0 + w1) + (w2 + w3).
-template <int NV>
+// Fixed-order sum of slots into one chunk (len * NV <= 1024 values).  Lane i
+// of a wave owns values i, i+64, ...; wave w sums the sources w, w+4, ... in
+// order; the four wave sums are added as (w0 + w1) + (w2 + w3).
+template <int NV, bool COH>
 __device__ void reduce_task(const RTask &t, const int *__restrict__ src, double *__restrict__ Wv,
                             double *__restrict__ Y, long long ldy, double *lds) {
-    double *red = lds;   // 4 x RCHUNK * NV
+    double *red = lds;   // 4 x 1024
     const int w = threadIdx.x >> 6;
-    for (int q = threadIdx.x & 63; q < RCHUNK * NV; q += 64) {
+    const int n = t.len * NV;
+    for (int q = threadIdx.x & 63; q < n; q += 64) {
         double s = 0.0;
-        if (q < t.len * NV) {
-            const double *base = Wv + (long long)t.soff * NV + q;
-            int j = t.src_beg + w;
-            for (; j + 12 < t.src_end; j += 16) {
-                const double a0 = base[(long long)src[j] * NV], a1 = base[(long long)src[j + 4] * NV];
-                const double a2 = base[(long long)src[j + 8] * NV], a3 = base[(long long)src[j + 12] * NV];
-                s += a0;
-                s += a1;
-                s += a2;
-                s += a3;
-            }
-            for (; j < t.src_end; j += 4) s += base[(long long)src[j] * NV];
+        const double *base = Wv + (long long)t.soff * NV + q;
+        int j = t.src_beg + w;
+        for (; j + 12 < t.src_end; j += 16) {
+            const double a0 = wld<COH>(base + (long long)src[j] * NV), a1 = wld<COH>(base + (long long)src[j + 4] * NV);
+            const double a2 = wld<COH>(base + (long long)src[j + 8] * NV), a3 = wld<COH>(base + (long long)src[j + 12] * NV);
+            s += a0;
+            s += a1;
+            s += a2;
+            s += a3;
         }
-        red[w * RCHUNK * NV + q] = s;
+        for (; j < t.src_end; j += 4) s += wld<COH>(base + (long long)src[j] * NV);
+        red[w * 1024 + q] = s;
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < t.len * NV; q += WG) {
-        const double s = (red[q] + red[RCHUNK * NV + q]) + (red[2 * RCHUNK * NV + q] + red[3 * RCHUNK * NV + q]);
+    for (int q = threadIdx.x; q < n; q += WG) {
+        const double s = (red[q] + red[1024 + q]) + (red[2048 + q] + red[3072 + q]);
         if (t.flags & 1) {
             const int i = q / NV, v = q - i * NV;
             Y[(long long)v * ldy + t.dst + i] = s;
         } else {
-            Wv[t.dst * NV + q] = s;
+            wst<COH>(Wv + t.dst * NV + q, s);
         }
     }
 }
@@ -370,13 +395,13 @@ __global__ __launch_bounds__(WG) void stage_kernel(const BTask *__restrict__ t64
     const int b = blockIdx.x;
     if (b < n64) {
         const BTask t = t64[b];
-        bgemv_tile<double, NV>(t, S64, vsrc, Wv, X, ldx, lds);
+        bgemv_tile<double, NV, false>(t, S64, vsrc, Wv, X, ldx, lds);
     } else if (b - n64 < n32) {
         const BTask t = t32[b - n64];
-        bgemv_tile<float, NV>(t, S32, vsrc, Wv, X, ldx, lds);
+        bgemv_tile<float, NV, false>(t, S32, vsrc, Wv, X, ldx, lds);
     } else if (b - n64 - n32 < nr) {
         const RTask t = rt[b - n64 - n32];
-        reduce_task<NV>(t, rsrc, Wv, Y, ldx, lds);
+        reduce_task<NV, false>(t, rsrc, Wv, Y, ldx, lds);
     }
 }
 
@@ -405,14 +430,124 @@ __global__ __launch_bounds__(WG) void stage_kernel_grouped(const GTask *__restri
     const MolPtrs m = mols[g.mol];
     if (g.kind == 0) {
         const BTask t = m.t64[g.idx];
-        bgemv_tile<double, NV>(t, m.S64, m.vsrc, m.Wv, m.X, m.ldx, lds);
+        bgemv_tile<double, NV, false>(t, m.S64, m.vsrc, m.Wv, m.X, m.ldx, lds);
     } else if (g.kind == 1) {
         const BTask t = m.t32[g.idx];
-        bgemv_tile<float, NV>(t, m.S32, m.vsrc, m.Wv, m.X, m.ldx, lds);
+        bgemv_tile<float, NV, false>(t, m.S32, m.vsrc, m.Wv, m.X, m.ldx, lds);
     } else {
         const RTask t = m.rt[g.idx];
-        reduce_task<NV>(t, m.rsrc, m.Wv, m.Y, m.ldx, lds);
+        reduce_task<NV, false>(t, m.rsrc, m.Wv, m.Y, m.ldx, lds);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent executor: the whole pass in one launch.
+// ---------------------------------------------------------------------------
+// The tasks of every stage of a plan, in a topological order computed on the
+// host.  Dependencies are per input list: a tile's in0/in1 or a reduce task's
+// source list is ready once every task writing one of its slots has counted
+// itself into the list's counter.  Workgroup b < nstatic runs task b first (those
+// tasks have no inputs to wait for); the remaining tasks are taken in chunks
+// of QCHUNK consecutive tasks from one ticket counter, the next chunk claimed
+// while the current one runs.  A task only ever waits for tasks of lower
+// index, and those are held by running workgroups, so the launch drains
+// whatever the residency.  Every wait is bounded: past `spin_limit` polls the
+// workgroup raises the error flag and goes on, so every wave exits.
+//
+// Data between workgroups goes through agent-coherent workspace accesses
+// (wld/wst with COH); a task's counters move only after every wave's stores
+// have completed, and a consumer reads only after its counters are complete.
+struct QTask {
+    int kind;         // 0: FP64 tile, 1: FP32 tile, 2: reduce task
+    int idx;          // index into t64 / t32 / rt
+    int in0, in1;     // input list counters to wait for (-1: none)
+    int out0, out1;   // list counters this task counts into (-1: none)
+    int pad_[2];
+};
+static_assert(sizeof(QTask) == 32, "QTask layout");
+
+// control words of the counter array (then one counter per list)
+static const int QC_TICKET = 0;    // next chunk to claim
+static const int QC_ERROR = 32;    // set when a wait exceeded its bound
+static const int QC_LISTS = 64;    // first list counter
+static const int QCHUNK = 1;       // tasks per claim: consecutive tasks are often dependent
+
+__device__ __forceinline__ void q_wait(int *ctr, int list, int need, int spin_limit) {
+    if (list < 0) return;
+    const int *c = ctr + QC_LISTS + list;
+    int it = 0;
+    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        if (++it > spin_limit) {
+            __hip_atomic_store(ctr + QC_ERROR, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// s_waitcnt vmcnt(0): every vector memory operation of this wave (stores
+// included on gfx9) has completed.
+__device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+template <int NV>
+__global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ tasks, int ntask, int nstatic,
+                                                      const int *__restrict__ need, int *ctr, int spin_limit,
+                                                      const BTask *__restrict__ t64, const BTask *__restrict__ t32,
+                                                      const RTask *__restrict__ rt, const int *__restrict__ vsrc,
+                                                      const int *__restrict__ rsrc, const double *__restrict__ S64,
+                                                      const float *__restrict__ S32, double *__restrict__ Wv,
+                                                      const double *__restrict__ X, double *__restrict__ Y,
+                                                      long long ldx) {
+    __shared__ double lds[LDS_DOUBLES];
+    __shared__ int s_next[2];
+    int t0, t1;
+    if ((int)blockIdx.x < nstatic) {
+        t0 = blockIdx.x;
+        t1 = t0 + 1;
+    } else {
+        if (threadIdx.x == 0) s_next[0] = __hip_atomic_fetch_add(ctr + QC_TICKET, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        t0 = nstatic + s_next[0] * QCHUNK;
+        t1 = min(t0 + QCHUNK, ntask);
+    }
+    for (int par = 1; t0 < ntask; par ^= 1) {
+        for (int t = t0; t < t1; t++) {
+            const QTask q = tasks[t];
+            if (threadIdx.x == 0) {
+                q_wait(ctr, q.in0, q.in0 >= 0 ? need[q.in0] : 0, spin_limit);
+                q_wait(ctr, q.in1, q.in1 >= 0 ? need[q.in1] : 0, spin_limit);
+                // the next chunk, claimed while this one runs
+                if (t == t0) s_next[par] = __hip_atomic_fetch_add(ctr + QC_TICKET, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            if (q.kind == 0) {
+                const BTask b = t64[q.idx];
+                bgemv_tile<double, NV, true>(b, S64, vsrc, Wv, X, ldx, lds);
+            } else if (q.kind == 1) {
+                const BTask b = t32[q.idx];
+                bgemv_tile<float, NV, true>(b, S32, vsrc, Wv, X, ldx, lds);
+            } else {
+                const RTask r = rt[q.idx];
+                reduce_task<NV, true>(r, rsrc, Wv, Y, ldx, lds);
+            }
+            wait_vm_stores();
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (q.out0 >= 0) __hip_atomic_fetch_add(ctr + QC_LISTS + q.out0, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (q.out1 >= 0) __hip_atomic_fetch_add(ctr + QC_LISTS + q.out1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        t0 = nstatic + s_next[par] * QCHUNK;
+        t1 = min(t0 + QCHUNK, ntask);
+    }
+}
+
+// Clears the ticket and list counters before a queue launch (the error flag
+// stays until the host reads it).
+__global__ void queue_reset_kernel(int *ctr, int nlist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == QC_TICKET) ctr[i] = 0;
+    if (i < nlist) ctr[QC_LISTS + i] = 0;
 }
 
 // x[i] = c_i * D[g_i >> 1], c_i = 2 when the low bit is set
@@ -549,6 +684,7 @@ struct Plan {
     long long nW = 0;   // workspace elements per vector
     double bytes_vec = 0;
     int n_fold = 0;     // fold reduce tasks
+    std::vector<std::pair<long long, int>> slots;   // workspace slots in allocation order: (offset, length)
 };
 
 // One timed launch of the last call (phase timing mode).
@@ -565,6 +701,12 @@ struct DevPlan {
     BTask *d_t64 = nullptr, *d_t32 = nullptr;
     RTask *d_rt = nullptr;
     int *d_rsrc = nullptr, *d_vsrc = nullptr;
+    // persistent executor
+    bool q_ready = false;
+    int q_n = 0, q_static = 0, q_nlist = 0, q_grid = 0;
+    double q_est_us = 0;
+    QTask *d_q = nullptr;
+    int *d_need = nullptr, *d_ctr = nullptr;
 };
 
 static const int NV_VARIANTS[4] = {1, 2, 4, 8};
@@ -575,6 +717,7 @@ struct DevState {
     int ub_bits = 64;
     double tile_bytes = 128.0 * 1024.0;
     int nf_schedule = 3;      // near-field placement, see H2ERI_set_schedule (two groups on a second stream)
+    int executor = 0;         // 0: one launch per stage, 1: one persistent launch per pass (H2ERI_set_executor)
     Storage st;
     bool st_ready = false;
     double *S64 = nullptr;
@@ -617,9 +760,10 @@ static void release_plans(DevState *d) {
     release_graph(d);
     for (DevPlan &p : d->plans) {
         void **ps[] = {(void **)&p.Wv, (void **)&p.d_t64, (void **)&p.d_t32, (void **)&p.d_rt, (void **)&p.d_rsrc,
-                       (void **)&p.d_vsrc};
+                       (void **)&p.d_vsrc, (void **)&p.d_q, (void **)&p.d_need, (void **)&p.d_ctr};
         for (void **q : ps) free_dev(*q);
         p.ready = false;
+        p.q_ready = false;
         p.P = Plan();
     }
 }
@@ -695,6 +839,7 @@ struct Planner {
     int nv;
     double tile_bytes;
     int nf_schedule;
+    int rchunk;  // output elements per reduce task
     int maxc;   // max tile columns for this batch width
     std::vector<VVec> vv;
     std::vector<PTile> tiles;
@@ -703,12 +848,13 @@ struct Planner {
     std::vector<double> sload;
     int nst = 0, sG = 0, sS = 0;   // stages: main 0..sG, near-field stream sS..nst-1 (schedule 2)
 
-    Planner(H2ERI_p h_, const Storage &s_, Plan &p_, int nv_, double tb, int nfs)
-        : h(h_), S(s_), P(p_), nv(nv_), tile_bytes(tb), nf_schedule(nfs), maxc(64 * cpl_of(nv_)) {}
+    Planner(H2ERI_p h_, const Storage &s_, Plan &p_, int nv_, double tb, int nfs, int rch)
+        : h(h_), S(s_), P(p_), nv(nv_), tile_bytes(tb), nf_schedule(nfs), rchunk(rch), maxc(64 * cpl_of(nv_)) {}
 
     long long walloc(long long n) {
         long long o = P.nW;
         P.nW += (n + 7) & ~7LL;
+        P.slots.push_back({o, (int)n});
         return o;
     }
     int new_vec(int len) {
@@ -799,11 +945,11 @@ struct Planner {
     void emit_reduce(int st, long long dst, bool external, int len, const std::vector<Src> &srcs) {
         const int sb = (int)P.rsrc.size();
         for (const Src &s : srcs) P.rsrc.push_back(s.off);
-        for (int e0 = 0; e0 < len; e0 += RCHUNK) {
+        for (int e0 = 0; e0 < len; e0 += rchunk) {
             RTask r;
             memset(&r, 0, sizeof(r));
             r.dst = dst + e0;
-            r.len = std::min(RCHUNK, len - e0);
+            r.len = std::min(rchunk, len - e0);
             r.src_beg = sb;
             r.src_end = (int)P.rsrc.size();
             r.soff = e0;
@@ -1078,7 +1224,7 @@ struct Planner {
         };
         if (!chk(P.t64, false) || !chk(P.t32, true)) return false;
         for (const RTask &r : P.rt) {
-            bool ok = r.len > 0 && r.len <= RCHUNK && r.dst >= 0 && r.soff >= 0 && r.src_beg >= 0 &&
+            bool ok = r.len > 0 && r.len <= rchunk && r.len * nv <= 1024 && r.dst >= 0 && r.soff >= 0 && r.src_beg >= 0 &&
                       r.src_end <= (int)P.rsrc.size() && r.src_beg < r.src_end;
             ok = ok && ((r.flags & 1) ? r.dst + r.len <= N : r.dst + r.len <= P.nW);
             for (int j = r.src_beg; ok && j < r.src_end; j++) ok = P.rsrc[j] >= 0 && (long long)P.rsrc[j] + r.soff + r.len <= P.nW;
@@ -1096,9 +1242,208 @@ struct Planner {
 
 }  // namespace
 
-static bool build_plan(H2ERI_p h, const Storage &S, Plan &P, int nv, double tb, int nfs) {
-    Planner pl(h, S, P, nv, tb, nfs);
+static bool build_plan(H2ERI_p h, const Storage &S, Plan &P, int nv, double tb, int nfs, int executor) {
+    Planner pl(h, S, P, nv, tb, nfs, rchunk_for(executor, nv));
     return pl.build();
+}
+
+// Queue form of a plan (persistent executor): the tasks of all its stages in
+// one topological order, with their input and output lists.
+struct QPlan {
+    std::vector<QTask> tasks;   // ticket order
+    std::vector<int> need;      // producer count of every list
+    int nstatic = 0;            // leading tasks without inputs, one per workgroup
+    double est_us = 0;          // list-scheduling estimate of the pass
+};
+
+// Builds the queue of plan P for G resident workgroups.  Lists are the input
+// vectors of the tiles (vsrc ranges) and the source lists of the reduce tasks
+// (rsrc ranges); a workspace slot belongs to at most one list, and a task
+// counts itself into the list of every slot it writes.  The order comes from
+// list scheduling on G workgroups with a bytes-proportional duration per
+// task, highest remaining path first, so the sweeps' critical path runs as
+// soon as it can and the near field fills the gaps.
+static bool build_queue(const Plan &P, int nv, int G, QPlan &Q) {
+    Q = QPlan();
+    struct TRef { int kind, idx; };
+    std::vector<TRef> T;
+    for (const Stage &st : P.stages) {
+        for (int i = 0; i < st.t64_cnt; i++) T.push_back({0, st.t64_beg + i});
+        for (int i = 0; i < st.t32_cnt; i++) T.push_back({1, st.t32_beg + i});
+        for (int i = 0; i < st.r_cnt; i++) T.push_back({2, st.r_beg + i});
+    }
+    const int n = (int)T.size();
+    // lists, keyed by the begin index of their range
+    std::vector<int> vlist(P.vsrc.size() + 1, -1), rlist(P.rsrc.size() + 1, -1);
+    std::vector<std::pair<int, int>> lrange;   // [b, e) in vsrc (kind 0) or rsrc (kind 1)
+    std::vector<char> lkind;
+    auto list_of = [&](int kind, int b, int e) -> int {
+        std::vector<int> &m = kind == 0 ? vlist : rlist;
+        const std::vector<int> &src = kind == 0 ? P.vsrc : P.rsrc;
+        if (b < 0 || e <= b || e > (int)src.size() || src[b] < 0) return -1;   // external input or nothing
+        if (m[b] < 0) {
+            m[b] = (int)lrange.size();
+            lrange.push_back({b, e});
+            lkind.push_back((char)kind);
+        } else if (lrange[m[b]].second != e) {
+            return -2;
+        }
+        return m[b];
+    };
+    std::vector<QTask> q(n);
+    for (int i = 0; i < n; i++) {
+        QTask &t = q[i];
+        memset(&t, 0, sizeof(t));
+        t.kind = T[i].kind;
+        t.idx = T[i].idx;
+        t.in0 = t.in1 = t.out0 = t.out1 = -1;
+        if (t.kind < 2) {
+            const BTask &b = (t.kind == 0 ? P.t64 : P.t32)[t.idx];
+            if (b.out0 >= 0) t.in0 = list_of(0, b.v0b, b.v0e);
+            if (b.out1 >= 0) t.in1 = list_of(0, b.v1b, b.v1e);
+        } else {
+            const RTask &r = P.rt[t.idx];
+            t.in0 = list_of(1, r.src_beg, r.src_end);
+        }
+        if (t.in0 == -2 || t.in1 == -2) { set_error(ERR_ARG, "queue plan: overlapping source lists"); return false; }
+    }
+    const int nlist = (int)lrange.size();
+    // slot -> list (slots are in increasing offset order)
+    const std::vector<std::pair<long long, int>> &SL = P.slots;
+    auto slot_at = [&](long long off) -> int {
+        auto it = std::upper_bound(SL.begin(), SL.end(), std::make_pair(off, INT_MAX));
+        if (it == SL.begin()) return -1;
+        const int k = (int)(it - SL.begin()) - 1;
+        return off < SL[k].first + SL[k].second ? k : -1;
+    };
+    std::vector<int> slot_list(SL.size(), -1);
+    for (int L = 0; L < nlist; L++) {
+        const std::vector<int> &src = lkind[L] == 0 ? P.vsrc : P.rsrc;
+        for (int j = lrange[L].first; j < lrange[L].second; j++) {
+            const int k = slot_at(src[j]);
+            if (k < 0 || SL[k].first != src[j]) { set_error(ERR_ARG, "queue plan: list entry %d is not a slot", src[j]); return false; }
+            if (slot_list[k] >= 0 && slot_list[k] != L) { set_error(ERR_ARG, "queue plan: slot in two lists"); return false; }
+            slot_list[k] = L;
+        }
+    }
+    // outputs -> lists
+    std::vector<int> nprod(nlist, 0);
+    auto out_list = [&](long long off, int len, int &out) -> bool {
+        const int k = slot_at(off);
+        if (k < 0 || off + len > SL[k].first + SL[k].second) { set_error(ERR_ARG, "queue plan: output outside its slot"); return false; }
+        out = slot_list[k];
+        return true;
+    };
+    for (int i = 0; i < n; i++) {
+        QTask &t = q[i];
+        if (t.kind < 2) {
+            const BTask &b = (t.kind == 0 ? P.t64 : P.t32)[t.idx];
+            if (b.out0 >= 0 && !out_list(b.out0, b.nrow, t.out0)) return false;
+            if (b.out1 >= 0 && !out_list(b.out1, b.ncol, t.out1)) return false;
+        } else {
+            const RTask &r = P.rt[t.idx];
+            if (!(r.flags & 1) && !out_list(r.dst, r.len, t.out0)) return false;
+        }
+        if (t.out1 >= 0 && t.out1 == t.out0) t.out1 = -1;
+        if (t.out0 >= 0) nprod[t.out0]++;
+        if (t.out1 >= 0) nprod[t.out1]++;
+    }
+    std::vector<std::vector<int>> cons(nlist);
+    for (int i = 0; i < n; i++) {
+        if (q[i].in1 == q[i].in0) q[i].in1 = -1;
+        for (int L : {q[i].in0, q[i].in1}) {
+            if (L < 0) continue;
+            if (nprod[L] == 0) { set_error(ERR_ARG, "queue plan: a list without producers is read"); return false; }
+            cons[L].push_back(i);
+        }
+    }
+    // duration model: fixed cost (the counter wait, input staging and output
+    // reductions of a task, as measured) plus bytes at the per-workgroup
+    // share of HBM
+    const double c0 = 6.0e-6, bw = 5.0e12 / std::max(G, 1);
+    std::vector<double> dur(n);
+    for (int i = 0; i < n; i++) {
+        const QTask &t = q[i];
+        double bytes;
+        if (t.kind < 2) {
+            const BTask &b = (t.kind == 0 ? P.t64 : P.t32)[t.idx];
+            bytes = (t.kind == 0 ? 8.0 : 4.0) * b.nrow * b.ncol;
+            if (b.out0 >= 0) bytes += 8.0 * nv * ((double)b.ncol * (b.v0e - b.v0b) + b.nrow);
+            if (b.out1 >= 0) bytes += 8.0 * nv * ((double)b.nrow * (b.v1e - b.v1b) + b.ncol);
+        } else {
+            const RTask &r = P.rt[t.idx];
+            bytes = 8.0 * nv * r.len * (double)(r.src_end - r.src_beg + 1);
+        }
+        dur[i] = c0 + bytes / bw;
+    }
+    // topological order (Kahn), then the longest remaining path of every task
+    std::vector<int> rem(n, 0), cnt(nlist, 0), topo;
+    topo.reserve(n);
+    for (int i = 0; i < n; i++) rem[i] = (q[i].in0 >= 0) + (q[i].in1 >= 0);
+    for (int i = 0; i < n; i++) if (rem[i] == 0) topo.push_back(i);
+    auto complete = [&](int i, std::vector<int> &cnt_, std::vector<int> &rem_, auto &&on_ready) {
+        for (int L : {q[i].out0, q[i].out1}) {
+            if (L < 0 || ++cnt_[L] != nprod[L]) continue;
+            for (int c : cons[L])
+                if (--rem_[c] == 0) on_ready(c);
+        }
+    };
+    {
+        std::vector<int> r2 = rem;
+        for (size_t h = 0; h < topo.size(); h++) complete(topo[h], cnt, r2, [&](int c) { topo.push_back(c); });
+    }
+    if ((int)topo.size() != n) { set_error(ERR_ARG, "queue plan: dependency cycle"); return false; }
+    std::vector<double> bl(n, 0.0), lbl(nlist, 0.0);
+    for (int h = n - 1; h >= 0; h--) {
+        const int i = topo[h];
+        double m = 0.0;
+        for (int L : {q[i].out0, q[i].out1}) if (L >= 0) m = std::max(m, lbl[L]);
+        bl[i] = dur[i] + m;
+        for (int L : {q[i].in0, q[i].in1}) if (L >= 0) lbl[L] = std::max(lbl[L], bl[i]);
+    }
+    // list scheduling on G workgroups, longest remaining path first
+    std::priority_queue<std::pair<double, int>> ready;
+    typedef std::pair<double, int> Ev;
+    std::priority_queue<Ev, std::vector<Ev>, std::greater<Ev>> running;
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int i = 0; i < n; i++) if (rem[i] == 0) ready.push({bl[i], i});
+    std::vector<int> order;
+    order.reserve(n);
+    int free_slots = std::max(G, 1);
+    double now = 0.0;
+    while ((int)order.size() < n) {
+        while (free_slots > 0 && !ready.empty()) {
+            const int i = ready.top().second;
+            ready.pop();
+            order.push_back(i);
+            running.push({now + dur[i], i});
+            free_slots--;
+        }
+        if (running.empty()) { set_error(ERR_ARG, "queue plan: scheduling stalled"); return false; }
+        const Ev e = running.top();
+        running.pop();
+        now = e.first;
+        free_slots++;
+        complete(e.second, cnt, rem, [&](int c) { ready.push({bl[c], c}); });
+    }
+    while (!running.empty()) { now = std::max(now, running.top().first); running.pop(); }
+    Q.est_us = now * 1e6;
+    // ticket order; every producer of a task's inputs precedes it
+    std::vector<int> pos(n);
+    for (int k = 0; k < n; k++) pos[order[k]] = k;
+    std::vector<int> last_prod(nlist, -1);
+    for (int i = 0; i < n; i++)
+        for (int L : {q[i].out0, q[i].out1}) if (L >= 0) last_prod[L] = std::max(last_prod[L], pos[i]);
+    Q.tasks.resize(n);
+    for (int k = 0; k < n; k++) {
+        const QTask &t = q[order[k]];
+        for (int L : {t.in0, t.in1})
+            if (L >= 0 && last_prod[L] >= k) { set_error(ERR_ARG, "queue plan: order is not topological"); return false; }
+        Q.tasks[k] = t;
+    }
+    while (Q.nstatic < std::min(n, G) && Q.tasks[Q.nstatic].in0 < 0 && Q.tasks[Q.nstatic].in1 < 0) Q.nstatic++;
+    Q.need = nprod;
+    return true;
 }
 
 static bool ensure_events(DevState *d, size_t n) {
@@ -1147,12 +1492,55 @@ static bool upload_storage(H2ERI_p h) {
     return true;
 }
 
+// Resident workgroups of the persistent kernel on the current device.
+static int queue_grid(int nv) {
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    hipError_t e;
+    switch (nv) {
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<1>, WG, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<2>, WG, 0); break;
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<4>, WG, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<8>, WG, 0); break;
+    }
+    return e == hipSuccess ? ncu * std::max(per, 1) : 0;
+}
+
+// Queue form of an uploaded plan: built on the host, uploaded, counters zeroed.
+static bool ensure_queue(DevState *d, DevPlan &dp) {
+    if (dp.q_ready) return true;
+    const int G = queue_grid(dp.P.nv);
+    if (G <= 0) { set_error(ERR_DEVICE, "persistent executor: occupancy query failed"); return false; }
+    QPlan Q;
+    if (!build_queue(dp.P, dp.P.nv, G, Q)) return false;
+    const size_t nq = std::max<size_t>(Q.tasks.size(), 1), nl = Q.need.size();
+    HIP_OK(hipMalloc(&dp.d_q, nq * sizeof(QTask)));
+    HIP_OK(hipMalloc(&dp.d_need, std::max<size_t>(nl, 1) * sizeof(int)));
+    HIP_OK(hipMalloc(&dp.d_ctr, (QC_LISTS + nl) * sizeof(int)));
+    if (!Q.tasks.empty()) HIP_OK(hipMemcpy(dp.d_q, Q.tasks.data(), Q.tasks.size() * sizeof(QTask), hipMemcpyHostToDevice));
+    if (nl) HIP_OK(hipMemcpy(dp.d_need, Q.need.data(), nl * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemset(dp.d_ctr, 0, (QC_LISTS + nl) * sizeof(int)));
+    HIP_OK(hipDeviceSynchronize());
+    dp.q_n = (int)Q.tasks.size();
+    dp.q_static = Q.nstatic;
+    dp.q_nlist = (int)nl;
+    dp.q_grid = std::min(G, std::max(dp.q_n, 1));
+    dp.q_est_us = Q.est_us;
+    dp.q_ready = true;
+    (void)d;
+    return true;
+}
+
 static DevPlan *get_plan(H2ERI_p h, int nv) {
     if (!upload_storage(h)) return nullptr;
     DevState *d = dstate(h);
     DevPlan &dp = d->plans[nv_index(nv)];
-    if (dp.ready) return &dp;
-    if (!build_plan(h, d->st, dp.P, nv, d->tile_bytes, d->nf_schedule)) return nullptr;
+    if (dp.ready) {
+        if (d->executor == 1 && !ensure_queue(d, dp)) return nullptr;
+        return &dp;
+    }
+    if (!build_plan(h, d->st, dp.P, nv, d->tile_bytes, d->nf_schedule, d->executor)) return nullptr;
     const Plan &P = dp.P;
     auto alloc_copy = [&](void **dst, const void *src, size_t bytes) -> bool {
         if (bytes == 0) bytes = 64;
@@ -1169,6 +1557,7 @@ static DevPlan *get_plan(H2ERI_p h, int nv) {
     if (!alloc_copy((void **)&dp.d_rsrc, P.rsrc.empty() ? nullptr : P.rsrc.data(), P.rsrc.size() * sizeof(int))) return nullptr;
     if (!alloc_copy((void **)&dp.d_vsrc, P.vsrc.empty() ? nullptr : P.vsrc.data(), P.vsrc.size() * sizeof(int))) return nullptr;
     dp.ready = true;
+    if (d->executor == 1 && !ensure_queue(d, dp)) return nullptr;
     return &dp;
 }
 
@@ -1182,9 +1571,51 @@ static void launch_stage(const Stage &st, const DevPlan &dp, DevState *d, const 
 
 static void collect_phase_ms(H2ERI_p h, DevState *d);
 
+// Spin bound of one wait of the persistent kernel: about a second of polling,
+// far beyond any dependency of a pass.
+static const int QUEUE_SPIN_LIMIT = 1 << 22;
+
+template <int NV>
+static void launch_queue(const DevPlan &dp, DevState *d, const double *X, double *Y, long long ld, hipStream_t s) {
+    hipLaunchKernelGGL(queue_reset_kernel, dim3((std::max(dp.q_nlist, QC_LISTS) + 255) / 256), dim3(256), 0, s, dp.d_ctr,
+                       dp.q_nlist);
+    hipLaunchKernelGGL(queue_kernel<NV>, dim3(dp.q_grid), dim3(WG), 0, s, dp.d_q, dp.q_n, dp.q_static, dp.d_need,
+                       dp.d_ctr, QUEUE_SPIN_LIMIT, dp.d_t64, dp.d_t32, dp.d_rt, dp.d_vsrc, dp.d_rsrc, d->S64, d->S32,
+                       dp.Wv, X, Y, ld);
+}
+
+// One pass of the persistent executor on stream s (reset + queue launch).
+static bool run_queue(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, double *Y, hipStream_t s) {
+    if (!dp.q_ready) { set_error(ERR_DEVICE, "persistent executor: queue not built"); return false; }
+    const long long ld = h->krnl_mat_size;
+    const bool tim = d->phase_timing && !d->capturing;
+    if (tim) {
+        collect_phase_ms(h, d);
+        if (!ensure_events(d, 2)) return false;
+        d->launches.clear();
+        d->collected = false;
+        HIP_OK(hipEventRecord(d->evs[0], s));
+    }
+    switch (dp.P.nv) {
+    case 1: launch_queue<1>(dp, d, X, Y, ld, s); break;
+    case 2: launch_queue<2>(dp, d, X, Y, ld, s); break;
+    case 4: launch_queue<4>(dp, d, X, Y, ld, s); break;
+    default: launch_queue<8>(dp, d, X, Y, ld, s); break;
+    }
+    HIP_OK(hipGetLastError());
+    if (tim) {
+        HIP_OK(hipEventRecord(d->evs[1], s));
+        double mb = 0;
+        for (const Stage &st : dp.P.stages) mb += st.mat_bytes;
+        d->launches.push_back(LaunchRec{PH_N, dp.q_grid, mb, 0.f});
+    }
+    return true;
+}
+
 // Enqueue one batched matvec of nv vectors: X (nv x N, vector-major) -> Y.
 // In phase-timing mode an event pair brackets every launch.
 static bool run_matvec(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, double *Y, hipStream_t s) {
+    if (d->executor == 1) return run_queue(h, d, dp, X, Y, s);
     const Plan &P = dp.P;
     const long long ld = h->krnl_mat_size;
     const bool tim = d->phase_timing && !d->capturing;
@@ -1242,7 +1673,8 @@ static void collect_phase_ms(H2ERI_p h, DevState *d) {
             hipEventElapsedTime(&ms, d->evs[2 * i], d->evs[2 * i + 1]) != hipSuccess)
             ms = 0.f;
         d->launches[i].ms = ms;
-        d->phase_ms[d->launches[i].phase] += ms;
+        if (d->launches[i].phase < PH_N) d->phase_ms[d->launches[i].phase] += ms;
+        else d->phase_ms[PH_D] += ms;   // one launch for the whole pass (persistent executor)
         d->phase_ms[6] += ms;
     }
     // reference timer slots (seconds): forward, intermediate, backward, dense, vector ops
@@ -1522,6 +1954,60 @@ void H2ERI_set_schedule(H2ERI_p h, const int mode) {
     if (d->nf_schedule != mode) { d->nf_schedule = mode; release_plans(d); }
 }
 
+// Executor of the matvec (re-plans): 0 launches every stage separately (the
+// near field on a second stream for schedules >= 2); 1 runs the whole pass as
+// one persistent launch whose workgroups take the plan's tasks in a
+// dependency-checked order (plus a small launch that clears its counters).
+// Both compute every output with the same operations in the same order, so
+// their results are bitwise equal.
+void H2ERI_set_executor(H2ERI_p h, const int mode) {
+    if (!h || (mode != 0 && mode != 1)) { set_error(ERR_ARG, "H2ERI_set_executor: 0 or 1"); return; }
+    DevState *d = dstate_get(h);
+    if (d->executor != mode) { d->executor = mode; release_plans(d); }
+}
+
+int H2ERI_get_executor(H2ERI_p h) { return h && h->dev ? dstate(h)->executor : 0; }
+
+// Waits for the device and reports whether a wait of the persistent executor
+// exceeded its bound since the last call (which would mean a planning fault:
+// the results of that pass are not valid).  Returns 0 when clean, 1 with the
+// error set, -1 on a device error.  Clears the flag.
+int H2ERI_device_status(H2ERI_p h) {
+    if (!h) return -1;
+    DevState *d = dstate_get(h);
+    if (hipDeviceSynchronize() != hipSuccess) { set_error(ERR_DEVICE, "H2ERI_device_status: device error"); return -1; }
+    int bad = 0;
+    for (DevPlan &p : d->plans) {
+        if (!p.q_ready) continue;
+        int e = 0;
+        if (hipMemcpy(&e, p.d_ctr + QC_ERROR, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error(ERR_DEVICE, "H2ERI_device_status: copy failed");
+            return -1;
+        }
+        if (e) {
+            bad = 1;
+            const int zero = 0;
+            if (hipMemcpy(p.d_ctr + QC_ERROR, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return -1;
+        }
+    }
+    if (bad) set_error(ERR_DEVICE, "persistent executor: a dependency wait exceeded its bound");
+    return bad;
+}
+
+// Queue statistics of the nv-vector plan (building it when needed):
+// out4 = {tasks, tasks started without a ticket, workgroups, list-scheduling
+// estimate of the pass in microseconds}.
+int H2ERI_queue_stats(H2ERI_p h, const int nv, double *out4) {
+    if (!h || !out4 || (nv != 1 && nv != 2 && nv != 4 && nv != 8)) { set_error(ERR_ARG, "H2ERI_queue_stats: bad argument"); return -1; }
+    DevPlan *dp = get_plan(h, nv);
+    if (!dp || !ensure_queue(dstate(h), *dp)) return -1;
+    out4[0] = dp->q_n;
+    out4[1] = dp->q_static;
+    out4[2] = dp->q_grid;
+    out4[3] = dp->q_est_us;
+    return 0;
+}
+
 // Tile size of the bi-GEMV tasks in bytes (tuning knob; re-plans).
 void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
     if (!h || bytes < 1024) return;
@@ -1632,7 +2118,7 @@ void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_
         if (!upload_storage(h)) { group_free(g); return nullptr; }
         DevState *d = dstate(h);
         DevPlan &dp = g->plans[k];
-        if (!build_plan(h, d->st, dp.P, 1, d->tile_bytes, d->nf_schedule)) { group_free(g); return nullptr; }
+        if (!build_plan(h, d->st, dp.P, 1, d->tile_bytes, d->nf_schedule, 0)) { group_free(g); return nullptr; }
         const Plan &P = dp.P;
         auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
             if (bytes == 0) bytes = 64;
@@ -1762,7 +2248,7 @@ int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double 
     S.build(h, ub_bits == 32);
     Plan P;
     DevState *ds = dstate_get(h);
-    if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule)) return -1;
+    if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, 0)) return -1;
     const long long N = h->krnl_mat_size;
     std::vector<double> Wv((size_t)P.nW * nv, std::nan(""));
     std::vector<int> wmark((size_t)P.nW, -1), rmark((size_t)P.nW, -1);
@@ -1858,6 +2344,118 @@ int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double 
     return conflicts;
 }
 
+// Verification of the persistent executor's queue on the host (no HIP call):
+// builds the plan and its queue for G = `workgroups`, executes the tasks in
+// ticket order with the kernels' semantics, and counts every read of a
+// workspace element whose writer is not a producer of the list the read goes
+// through (a dependency the counters would not wait for), every read of an
+// unwritten element, and every output element not written exactly once.
+// Returns that count, or -1 on failure.
+int H2ERI_queue_emulate(H2ERI_p h, const int ub_bits, const int nv, const int workgroups, const double *X, double *Y) {
+    if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !X || !Y || (nv != 1 && nv != 2 && nv != 4 && nv != 8) ||
+        workgroups < 1) {
+        set_error(ERR_ARG, "queue_emulate: bad args");
+        return -1;
+    }
+    Storage S;
+    S.build(h, ub_bits == 32);
+    Plan P;
+    DevState *ds = dstate_get(h);
+    if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, 1)) return -1;
+    QPlan Q;
+    if (!build_queue(P, nv, workgroups, Q)) return -1;
+    const long long N = h->krnl_mat_size;
+    std::vector<double> Wv((size_t)P.nW * nv, 0.0);
+    std::vector<int> writer((size_t)P.nW, -1), ymark((size_t)N, -1);
+    int conflicts = 0;
+    for (size_t k = 0; k < Q.tasks.size(); k++) {
+        const QTask &q = Q.tasks[k];
+        auto rd = [&](long long e, int list) -> double {
+            const int w = e >= 0 && e < P.nW ? writer[e] : -2;
+            if (w < 0) {
+                conflicts++;
+                return 0.0;
+            }
+            const QTask &wq = Q.tasks[w];
+            if (list < 0 || (wq.out0 != list && wq.out1 != list)) conflicts++;
+            return 1.0;
+        };
+        auto vin = [&](int b, int e, long long idx, int v, int list) -> double {
+            double s = 0.0;
+            for (int j = b; j < e; j++) {
+                const int src = P.vsrc[j];
+                if (src < 0) return X[(long long)v * N + (-src - 1) + idx];
+                rd(src + idx, list);
+                s += Wv[(size_t)(src + idx) * nv + v];
+            }
+            return s;
+        };
+        auto wr = [&](long long e) {
+            if (writer[e] >= 0) conflicts++;
+            writer[e] = (int)k;
+        };
+        if (q.kind < 2) {
+            const bool f32 = q.kind == 1;
+            const BTask &t = (f32 ? P.t32 : P.t64)[q.idx];
+            auto M = [&](int r, int c) -> double {
+                const long long e = t.mat + (long long)r * t.ld + c;
+                return f32 ? (double)S.h32[e] : S.h64[e];
+            };
+            std::vector<std::vector<double>> o0(nv), o1(nv);
+            for (int v = 0; v < nv; v++) {
+                std::vector<double> x0(t.ncol, 0.0), y1(t.nrow, 0.0);
+                if (t.out0 >= 0)
+                    for (int c = 0; c < t.ncol; c++) x0[c] = vin(t.v0b, t.v0e, t.v0off + c, v, q.in0);
+                if (t.out1 >= 0)
+                    for (int r = 0; r < t.nrow; r++) y1[r] = vin(t.v1b, t.v1e, t.v1off + r, v, q.in1 >= 0 ? q.in1 : q.in0);
+                if (t.out0 >= 0)
+                    for (int r = 0; r < t.nrow; r++) {
+                        double d = 0.0;
+                        for (int c = 0; c < t.ncol; c++) d += M(r, c) * x0[c];
+                        o0[v].push_back(d);
+                    }
+                if (t.out1 >= 0)
+                    for (int c = 0; c < t.ncol; c++) {
+                        double s = 0.0;
+                        for (int r = 0; r < t.nrow; r++) s += M(r, c) * y1[r];
+                        o1[v].push_back(s);
+                    }
+            }
+            if (t.out0 >= 0)
+                for (int r = 0; r < t.nrow; r++) {
+                    wr(t.out0 + r);
+                    for (int v = 0; v < nv; v++) Wv[(size_t)(t.out0 + r) * nv + v] = o0[v][r];
+                }
+            if (t.out1 >= 0)
+                for (int c = 0; c < t.ncol; c++) {
+                    wr(t.out1 + c);
+                    for (int v = 0; v < nv; v++) Wv[(size_t)(t.out1 + c) * nv + v] = o1[v][c];
+                }
+        } else {
+            const RTask &r = P.rt[q.idx];
+            for (int i = 0; i < r.len; i++) {
+                std::vector<double> s(nv, 0.0);
+                for (int j = r.src_beg; j < r.src_end; j++) {
+                    const long long e = P.rsrc[j] + r.soff + i;
+                    rd(e, q.in0);
+                    for (int v = 0; v < nv; v++) s[v] += Wv[(size_t)e * nv + v];
+                }
+                if (r.flags & 1) {
+                    if (ymark[r.dst + i] >= 0) conflicts++;
+                    ymark[r.dst + i] = (int)k;
+                    for (int v = 0; v < nv; v++) Y[(size_t)v * N + r.dst + i] = s[v];
+                } else {
+                    wr(r.dst + i);
+                    for (int v = 0; v < nv; v++) Wv[(size_t)(r.dst + i) * nv + v] = s[v];
+                }
+            }
+        }
+    }
+    for (long long i = 0; i < N; i++)
+        if (ymark[i] < 0) conflicts++;
+    return conflicts;
+}
+
 // Builds the device plans of every batch width on the host only and runs
 // their bounds checks; no HIP call is made.  Returns the number of tile
 // tasks of the single-vector plan, or -1 on failure.
@@ -1867,11 +2465,12 @@ int H2ERI_plan_check(H2ERI_p h, const int ub_bits) {
     S.build(h, ub_bits == 32);
     DevState *ds = dstate_get(h);
     int tasks = 0;
-    for (int nv : NV_VARIANTS) {
-        Plan P;
-        if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule)) return -1;
-        if (nv == 1) tasks = (int)(P.t64.size() + P.t32.size());
-    }
+    for (int nv : NV_VARIANTS)
+        for (int ex = 0; ex < 2; ex++) {
+            Plan P;
+            if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, ex)) return -1;
+            if (nv == 1 && ex == 0) tasks = (int)(P.t64.size() + P.t32.size());
+        }
     return tasks;
 }
 

@@ -206,6 +206,36 @@ class H2ERI:
         self.lib.H2ERI_set_schedule(self.h, mode)
         _check(self.lib)
 
+    def set_executor(self, mode):
+        """0: one launch per stage; 1: one persistent launch per pass."""
+        self.lib.H2ERI_set_executor(self.h, mode)
+        _check(self.lib)
+
+    def executor(self):
+        return self.lib.H2ERI_get_executor(self.h)
+
+    def device_status(self):
+        """Synchronises; raises if a persistent-executor wait exceeded its bound."""
+        r = self.lib.H2ERI_device_status(self.h)
+        _check(self.lib)
+        return r
+
+    def queue_stats(self, nv=1):
+        out = np.zeros(4)
+        self.lib.H2ERI_queue_stats(self.h, nv, _dptr(out))
+        _check(self.lib)
+        return dict(tasks=int(out[0]), static=int(out[1]), workgroups=int(out[2]), est_us=out[3])
+
+    def queue_emulate(self, x, bits=64, workgroups=1024):
+        """Host execution of the persistent executor's queue (tests only):
+        (y, uncovered reads + output errors)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        nv = 1 if x.ndim == 1 else x.shape[0]
+        y = np.zeros_like(x)
+        r = self.lib.H2ERI_queue_emulate(self.h, bits, nv, workgroups, _dptr(x), _dptr(y))
+        _check(self.lib)
+        return y, r
+
     def upload(self):
         r = self.lib.H2ERI_upload(self.h)
         _check(self.lib)
@@ -316,7 +346,7 @@ class H2ERI:
         n = self.lib.H2ERI_get_launch_stats(self.h, 0, None)
         out = np.zeros(5 * max(n, 1))
         n = self.lib.H2ERI_get_launch_stats(self.h, n, _dptr(out))
-        names = ["up", "B", "y1", "down", "D", "gather"]
+        names = ["up", "B", "y1", "down", "D", "gather", "pass"]
         return [dict(phase=names[int(out[5 * i])], blocks=int(out[5 * i + 2]), mat_bytes=out[5 * i + 3],
                      ms=out[5 * i + 4]) for i in range(n)]
 

@@ -113,6 +113,11 @@ SIGNATURES = {
     "H2ERI_group_destroy": (None, [C.c_void_p]),
     "H2ERI_plan_check": (C.c_int, [H2ERI_p, C.c_int]),
     "H2ERI_plan_emulate": (C.c_int, [H2ERI_p, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
+    "H2ERI_set_executor": (None, [H2ERI_p, C.c_int]),
+    "H2ERI_get_executor": (C.c_int, [H2ERI_p]),
+    "H2ERI_device_status": (C.c_int, [H2ERI_p]),
+    "H2ERI_queue_stats": (C.c_int, [H2ERI_p, C.c_int, c_dbl_p]),
+    "H2ERI_queue_emulate": (C.c_int, [H2ERI_p, C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
 }
 
 _lib = None

@@ -342,6 +342,27 @@ void H2ERI_set_tile_bytes(H2ERI_p h2eri, const double bytes);
  * launches. */
 void H2ERI_set_schedule(H2ERI_p h2eri, const int mode);
 
+/* Matvec executor (re-plans; no reference counterpart): 0 launches every
+ * stage of the plan separately, 1 runs the whole pass as one persistent
+ * launch (plus a small counter-reset launch) whose workgroups take the plan's
+ * tasks by ticket in a topological order and wait only for the producers of
+ * their own inputs.  Results of the two are bitwise equal.  With 1, phase
+ * timing sees one launch per pass: its time goes to the near-field slot of
+ * H2ERI_get_phase_ms and of the reference's timers. */
+void H2ERI_set_executor(H2ERI_p h2eri, const int mode);
+int  H2ERI_get_executor(H2ERI_p h2eri);
+
+/* Synchronises the device and reports whether a dependency wait of the
+ * persistent executor exceeded its bound (about a second) since the last
+ * call; such a pass's results are invalid.  0 clean, 1 with the error set,
+ * -1 on a device error.  Clears the flag. */
+int H2ERI_device_status(H2ERI_p h2eri);
+
+/* Queue of the nv-vector plan for the persistent executor (built when
+ * needed): out4 = {tasks, tasks started without a ticket, workgroups,
+ * list-scheduling estimate of the pass in microseconds}.  0, -1 on error. */
+int H2ERI_queue_stats(H2ERI_p h2eri, const int nv, double *out4);
+
 /* Molecule groups (uneven molecules in one launch per stage index):
  * H2ERI_group_create plans nh uploaded representations together for fixed
  * device vectors (y_k := A_k x_k, one vector each); H2ERI_group_matvec
@@ -369,6 +390,15 @@ int H2ERI_plan_check(H2ERI_p h2eri, const int ub_bits);
  * intra-launch read/write conflicts plus output elements not written exactly
  * once (0 expected), -1 on failure.  The hot-path entry points never call it. */
 int H2ERI_plan_emulate(H2ERI_p h2eri, const int ub_bits, const int nv, const double *X, double *Y);
+
+/* Host verification of the persistent executor's queue (no HIP call, tests
+ * only): builds the queue for `workgroups` resident workgroups, executes the
+ * tasks in ticket order with the kernels' semantics into Y, and returns the
+ * number of workspace reads not covered by the counters the reading task
+ * waits for, reads of unwritten elements and output elements not written
+ * exactly once (0 expected), -1 on failure. */
+int H2ERI_queue_emulate(H2ERI_p h2eri, const int ub_bits, const int nv, const int workgroups, const double *X,
+                        double *Y);
 
 /* sizeof(struct H2ERI) as compiled into the library (ABI check for
  * bindings that mirror the struct). */

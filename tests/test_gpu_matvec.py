@@ -149,3 +149,60 @@ def test_graph_replay_matches_eager(reps, nvec):
         h.matvec_multi_dev(nvec, Xd.data_ptr(), Ye.data_ptr(), s)
         torch.cuda.synchronize()
         assert torch.equal(Yd, Ye)
+
+
+@pytest.mark.parametrize("name", [c[0] for c in CASES])
+def test_persistent_executor_equals_staged(reps, name):
+    """One persistent launch per pass gives bitwise the staged executor's
+    results (FP64 and FP32-stored U/B, 1 to 15 vectors) and reports no
+    dependency wait past its bound."""
+    h, path = reps[name]
+    X = np.random.default_rng(30).standard_normal((15, h.n))
+    try:
+        for bits in (64, 32):
+            h.set_precision(bits)
+            h.set_executor(0)
+            Y0, y0 = h.matvec_multi(X), h.matvec(X[0])
+            h.set_executor(1)
+            Y1, y1 = h.matvec_multi(X), h.matvec(X[0])
+            assert h.device_status() == 0
+            assert np.array_equal(Y0, Y1)
+            assert np.array_equal(y0, y1)
+    finally:
+        h.set_executor(0)
+        h.set_precision(64)
+    o = oracle_bind.Oracle(path)
+    h.set_executor(1)
+    try:
+        assert relerr(h.matvec(X[1]), o.matvec(X[1])) < 1e-13
+    finally:
+        h.set_executor(0)
+
+
+@pytest.mark.parametrize("nvec", [1, 8])
+def test_persistent_executor_graph_replay(reps, nvec):
+    """The persistent pass captured once in a graph: every replay (new inputs
+    and eager work between replays) equals the staged eager call of a second
+    handle on the same representation, bitwise."""
+    import torch
+    import h2eri
+    h, path = reps["cube"]
+    ref = h2eri.H2ERI.load(path)   # staged executor
+    rng = np.random.default_rng(40 + nvec)
+    Xd = torch.empty((nvec, h.n), dtype=torch.float64, device="cuda:0")
+    Yd = torch.empty_like(Xd)
+    Ye = torch.empty_like(Xd)
+    s = torch.cuda.current_stream().cuda_stream
+    try:
+        h.set_executor(1)
+        h.graph_capture("matvec", nvec, Xd.data_ptr(), Yd.data_ptr())
+        for _ in range(3):
+            Xd.copy_(torch.from_numpy(rng.standard_normal((nvec, h.n))))
+            Yd.zero_()
+            h.graph_replay(s)
+            ref.matvec_multi_dev(nvec, Xd.data_ptr(), Ye.data_ptr(), s)
+            torch.cuda.synchronize()
+            assert torch.equal(Yd, Ye)
+        assert h.device_status() == 0
+    finally:
+        h.set_executor(0)

@@ -119,6 +119,34 @@ def test_device_plan_semantics_batched(tmp_path, nv, seed, n, leaf, mr, shape):
         assert relerr(Y[v], o.matvec(X[v])) < 1e-13
 
 
+@pytest.mark.parametrize("seed,n,leaf,mr,shape", [(1, 900, 30, 12, "chain"), (4, 1500, 40, 20, "cube"),
+                                                  (5, 40, 300, 8, "chain"), (6, 2000, 120, 600, "chain")])
+def test_queue_order_covers_every_dependency(tmp_path, seed, n, leaf, mr, shape):
+    """The persistent executor's queue, executed in ticket order, reads no
+    workspace element that is not produced by a task counted in the list it
+    reads through, and reproduces the oracle; for several workgroup counts
+    (which change the order), schedules and batch widths."""
+    import oracle_bind
+    from conftest import relerr
+    h = make_synthetic(tmp_path / "r.rep", n, seed, leaf=leaf, max_rank=mr, lr_frac=0.5, diag_lr_frac=0.4,
+                       shape=shape)
+    o = oracle_bind.Oracle(str(tmp_path / "r.rep"))
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(h.n)
+    yo = o.matvec(x)
+    for sched in (3, 0):
+        h.set_schedule(sched)
+        for wg in (1, 7, 1024):
+            y, bad = h.queue_emulate(x, 64, workgroups=wg)
+            assert bad == 0
+            assert relerr(y, yo) < 1e-13
+    X = rng.standard_normal((4, h.n))
+    Y, bad = h.queue_emulate(X, 32, workgroups=256)
+    assert bad == 0
+    for v in range(4):
+        assert relerr(Y[v], o.matvec(X[v])) < 2e-6
+
+
 def test_exchange_needs_molecule(tmp_path):
     """A synthetic (chemistry-free) representation has no shell pairs: the
     exchange build reports an error before touching the device."""
