@@ -75,11 +75,10 @@ struct RTask {
 static_assert(sizeof(RTask) == 32, "RTask layout");
 
 static const int WG = 256;       // threads per workgroup (4 waves)
-// Output elements per reduce task: 64 for the launch-per-stage executor (many
-// small tasks spread a launch over the chip), 1024 / NV for the persistent
-// executor (fewer tasks, fewer counter operations).  Both sum every element
-// in the same order.
-static inline int rchunk_for(int executor, int nv) { return executor == 1 ? 1024 / nv : 64; }
+// Output elements per reduce task: 1024 / NV values fill the task's 256
+// threads four deep.  Both executors use the same chunks, so their plans and
+// summation orders are identical.
+static inline int rchunk_for(int executor, int nv) { (void)executor; return 1024 / nv; }
 static const int LDS_DOUBLES = 4096;   // 32 KB of LDS per workgroup
 static const int XIN_DOUBLES = 1024;   // in0 staging (ncol * NV <= 1024)
 static const int YIN_DOUBLES = LDS_DOUBLES - XIN_DOUBLES;   // in1 staging (nrow * NV)
@@ -124,45 +123,39 @@ __device__ __forceinline__ void wst(double *p, double v) {
     else *p = v;
 }
 
-// Element idx of the virtual vector vsrc[b..e): NV values into out.
+// Stages n elements (NV values each) of the virtual vector vsrc[b..e) at
+// element offset `off` into dst[n * NV] (LDS).  From the workspace the
+// threads take values q = tid, tid + WG, ... (contiguous in its
+// NV-interleaved layout), so every thread works for any NV, and each sums
+// the slots in list order with eight loads in flight.  From the external
+// input the threads take elements and read the NV vectors.
 template <int NV, bool COH>
-__device__ __forceinline__ void vsum(const int *__restrict__ vsrc, int b, int e, int idx, const double *__restrict__ Wv,
-                                     const double *__restrict__ X, long long ldx, double (&out)[NV]) {
-    #pragma unroll
-    for (int v = 0; v < NV; v++) out[v] = 0.0;
+__device__ __forceinline__ void stage_vec(const int *__restrict__ vsrc, int b, int e, int off, int n,
+                                          const double *__restrict__ Wv, const double *__restrict__ X, long long ldx,
+                                          double *dst) {
     const int s0 = vsrc[b];
     if (s0 < 0) {
-        const double *p = X + (long long)(-s0 - 1 + idx);
-        #pragma unroll
-        for (int v = 0; v < NV; v++) out[v] = p[(long long)v * ldx];
+        const double *p = X + (long long)(-s0 - 1 + off);
+        for (int i = threadIdx.x; i < n; i += WG) {
+            #pragma unroll
+            for (int v = 0; v < NV; v++) dst[i * NV + v] = p[(long long)v * ldx + i];
+        }
         return;
     }
-    int j = b;
-    for (; j + 3 < e; j += 4) {
-        const double *p0 = Wv + (long long)(vsrc[j] + idx) * NV;
-        const double *p1 = Wv + (long long)(vsrc[j + 1] + idx) * NV;
-        const double *p2 = Wv + (long long)(vsrc[j + 2] + idx) * NV;
-        const double *p3 = Wv + (long long)(vsrc[j + 3] + idx) * NV;
-        double a0[NV], a1[NV], a2[NV], a3[NV];
-        #pragma unroll
-        for (int v = 0; v < NV; v++) {
-            a0[v] = wld<COH>(p0 + v);
-            a1[v] = wld<COH>(p1 + v);
-            a2[v] = wld<COH>(p2 + v);
-            a3[v] = wld<COH>(p3 + v);
+    const int nq = n * NV;
+    for (int q = threadIdx.x; q < nq; q += WG) {
+        const double *base = Wv + (long long)off * NV + q;
+        double sum = 0.0;
+        int j = b;
+        for (; j + 7 < e; j += 8) {
+            double a[8];
+            #pragma unroll
+            for (int u = 0; u < 8; u++) a[u] = wld<COH>(base + (long long)vsrc[j + u] * NV);
+            #pragma unroll
+            for (int u = 0; u < 8; u++) sum += a[u];
         }
-        #pragma unroll
-        for (int v = 0; v < NV; v++) {
-            out[v] += a0[v];
-            out[v] += a1[v];
-            out[v] += a2[v];
-            out[v] += a3[v];
-        }
-    }
-    for (; j < e; j++) {
-        const double *p = Wv + (long long)(vsrc[j] + idx) * NV;
-        #pragma unroll
-        for (int v = 0; v < NV; v++) out[v] += wld<COH>(p + v);
+        for (; j < e; j++) sum += wld<COH>(base + (long long)vsrc[j] * NV);
+        dst[q] = sum;
     }
 }
 
@@ -205,20 +198,8 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
     load_rows(nxt, grp);
 
     // stage the input vectors once per tile (sums of their slots)
-    if (do0)
-        for (int q = tid; q < C; q += WG) {
-            double tmp[NV];
-            vsum<NV, COH>(vsrc, t.v0b, t.v0e, t.v0off + q, Wv, X, ldx, tmp);
-            #pragma unroll
-            for (int v = 0; v < NV; v++) xin[q * NV + v] = tmp[v];
-        }
-    if (do1)
-        for (int q = tid; q < R; q += WG) {
-            double tmp[NV];
-            vsum<NV, COH>(vsrc, t.v1b, t.v1e, t.v1off + q, Wv, X, ldx, tmp);
-            #pragma unroll
-            for (int v = 0; v < NV; v++) yin[q * NV + v] = tmp[v];
-        }
+    if (do0) stage_vec<NV, COH>(vsrc, t.v0b, t.v0e, t.v0off, C, Wv, X, ldx, xin);
+    if (do1) stage_vec<NV, COH>(vsrc, t.v1b, t.v1e, t.v1off, R, Wv, X, ldx, yin);
     __syncthreads();
 
     double x0[KP][W][NV];
@@ -346,39 +327,62 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
     }
 }
 
-// Fixed-order sum of slots into one chunk (len * NV <= 1024 values).  Lane i
-// of a wave owns values i, i+64, ...; wave w sums the sources w, w+4, ... in
-// order; the four wave sums are added as (w0 + w1) + (w2 + w3).
+// Fixed-order sum of slots into one chunk (n = len * NV <= 1024 values).
+// K = 4, 2 or 1 thread groups (n <= 64, <= 128, larger) split the sources:
+// group g sums sources g, g + K, ... in order, each thread up to four values
+// with all their loads of four sources in flight; the group sums are added
+// as (g0 + g1) + (g2 + g3), or g0 + g1.
 template <int NV, bool COH>
 __device__ void reduce_task(const RTask &t, const int *__restrict__ src, double *__restrict__ Wv,
                             double *__restrict__ Y, long long ldy, double *lds) {
-    double *red = lds;   // 4 x 1024
-    const int w = threadIdx.x >> 6;
     const int n = t.len * NV;
-    for (int q = threadIdx.x & 63; q < n; q += 64) {
-        double s = 0.0;
-        const double *base = Wv + (long long)t.soff * NV + q;
-        int j = t.src_beg + w;
-        for (; j + 12 < t.src_end; j += 16) {
-            const double a0 = wld<COH>(base + (long long)src[j] * NV), a1 = wld<COH>(base + (long long)src[j + 4] * NV);
-            const double a2 = wld<COH>(base + (long long)src[j + 8] * NV), a3 = wld<COH>(base + (long long)src[j + 12] * NV);
-            s += a0;
-            s += a1;
-            s += a2;
-            s += a3;
+    const int K = n <= 64 ? 4 : (n <= 128 ? 2 : 1);
+    const int gsz = WG / K;
+    const int g = threadIdx.x / gsz, lt = threadIdx.x - g * gsz;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double *base = Wv + (long long)t.soff * NV + lt;
+    int j = t.src_beg + g;
+    for (; j + 3 * K < t.src_end; j += 4 * K) {
+        const long long o0 = (long long)src[j] * NV, o1 = (long long)src[j + K] * NV;
+        const long long o2 = (long long)src[j + 2 * K] * NV, o3 = (long long)src[j + 3 * K] * NV;
+        #pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (lt + k * gsz < n) {
+                const double *p = base + k * gsz;
+                const double a0 = wld<COH>(p + o0), a1 = wld<COH>(p + o1), a2 = wld<COH>(p + o2), a3 = wld<COH>(p + o3);
+                acc[k] += a0;
+                acc[k] += a1;
+                acc[k] += a2;
+                acc[k] += a3;
+            }
         }
-        for (; j < t.src_end; j += 4) s += wld<COH>(base + (long long)src[j] * NV);
-        red[w * 1024 + q] = s;
     }
+    for (; j < t.src_end; j += K) {
+        const long long o = (long long)src[j] * NV;
+        #pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (lt + k * gsz < n) acc[k] += wld<COH>(base + k * gsz + o);
+    }
+    auto put = [&](int q, double v) {
+        if (t.flags & 1) {
+            const int i = q / NV, w = q - i * NV;
+            Y[(long long)w * ldy + t.dst + i] = v;
+        } else {
+            wst<COH>(Wv + t.dst * NV + q, v);
+        }
+    };
+    if (K == 1) {
+        #pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (lt + k * WG < n) put(lt + k * WG, acc[k]);
+        return;
+    }
+    double *red = lds;   // K x 1024
+    if (lt < n) red[g * 1024 + lt] = acc[0];
     __syncthreads();
     for (int q = threadIdx.x; q < n; q += WG) {
-        const double s = (red[q] + red[1024 + q]) + (red[2048 + q] + red[3072 + q]);
-        if (t.flags & 1) {
-            const int i = q / NV, v = q - i * NV;
-            Y[(long long)v * ldy + t.dst + i] = s;
-        } else {
-            wst<COH>(Wv + t.dst * NV + q, s);
-        }
+        const double v = K == 4 ? (red[q] + red[1024 + q]) + (red[2048 + q] + red[3072 + q]) : red[q] + red[1024 + q];
+        put(q, v);
     }
 }
 
@@ -1200,7 +1204,7 @@ struct Planner {
         };
         auto chk = [&](const std::vector<BTask> &tl, bool f32) {
             const long long nstore = f32 ? (long long)S.h32.size() : (long long)S.h64.size();
-            const int W = !f32 ? 2 : (nv == 8 ? 2 : 4);
+            const int W = !f32 ? 2 : 4;   // widest vector load of any tile path (FP32: float4)
             for (const BTask &t : tl) {
                 bool ok = t.nrow > 0 && t.ncol > 0 && t.ncol <= maxc && t.ncol * nv <= XIN_DOUBLES && t.ld >= t.ncol &&
                           t.ld % W == 0 && t.mat % W == 0 && t.mat >= 0 && (t.lpr & (t.lpr - 1)) == 0 && t.lpr >= 4 &&
@@ -2016,12 +2020,12 @@ void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
     release_plans(d);
 }
 
-// Autotuner: times the device matvec of nv vectors for every pair of tile
-// size and near-field schedule in the candidate lists (HIP events around
-// `reps` back-to-back calls after one warm-up) and keeps the fastest pair.
-// out3 = {best ms per call, best tile bytes, best schedule}.  Returns 0, or
-// -1 with the error set.
-int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out3) {
+// Autotuner: times the device matvec of nv vectors for every combination of
+// executor, tile size and near-field schedule in the candidate lists (HIP
+// events around `reps` back-to-back calls after one warm-up) and keeps the
+// fastest.  out4 = {best ms per call, tile bytes, schedule, executor}.
+// Returns 0, or -1 with the error set.
+int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out4) {
     if (!h || (nv != 1 && nv != 2 && nv != 4 && nv != 8) || reps < 1) { set_error(ERR_ARG, "H2ERI_autotune: bad argument"); return -1; }
     if (!upload_storage(h)) return -1;
     DevState *d = dstate(h);
@@ -2035,13 +2039,15 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out3) {
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { set_error(ERR_DEVICE, "autotune: events"); return -1; }
     double best = 1e300, best_tile = d->tile_bytes;
-    int best_sched = d->nf_schedule;
+    int best_sched = d->nf_schedule, best_ex = d->executor;
     const bool was_timing = d->phase_timing;
     d->phase_timing = false;
     bool ok = true;
+    for (int ex = 0; ok && ex < 2; ex++)
     for (double tb : tiles) {
         for (int sc : scheds) {
             release_plans(d);
+            d->executor = ex;
             d->tile_bytes = tb;
             d->nf_schedule = sc;
             ok = matvec_multi_dev_impl(h, nv, d->d_x, d->d_y, nullptr);   // plans, uploads, warms up
@@ -2051,7 +2057,7 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out3) {
             float ms = 0.f;
             ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
             if (!ok) break;
-            if (ms / reps < best) { best = ms / reps; best_tile = tb; best_sched = sc; }
+            if (ms / reps < best) { best = ms / reps; best_tile = tb; best_sched = sc; best_ex = ex; }
         }
         if (!ok) break;
     }
@@ -2061,11 +2067,12 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out3) {
     release_plans(d);
     d->tile_bytes = best_tile;
     d->nf_schedule = best_sched;
+    d->executor = best_ex;
     if (!ok) {
         if (!has_error()) set_error(ERR_DEVICE, "autotune: timing failed");
         return -1;
     }
-    if (out3) { out3[0] = best; out3[1] = best_tile; out3[2] = best_sched; }
+    if (out4) { out4[0] = best; out4[1] = best_tile; out4[2] = best_sched; out4[3] = best_ex; }
     return 0;
 }
 

@@ -327,11 +327,12 @@ class H2ERI:
         return dict(terms=n, **dict(zip(["gemm_flops", "terms_f", "resident_bytes", "scratch_doubles"], out.tolist())))
 
     def autotune(self, nv=1, reps=10):
-        """Times tile sizes x near-field schedules on the device, keeps the best."""
-        out = np.zeros(3)
+        """Times executors x tile sizes x near-field schedules on the device,
+        keeps the best."""
+        out = np.zeros(4)
         self.lib.H2ERI_autotune(self.h, nv, reps, _dptr(out))
         _check(self.lib)
-        return dict(ms=out[0], tile_bytes=out[1], schedule=int(out[2]))
+        return dict(ms=out[0], tile_bytes=out[1], schedule=int(out[2]), executor=int(out[3]))
 
     def set_phase_timing(self, on=True):
         self.lib.H2ERI_set_phase_timing(self.h, 1 if on else 0)

@@ -374,10 +374,11 @@ int   H2ERI_group_matvec(void *group, void *stream);
 void  H2ERI_group_destroy(void *group);
 
 /* Autotuner: times nv-vector device matvecs (reps calls after a warm-up)
- * for each tile size in {64, 128, 256} KiB and near-field schedule in
- * {0, 2, 3, 4}, and keeps the fastest pair for later calls.  out3 = {ms per
- * call, tile bytes, schedule} of the pair kept.  Returns 0, -1 on error. */
-int H2ERI_autotune(H2ERI_p h2eri, const int nv, const int reps, double *out3);
+ * for every executor (H2ERI_set_executor) x tile size {64, 128, 256} KB x
+ * near-field schedule {0, 2, 3, 4}, and keeps the fastest combination for
+ * later calls.  out4 = {ms per call, tile bytes, schedule, executor} of the
+ * combination kept.  Returns 0, -1 on error. */
+int H2ERI_autotune(H2ERI_p h2eri, const int nv, const int reps, double *out4);
 
 /* Builds the device execution plan on the host only (no HIP call) and
  * checks every access of every launch against its buffer.  Returns the
