@@ -1581,8 +1581,6 @@ static const int QUEUE_SPIN_LIMIT = 1 << 22;
 
 template <int NV>
 static void launch_queue(const DevPlan &dp, DevState *d, const double *X, double *Y, long long ld, hipStream_t s) {
-    hipLaunchKernelGGL(queue_reset_kernel, dim3((std::max(dp.q_nlist, QC_LISTS) + 255) / 256), dim3(256), 0, s, dp.d_ctr,
-                       dp.q_nlist);
     hipLaunchKernelGGL(queue_kernel<NV>, dim3(dp.q_grid), dim3(WG), 0, s, dp.d_q, dp.q_n, dp.q_static, dp.d_need,
                        dp.d_ctr, QUEUE_SPIN_LIMIT, dp.d_t64, dp.d_t32, dp.d_rt, dp.d_vsrc, dp.d_rsrc, d->S64, d->S32,
                        dp.Wv, X, Y, ld);
@@ -1593,7 +1591,10 @@ static bool run_queue(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, doub
     if (!dp.q_ready) { set_error(ERR_DEVICE, "persistent executor: queue not built"); return false; }
     const long long ld = h->krnl_mat_size;
     const bool tim = d->phase_timing && !d->capturing;
-    if (tim) {
+    hipLaunchKernelGGL(queue_reset_kernel, dim3((std::max(dp.q_nlist, QC_LISTS) + 255) / 256), dim3(256), 0, s, dp.d_ctr,
+                       dp.q_nlist);
+    HIP_OK(hipGetLastError());
+    if (tim) {   // the event pair brackets the persistent launch only
         collect_phase_ms(h, d);
         if (!ensure_events(d, 2)) return false;
         d->launches.clear();
