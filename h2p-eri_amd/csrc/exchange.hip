@@ -120,13 +120,24 @@ __global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, c
     for (int ql = tk.pad_; ql < qend; ql++) {
         const int q0 = qptr[nj.q_ptr + ql], q1 = qptr[nj.q_ptr + ql + 1];
         d4 acc = d4{0.0, 0.0, 0.0, 0.0};
-        for (int e0 = q0; e0 < q1; e0 += 4) {
-            const int e = e0 + kl;
-            const bool ev = e < q1;
-            const int2 cp = ev ? qent[e] : make_int2(0, 0);
-            const double a = (ev && tv) ? R[(long long)cp.x * t.Rrs + (long long)ta * t.Rcs] : 0.0;
-            const double b = (ev && nv) ? Drow[cp.y] : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        // four MFMA steps per iteration, every index and operand load issued
+        // before the first of them (entries past the list give zero operands)
+        for (int e0 = q0; e0 < q1; e0 += 16) {
+            int2 cp[4];
+            #pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + 4 * u + kl;
+                cp[u] = e < q1 ? qent[e] : make_int2(-1, 0);
+            }
+            double a[4], b[4];
+            #pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const bool ev = cp[u].x >= 0;
+                a[u] = (ev && tv) ? R[(long long)cp[u].x * t.Rrs + (long long)ta * t.Rcs] : 0.0;
+                b[u] = (ev && nv) ? Drow[cp[u].y] : 0.0;
+            }
+            #pragma unroll
+            for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
         }
         #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -138,10 +149,11 @@ __global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, c
 
 // Bra on MFMA: K[m][q] += sum_n sum_{(r,m) of n} sum_t L[r][t] G[n][t][q].
 // For each n the rows r that pair with n form a GEMM (rows: those entries,
-// columns q, inner dimension t); an entry's local m is unique within one n,
-// so the 16 x 16 output tiles of one n add into an LDS block without
-// conflicts; a barrier separates successive n.  Task = (term, 16 functions q,
-// split of KSPLIT functions m); the block is added into K at the end.
+// columns q, inner dimension t).  Task = (term, 16 functions q, split of
+// KSPLIT functions m); wave w takes the functions n = w, w + 4, ... and adds
+// its 16 x 16 output tiles into an LDS block of the split with FP64 LDS
+// atomics (tiles of different n may share an m); the block is added into K
+// at the end.  The t loop issues the operands of four MFMA steps at once.
 __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
                                                const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
                                                const int *__restrict__ nspl, const int2 *__restrict__ nent,
@@ -162,30 +174,35 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
     const int ql = ql0 + (lane & 15);
     const bool qv = ql < nj.nbf;
     const int ns1 = ni.nsplit + 1;
-    for (int nl = 0; nl < ni.nbf; nl++) {
+    for (int nl = w; nl < ni.nbf; nl += KWG / 64) {
         const int eb = nspl[ni.s_ptr + nl * ns1 + split], ee = nspl[ni.s_ptr + nl * ns1 + split + 1];
-        for (int e0 = eb + 16 * w; e0 < ee; e0 += 64) {
+        const double *Gn = G + (size_t)nl * t.kk * nj.nbf + ql;
+        for (int e0 = eb; e0 < ee; e0 += 16) {
             const int e = e0 + (lane & 15);
             const bool ev = e < ee;
             const int row = ev ? nent[e].x : 0;
             const double *Lr = L + (long long)row * t.Lld;
-            const double *Gn = G + (size_t)nl * t.kk * nj.nbf + ql;
             d4 acc = d4{0.0, 0.0, 0.0, 0.0};
-            for (int t0 = 0; t0 < t.kk; t0 += 4) {
-                const int tt = t0 + kl;
-                const bool tv = tt < t.kk;
-                const double a = (ev && tv) ? Lr[tt] : 0.0;
-                const double b = (qv && tv) ? Gn[(size_t)tt * nj.nbf] : 0.0;
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            for (int t0 = 0; t0 < t.kk; t0 += 16) {
+                double a[4], b[4];
+                #pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int tt = t0 + 4 * u + kl;
+                    const bool tv = tt < t.kk;
+                    a[u] = (ev && tv) ? Lr[tt] : 0.0;
+                    b[u] = (qv && tv) ? Gn[(size_t)tt * nj.nbf] : 0.0;
+                }
+                #pragma unroll
+                for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
             }
             #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int er = e0 + kl + 4 * r;
-                if (er < ee && qv) Kb[(nent[er].y - ml0) * 16 + (lane & 15)] += acc[r];
+                if (er < ee && qv) atomicAdd(&Kb[(nent[er].y - ml0) * 16 + (lane & 15)], acc[r]);
             }
         }
-        __syncthreads();
     }
+    __syncthreads();
     double *Kout = t.diag ? Kd : K1;
     const int mlN = min(KSPLIT, ni.nbf - ml0);
     for (int i = threadIdx.x; i < mlN * 16; i += KWG) {
