@@ -62,7 +62,7 @@ struct KTerm {
     long long Loff;
     int Rsrc, Rrs, Rcs;   // R[c][t] = src[Roff + c*Rrs + t*Rcs]
     long long Roff;
-    long long Goff;       // G (|B(i)| x kk x |B(j)|) in scratch
+    long long Goff;       // G (|B(i)| x kk x gld(|B(j)|)) in scratch, q fastest
     int pad_[2];
 };
 
@@ -91,12 +91,18 @@ struct KNode {           // per row/column range: basis-function set and the ori
 };
 
 static const int KWG = 256;
+// Row length of G: |B(j)| rounded up to 4, so that four consecutive q of a
+// (n, t) row form one aligned 32-byte store.
+__host__ __device__ inline int gld(int nbf_j) { return (nbf_j + 3) & ~3; }
 static const int KQC = 16;      // column functions q per ket task
 static const int KSPLIT = 256;  // local m rows per bra task (LDS block KSPLIT x 16)
 
 // Ket on MFMA: G[n][t][q] = sum over the entries (c, p) of q of R[c][t] D[n][p].
 // Per q this is a GEMM (rows t, columns n, inner dimension the entries of q).
 // Task = (term, t tile of 16, 64 functions n (16 per wave), KQC functions q).
+// A wave computes four consecutive q together and stores each (n, t) row of
+// the four as one aligned 32-byte write (single 8-byte writes scattered over
+// 16 rows cost four times their bytes in HBM write traffic).
 __global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
                                                const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
                                                const int *__restrict__ qptr, const int2 *__restrict__ qent,
@@ -115,34 +121,42 @@ __global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, c
     const int ta = tk.t0 + (lane & 15);   // A operand row (t)
     const bool tv = ta < t.kk;
     const int kl = lane >> 4;
+    const int ldq = gld(nj.nbf);
     double *G = SC + t.Goff;
     const int qend = min(tk.pad_ + KQC, nj.nbf);
-    for (int ql = tk.pad_; ql < qend; ql++) {
-        const int q0 = qptr[nj.q_ptr + ql], q1 = qptr[nj.q_ptr + ql + 1];
-        d4 acc = d4{0.0, 0.0, 0.0, 0.0};
-        // four MFMA steps per iteration, every index and operand load issued
-        // before the first of them (entries past the list give zero operands)
-        for (int e0 = q0; e0 < q1; e0 += 16) {
-            int2 cp[4];
-            #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int e = e0 + 4 * u + kl;
-                cp[u] = e < q1 ? qent[e] : make_int2(-1, 0);
+    for (int qg = tk.pad_; qg < qend; qg += 4) {
+        d4 acc[4];
+        #pragma unroll
+        for (int j = 0; j < 4; j++) {
+            acc[j] = d4{0.0, 0.0, 0.0, 0.0};
+            const int ql = qg + j;
+            if (ql >= qend) continue;
+            const int q0 = qptr[nj.q_ptr + ql], q1 = qptr[nj.q_ptr + ql + 1];
+            // four MFMA steps per iteration, every index and operand load
+            // issued before the first of them (entries past the list: zeros)
+            for (int e0 = q0; e0 < q1; e0 += 16) {
+                int2 cp[4];
+                #pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int e = e0 + 4 * u + kl;
+                    cp[u] = e < q1 ? qent[e] : make_int2(-1, 0);
+                }
+                double a[4], b[4];
+                #pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const bool ev = cp[u].x >= 0;
+                    a[u] = (ev && tv) ? R[(long long)cp[u].x * t.Rrs + (long long)ta * t.Rcs] : 0.0;
+                    b[u] = (ev && nv) ? Drow[cp[u].y] : 0.0;
+                }
+                #pragma unroll
+                for (int u = 0; u < 4; u++) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc[j], 0, 0, 0);
             }
-            double a[4], b[4];
-            #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const bool ev = cp[u].x >= 0;
-                a[u] = (ev && tv) ? R[(long long)cp[u].x * t.Rrs + (long long)ta * t.Rcs] : 0.0;
-                b[u] = (ev && nv) ? Drow[cp[u].y] : 0.0;
-            }
-            #pragma unroll
-            for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
         }
         #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int tt = tk.t0 + kl + 4 * r;
-            if (tt < t.kk && nv) G[((size_t)nl * t.kk + tt) * nj.nbf + ql] = acc[r];
+            if (tt < t.kk && nv)
+                *reinterpret_cast<d4 *>(G + ((size_t)nl * t.kk + tt) * ldq + qg) = d4{acc[0][r], acc[1][r], acc[2][r], acc[3][r]};
         }
     }
 }
@@ -176,7 +190,7 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
     const int ns1 = ni.nsplit + 1;
     for (int nl = w; nl < ni.nbf; nl += KWG / 64) {
         const int eb = nspl[ni.s_ptr + nl * ns1 + split], ee = nspl[ni.s_ptr + nl * ns1 + split + 1];
-        const double *Gn = G + (size_t)nl * t.kk * nj.nbf + ql;
+        const double *Gn = G + (size_t)nl * t.kk * gld(nj.nbf) + ql;
         for (int e0 = eb; e0 < ee; e0 += 16) {
             const int e = e0 + (lane & 15);
             const bool ev = e < ee;
@@ -190,7 +204,7 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
                     const int tt = t0 + 4 * u + kl;
                     const bool tv = tt < t.kk;
                     a[u] = (ev && tv) ? Lr[tt] : 0.0;
-                    b[u] = (qv && tv) ? Gn[(size_t)tt * nj.nbf] : 0.0;
+                    b[u] = (qv && tv) ? Gn[(size_t)tt * gld(nj.nbf)] : 0.0;
                 }
                 #pragma unroll
                 for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
@@ -544,7 +558,7 @@ struct KBuilder {
             if (bas1) { t.Rsrc = KS_SC; t.Roff = gemm(pb_off[n1], k1, m1, k1, c.VT, k, k); t.Rrs = k; t.Rcs = 1; }
             else { t.Rsrc = KS_CB; t.Roff = c.VT; t.Rrs = k; t.Rcs = 1; }
         }
-        t.Goff = salloc((long long)t.kk * P.nodes[t.ni].nbf * P.nodes[t.nj].nbf);
+        t.Goff = salloc((long long)t.kk * P.nodes[t.ni].nbf * gld(P.nodes[t.nj].nbf));
         P.terms.push_back(t);
     }
 
@@ -624,7 +638,7 @@ struct KBuilder {
             if (!t.direct) {
                 const long long last = t.Roff + (long long)(t.m1 - 1) * t.Rrs + (long long)(t.kk - 1) * t.Rcs;
                 ok = ok && t.Roff >= 0 && last < size_of(t.Rsrc) && t.Goff >= 0 &&
-                     t.Goff + (long long)t.kk * ni.nbf * nj.nbf <= nSC;
+                     t.Goff % 4 == 0 && t.Goff + (long long)t.kk * ni.nbf * gld(nj.nbf) <= nSC;
             }
             if (!ok) { set_error(ERR_ARG, "exchange plan check failed: term"); return false; }
         }
