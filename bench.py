@@ -124,8 +124,8 @@ def build_molecule(h2eri, M, atoms, reltol, label):
 def batching_config(args):
     """Batching workload: 16 densities on C100H202 per graph replay (passes of
     8 densities over the matrix data, tile shape from the autotuner), and 8
-    alkanes C20..C160 of uneven size as one molecule group (one launch per
-    stage index), replayed 20 times."""
+    alkanes C20..C160 of uneven size as one molecule group (one persistent
+    launch over all their tasks), replayed 20 times."""
     import numpy as np
     import torch
 
@@ -194,6 +194,7 @@ def batching_config(args):
     e1.record(stream)
     torch.cuda.synchronize()
     ms_group = e0.elapsed_time(e1) / 20
+    g.status()   # raises if a dependency wait of the persistent launch exceeded its bound
     Yg = [y.cpu().numpy() for y in Y]
     e0.record(stream)
     for _ in range(20):

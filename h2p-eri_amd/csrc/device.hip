@@ -409,8 +409,7 @@ __global__ __launch_bounds__(WG) void stage_kernel(const BTask *__restrict__ t64
     }
 }
 
-// Several molecules in one launch: every task names its molecule, whose
-// storage, plan arrays, workspace and vectors come from a pointer table.
+// Pointers of one representation, for launches over several molecules.
 struct MolPtrs {
     const double *S64;
     const float *S32;
@@ -422,28 +421,6 @@ struct MolPtrs {
     double *Y;
     long long ldx;
 };
-struct GTask {
-    int mol, kind, idx, pad_;   // kind 0: FP64 tile, 1: FP32 tile, 2: reduce task
-};
-
-template <int NV>
-__global__ __launch_bounds__(WG) void stage_kernel_grouped(const GTask *__restrict__ tasks,
-                                                           const MolPtrs *__restrict__ mols) {
-    __shared__ double lds[LDS_DOUBLES];
-    const GTask g = tasks[blockIdx.x];
-    const MolPtrs m = mols[g.mol];
-    if (g.kind == 0) {
-        const BTask t = m.t64[g.idx];
-        bgemv_tile<double, NV, false>(t, m.S64, m.vsrc, m.Wv, m.X, m.ldx, lds);
-    } else if (g.kind == 1) {
-        const BTask t = m.t32[g.idx];
-        bgemv_tile<float, NV, false>(t, m.S32, m.vsrc, m.Wv, m.X, m.ldx, lds);
-    } else {
-        const RTask t = m.rt[g.idx];
-        reduce_task<NV, false>(t, m.rsrc, m.Wv, m.Y, m.ldx, lds);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Persistent executor: the whole pass in one launch.
 // ---------------------------------------------------------------------------
@@ -493,15 +470,17 @@ __device__ __forceinline__ void q_wait(int *ctr, int list, int need, int spin_li
 // included on gfx9) has completed.
 __device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-template <int NV>
+// GROUP = false: one representation, pointers from the arguments; GROUP =
+// true: QTask::pad_[0] names the molecule whose pointers come from mols.
+template <int NV, bool GROUP>
 __global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ tasks, int ntask, int nstatic,
                                                       const int *__restrict__ need, int *ctr, int spin_limit,
-                                                      const BTask *__restrict__ t64, const BTask *__restrict__ t32,
-                                                      const RTask *__restrict__ rt, const int *__restrict__ vsrc,
-                                                      const int *__restrict__ rsrc, const double *__restrict__ S64,
-                                                      const float *__restrict__ S32, double *__restrict__ Wv,
-                                                      const double *__restrict__ X, double *__restrict__ Y,
-                                                      long long ldx) {
+                                                      const MolPtrs *__restrict__ mols, const BTask *__restrict__ t64,
+                                                      const BTask *__restrict__ t32, const RTask *__restrict__ rt,
+                                                      const int *__restrict__ vsrc, const int *__restrict__ rsrc,
+                                                      const double *__restrict__ S64, const float *__restrict__ S32,
+                                                      double *__restrict__ Wv, const double *__restrict__ X,
+                                                      double *__restrict__ Y, long long ldx) {
     __shared__ double lds[LDS_DOUBLES];
     __shared__ int s_next[2];
     int t0, t1;
@@ -524,15 +503,15 @@ __global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ 
                 if (t == t0) s_next[par] = __hip_atomic_fetch_add(ctr + QC_TICKET, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             __syncthreads();
-            if (q.kind == 0) {
-                const BTask b = t64[q.idx];
-                bgemv_tile<double, NV, true>(b, S64, vsrc, Wv, X, ldx, lds);
-            } else if (q.kind == 1) {
-                const BTask b = t32[q.idx];
-                bgemv_tile<float, NV, true>(b, S32, vsrc, Wv, X, ldx, lds);
+            if constexpr (GROUP) {
+                const MolPtrs m = mols[q.pad_[0]];
+                if (q.kind == 0) bgemv_tile<double, NV, true>(m.t64[q.idx], m.S64, m.vsrc, m.Wv, m.X, m.ldx, lds);
+                else if (q.kind == 1) bgemv_tile<float, NV, true>(m.t32[q.idx], m.S32, m.vsrc, m.Wv, m.X, m.ldx, lds);
+                else reduce_task<NV, true>(m.rt[q.idx], m.rsrc, m.Wv, m.Y, m.ldx, lds);
             } else {
-                const RTask r = rt[q.idx];
-                reduce_task<NV, true>(r, rsrc, Wv, Y, ldx, lds);
+                if (q.kind == 0) bgemv_tile<double, NV, true>(t64[q.idx], S64, vsrc, Wv, X, ldx, lds);
+                else if (q.kind == 1) bgemv_tile<float, NV, true>(t32[q.idx], S32, vsrc, Wv, X, ldx, lds);
+                else reduce_task<NV, true>(rt[q.idx], rsrc, Wv, Y, ldx, lds);
             }
             wait_vm_stores();
             __syncthreads();
@@ -1260,15 +1239,19 @@ struct QPlan {
     double est_us = 0;          // list-scheduling estimate of the pass
 };
 
-// Builds the queue of plan P for G resident workgroups.  Lists are the input
-// vectors of the tiles (vsrc ranges) and the source lists of the reduce tasks
-// (rsrc ranges); a workspace slot belongs to at most one list, and a task
-// counts itself into the list of every slot it writes.  The order comes from
-// list scheduling on G workgroups with a bytes-proportional duration per
-// task, highest remaining path first, so the sweeps' critical path runs as
-// soon as it can and the near field fills the gaps.
-static bool build_queue(const Plan &P, int nv, int G, QPlan &Q) {
-    Q = QPlan();
+// Dependency graph of one plan's tasks for the persistent executor.  Lists
+// are the input vectors of the tiles (vsrc ranges) and the source lists of
+// the reduce tasks (rsrc ranges); a workspace slot belongs to at most one
+// list, and a task counts itself into the list of every slot it writes.
+struct QGraph {
+    std::vector<QTask> q;                  // plan order; list ids local to the plan
+    std::vector<int> nprod;                // producers per list
+    std::vector<std::vector<int>> cons;    // consumers per list
+    std::vector<double> dur;               // modelled duration per task
+};
+
+static bool queue_graph(const Plan &P, int nv, int G, QGraph &Gr) {
+    Gr = QGraph();
     struct TRef { int kind, idx; };
     std::vector<TRef> T;
     for (const Stage &st : P.stages) {
@@ -1380,6 +1363,41 @@ static bool build_queue(const Plan &P, int nv, int G, QPlan &Q) {
         }
         dur[i] = c0 + bytes / bw;
     }
+    Gr.q = std::move(q);
+    Gr.nprod = std::move(nprod);
+    Gr.cons = std::move(cons);
+    Gr.dur = std::move(dur);
+    return true;
+}
+
+// Orders the tasks of several graphs (disjoint lists: one per molecule of a
+// group, or one) by list scheduling on G workgroups with the modelled
+// durations, longest remaining path first, so the sweeps' critical paths run
+// as soon as they can and the near field fills the gaps.  List ids become
+// global (offset per graph) and QTask::pad_[0] names the graph.
+static bool queue_schedule(const std::vector<QGraph> &gs, int G, QPlan &Q) {
+    Q = QPlan();
+    std::vector<QTask> q;
+    std::vector<int> nprod;
+    std::vector<std::vector<int>> cons;
+    std::vector<double> dur;
+    for (size_t gi = 0; gi < gs.size(); gi++) {
+        const QGraph &g = gs[gi];
+        const int tb = (int)q.size(), lb = (int)nprod.size();
+        for (QTask t : g.q) {
+            for (int *L : {&t.in0, &t.in1, &t.out0, &t.out1})
+                if (*L >= 0) *L += lb;
+            t.pad_[0] = (int)gi;
+            q.push_back(t);
+        }
+        nprod.insert(nprod.end(), g.nprod.begin(), g.nprod.end());
+        for (const std::vector<int> &c : g.cons) {
+            cons.emplace_back();
+            for (int i : c) cons.back().push_back(tb + i);
+        }
+        dur.insert(dur.end(), g.dur.begin(), g.dur.end());
+    }
+    const int n = (int)q.size(), nlist = (int)nprod.size();
     // topological order (Kahn), then the longest remaining path of every task
     std::vector<int> rem(n, 0), cnt(nlist, 0), topo;
     topo.reserve(n);
@@ -1450,6 +1468,12 @@ static bool build_queue(const Plan &P, int nv, int G, QPlan &Q) {
     return true;
 }
 
+static bool build_queue(const Plan &P, int nv, int G, QPlan &Q) {
+    std::vector<QGraph> gs(1);
+    if (!queue_graph(P, nv, G, gs[0])) return false;
+    return queue_schedule(gs, G, Q);
+}
+
 static bool ensure_events(DevState *d, size_t n) {
     while (d->evs.size() < n) {
         hipEvent_t e;
@@ -1503,10 +1527,10 @@ static int queue_grid(int nv) {
         return 0;
     hipError_t e;
     switch (nv) {
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<1>, WG, 0); break;
-    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<2>, WG, 0); break;
-    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<4>, WG, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<8>, WG, 0); break;
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<1, false>, WG, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<2, false>, WG, 0); break;
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<4, false>, WG, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<8, false>, WG, 0); break;
     }
     return e == hipSuccess ? ncu * std::max(per, 1) : 0;
 }
@@ -1581,9 +1605,9 @@ static const int QUEUE_SPIN_LIMIT = 1 << 22;
 
 template <int NV>
 static void launch_queue(const DevPlan &dp, DevState *d, const double *X, double *Y, long long ld, hipStream_t s) {
-    hipLaunchKernelGGL(queue_kernel<NV>, dim3(dp.q_grid), dim3(WG), 0, s, dp.d_q, dp.q_n, dp.q_static, dp.d_need,
-                       dp.d_ctr, QUEUE_SPIN_LIMIT, dp.d_t64, dp.d_t32, dp.d_rt, dp.d_vsrc, dp.d_rsrc, d->S64, d->S32,
-                       dp.Wv, X, Y, ld);
+    hipLaunchKernelGGL((queue_kernel<NV, false>), dim3(dp.q_grid), dim3(WG), 0, s, dp.d_q, dp.q_n, dp.q_static,
+                       dp.d_need, dp.d_ctr, QUEUE_SPIN_LIMIT, (const MolPtrs *)nullptr, dp.d_t64, dp.d_t32, dp.d_rt,
+                       dp.d_vsrc, dp.d_rsrc, d->S64, d->S32, dp.Wv, X, Y, ld);
 }
 
 // One pass of the persistent executor on stream s (reset + queue launch).
@@ -2078,18 +2102,16 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out4) {
 }
 
 // ---------------------------------------------------------------------------
-// Molecule groups: one launch per stage index for several representations
+// Molecule groups: one persistent launch over the tasks of several representations
 // ---------------------------------------------------------------------------
 struct H2EGroup {
     std::vector<H2ERI_p> hs;
-    std::vector<DevPlan> plans;          // one plan per molecule, the handle's near-field schedule
-    std::vector<int> launch_beg;          // GTask ranges: main-stream launches, then near-field launches
-    GTask *d_tasks = nullptr;
+    std::vector<DevPlan> plans;   // per molecule: workspace and task arrays (one vector)
     MolPtrs *d_mols = nullptr;
-    int nmain = 0, nside = 0;
-    size_t ntask = 0;
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    QTask *d_q = nullptr;
+    int *d_need = nullptr, *d_ctr = nullptr;
+    int q_n = 0, q_static = 0, q_nlist = 0, q_grid = 0;
+    double q_est_us = 0;
 };
 
 static void group_free(H2EGroup *g) {
@@ -2098,35 +2120,41 @@ static void group_free(H2EGroup *g) {
                        (void **)&p.d_vsrc};
         for (void **q : ps) free_dev(*q);
     }
-    free_dev((void *&)g->d_tasks);
-    free_dev((void *&)g->d_mols);
-    if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
-    if (g->ev_join) (void)hipEventDestroy(g->ev_join);
-    if (g->side) (void)hipStreamDestroy(g->side);
+    void **ps[] = {(void **)&g->d_mols, (void **)&g->d_q, (void **)&g->d_need, (void **)&g->d_ctr};
+    for (void **q : ps) free_dev(*q);
     delete g;
 }
 
 // Groups nh representations (single vector each) for H2ERI_group_matvec: y_k
 // := A_k x_k for k < nh with fixed device buffers X_dev[k], Y_dev[k].  The
-// main-stream stages of all molecules are right-aligned, so that every
-// molecule's gather falls in the last main launch: launch l runs stage
-// l - (L - L_k) of molecule k.  Their near-field stages run left-aligned on a
-// second stream joined before that last launch.  Molecules of uneven size so
-// fill the device together.  The handles must outlive the group and keep
-// their precision; returns NULL with the error set on failure.
+// tasks of all the molecules' plans form one persistent queue: list
+// scheduling orders them jointly on the resident workgroups, longest
+// remaining path first, so the small molecules' work fills the gaps of the
+// big ones' critical paths and one launch (plus the counter reset) runs the
+// whole group.  The handles must outlive the group and keep their precision;
+// returns NULL with the error set on failure.
 void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_dev, double *const *Y_dev) {
     if (nh < 1 || !handles || !X_dev || !Y_dev) { set_error(ERR_ARG, "H2ERI_group_create: bad argument"); return nullptr; }
     H2EGroup *g = new H2EGroup();
     g->plans.resize(nh);
     std::vector<MolPtrs> mols(nh);
-    std::vector<std::vector<GTask>> per_launch;
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<1, true>, WG, 0) != hipSuccess) {
+        set_error(ERR_DEVICE, "H2ERI_group_create: occupancy query failed");
+        delete g;
+        return nullptr;
+    }
+    const int G = ncu * std::max(per, 1);
+    std::vector<QGraph> graphs(nh);
     for (int k = 0; k < nh; k++) {
         H2ERI_p h = handles[k];
         if (!h || !X_dev[k] || !Y_dev[k]) { set_error(ERR_ARG, "H2ERI_group_create: null handle or buffer %d", k); group_free(g); return nullptr; }
         if (!upload_storage(h)) { group_free(g); return nullptr; }
         DevState *d = dstate(h);
         DevPlan &dp = g->plans[k];
-        if (!build_plan(h, d->st, dp.P, 1, d->tile_bytes, d->nf_schedule, 0)) { group_free(g); return nullptr; }
+        if (!build_plan(h, d->st, dp.P, 1, d->tile_bytes, d->nf_schedule, 1)) { group_free(g); return nullptr; }
+        if (!queue_graph(dp.P, 1, G, graphs[k])) { group_free(g); return nullptr; }
         const Plan &P = dp.P;
         auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
             if (bytes == 0) bytes = 64;
@@ -2146,60 +2174,27 @@ void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_
                           (long long)h->krnl_mat_size};
         g->hs.push_back(h);
     }
-    // main-stream stages right-aligned, near-field stages left-aligned
-    int L = 0, Ls = 0;
-    for (const DevPlan &dp : g->plans) {
-        int m = 0, sd = 0;
-        for (const Stage &st : dp.P.stages) (st.stream == 0 ? m : sd)++;
-        L = std::max(L, m);
-        Ls = std::max(Ls, sd);
-    }
-    std::vector<std::vector<GTask>> mainL(L), sideL(Ls);
-    for (int k = 0; k < nh; k++) {
-        const Plan &P = g->plans[k].P;
-        int m = 0;
-        for (const Stage &st : P.stages) m += st.stream == 0;
-        int li = L - m, si = 0;
-        for (const Stage &st : P.stages) {
-            std::vector<GTask> &dst = st.stream == 0 ? mainL[li++] : sideL[si++];
-            for (int i = 0; i < st.t64_cnt; i++) dst.push_back(GTask{k, 0, st.t64_beg + i, 0});
-            for (int i = 0; i < st.t32_cnt; i++) dst.push_back(GTask{k, 1, st.t32_beg + i, 0});
-            for (int i = 0; i < st.r_cnt; i++) dst.push_back(GTask{k, 2, st.r_beg + i, 0});
-        }
-    }
-    // within a launch: largest tasks first across all molecules (reduce tasks last)
-    auto weight = [&](const GTask &t) -> long long {
-        const Plan &P = g->plans[t.mol].P;
-        if (t.kind == 2) return 0;
-        const BTask &b = t.kind == 0 ? P.t64[t.idx] : P.t32[t.idx];
-        return (long long)b.nrow * b.ncol * (t.kind == 0 ? 2 : 1);
-    };
-    std::vector<GTask> all;
-    for (auto *lists : {&mainL, &sideL})
-        for (auto &v : *lists) {
-            std::stable_sort(v.begin(), v.end(), [&](const GTask &a, const GTask &b) { return weight(a) > weight(b); });
-            g->launch_beg.push_back((int)all.size());
-            all.insert(all.end(), v.begin(), v.end());
-        }
-    g->launch_beg.push_back((int)all.size());
-    g->nmain = L;
-    g->nside = Ls;
-    g->ntask = all.size();
-    if (Ls > 0 && (hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
-                   hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming) != hipSuccess ||
-                   hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming) != hipSuccess)) {
-        set_error(ERR_DEVICE, "H2ERI_group_create: stream setup failed");
-        group_free(g);
-        return nullptr;
-    }
-    if (hipMalloc(&g->d_tasks, sizeof(GTask) * std::max<size_t>(all.size(), 1)) != hipSuccess ||
-        hipMalloc(&g->d_mols, sizeof(MolPtrs) * nh) != hipSuccess ||
-        (!all.empty() && hipMemcpy(g->d_tasks, all.data(), sizeof(GTask) * all.size(), hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemcpy(g->d_mols, mols.data(), sizeof(MolPtrs) * nh, hipMemcpyHostToDevice) != hipSuccess) {
+    QPlan Q;
+    if (!queue_schedule(graphs, G, Q)) { group_free(g); return nullptr; }
+    const size_t nq = std::max<size_t>(Q.tasks.size(), 1), nl = Q.need.size();
+    bool ok = hipMalloc(&g->d_mols, sizeof(MolPtrs) * nh) == hipSuccess &&
+              hipMemcpy(g->d_mols, mols.data(), sizeof(MolPtrs) * nh, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMalloc(&g->d_q, nq * sizeof(QTask)) == hipSuccess &&
+              (Q.tasks.empty() || hipMemcpy(g->d_q, Q.tasks.data(), Q.tasks.size() * sizeof(QTask), hipMemcpyHostToDevice) == hipSuccess) &&
+              hipMalloc(&g->d_need, std::max<size_t>(nl, 1) * sizeof(int)) == hipSuccess &&
+              (nl == 0 || hipMemcpy(g->d_need, Q.need.data(), nl * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
+              hipMalloc(&g->d_ctr, (QC_LISTS + nl) * sizeof(int)) == hipSuccess &&
+              hipMemset(g->d_ctr, 0, (QC_LISTS + nl) * sizeof(int)) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+    if (!ok) {
         set_error(ERR_DEVICE, "H2ERI_group_create: upload failed");
         group_free(g);
         return nullptr;
     }
+    g->q_n = (int)Q.tasks.size();
+    g->q_static = Q.nstatic;
+    g->q_nlist = (int)nl;
+    g->q_grid = std::min(G, std::max(g->q_n, 1));
+    g->q_est_us = Q.est_us;
     return g;
 }
 
@@ -2208,30 +2203,32 @@ int H2ERI_group_matvec(void *group, void *stream) {
     H2EGroup *g = (H2EGroup *)group;
     if (!g) { set_error(ERR_ARG, "H2ERI_group_matvec: null group"); return -1; }
     hipStream_t s = (hipStream_t)stream;
-    auto launch = [&](int l, hipStream_t q) -> bool {
-        const int nb = g->launch_beg[l + 1] - g->launch_beg[l];
-        if (nb == 0) return true;
-        hipLaunchKernelGGL(stage_kernel_grouped<1>, dim3(nb), dim3(WG), 0, q, g->d_tasks + g->launch_beg[l], g->d_mols);
-        if (hipGetLastError() != hipSuccess) { set_error(ERR_DEVICE, "H2ERI_group_matvec: launch failed"); return false; }
-        return true;
-    };
-    if (g->nside > 0) {
-        if (hipEventRecord(g->ev_fork, s) != hipSuccess || hipStreamWaitEvent(g->side, g->ev_fork, 0) != hipSuccess) {
-            set_error(ERR_DEVICE, "H2ERI_group_matvec: fork failed");
-            return -1;
-        }
-        for (int l = 0; l < g->nside; l++)
-            if (!launch(g->nmain + l, g->side)) return -1;
-        if (hipEventRecord(g->ev_join, g->side) != hipSuccess) { set_error(ERR_DEVICE, "H2ERI_group_matvec: join failed"); return -1; }
-    }
-    for (int l = 0; l < g->nmain; l++) {
-        if (g->nside > 0 && l == g->nmain - 1 && hipStreamWaitEvent(s, g->ev_join, 0) != hipSuccess) {
-            set_error(ERR_DEVICE, "H2ERI_group_matvec: join failed");
-            return -1;
-        }
-        if (!launch(l, s)) return -1;
-    }
+    hipLaunchKernelGGL(queue_reset_kernel, dim3((std::max(g->q_nlist, QC_LISTS) + 255) / 256), dim3(256), 0, s, g->d_ctr,
+                       g->q_nlist);
+    hipLaunchKernelGGL((queue_kernel<1, true>), dim3(g->q_grid), dim3(WG), 0, s, g->d_q, g->q_n, g->q_static, g->d_need,
+                       g->d_ctr, QUEUE_SPIN_LIMIT, g->d_mols, (const BTask *)nullptr, (const BTask *)nullptr,
+                       (const RTask *)nullptr, (const int *)nullptr, (const int *)nullptr, (const double *)nullptr,
+                       (const float *)nullptr, (double *)nullptr, (const double *)nullptr, (double *)nullptr, 0LL);
+    if (hipGetLastError() != hipSuccess) { set_error(ERR_DEVICE, "H2ERI_group_matvec: launch failed"); return -1; }
     return 0;
+}
+
+// Synchronises and reports whether a dependency wait of the group's
+// persistent launch exceeded its bound since the last call (0 clean, 1 with
+// the error set, -1 on a device error); clears the flag.
+int H2ERI_group_status(void *group) {
+    H2EGroup *g = (H2EGroup *)group;
+    if (!g) { set_error(ERR_ARG, "H2ERI_group_status: null group"); return -1; }
+    int e = 0;
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(&e, g->d_ctr + QC_ERROR, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error(ERR_DEVICE, "H2ERI_group_status: device error");
+        return -1;
+    }
+    if (!e) return 0;
+    const int zero = 0;
+    if (hipMemcpy(g->d_ctr + QC_ERROR, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    set_error(ERR_DEVICE, "group: a dependency wait exceeded its bound");
+    return 1;
 }
 
 void H2ERI_group_destroy(void *group) {

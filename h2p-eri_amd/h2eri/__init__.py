@@ -46,8 +46,8 @@ def _darr(p, n):
 
 
 class Group:
-    """Several representations whose matvecs run together, one launch per
-    stage index (H2ERI_group_create / _matvec / _destroy)."""
+    """Several representations whose matvecs run together in one persistent
+    launch (H2ERI_group_create / _matvec / _status / _destroy)."""
 
     def __init__(self, handles, x_ptrs, y_ptrs):
         self.lib = capi.load()
@@ -62,6 +62,12 @@ class Group:
     def matvec(self, stream=None):
         self.lib.H2ERI_group_matvec(self.g, C.c_void_p(stream or 0))
         _check(self.lib)
+
+    def status(self):
+        """Synchronises; raises if a dependency wait exceeded its bound."""
+        r = self.lib.H2ERI_group_status(self.g)
+        _check(self.lib)
+        return r
 
     def close(self):
         if self.g:

@@ -363,14 +363,18 @@ int H2ERI_device_status(H2ERI_p h2eri);
  * list-scheduling estimate of the pass in microseconds}.  0, -1 on error. */
 int H2ERI_queue_stats(H2ERI_p h2eri, const int nv, double *out4);
 
-/* Molecule groups (uneven molecules in one launch per stage index):
- * H2ERI_group_create plans nh uploaded representations together for fixed
- * device vectors (y_k := A_k x_k, one vector each); H2ERI_group_matvec
- * enqueues one matvec of every molecule on `stream` (stage l of all molecules
- * in launch l; graph-capturable); H2ERI_group_destroy frees the group.  The
- * handles must outlive the group and keep their precision. */
+/* Molecule groups (uneven molecules in one persistent launch; no reference
+ * counterpart): H2ERI_group_create plans nh uploaded representations together
+ * for fixed device vectors (y_k := A_k x_k, one vector each) and orders all
+ * their tasks in one queue; H2ERI_group_matvec enqueues one matvec of every
+ * molecule on `stream` (a counter reset and one persistent launch;
+ * graph-capturable); H2ERI_group_status synchronises and reports a dependency
+ * wait past its bound (0 clean, 1 error set, -1 device error);
+ * H2ERI_group_destroy frees the group.  The handles must outlive the group
+ * and keep their precision. */
 void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_dev, double *const *Y_dev);
 int   H2ERI_group_matvec(void *group, void *stream);
+int   H2ERI_group_status(void *group);
 void  H2ERI_group_destroy(void *group);
 
 /* Autotuner: times nv-vector device matvecs (reps calls after a warm-up)

@@ -94,7 +94,8 @@ def test_uneven_molecules(tmp_path_factory):
 
 
 def test_molecule_group(tmp_path_factory):
-    """Uneven molecules in one launch per stage index (H2ERI_group_*)."""
+    """Uneven molecules in one persistent launch (H2ERI_group_*): each equals
+    the oracle, and bitwise the molecule's own matvec (same plan, same sums)."""
     import torch
     import h2eri
     from h2eri import molecules as M
@@ -114,8 +115,10 @@ def test_molecule_group(tmp_path_factory):
             y.zero_()
         g.matvec(torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
+        assert g.status() == 0
         for k in range(3):
             assert relerr(Y[k].cpu().numpy(), oracle_bind.Oracle(paths[k]).matvec(xs[k])) < 1e-13
+            assert np.array_equal(Y[k].cpu().numpy(), hs[k].matvec(xs[k]))
     g.close()
 
 
