@@ -230,7 +230,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batches", default="1,2,4,8", help="densities per pass to time")
-    ap.add_argument("--tuned", default="", help="EXECUTOR,TILE_BYTES,SCHEDULE instead of running the autotuner "
+    ap.add_argument("--tuned", default="", help="EXECUTOR,TILE_BYTES,SCHEDULE[,WAVE_TILE_BYTES] instead of running the autotuner "
                     "(profiling runs, where the profiler distorts the tuner's timings)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MB first (HBM counter calibration)")
@@ -293,11 +293,14 @@ def main():
     h.set_precision(32)
     h.upload()
     if args.tuned:
-        ex, tb, sc = (int(v) for v in args.tuned.split(","))
+        vals = [int(v) for v in args.tuned.split(",")]
+        ex, tb, sc = vals[:3]
+        wb = vals[3] if len(vals) > 3 else 32768
         h.set_executor(ex)
         h.set_tile_bytes(tb)
         h.set_schedule(sc)
-        tune = dict(ms=0.0, tile_bytes=tb, schedule=sc, executor=ex)
+        h.set_wave_tile_bytes(wb)
+        tune = dict(ms=0.0, tile_bytes=tb, schedule=sc, executor=ex, wave_tile_bytes=wb)
         log(f"[bench] configuration given: {tune}")
     else:
         tune = h.autotune(nv=1, reps=10)
@@ -411,7 +414,8 @@ def main():
                    "alg_MB_fp64": round(r64["batch"][1]["alg"] / 1e6, 2),
                    "cache_resident": r32["batch"][1]["alg"] < MALL_BYTES, "timing": "hipGraph replays",
                    "autotune": {"executor": ["per stage", "persistent"][tune["executor"]],
-                                "tile_bytes": int(tune["tile_bytes"]), "schedule": tune["schedule"]}},
+                                "tile_bytes": int(tune["tile_bytes"]), "schedule": tune["schedule"],
+                                "wave_tile_bytes": int(tune["wave_tile_bytes"])}},
         "fp64": {"ms": round(fp64_plain, 5), "GBs": round(r64["batch"][1]["GBs"], 2)},
         "fp32_speedup_vs_fp64": round(fp64_plain / r32["batch"][1]["ms"], 4),
         "batched": {f"fp{bits}": {str(nv): {k: (float(f"{v:.5g}") if isinstance(v, float) else v)

@@ -129,21 +129,21 @@ __device__ __forceinline__ void wst(double *p, double v) {
 // NV-interleaved layout), so every thread works for any NV, and each sums
 // the slots in list order with eight loads in flight.  From the external
 // input the threads take elements and read the NV vectors.
-template <int NV, bool COH>
+template <int NV, bool COH, int NT>
 __device__ __forceinline__ void stage_vec(const int *__restrict__ vsrc, int b, int e, int off, int n,
                                           const double *__restrict__ Wv, const double *__restrict__ X, long long ldx,
-                                          double *dst) {
+                                          double *dst, int tid) {
     const int s0 = vsrc[b];
     if (s0 < 0) {
         const double *p = X + (long long)(-s0 - 1 + off);
-        for (int i = threadIdx.x; i < n; i += WG) {
+        for (int i = tid; i < n; i += NT) {
             #pragma unroll
             for (int v = 0; v < NV; v++) dst[i * NV + v] = p[(long long)v * ldx + i];
         }
         return;
     }
     const int nq = n * NV;
-    for (int q = threadIdx.x; q < nq; q += WG) {
+    for (int q = tid; q < nq; q += NT) {
         const double *base = Wv + (long long)off * NV + q;
         double sum = 0.0;
         int j = b;
@@ -159,21 +159,43 @@ __device__ __forceinline__ void stage_vec(const int *__restrict__ vsrc, int b, i
     }
 }
 
-template <typename T, int NV, bool COH>
+// Synchronisation of the threads of one tile: the workgroup, or one wave
+// (a wave's LDS operations complete in order; the fences keep the compiler
+// from moving memory operations across the point).
+template <int NT>
+__device__ __forceinline__ void tile_sync() {
+    if constexpr (NT == WG) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// A tile of NT threads: the workgroup (NT = WG) or one wave (NT = 64, a
+// "wave tile": four of them run side by side in a workgroup, each with a
+// quarter of the LDS).
+template <int NT> struct TileLDS {
+    static const int X = NT == WG ? XIN_DOUBLES : XIN_DOUBLES / 4;      // in0 staging (ncol * NV)
+    static const int Y = NT == WG ? YIN_DOUBLES : YIN_DOUBLES / 4;      // in1 staging (nrow * NV)
+};
+
+template <typename T, int NV, bool COH, int NT = WG>
 __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *__restrict__ vsrc,
                            double *__restrict__ Wv, const double *__restrict__ X, long long ldx, double *lds) {
     typedef Cfg<T, NV> C_;
     typedef typename C_::V V;
     constexpr int W = C_::W, KP = C_::KP, RU = C_::RU, CPL = W * KP;
     const int lpr = t.lpr;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x & (NT - 1);
     const int lane = tid & (lpr - 1);
     const int grp = tid / lpr;
-    const int G = WG / lpr;
+    const int G = NT / lpr;
     const int R = t.nrow, C = t.ncol;
     const bool do0 = t.out0 >= 0, do1 = t.out1 >= 0;
     double *xin = lds;
-    double *yin = lds + XIN_DOUBLES;
+    double *yin = lds + TileLDS<NT>::X;
     const T *base = S + t.mat;
     // Row blocks of RU rows; every lane issues RU*KP vector loads per block.
     // Loads are unconditional: rows past the tile are clamped to its last row
@@ -198,9 +220,9 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
     load_rows(nxt, grp);
 
     // stage the input vectors once per tile (sums of their slots)
-    if (do0) stage_vec<NV, COH>(vsrc, t.v0b, t.v0e, t.v0off, C, Wv, X, ldx, xin);
-    if (do1) stage_vec<NV, COH>(vsrc, t.v1b, t.v1e, t.v1off, R, Wv, X, ldx, yin);
-    __syncthreads();
+    if (do0) stage_vec<NV, COH, NT>(vsrc, t.v0b, t.v0e, t.v0off, C, Wv, X, ldx, xin, tid);
+    if (do1) stage_vec<NV, COH, NT>(vsrc, t.v1b, t.v1e, t.v1off, R, Wv, X, ldx, yin, tid);
+    tile_sync<NT>();
 
     double x0[KP][W][NV];
     double a1[KP][W][NV];
@@ -306,6 +328,21 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
                 #pragma unroll
                 for (int v = 0; v < NV; v++)
                     for (int s = lpr; s < 64; s <<= 1) a1[k][w][v] += __shfl_xor(a1[k][w][v], s, 64);
+        if constexpr (NT != WG) {   // one wave: lanes below lpr hold the column sums
+            if (lane == tid) {
+                #pragma unroll
+                for (int k = 0; k < KP; k++)
+                    #pragma unroll
+                    for (int w = 0; w < W; w++) {
+                        const int c = (k * lpr + lane) * W + w;
+                        if (c < C) {
+                            #pragma unroll
+                            for (int v = 0; v < NV; v++) wst<COH>(Wv + (long long)(t.out1 + c) * NV + v, a1[k][w][v]);
+                        }
+                    }
+            }
+            return;
+        }
         __syncthreads();   // the staged in1 is dead from here on
         double *red = lds;
         const int wave = tid >> 6;
@@ -387,11 +424,13 @@ __device__ void reduce_task(const RTask &t, const int *__restrict__ src, double 
 }
 
 // One launch per stage: n64 FP64 tiles, then n32 FP32 tiles, then nr reduce
-// tasks (all independent of each other within the launch).
+// tasks, then the wave tiles, four per workgroup (FP64 first), all
+// independent of each other within the launch.
 template <int NV>
-__global__ __launch_bounds__(WG) void stage_kernel(const BTask *__restrict__ t64, int n64, const BTask *__restrict__ t32,
+__global__ __launch_bounds__(WG, 4) void stage_kernel(const BTask *__restrict__ t64, int n64, const BTask *__restrict__ t32,
                                                    int n32, const RTask *__restrict__ rt, int nr,
-                                                   const int *__restrict__ vsrc, const int *__restrict__ rsrc,
+                                                   const BTask *__restrict__ w64, int nw64, const BTask *__restrict__ w32,
+                                                   int nw32, const int *__restrict__ vsrc, const int *__restrict__ rsrc,
                                                    const double *__restrict__ S64, const float *__restrict__ S32,
                                                    double *__restrict__ Wv, const double *__restrict__ X,
                                                    double *__restrict__ Y, long long ldx) {
@@ -406,6 +445,18 @@ __global__ __launch_bounds__(WG) void stage_kernel(const BTask *__restrict__ t64
     } else if (b - n64 - n32 < nr) {
         const RTask t = rt[b - n64 - n32];
         reduce_task<NV, false>(t, rsrc, Wv, Y, ldx, lds);
+    } else {
+        const int wave = threadIdx.x >> 6;
+        const int i = 4 * (b - n64 - n32 - nr) + wave;   // wave tile of this wave
+        double *wl = lds + wave * (LDS_DOUBLES / 4);
+        const int j = i - ((nw64 + 3) & ~3);   // FP32 wave tiles start at a new workgroup
+        if (i < nw64) {
+            const BTask t = w64[i];
+            bgemv_tile<double, NV, false, 64>(t, S64, vsrc, Wv, X, ldx, wl);
+        } else if (j >= 0 && j < nw32) {
+            const BTask t = w32[j];
+            bgemv_tile<float, NV, false, 64>(t, S32, vsrc, Wv, X, ldx, wl);
+        }
     }
 }
 
@@ -655,12 +706,14 @@ struct Stage {
     int stream;         // 0: main stream, 1: near-field stream (joins before the gather)
     int t64_beg, t64_cnt, t32_beg, t32_cnt;
     int r_beg, r_cnt;
+    int w64_beg, w64_cnt, w32_beg, w32_cnt;   // wave tiles (four per workgroup)
     double mat_bytes;   // matrix bytes streamed by this launch
 };
 
 struct Plan {
     int nv = 1;
     std::vector<BTask> t64, t32;
+    std::vector<BTask> w64, w32;   // wave tiles
     std::vector<RTask> rt;
     std::vector<int> rsrc, vsrc;
     std::vector<Stage> stages;
@@ -681,7 +734,7 @@ struct DevPlan {
     Plan P;
     bool ready = false;
     double *Wv = nullptr;
-    BTask *d_t64 = nullptr, *d_t32 = nullptr;
+    BTask *d_t64 = nullptr, *d_t32 = nullptr, *d_w64 = nullptr, *d_w32 = nullptr;
     RTask *d_rt = nullptr;
     int *d_rsrc = nullptr, *d_vsrc = nullptr;
     // persistent executor
@@ -701,6 +754,7 @@ struct DevState {
     double tile_bytes = 128.0 * 1024.0;
     int nf_schedule = 3;      // near-field placement, see H2ERI_set_schedule (two groups on a second stream)
     int executor = 0;         // 0: one launch per stage, 1: one persistent launch per pass (H2ERI_set_executor)
+    double wave_bytes = 32768.0;   // staged executor: matrix parts up to this size run as wave tiles
     Storage st;
     bool st_ready = false;
     double *S64 = nullptr;
@@ -742,8 +796,9 @@ static void release_graph(DevState *d);
 static void release_plans(DevState *d) {
     release_graph(d);
     for (DevPlan &p : d->plans) {
-        void **ps[] = {(void **)&p.Wv, (void **)&p.d_t64, (void **)&p.d_t32, (void **)&p.d_rt, (void **)&p.d_rsrc,
-                       (void **)&p.d_vsrc, (void **)&p.d_q, (void **)&p.d_need, (void **)&p.d_ctr};
+        void **ps[] = {(void **)&p.Wv, (void **)&p.d_t64, (void **)&p.d_t32, (void **)&p.d_w64, (void **)&p.d_w32,
+                       (void **)&p.d_rt, (void **)&p.d_rsrc, (void **)&p.d_vsrc, (void **)&p.d_q, (void **)&p.d_need,
+                       (void **)&p.d_ctr};
         for (void **q : ps) free_dev(*q);
         p.ready = false;
         p.q_ready = false;
@@ -812,6 +867,7 @@ struct PTile {
     BTask t;
     int st;
     bool f32;
+    bool wave;   // a wave tile
     int iv0, iv1;
 };
 
@@ -823,6 +879,7 @@ struct Planner {
     double tile_bytes;
     int nf_schedule;
     int rchunk;  // output elements per reduce task
+    double wave_bytes;   // matrix parts up to this size become wave tiles (0: none)
     int maxc;   // max tile columns for this batch width
     std::vector<VVec> vv;
     std::vector<PTile> tiles;
@@ -831,8 +888,9 @@ struct Planner {
     std::vector<double> sload;
     int nst = 0, sG = 0, sS = 0;   // stages: main 0..sG, near-field stream sS..nst-1 (schedule 2)
 
-    Planner(H2ERI_p h_, const Storage &s_, Plan &p_, int nv_, double tb, int nfs, int rch)
-        : h(h_), S(s_), P(p_), nv(nv_), tile_bytes(tb), nf_schedule(nfs), rchunk(rch), maxc(64 * cpl_of(nv_)) {}
+    Planner(H2ERI_p h_, const Storage &s_, Plan &p_, int nv_, double tb, int nfs, int rch, double wb)
+        : h(h_), S(s_), P(p_), nv(nv_), tile_bytes(tb), nf_schedule(nfs), rchunk(rch), wave_bytes(wb),
+          maxc(64 * cpl_of(nv_)) {}
 
     long long walloc(long long n) {
         long long o = P.nW;
@@ -859,10 +917,13 @@ struct Planner {
         const int ncol = m.ncol;
         if (nrow <= 0 || ncol <= 0 || (d0 < 0 && d1 < 0)) return;
         const double esz = m.f32 ? 4.0 : 8.0;
-        const int nct = (ncol + maxc - 1) / maxc;
-        const int ctw = ((ncol + nct - 1) / nct + 7) / 8 * 8;   // multiple of 8 (vector alignment), <= maxc
-        int rows = (int)std::max(1.0, tile_bytes / (esz * ctw));
-        if (d1 >= 0) rows = std::min(rows, YIN_DOUBLES / nv);
+        // a small part is one wave tile: one wave of a workgroup, a quarter of its LDS
+        const bool wave = esz * nrow * ncol <= wave_bytes && ncol <= maxc &&
+                          ncol * nv <= TileLDS<64>::X && (d1 < 0 || nrow * nv <= TileLDS<64>::Y);
+        const int nct = wave ? 1 : (ncol + maxc - 1) / maxc;
+        const int ctw = wave ? ncol : ((ncol + nct - 1) / nct + 7) / 8 * 8;   // multiple of 8 (vector alignment), <= maxc
+        int rows = wave ? nrow : (int)std::max(1.0, tile_bytes / (esz * ctw));
+        if (d1 >= 0 && !wave) rows = std::min(rows, YIN_DOUBLES / nv);
         const int nrt = (nrow + rows - 1) / rows;
         rows = (nrow + nrt - 1) / nrt;
         std::vector<long long> s0(nct, -1), s1(nrt, -1);
@@ -888,6 +949,7 @@ struct Planner {
                 t.out1 = d1 >= 0 ? (int)(s1[rt] + c0) : -1;
                 p.st = st;
                 p.f32 = m.f32;
+                p.wave = wave;
                 p.iv0 = d0 >= 0 ? v0 : -1;
                 p.iv1 = d1 >= 0 ? v1 : -1;
                 if (p.iv0 >= 0) use(v0, st, nc);
@@ -1140,32 +1202,43 @@ struct Planner {
     }
 
     void finish() {
-        std::vector<std::vector<BTask>> S64(nst), S32(nst);
+        std::vector<std::vector<BTask>> S64(nst), S32(nst), W64(nst), W32(nst);
         for (PTile &p : tiles) {
             if (p.iv0 >= 0) { auto r = emit_vec(p.iv0); p.t.v0b = r.first; p.t.v0e = r.second; }
             if (p.iv1 >= 0) { auto r = emit_vec(p.iv1); p.t.v1b = r.first; p.t.v1e = r.second; }
-            (p.f32 ? S32 : S64)[p.st].push_back(p.t);
+            if (p.wave) (p.f32 ? W32 : W64)[p.st].push_back(p.t);
+            else (p.f32 ? S32 : S64)[p.st].push_back(p.t);
         }
         auto key = [](const BTask &t) { return (long long)t.nrow * t.ncol; };
         for (int st = 0; st < nst; st++) {
             std::stable_sort(S64[st].begin(), S64[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
             std::stable_sort(S32[st].begin(), S32[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
+            std::stable_sort(W64[st].begin(), W64[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
+            std::stable_sort(W32[st].begin(), W32[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
             Stage s;
             s.phase = sphase[st];
             s.stream = st > sG ? 1 : 0;
             s.mat_bytes = 0;
             for (const BTask &t : S64[st]) s.mat_bytes += 8.0 * t.nrow * t.ncol;
             for (const BTask &t : S32[st]) s.mat_bytes += 4.0 * t.nrow * t.ncol;
+            for (const BTask &t : W64[st]) s.mat_bytes += 8.0 * t.nrow * t.ncol;
+            for (const BTask &t : W32[st]) s.mat_bytes += 4.0 * t.nrow * t.ncol;
             s.t64_beg = (int)P.t64.size();
             s.t64_cnt = (int)S64[st].size();
             s.t32_beg = (int)P.t32.size();
             s.t32_cnt = (int)S32[st].size();
             P.t64.insert(P.t64.end(), S64[st].begin(), S64[st].end());
             P.t32.insert(P.t32.end(), S32[st].begin(), S32[st].end());
+            s.w64_beg = (int)P.w64.size();
+            s.w64_cnt = (int)W64[st].size();
+            s.w32_beg = (int)P.w32.size();
+            s.w32_cnt = (int)W32[st].size();
+            P.w64.insert(P.w64.end(), W64[st].begin(), W64[st].end());
+            P.w32.insert(P.w32.end(), W32[st].begin(), W32[st].end());
             s.r_beg = (int)P.rt.size();
             s.r_cnt = (int)SR[st].size();
             P.rt.insert(P.rt.end(), SR[st].begin(), SR[st].end());
-            if (s.t64_cnt + s.t32_cnt + s.r_cnt > 0) P.stages.push_back(s);
+            if (s.t64_cnt + s.t32_cnt + s.r_cnt + s.w64_cnt + s.w32_cnt > 0) P.stages.push_back(s);
         }
     }
 
@@ -1181,18 +1254,18 @@ struct Planner {
             }
             return true;
         };
-        auto chk = [&](const std::vector<BTask> &tl, bool f32) {
+        auto chk = [&](const std::vector<BTask> &tl, bool f32, int xcap, int ycap) {
             const long long nstore = f32 ? (long long)S.h32.size() : (long long)S.h64.size();
             const int W = !f32 ? 2 : 4;   // widest vector load of any tile path (FP32: float4)
             for (const BTask &t : tl) {
-                bool ok = t.nrow > 0 && t.ncol > 0 && t.ncol <= maxc && t.ncol * nv <= XIN_DOUBLES && t.ld >= t.ncol &&
+                bool ok = t.nrow > 0 && t.ncol > 0 && t.ncol <= maxc && t.ncol * nv <= xcap && t.ld >= t.ncol &&
                           t.ld % W == 0 && t.mat % W == 0 && t.mat >= 0 && (t.lpr & (t.lpr - 1)) == 0 && t.lpr >= 4 &&
                           t.lpr <= 64 && t.lpr * cpl_of(nv) >= t.ncol && (t.out0 >= 0 || t.out1 >= 0);
                 const long long last = t.mat + (long long)(t.nrow - 1) * t.ld + ((t.ncol - 1) / W) * W + W;
                 ok = ok && last <= nstore;
                 if (t.out0 >= 0) ok = ok && src_ok(t.v0b, t.v0e, t.v0off, t.ncol) && t.out0 + (long long)t.nrow <= P.nW;
                 if (t.out1 >= 0)
-                    ok = ok && t.nrow * nv <= YIN_DOUBLES && src_ok(t.v1b, t.v1e, t.v1off, t.nrow) &&
+                    ok = ok && t.nrow * nv <= ycap && src_ok(t.v1b, t.v1e, t.v1off, t.nrow) &&
                          t.out1 + (long long)t.ncol <= P.nW;
                 if (!ok) {
                     set_error(ERR_ARG,
@@ -1205,7 +1278,9 @@ struct Planner {
             }
             return true;
         };
-        if (!chk(P.t64, false) || !chk(P.t32, true)) return false;
+        if (!chk(P.t64, false, XIN_DOUBLES, YIN_DOUBLES) || !chk(P.t32, true, XIN_DOUBLES, YIN_DOUBLES) ||
+            !chk(P.w64, false, TileLDS<64>::X, TileLDS<64>::Y) || !chk(P.w32, true, TileLDS<64>::X, TileLDS<64>::Y))
+            return false;
         for (const RTask &r : P.rt) {
             bool ok = r.len > 0 && r.len <= rchunk && r.len * nv <= 1024 && r.dst >= 0 && r.soff >= 0 && r.src_beg >= 0 &&
                       r.src_end <= (int)P.rsrc.size() && r.src_beg < r.src_end;
@@ -1215,7 +1290,8 @@ struct Planner {
         }
         for (const Stage &s : P.stages)
             if (s.t64_beg + s.t64_cnt > (int)P.t64.size() || s.t32_beg + s.t32_cnt > (int)P.t32.size() ||
-                s.r_beg + s.r_cnt > (int)P.rt.size()) {
+                s.r_beg + s.r_cnt > (int)P.rt.size() || s.w64_beg + s.w64_cnt > (int)P.w64.size() ||
+                s.w32_beg + s.w32_cnt > (int)P.w32.size()) {
                 set_error(ERR_ARG, "device plan check failed: stage ranges");
                 return false;
             }
@@ -1225,8 +1301,10 @@ struct Planner {
 
 }  // namespace
 
-static bool build_plan(H2ERI_p h, const Storage &S, Plan &P, int nv, double tb, int nfs, int executor) {
-    Planner pl(h, S, P, nv, tb, nfs, rchunk_for(executor, nv));
+// Wave tiles (parts up to wave_bytes) belong to the staged executor only.
+static bool build_plan(H2ERI_p h, const Storage &S, Plan &P, int nv, double tb, int nfs, int executor,
+                       double wave_bytes) {
+    Planner pl(h, S, P, nv, tb, nfs, rchunk_for(executor, nv), executor == 0 ? wave_bytes : 0.0);
     return pl.build();
 }
 
@@ -1252,6 +1330,7 @@ struct QGraph {
 
 static bool queue_graph(const Plan &P, int nv, int G, QGraph &Gr) {
     Gr = QGraph();
+    if (!P.w64.empty() || !P.w32.empty()) { set_error(ERR_ARG, "queue plan: wave tiles are not queued"); return false; }
     struct TRef { int kind, idx; };
     std::vector<TRef> T;
     for (const Stage &st : P.stages) {
@@ -1568,7 +1647,7 @@ static DevPlan *get_plan(H2ERI_p h, int nv) {
         if (d->executor == 1 && !ensure_queue(d, dp)) return nullptr;
         return &dp;
     }
-    if (!build_plan(h, d->st, dp.P, nv, d->tile_bytes, d->nf_schedule, d->executor)) return nullptr;
+    if (!build_plan(h, d->st, dp.P, nv, d->tile_bytes, d->nf_schedule, d->executor, d->wave_bytes)) return nullptr;
     const Plan &P = dp.P;
     auto alloc_copy = [&](void **dst, const void *src, size_t bytes) -> bool {
         if (bytes == 0) bytes = 64;
@@ -1581,6 +1660,8 @@ static DevPlan *get_plan(H2ERI_p h, int nv) {
     if (hipMemset(dp.Wv, 0, (size_t)P.nW * nv * 8) != hipSuccess) { set_error(ERR_DEVICE, "workspace memset"); return nullptr; }
     if (!alloc_copy((void **)&dp.d_t64, P.t64.empty() ? nullptr : P.t64.data(), P.t64.size() * sizeof(BTask))) return nullptr;
     if (!alloc_copy((void **)&dp.d_t32, P.t32.empty() ? nullptr : P.t32.data(), P.t32.size() * sizeof(BTask))) return nullptr;
+    if (!alloc_copy((void **)&dp.d_w64, P.w64.empty() ? nullptr : P.w64.data(), P.w64.size() * sizeof(BTask))) return nullptr;
+    if (!alloc_copy((void **)&dp.d_w32, P.w32.empty() ? nullptr : P.w32.data(), P.w32.size() * sizeof(BTask))) return nullptr;
     if (!alloc_copy((void **)&dp.d_rt, P.rt.empty() ? nullptr : P.rt.data(), P.rt.size() * sizeof(RTask))) return nullptr;
     if (!alloc_copy((void **)&dp.d_rsrc, P.rsrc.empty() ? nullptr : P.rsrc.data(), P.rsrc.size() * sizeof(int))) return nullptr;
     if (!alloc_copy((void **)&dp.d_vsrc, P.vsrc.empty() ? nullptr : P.vsrc.data(), P.vsrc.size() * sizeof(int))) return nullptr;
@@ -1589,12 +1670,18 @@ static DevPlan *get_plan(H2ERI_p h, int nv) {
     return &dp;
 }
 
+// Workgroups of one stage launch: a tile or reduce task each, four wave tiles each.
+static inline int stage_blocks(const Stage &st) {
+    return st.t64_cnt + st.t32_cnt + st.r_cnt + (st.w64_cnt + 3) / 4 + (st.w32_cnt + 3) / 4;
+}
+
 template <int NV>
 static void launch_stage(const Stage &st, const DevPlan &dp, DevState *d, const double *X, double *Y, long long ld,
                          hipStream_t s) {
-    hipLaunchKernelGGL(stage_kernel<NV>, dim3(st.t64_cnt + st.t32_cnt + st.r_cnt), dim3(WG), 0, s,
-                       dp.d_t64 + st.t64_beg, st.t64_cnt, dp.d_t32 + st.t32_beg, st.t32_cnt, dp.d_rt + st.r_beg,
-                       st.r_cnt, dp.d_vsrc, dp.d_rsrc, d->S64, d->S32, dp.Wv, X, Y, ld);
+    hipLaunchKernelGGL(stage_kernel<NV>, dim3(stage_blocks(st)), dim3(WG), 0, s, dp.d_t64 + st.t64_beg, st.t64_cnt,
+                       dp.d_t32 + st.t32_beg, st.t32_cnt, dp.d_rt + st.r_beg, st.r_cnt, dp.d_w64 + st.w64_beg,
+                       st.w64_cnt, dp.d_w32 + st.w32_beg, st.w32_cnt, dp.d_vsrc, dp.d_rsrc, d->S64, d->S32, dp.Wv, X, Y,
+                       ld);
 }
 
 static void collect_phase_ms(H2ERI_p h, DevState *d);
@@ -1667,7 +1754,7 @@ static bool run_matvec(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, dou
         HIP_OK(hipGetLastError());
         if (tim) {
             HIP_OK(hipEventRecord(d->evs[2 * d->launches.size() + 1], q));
-            d->launches.push_back(LaunchRec{st.phase, st.t64_cnt + st.t32_cnt + st.r_cnt, st.mat_bytes, 0.f});
+            d->launches.push_back(LaunchRec{st.phase, stage_blocks(st), st.mat_bytes, 0.f});
         }
         return true;
     };
@@ -1987,8 +2074,9 @@ void H2ERI_set_schedule(H2ERI_p h, const int mode) {
 // near field on a second stream for schedules >= 2); 1 runs the whole pass as
 // one persistent launch whose workgroups take the plan's tasks in a
 // dependency-checked order (plus a small launch that clears its counters).
-// Both compute every output with the same operations in the same order, so
-// their results are bitwise equal.
+// The staged plan runs matrix parts up to 32 KB as wave tiles, whose column
+// sums are added in another order, so the two agree to rounding (bitwise
+// when the plan has no such part).
 void H2ERI_set_executor(H2ERI_p h, const int mode) {
     if (!h || (mode != 0 && mode != 1)) { set_error(ERR_ARG, "H2ERI_set_executor: 0 or 1"); return; }
     DevState *d = dstate_get(h);
@@ -2037,6 +2125,14 @@ int H2ERI_queue_stats(H2ERI_p h, const int nv, double *out4) {
     return 0;
 }
 
+// Matrix parts up to `bytes` run as wave tiles in the staged executor (one
+// wave each, four per workgroup); 0 disables them (tuning knob; re-plans).
+void H2ERI_set_wave_tile_bytes(H2ERI_p h, const double bytes) {
+    if (!h || bytes < 0) { set_error(ERR_ARG, "H2ERI_set_wave_tile_bytes: bytes >= 0"); return; }
+    DevState *d = dstate_get(h);
+    if (d->wave_bytes != bytes) { d->wave_bytes = bytes; release_plans(d); }
+}
+
 // Tile size of the bi-GEMV tasks in bytes (tuning knob; re-plans).
 void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
     if (!h || bytes < 1024) return;
@@ -2046,11 +2142,12 @@ void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
 }
 
 // Autotuner: times the device matvec of nv vectors for every combination of
-// executor, tile size and near-field schedule in the candidate lists (HIP
-// events around `reps` back-to-back calls after one warm-up) and keeps the
-// fastest.  out4 = {best ms per call, tile bytes, schedule, executor}.
-// Returns 0, or -1 with the error set.
-int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out4) {
+// executor, tile size, near-field schedule and (staged executor) wave-tile
+// limit in the candidate lists (HIP events around `reps` back-to-back calls
+// after one warm-up) and keeps the fastest.  out5 = {best ms per call, tile
+// bytes, schedule, executor, wave-tile bytes}.  Returns 0, or -1 with the
+// error set.
+int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
     if (!h || (nv != 1 && nv != 2 && nv != 4 && nv != 8) || reps < 1) { set_error(ERR_ARG, "H2ERI_autotune: bad argument"); return -1; }
     if (!upload_storage(h)) return -1;
     DevState *d = dstate(h);
@@ -2063,16 +2160,19 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out4) {
     const int scheds[] = {0, 2, 3, 4};
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { set_error(ERR_DEVICE, "autotune: events"); return -1; }
-    double best = 1e300, best_tile = d->tile_bytes;
+    double best = 1e300, best_tile = d->tile_bytes, best_wave = d->wave_bytes;
     int best_sched = d->nf_schedule, best_ex = d->executor;
+    const double waves[] = {0.0, 32768.0};
     const bool was_timing = d->phase_timing;
     d->phase_timing = false;
     bool ok = true;
     for (int ex = 0; ok && ex < 2; ex++)
+    for (int wi = 0; ok && wi < (ex == 0 ? 2 : 1); wi++)
     for (double tb : tiles) {
         for (int sc : scheds) {
             release_plans(d);
             d->executor = ex;
+            d->wave_bytes = ex == 0 ? waves[wi] : 0.0;
             d->tile_bytes = tb;
             d->nf_schedule = sc;
             ok = matvec_multi_dev_impl(h, nv, d->d_x, d->d_y, nullptr);   // plans, uploads, warms up
@@ -2082,7 +2182,7 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out4) {
             float ms = 0.f;
             ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
             if (!ok) break;
-            if (ms / reps < best) { best = ms / reps; best_tile = tb; best_sched = sc; best_ex = ex; }
+            if (ms / reps < best) { best = ms / reps; best_tile = tb; best_sched = sc; best_ex = ex; best_wave = d->wave_bytes; }
         }
         if (!ok) break;
     }
@@ -2093,11 +2193,12 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out4) {
     d->tile_bytes = best_tile;
     d->nf_schedule = best_sched;
     d->executor = best_ex;
+    d->wave_bytes = best_wave;
     if (!ok) {
         if (!has_error()) set_error(ERR_DEVICE, "autotune: timing failed");
         return -1;
     }
-    if (out4) { out4[0] = best; out4[1] = best_tile; out4[2] = best_sched; out4[3] = best_ex; }
+    if (out5) { out5[0] = best; out5[1] = best_tile; out5[2] = best_sched; out5[3] = best_ex; out5[4] = best_wave; }
     return 0;
 }
 
@@ -2153,7 +2254,7 @@ void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_
         if (!upload_storage(h)) { group_free(g); return nullptr; }
         DevState *d = dstate(h);
         DevPlan &dp = g->plans[k];
-        if (!build_plan(h, d->st, dp.P, 1, d->tile_bytes, d->nf_schedule, 1)) { group_free(g); return nullptr; }
+        if (!build_plan(h, d->st, dp.P, 1, d->tile_bytes, d->nf_schedule, 1, 0.0)) { group_free(g); return nullptr; }
         if (!queue_graph(dp.P, 1, G, graphs[k])) { group_free(g); return nullptr; }
         const Plan &P = dp.P;
         auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
@@ -2253,7 +2354,7 @@ int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double 
     S.build(h, ub_bits == 32);
     Plan P;
     DevState *ds = dstate_get(h);
-    if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, 0)) return -1;
+    if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, 0, ds->wave_bytes)) return -1;
     const long long N = h->krnl_mat_size;
     std::vector<double> Wv((size_t)P.nW * nv, std::nan(""));
     std::vector<int> wmark((size_t)P.nW, -1), rmark((size_t)P.nW, -1);
@@ -2293,10 +2394,16 @@ int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double 
         std::fill(rmark.begin(), rmark.end(), -1);
         in_side = st.stream == 1;
         before_join = st.phase != PH_GATHER;
-        const int ntile = st.t64_cnt + st.t32_cnt;
+        const int ntile = st.t64_cnt + st.t32_cnt + st.w64_cnt + st.w32_cnt;
         for (int q = 0; q < ntile; q++) {
-            const bool f32 = q >= st.t64_cnt;
-            const BTask &t = f32 ? P.t32[st.t32_beg + q - st.t64_cnt] : P.t64[st.t64_beg + q];
+            int k = q;
+            const BTask *tp;
+            bool f32;
+            if (k < st.t64_cnt) { tp = &P.t64[st.t64_beg + k]; f32 = false; }
+            else if ((k -= st.t64_cnt) < st.t32_cnt) { tp = &P.t32[st.t32_beg + k]; f32 = true; }
+            else if ((k -= st.t32_cnt) < st.w64_cnt) { tp = &P.w64[st.w64_beg + k]; f32 = false; }
+            else { k -= st.w64_cnt; tp = &P.w32[st.w32_beg + k]; f32 = true; }
+            const BTask &t = *tp;
             auto M = [&](int r, int c) -> double {
                 const long long e = t.mat + (long long)r * t.ld + c;
                 return f32 ? (double)S.h32[e] : S.h64[e];
@@ -2366,7 +2473,7 @@ int H2ERI_queue_emulate(H2ERI_p h, const int ub_bits, const int nv, const int wo
     S.build(h, ub_bits == 32);
     Plan P;
     DevState *ds = dstate_get(h);
-    if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, 1)) return -1;
+    if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, 1, 0.0)) return -1;
     QPlan Q;
     if (!build_queue(P, nv, workgroups, Q)) return -1;
     const long long N = h->krnl_mat_size;
@@ -2473,8 +2580,8 @@ int H2ERI_plan_check(H2ERI_p h, const int ub_bits) {
     for (int nv : NV_VARIANTS)
         for (int ex = 0; ex < 2; ex++) {
             Plan P;
-            if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, ex)) return -1;
-            if (nv == 1 && ex == 0) tasks = (int)(P.t64.size() + P.t32.size());
+            if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, ex, ds->wave_bytes)) return -1;
+            if (nv == 1 && ex == 0) tasks = (int)(P.t64.size() + P.t32.size() + P.w64.size() + P.w32.size());
         }
     return tasks;
 }

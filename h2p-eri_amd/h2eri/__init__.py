@@ -212,6 +212,11 @@ class H2ERI:
         self.lib.H2ERI_set_schedule(self.h, mode)
         _check(self.lib)
 
+    def set_wave_tile_bytes(self, nbytes):
+        """Staged executor: matrix parts up to nbytes run as wave tiles (0: none)."""
+        self.lib.H2ERI_set_wave_tile_bytes(self.h, float(nbytes))
+        _check(self.lib)
+
     def set_executor(self, mode):
         """0: one launch per stage; 1: one persistent launch per pass."""
         self.lib.H2ERI_set_executor(self.h, mode)
@@ -335,10 +340,11 @@ class H2ERI:
     def autotune(self, nv=1, reps=10):
         """Times executors x tile sizes x near-field schedules on the device,
         keeps the best."""
-        out = np.zeros(4)
+        out = np.zeros(5)
         self.lib.H2ERI_autotune(self.h, nv, reps, _dptr(out))
         _check(self.lib)
-        return dict(ms=out[0], tile_bytes=out[1], schedule=int(out[2]), executor=int(out[3]))
+        return dict(ms=float(out[0]), tile_bytes=int(out[1]), schedule=int(out[2]), executor=int(out[3]),
+                    wave_tile_bytes=int(out[4]))
 
     def set_phase_timing(self, on=True):
         self.lib.H2ERI_set_phase_timing(self.h, 1 if on else 0)

@@ -346,7 +346,8 @@ void H2ERI_set_schedule(H2ERI_p h2eri, const int mode);
  * stage of the plan separately, 1 runs the whole pass as one persistent
  * launch (plus a small counter-reset launch) whose workgroups take the plan's
  * tasks by ticket in a topological order and wait only for the producers of
- * their own inputs.  Results of the two are bitwise equal.  With 1, phase
+ * their own inputs.  The two agree to rounding (the staged plan runs small
+ * matrix parts as wave tiles, which sum columns in another order).  With 1, phase
  * timing sees one launch per pass: its time goes to the near-field slot of
  * H2ERI_get_phase_ms and of the reference's timers. */
 void H2ERI_set_executor(H2ERI_p h2eri, const int mode);
@@ -379,10 +380,17 @@ void  H2ERI_group_destroy(void *group);
 
 /* Autotuner: times nv-vector device matvecs (reps calls after a warm-up)
  * for every executor (H2ERI_set_executor) x tile size {64, 128, 256} KB x
- * near-field schedule {0, 2, 3, 4}, and keeps the fastest combination for
- * later calls.  out4 = {ms per call, tile bytes, schedule, executor} of the
- * combination kept.  Returns 0, -1 on error. */
-int H2ERI_autotune(H2ERI_p h2eri, const int nv, const int reps, double *out4);
+ * near-field schedule {0, 2, 3, 4} x (staged executor) wave-tile limit {0,
+ * 32} KB, and keeps the fastest combination for later calls.  out5 = {ms per
+ * call, tile bytes, schedule, executor, wave-tile bytes} of the combination
+ * kept.  Returns 0, -1 on error. */
+int H2ERI_autotune(H2ERI_p h2eri, const int nv, const int reps, double *out5);
+
+/* Staged executor: matrix parts up to `bytes` (default 32768) run as wave
+ * tiles, one wave each and four per workgroup, which pays off for the many
+ * small coupling blocks of globular molecules; 0 disables them (tuning;
+ * re-plans; no reference counterpart). */
+void H2ERI_set_wave_tile_bytes(H2ERI_p h2eri, const double bytes);
 
 /* Builds the device execution plan on the host only (no HIP call) and
  * checks every access of every launch against its buffer.  Returns the

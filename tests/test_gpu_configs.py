@@ -95,7 +95,8 @@ def test_uneven_molecules(tmp_path_factory):
 
 def test_molecule_group(tmp_path_factory):
     """Uneven molecules in one persistent launch (H2ERI_group_*): each equals
-    the oracle, and bitwise the molecule's own matvec (same plan, same sums)."""
+    the oracle, and bitwise the molecule's own persistent matvec (same plan,
+    same sums)."""
     import torch
     import h2eri
     from h2eri import molecules as M
@@ -118,7 +119,9 @@ def test_molecule_group(tmp_path_factory):
         assert g.status() == 0
         for k in range(3):
             assert relerr(Y[k].cpu().numpy(), oracle_bind.Oracle(paths[k]).matvec(xs[k])) < 1e-13
+            hs[k].set_executor(1)
             assert np.array_equal(Y[k].cpu().numpy(), hs[k].matvec(xs[k]))
+            hs[k].set_executor(0)
     g.close()
 
 
@@ -127,6 +130,6 @@ def test_autotune_keeps_results(tmp_path):
     h = make_synthetic(tmp_path / "a.rep", 2000, 21, leaf=60, max_rank=30, lr_frac=0.4, diag_lr_frac=0.3)
     res = h.autotune(nv=1, reps=3)
     assert res["ms"] > 0 and res["tile_bytes"] in (65536, 131072, 262144) and res["schedule"] in (0, 2, 3, 4)
-    assert res["executor"] in (0, 1) and h.executor() == res["executor"]
+    assert res["executor"] in (0, 1) and h.executor() == res["executor"] and res["wave_tile_bytes"] in (0, 32768)
     x = np.random.default_rng(2).standard_normal(h.n)
     assert relerr(h.matvec(x), oracle_bind.Oracle(str(tmp_path / "a.rep")).matvec(x)) < 1e-13
