@@ -78,7 +78,7 @@ static const int WG = 256;       // threads per workgroup (4 waves)
 // Output elements per reduce task: 1024 / NV values fill the task's 256
 // threads four deep.  Both executors use the same chunks, so their plans and
 // summation orders are identical.
-static inline int rchunk_for(int executor, int nv) { (void)executor; return 1024 / nv; }
+static inline int rchunk_for(int nv) { return 1024 / nv; }
 static const int LDS_DOUBLES = 4096;   // 32 KB of LDS per workgroup
 static const int XIN_DOUBLES = 1024;   // in0 staging (ncol * NV <= 1024)
 static const int YIN_DOUBLES = LDS_DOUBLES - XIN_DOUBLES;   // in1 staging (nrow * NV)
@@ -1304,7 +1304,7 @@ struct Planner {
 // Wave tiles (parts up to wave_bytes) belong to the staged executor only.
 static bool build_plan(H2ERI_p h, const Storage &S, Plan &P, int nv, double tb, int nfs, int executor,
                        double wave_bytes) {
-    Planner pl(h, S, P, nv, tb, nfs, rchunk_for(executor, nv), executor == 0 ? wave_bytes : 0.0);
+    Planner pl(h, S, P, nv, tb, nfs, rchunk_for(nv), executor == 0 ? wave_bytes : 0.0);
     return pl.build();
 }
 
