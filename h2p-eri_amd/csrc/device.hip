@@ -39,6 +39,7 @@
 #include <cmath>
 #include <numeric>
 #include <queue>
+#include <type_traits>
 
 #include "h2eri_internal.h"
 
@@ -73,6 +74,34 @@ struct RTask {
     int pad_;
 };
 static_assert(sizeof(RTask) == 32, "RTask layout");
+
+// Task descriptors are the same for every lane of a wave.  Reading them
+// through v_readfirstlane lets the compiler keep them in scalar registers
+// and treat branches on them as uniform; a branch it must assume divergent
+// keeps the registers of both sides live at once.
+__device__ __forceinline__ int ufl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long ufl(long long v) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(v & 0xffffffff));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ BTask uniform(const BTask &a) {
+    BTask t;
+    t.mat = ufl(a.mat);
+    t.nrow = ufl(a.nrow); t.ncol = ufl(a.ncol); t.ld = ufl(a.ld); t.lpr = ufl(a.lpr);
+    t.v0b = ufl(a.v0b); t.v0e = ufl(a.v0e); t.v0off = ufl(a.v0off);
+    t.v1b = ufl(a.v1b); t.v1e = ufl(a.v1e); t.v1off = ufl(a.v1off);
+    t.out0 = ufl(a.out0); t.out1 = ufl(a.out1);
+    t.pad_[0] = t.pad_[1] = 0;
+    return t;
+}
+__device__ __forceinline__ RTask uniform(const RTask &a) {
+    RTask t;
+    t.dst = ufl(a.dst);
+    t.len = ufl(a.len); t.src_beg = ufl(a.src_beg); t.src_end = ufl(a.src_end);
+    t.soff = ufl(a.soff); t.flags = ufl(a.flags); t.pad_ = 0;
+    return t;
+}
 
 static const int WG = 256;       // threads per workgroup (4 waves)
 // Output elements per reduce task: 1024 / NV values fill the task's 256
@@ -181,9 +210,83 @@ template <int NT> struct TileLDS {
     static const int Y = NT == WG ? YIN_DOUBLES : YIN_DOUBLES / 4;      // in1 staging (nrow * NV)
 };
 
+// Row-side-only tile at NV = 8 (y = M x with no transposed product, the
+// U^T and B products of the sweeps): RD_L = 4 lanes share a row and each
+// holds RB = 2 rows, so a lane keeps 2 x 8 sums and reads its x values from
+// LDS per column instead of holding a register copy of its columns (what
+// limits the general tile at NV = 8).  The four partial sums of a row are
+// combined by a butterfly; lane q % RD_L stores value q of the row pair.
+static const int RD_L = 4;
+template <typename T, int NV, bool COH>
+__device__ void rowdot_tile(const BTask &t, const T *__restrict__ S, double *__restrict__ Wv, const double *xin) {
+    constexpr int RB = 2;
+    constexpr int W = 2;
+    typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type V;
+    constexpr int NG = WG / RD_L;
+    const int tid = threadIdx.x, lr = tid % RD_L, rg = tid / RD_L;
+    const int R = t.nrow, C = t.ncol;
+    const int nstep = (C + RD_L * W - 1) / (RD_L * W);
+    const int cmax = ((C - 1) / W) * W;
+    const T *base = S + t.mat;
+    for (int r0 = 0; r0 < R; r0 += NG * RB) {
+        const T *rowp[RB];
+        #pragma unroll
+        for (int b = 0; b < RB; b++) rowp[b] = base + (long long)min(r0 + rg + NG * b, R - 1) * t.ld;
+        double d[RB][NV];
+        #pragma unroll
+        for (int b = 0; b < RB; b++)
+            #pragma unroll
+            for (int v = 0; v < NV; v++) d[b][v] = 0.0;
+        V nxt[RB];
+        #pragma unroll
+        for (int b = 0; b < RB; b++) nxt[b] = *reinterpret_cast<const V *>(rowp[b] + min(lr * W, cmax));
+        for (int k = 0; k < nstep; k++) {
+            V m[RB];
+            #pragma unroll
+            for (int b = 0; b < RB; b++) m[b] = nxt[b];
+            if (k + 1 < nstep) {
+                const int cn = min(((k + 1) * RD_L + lr) * W, cmax);
+                #pragma unroll
+                for (int b = 0; b < RB; b++) nxt[b] = *reinterpret_cast<const V *>(rowp[b] + cn);
+            }
+            const int c0 = (k * RD_L + lr) * W;
+            #pragma unroll
+            for (int w = 0; w < W; w++) {
+                const int col = c0 + w;
+                const bool in = col < C;
+                double xv[NV];
+                #pragma unroll
+                for (int v = 0; v < NV; v++) xv[v] = in ? xin[col * NV + v] : 0.0;
+                #pragma unroll
+                for (int b = 0; b < RB; b++) {
+                    const double mv = in ? vget(m[b], w) : 0.0;
+                    #pragma unroll
+                    for (int v = 0; v < NV; v++) d[b][v] = fma(mv, xv[v], d[b][v]);
+                }
+            }
+        }
+        #pragma unroll
+        for (int b = 0; b < RB; b++)
+            #pragma unroll
+            for (int v = 0; v < NV; v++)
+                #pragma unroll
+                for (int sft = 1; sft < RD_L; sft <<= 1) d[b][v] += __shfl_xor(d[b][v], sft, RD_L);
+        #pragma unroll
+        for (int q = 0; q < RB * NV; q++) {
+            const int b = q / NV, v = q - b * NV;
+            const int r = r0 + rg + NG * b;
+            if (q % RD_L == lr && r < R) wst<COH>(Wv + (long long)(t.out0 + r) * NV + v, d[b][v]);
+        }
+    }
+}
+
 template <typename T, int NV, bool COH, int NT = WG>
-__device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *__restrict__ vsrc,
+__device__ void bgemv_tile(const BTask &task, const T *__restrict__ S, const int *__restrict__ vsrc,
                            double *__restrict__ Wv, const double *__restrict__ X, long long ldx, double *lds) {
+    // At NV = 8 the descriptor stays as loaded: the register allocation of
+    // the two tile paths together fits 128 VGPRs only that way (measured on
+    // the compiler's output), and the path choice is made uniform below.
+    const BTask t = NV == 8 ? task : uniform(task);
     typedef Cfg<T, NV> C_;
     typedef typename C_::V V;
     constexpr int W = C_::W, KP = C_::KP, RU = C_::RU, CPL = W * KP;
@@ -196,6 +299,16 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
     const bool do0 = t.out0 >= 0, do1 = t.out1 >= 0;
     double *xin = lds;
     double *yin = lds + TileLDS<NT>::X;
+    if constexpr (NV == 8 && NT == WG) {
+        // readfirstlane: a branch known to be uniform does not keep the
+        // registers of both paths live at once
+        if (__builtin_amdgcn_readfirstlane(t.out1) < 0) {
+            stage_vec<NV, COH, NT>(vsrc, t.v0b, t.v0e, t.v0off, C, Wv, X, ldx, xin, threadIdx.x);
+            __syncthreads();
+            rowdot_tile<T, NV, COH>(t, S, Wv, xin);
+            return;
+        }
+    }
     const T *base = S + t.mat;
     // Row blocks of RU rows; every lane issues RU*KP vector loads per block.
     // Loads are unconditional: rows past the tile are clamped to its last row
@@ -370,8 +483,9 @@ __device__ void bgemv_tile(const BTask &t, const T *__restrict__ S, const int *_
 // with all their loads of four sources in flight; the group sums are added
 // as (g0 + g1) + (g2 + g3), or g0 + g1.
 template <int NV, bool COH>
-__device__ void reduce_task(const RTask &t, const int *__restrict__ src, double *__restrict__ Wv,
+__device__ void reduce_task(const RTask &task, const int *__restrict__ src, double *__restrict__ Wv,
                             double *__restrict__ Y, long long ldy, double *lds) {
+    const RTask t = uniform(task);
     const int n = t.len * NV;
     const int K = n <= 64 ? 4 : (n <= 128 ? 2 : 1);
     const int gsz = WG / K;
@@ -546,7 +660,9 @@ __global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ 
     }
     for (int par = 1; t0 < ntask; par ^= 1) {
         for (int t = t0; t < t1; t++) {
-            const QTask q = tasks[t];
+            QTask q = tasks[t];
+            q.kind = ufl(q.kind); q.idx = ufl(q.idx); q.in0 = ufl(q.in0); q.in1 = ufl(q.in1);
+            q.out0 = ufl(q.out0); q.out1 = ufl(q.out1); q.pad_[0] = ufl(q.pad_[0]);
             if (threadIdx.x == 0) {
                 q_wait(ctr, q.in0, q.in0 >= 0 ? need[q.in0] : 0, spin_limit);
                 q_wait(ctr, q.in1, q.in1 >= 0 ? need[q.in1] : 0, spin_limit);
