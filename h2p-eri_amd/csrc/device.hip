@@ -158,7 +158,7 @@ __device__ __forceinline__ void wst(double *p, double v) {
 // NV-interleaved layout), so every thread works for any NV, and each sums
 // the slots in list order with eight loads in flight.  From the external
 // input the threads take elements and read the NV vectors.
-template <int NV, bool COH, int NT>
+template <int NV, bool COH, int NT, int XS = NV>
 __device__ __forceinline__ void stage_vec(const int *__restrict__ vsrc, int b, int e, int off, int n,
                                           const double *__restrict__ Wv, const double *__restrict__ X, long long ldx,
                                           double *dst, int tid) {
@@ -167,7 +167,7 @@ __device__ __forceinline__ void stage_vec(const int *__restrict__ vsrc, int b, i
         const double *p = X + (long long)(-s0 - 1 + off);
         for (int i = tid; i < n; i += NT) {
             #pragma unroll
-            for (int v = 0; v < NV; v++) dst[i * NV + v] = p[(long long)v * ldx + i];
+            for (int v = 0; v < NV; v++) dst[i * XS + v] = p[(long long)v * ldx + i];
         }
         return;
     }
@@ -184,7 +184,7 @@ __device__ __forceinline__ void stage_vec(const int *__restrict__ vsrc, int b, i
             for (int u = 0; u < 8; u++) sum += a[u];
         }
         for (; j < e; j++) sum += wld<COH>(base + (long long)vsrc[j] * NV);
-        dst[q] = sum;
+        dst[XS == NV ? q : (q / NV) * XS + q % NV] = sum;
     }
 }
 
@@ -210,23 +210,28 @@ template <int NT> struct TileLDS {
     static const int Y = NT == WG ? YIN_DOUBLES : YIN_DOUBLES / 4;      // in1 staging (nrow * NV)
 };
 
-// Row-side-only tile at NV = 8 (y = M x with no transposed product, the
-// U^T and B products of the sweeps): RD_L = 4 lanes share a row and each
-// holds RB = 2 rows, so a lane keeps 2 x 8 sums and reads its x values from
-// LDS per column instead of holding a register copy of its columns (what
-// limits the general tile at NV = 8).  The four partial sums of a row are
-// combined by a butterfly; lane q % RD_L stores value q of the row pair.
+// Row-side-only tile at NV = 8 (y = M x with no transposed product: the
+// down sweep and the first pass of a low-rank block): RD_L = 4 lanes share a
+// row and each holds RB = 2 rows, so a lane keeps 2 x 8 sums and reads its x
+// values from LDS per column instead of holding a register copy of its
+// columns (what limits the general tile at NV = 8).  Two register buffers
+// take the steps alternately, so the loads of the next two steps are in
+// flight while a step is consumed (a step is short, and a rotating buffer
+// would wait for its loads at every copy).  The four partial sums of a row
+// are combined by a butterfly; lane q % RD_L stores value q of the row pair.
 static const int RD_L = 4;
-template <typename T, int NV, bool COH>
+template <typename T, int NV, bool COH, int XS, int W>
 __device__ void rowdot_tile(const BTask &t, const T *__restrict__ S, double *__restrict__ Wv, const double *xin) {
     constexpr int RB = 2;
-    constexpr int W = 2;
-    typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type V;
+    // W columns per lane and step, as NL loads of E elements
+    constexpr int E = (W * sizeof(T) >= 16) ? 16 / (int)sizeof(T) : W;
+    constexpr int NL = W / E;
+    typedef typename std::conditional<sizeof(T) == 8, double2, typename std::conditional<E == 4, float4, float2>::type>::type V;
     constexpr int NG = WG / RD_L;
     const int tid = threadIdx.x, lr = tid % RD_L, rg = tid / RD_L;
     const int R = t.nrow, C = t.ncol;
     const int nstep = (C + RD_L * W - 1) / (RD_L * W);
-    const int cmax = ((C - 1) / W) * W;
+    const int cmax = ((C - 1) / E) * E;   // last load position in a row
     const T *base = S + t.mat;
     for (int r0 = 0; r0 < R; r0 += NG * RB) {
         const T *rowp[RB];
@@ -237,33 +242,48 @@ __device__ void rowdot_tile(const BTask &t, const T *__restrict__ S, double *__r
         for (int b = 0; b < RB; b++)
             #pragma unroll
             for (int v = 0; v < NV; v++) d[b][v] = 0.0;
-        V nxt[RB];
-        #pragma unroll
-        for (int b = 0; b < RB; b++) nxt[b] = *reinterpret_cast<const V *>(rowp[b] + min(lr * W, cmax));
-        for (int k = 0; k < nstep; k++) {
-            V m[RB];
+        // steps past the end re-read the last load position
+        auto load = [&](V (&buf)[RB][NL], int k) {
+            const int c0 = (k * RD_L + lr) * W;
             #pragma unroll
-            for (int b = 0; b < RB; b++) m[b] = nxt[b];
-            if (k + 1 < nstep) {
-                const int cn = min(((k + 1) * RD_L + lr) * W, cmax);
+            for (int b = 0; b < RB; b++)
                 #pragma unroll
-                for (int b = 0; b < RB; b++) nxt[b] = *reinterpret_cast<const V *>(rowp[b] + cn);
-            }
+                for (int l = 0; l < NL; l++) buf[b][l] = *reinterpret_cast<const V *>(rowp[b] + min(c0 + l * E, cmax));
+        };
+        auto consume = [&](const V (&m)[RB][NL], int k) {
             const int c0 = (k * RD_L + lr) * W;
             #pragma unroll
             for (int w = 0; w < W; w++) {
                 const int col = c0 + w;
                 const bool in = col < C;
+                const double *xc = xin + min(col, C - 1) * XS;
                 double xv[NV];
                 #pragma unroll
-                for (int v = 0; v < NV; v++) xv[v] = in ? xin[col * NV + v] : 0.0;
+                for (int v = 0; v < NV; v++) xv[v] = xc[v];
                 #pragma unroll
                 for (int b = 0; b < RB; b++) {
-                    const double mv = in ? vget(m[b], w) : 0.0;
+                    const double mv = in ? vget(m[b][w / E], w % E) : 0.0;
                     #pragma unroll
                     for (int v = 0; v < NV; v++) d[b][v] = fma(mv, xv[v], d[b][v]);
                 }
             }
+        };
+        // The loop body is straight-line: every load is issued and every step
+        // consumed (a step past the end meets zero matrix values), and the
+        // scheduling barriers keep the two buffers' loads in the same order
+        // before and inside the loop, so the compiler's wait for one buffer's
+        // loads does not also wait for the other's.
+        V ba[RB][NL], bb[RB][NL];
+        load(ba, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        load(bb, 1);
+        for (int k = 0; k < nstep; k += 2) {
+            consume(ba, k);
+            load(ba, k + 2);
+            __builtin_amdgcn_sched_barrier(0);
+            consume(bb, k + 1);
+            load(bb, k + 3);
+            __builtin_amdgcn_sched_barrier(0);
         }
         #pragma unroll
         for (int b = 0; b < RB; b++)
@@ -279,6 +299,92 @@ __device__ void rowdot_tile(const BTask &t, const T *__restrict__ S, double *__r
         }
     }
 }
+
+// Column-side-only tile at NV = 8 (M^T y with no row product: the up sweep
+// and the last pass of a low-rank block).  A lane owns W = 2 adjacent columns
+// (the plan's lanes per row cover the tile's columns at two per lane) and
+// sums over the rows of its row group; it loads RC rows per block, and two
+// register buffers take the blocks alternately, so two blocks' loads are in
+// flight while a block is consumed.  Rows past the
+// tile are clamped to its last row and meet a zero matrix value.  The column
+// sums are combined across the row groups of a wave by shuffles and across
+// the four waves through LDS, in a fixed order.
+template <typename T, int NV, bool COH, int RC>
+__device__ void coldot_tile(const BTask &t, const T *__restrict__ S, double *__restrict__ Wv, const double *yin,
+                            double *lds) {
+    constexpr int W = 2;
+    typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type V;
+    const int lpr = t.lpr;
+    const int tid = threadIdx.x, lane = tid & (lpr - 1), grp = tid / lpr, G = WG / lpr;
+    const int R = t.nrow, C = t.ncol;
+    const T *base = S + t.mat + min(lane * W, ((C - 1) / W) * W);
+    double acc[W][NV];
+    #pragma unroll
+    for (int w = 0; w < W; w++)
+        #pragma unroll
+        for (int v = 0; v < NV; v++) acc[w][v] = 0.0;
+    auto load = [&](V (&mm)[RC], int r0) {
+        #pragma unroll
+        for (int u = 0; u < RC; u++) mm[u] = *reinterpret_cast<const V *>(base + (long long)min(r0 + u * G, R - 1) * t.ld);
+    };
+    auto consume = [&](const V (&m)[RC], int r0) {
+        #pragma unroll
+        for (int u = 0; u < RC; u++) {
+            const int r = r0 + u * G;
+            const bool in = r < R;
+            const double *yr = yin + min(r, R - 1) * NV;
+            double y[NV];
+            #pragma unroll
+            for (int v = 0; v < NV; v++) y[v] = yr[v];
+            #pragma unroll
+            for (int w = 0; w < W; w++) {
+                const double mv = in ? vget(m[u], w) : 0.0;
+                #pragma unroll
+                for (int v = 0; v < NV; v++) acc[w][v] = fma(mv, y[v], acc[w][v]);
+            }
+        }
+    };
+    // two buffers take the blocks alternately, with a straight-line loop
+    // body (see rowdot_tile; rows past the end meet zero matrix values)
+    V ba[RC], bb[RC];
+    const int blk = RC * G;
+    load(ba, grp);
+    __builtin_amdgcn_sched_barrier(0);
+    load(bb, grp + blk);
+    for (int r0 = grp; r0 < R; r0 += 2 * blk) {
+        consume(ba, r0);
+        load(ba, r0 + 2 * blk);
+        __builtin_amdgcn_sched_barrier(0);
+        consume(bb, r0 + blk);
+        load(bb, r0 + 3 * blk);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    #pragma unroll
+    for (int w = 0; w < W; w++)
+        #pragma unroll
+        for (int v = 0; v < NV; v++)
+            for (int s = lpr; s < 64; s <<= 1) acc[w][v] += __shfl_xor(acc[w][v], s, 64);
+    __syncthreads();   // the staged y is dead from here on
+    double *red = lds;
+    const int wave = tid >> 6, lw = tid & 63;
+    constexpr int STRIDE = 64 * W * NV;   // doubles per wave
+    static_assert(4 * STRIDE <= LDS_DOUBLES, "column sums fit the LDS");
+    if (lw < lpr) {
+        #pragma unroll
+        for (int w = 0; w < W; w++)
+            #pragma unroll
+            for (int v = 0; v < NV; v++) red[wave * STRIDE + (lw * W + w) * NV + v] = acc[w][v];
+    }
+    __syncthreads();
+    for (int q = tid; q < C * NV; q += WG) {
+        const double s = (red[q] + red[STRIDE + q]) + (red[2 * STRIDE + q] + red[3 * STRIDE + q]);
+        wst<COH>(Wv + (long long)t.out1 * NV + q, s);
+    }
+}
+
+// NV = 8 paths: rows per block of the column tile, and the stride (doubles
+// per element) of the row-dot tile's staged x.
+template <typename T> struct Dot8 { static const int RC = 4, XS = 10, RW = 4; };
 
 template <typename T, int NV, bool COH, int NT = WG>
 __device__ void bgemv_tile(const BTask &task, const T *__restrict__ S, const int *__restrict__ vsrc,
@@ -303,9 +409,15 @@ __device__ void bgemv_tile(const BTask &task, const T *__restrict__ S, const int
         // readfirstlane: a branch known to be uniform does not keep the
         // registers of both paths live at once
         if (__builtin_amdgcn_readfirstlane(t.out1) < 0) {
-            stage_vec<NV, COH, NT>(vsrc, t.v0b, t.v0e, t.v0off, C, Wv, X, ldx, xin, threadIdx.x);
+            stage_vec<NV, COH, NT, Dot8<T>::XS>(vsrc, t.v0b, t.v0e, t.v0off, C, Wv, X, ldx, lds, threadIdx.x);
             __syncthreads();
-            rowdot_tile<T, NV, COH>(t, S, Wv, xin);
+            rowdot_tile<T, NV, COH, Dot8<T>::XS, Dot8<T>::RW>(t, S, Wv, lds);
+            return;
+        }
+        if (__builtin_amdgcn_readfirstlane(t.out0) < 0) {
+            stage_vec<NV, COH, NT>(vsrc, t.v1b, t.v1e, t.v1off, R, Wv, X, ldx, yin, threadIdx.x);
+            __syncthreads();
+            coldot_tile<T, NV, COH, Dot8<T>::RC>(t, S, Wv, yin, lds);
             return;
         }
     }
