@@ -8,7 +8,9 @@ uploaded once.  A step is one H2 matvec y = A_H2 x on the uncontracted
 density, replayed from a captured HIP graph, with x and y resident in HBM.
 Timed forms: FP64 storage (the drop-in) and FP32-stored U/B (FP64 near field
 and accumulation), each for 1 density and for batches of 2, 4 and 8 densities
-per pass over the matrix data.  `value` is the FP32-stored single-density
+per pass over the matrix data.  The single-density form runs the configuration
+the project's autotuner picks for it (FP32-stored), each batch width the one it
+picks for that width and precision.  `value` is the FP32-stored single-density
 form's algorithmic HBM throughput; the batched forms are reported beside it.
 
 Prints one JSON line on stdout; progress goes to stderr.
@@ -307,6 +309,16 @@ def main():
         log(f"[bench] autotune (FP32-stored, 1 density per pass): {tune}")
     kernel_name = "queue_kernel<1>" if tune["executor"] == 1 else "stage_kernel<1>"
 
+    def apply_config(cfg):
+        h.set_executor(cfg["executor"])
+        h.set_tile_bytes(cfg["tile_bytes"])
+        h.set_schedule(cfg["schedule"])
+        h.set_wave_tile_bytes(cfg["wave_tile_bytes"])
+
+    def config_record(cfg):
+        return {"executor": ["per stage", "persistent"][cfg["executor"]], "tile_bytes": int(cfg["tile_bytes"]),
+                "schedule": cfg["schedule"], "wave_tile_bytes": int(cfg["wave_tile_bytes"])}
+
     res = {}
     for bits in (64, 32):
         h.set_precision(bits)
@@ -315,11 +327,16 @@ def main():
         mat = nb["U"] + nb["B"] + nb["D"]
         r = dict(resident=resident, nb=nb, batch={})
         for nv in batches:
+            # one density: the headline configuration; a batch: the autotuner's
+            # choice for that width and storage precision
+            cfg = tune if (nv == 1 or args.tuned) else h.autotune(nv=nv, reps=5)
+            apply_config(cfg)
             h.graph_capture("matvec", nv, X_dev.data_ptr(), Y_dev.data_ptr())
             steps = args.steps if nv == 1 else max(10, args.steps // 2)
             ms = time_graph(h, torch, stream, steps, args.warmup)
             alg = mat + nv * nb["vec"]
-            r["batch"][nv] = dict(ms=ms, ms_per_density=ms / nv, alg=alg, GBs=alg / ms / 1e6)
+            r["batch"][nv] = dict(ms=ms, ms_per_density=ms / nv, alg=alg, GBs=alg / ms / 1e6,
+                                  config=config_record(cfg))
             if nv == 1:
                 Y1 = Y_dev[0].cpu().numpy()
             else:   # every density of the batch equals the single-density product (linearity)
@@ -330,6 +347,7 @@ def main():
             log(f"[bench] FP{bits}-stored, {nv} densities/pass: {ms:.4f} ms/pass, {ms / nv:.4f} ms/density, "
                 f"{alg / ms / 1e6:.0f} GB/s algorithmic")
         r["y"] = Y1
+        apply_config(tune)
         r["prof"] = launch_profile(h, torch, X_dev[0], Y_dev[0], stream, 10)
         h.build_coulomb_dev(D_dev.data_ptr(), J_dev.data_ptr(), stream.cuda_stream)
         torch.cuda.synchronize()
@@ -413,9 +431,7 @@ def main():
                    "alg_MB_fp32": round(r32["batch"][1]["alg"] / 1e6, 2),
                    "alg_MB_fp64": round(r64["batch"][1]["alg"] / 1e6, 2),
                    "cache_resident": r32["batch"][1]["alg"] < MALL_BYTES, "timing": "hipGraph replays",
-                   "autotune": {"executor": ["per stage", "persistent"][tune["executor"]],
-                                "tile_bytes": int(tune["tile_bytes"]), "schedule": tune["schedule"],
-                                "wave_tile_bytes": int(tune["wave_tile_bytes"])}},
+                   "autotune": config_record(tune)},
         "fp64": {"ms": round(fp64_plain, 5), "GBs": round(r64["batch"][1]["GBs"], 2)},
         "fp32_speedup_vs_fp64": round(fp64_plain / r32["batch"][1]["ms"], 4),
         "batched": {f"fp{bits}": {str(nv): {k: (float(f"{v:.5g}") if isinstance(v, float) else v)

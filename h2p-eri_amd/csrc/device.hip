@@ -1108,6 +1108,7 @@ struct Planner {
     int nf_schedule;
     int rchunk;  // output elements per reduce task
     double wave_bytes;   // matrix parts up to this size become wave tiles (0: none)
+    bool split2;         // two-sided tiles become two one-sided tasks (staged executor, NV = 8)
     int maxc;   // max tile columns for this batch width
     std::vector<VVec> vv;
     std::vector<PTile> tiles;
@@ -1116,9 +1117,9 @@ struct Planner {
     std::vector<double> sload;
     int nst = 0, sG = 0, sS = 0;   // stages: main 0..sG, near-field stream sS..nst-1 (schedule 2)
 
-    Planner(H2ERI_p h_, const Storage &s_, Plan &p_, int nv_, double tb, int nfs, int rch, double wb)
+    Planner(H2ERI_p h_, const Storage &s_, Plan &p_, int nv_, double tb, int nfs, int rch, double wb, bool sp)
         : h(h_), S(s_), P(p_), nv(nv_), tile_bytes(tb), nf_schedule(nfs), rchunk(rch), wave_bytes(wb),
-          maxc(64 * cpl_of(nv_)) {}
+          split2(sp && nv_ == 8), maxc(64 * cpl_of(nv_)) {}
 
     long long walloc(long long n) {
         long long o = P.nW;
@@ -1438,9 +1439,41 @@ struct Planner {
             else (p.f32 ? S32 : S64)[p.st].push_back(p.t);
         }
         auto key = [](const BTask &t) { return (long long)t.nrow * t.ncol; };
+        // Staged executor at 8 densities per pass: a tile with both outputs
+        // runs as two one-sided tasks (row-dot and column tile; together
+        // faster than the two-sided tile at this width, whose row sums need a
+        // reduction over up to 64 lanes per row).  The halves of eight
+        // consecutive tiles are placed eight blocks apart: workgroups go to the
+        // 8 XCDs round robin, so the two halves of a tile run on one XCD at
+        // about the same time and the second read of the matrix is mostly
+        // served by its L2.  (The persistent executor's claim order gives no
+        // such placement, and there the split is slower.)
+        auto split_two_sided = [&](std::vector<BTask> &L) {
+            if (!split2) return;
+            std::vector<BTask> out, ga, gb;
+            out.reserve(2 * L.size());
+            for (const BTask &t : L) {
+                if (t.out0 < 0 || t.out1 < 0) { out.push_back(t); continue; }
+                BTask a = t, b = t;
+                a.out1 = -1;
+                b.out0 = -1;
+                ga.push_back(a);
+                gb.push_back(b);
+                if (ga.size() == 8) {
+                    out.insert(out.end(), ga.begin(), ga.end());
+                    out.insert(out.end(), gb.begin(), gb.end());
+                    ga.clear();
+                    gb.clear();
+                }
+            }
+            for (size_t i = 0; i < ga.size(); i++) { out.push_back(ga[i]); out.push_back(gb[i]); }
+            L.swap(out);
+        };
         for (int st = 0; st < nst; st++) {
             std::stable_sort(S64[st].begin(), S64[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
             std::stable_sort(S32[st].begin(), S32[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
+            split_two_sided(S64[st]);
+            split_two_sided(S32[st]);
             std::stable_sort(W64[st].begin(), W64[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
             std::stable_sort(W32[st].begin(), W32[st].end(), [&](const BTask &x, const BTask &y) { return key(x) > key(y); });
             Stage s;
@@ -1529,10 +1562,11 @@ struct Planner {
 
 }  // namespace
 
-// Wave tiles (parts up to wave_bytes) belong to the staged executor only.
+// Wave tiles (parts up to wave_bytes) and split two-sided tiles belong to the
+// staged executor only.
 static bool build_plan(H2ERI_p h, const Storage &S, Plan &P, int nv, double tb, int nfs, int executor,
                        double wave_bytes) {
-    Planner pl(h, S, P, nv, tb, nfs, rchunk_for(nv), executor == 0 ? wave_bytes : 0.0);
+    Planner pl(h, S, P, nv, tb, nfs, rchunk_for(nv), executor == 0 ? wave_bytes : 0.0, executor == 0);
     return pl.build();
 }
 

@@ -140,11 +140,12 @@ def test_queue_order_covers_every_dependency(tmp_path, seed, n, leaf, mr, shape)
             y, bad = h.queue_emulate(x, 64, workgroups=wg)
             assert bad == 0
             assert relerr(y, yo) < 1e-13
-    X = rng.standard_normal((4, h.n))
-    Y, bad = h.queue_emulate(X, 32, workgroups=256)
-    assert bad == 0
-    for v in range(4):
-        assert relerr(Y[v], o.matvec(X[v])) < 2e-6
+    for nv in (4, 8):
+        X = rng.standard_normal((nv, h.n))
+        Y, bad = h.queue_emulate(X, 32, workgroups=256)
+        assert bad == 0
+        for v in range(nv):
+            assert relerr(Y[v], o.matvec(X[v])) < 2e-6
 
 
 def test_exchange_needs_molecule(tmp_path):
