@@ -775,11 +775,14 @@ __global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ 
             QTask q = tasks[t];
             q.kind = ufl(q.kind); q.idx = ufl(q.idx); q.in0 = ufl(q.in0); q.in1 = ufl(q.in1);
             q.out0 = ufl(q.out0); q.out1 = ufl(q.out1); q.pad_[0] = ufl(q.pad_[0]);
+            // the next chunk, claimed while this one runs: the ticket stays in
+            // a register of thread 0 and reaches LDS after the task, so its
+            // atomic's latency is not on the task's path
+            int next_ticket = 0;
             if (threadIdx.x == 0) {
+                if (t == t0) next_ticket = __hip_atomic_fetch_add(ctr + QC_TICKET, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 q_wait(ctr, q.in0, q.in0 >= 0 ? need[q.in0] : 0, spin_limit);
                 q_wait(ctr, q.in1, q.in1 >= 0 ? need[q.in1] : 0, spin_limit);
-                // the next chunk, claimed while this one runs
-                if (t == t0) s_next[par] = __hip_atomic_fetch_add(ctr + QC_TICKET, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             __syncthreads();
             if constexpr (GROUP) {
@@ -792,6 +795,7 @@ __global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ 
                 else if (q.kind == 1) bgemv_tile<float, NV, true>(t32[q.idx], S32, vsrc, Wv, X, ldx, lds);
                 else reduce_task<NV, true>(rt[q.idx], rsrc, Wv, Y, ldx, lds);
             }
+            if (threadIdx.x == 0 && t == t0) s_next[par] = next_ticket;
             wait_vm_stores();
             __syncthreads();
             if (threadIdx.x == 0) {
