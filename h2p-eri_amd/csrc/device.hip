@@ -980,13 +980,18 @@ struct DevPlan {
 static const int NV_VARIANTS[4] = {1, 2, 4, 8};
 static inline int nv_index(int nv) { return nv == 1 ? 0 : (nv == 2 ? 1 : (nv == 4 ? 2 : 3)); }
 
-struct DevState {
-    int dev = -1;
-    int ub_bits = 64;
+// Tuning configuration of one batch width.
+struct Tune {
     double tile_bytes = 128.0 * 1024.0;
     int nf_schedule = 3;      // near-field placement, see H2ERI_set_schedule (two groups on a second stream)
     int executor = 0;         // 0: one launch per stage, 1: one persistent launch per pass (H2ERI_set_executor)
     double wave_bytes = 32768.0;   // staged executor: matrix parts up to this size run as wave tiles
+};
+
+struct DevState {
+    int dev = -1;
+    int ub_bits = 64;
+    Tune cfg[4];              // per batch width (nv_index); the setters change all four, the autotuner one
     Storage st;
     bool st_ready = false;
     double *S64 = nullptr;
@@ -1025,17 +1030,20 @@ static void free_dev(void *&p) {
 }
 
 static void release_graph(DevState *d);
-static void release_plans(DevState *d) {
+static void release_plan(DevState *d, int i) {
     release_graph(d);
-    for (DevPlan &p : d->plans) {
-        void **ps[] = {(void **)&p.Wv, (void **)&p.d_t64, (void **)&p.d_t32, (void **)&p.d_w64, (void **)&p.d_w32,
-                       (void **)&p.d_rt, (void **)&p.d_rsrc, (void **)&p.d_vsrc, (void **)&p.d_q, (void **)&p.d_need,
-                       (void **)&p.d_ctr};
-        for (void **q : ps) free_dev(*q);
-        p.ready = false;
-        p.q_ready = false;
-        p.P = Plan();
-    }
+    DevPlan &p = d->plans[i];
+    void **ps[] = {(void **)&p.Wv, (void **)&p.d_t64, (void **)&p.d_t32, (void **)&p.d_w64, (void **)&p.d_w32,
+                   (void **)&p.d_rt, (void **)&p.d_rsrc, (void **)&p.d_vsrc, (void **)&p.d_q, (void **)&p.d_need,
+                   (void **)&p.d_ctr};
+    for (void **q : ps) free_dev(*q);
+    p.ready = false;
+    p.q_ready = false;
+    p.P = Plan();
+}
+
+static void release_plans(DevState *d) {
+    for (int i = 0; i < 4; i++) release_plan(d, i);
 }
 
 static void release_graph(DevState *d) {
@@ -1909,11 +1917,12 @@ static DevPlan *get_plan(H2ERI_p h, int nv) {
     if (!upload_storage(h)) return nullptr;
     DevState *d = dstate(h);
     DevPlan &dp = d->plans[nv_index(nv)];
+    const Tune &c = d->cfg[nv_index(nv)];
     if (dp.ready) {
-        if (d->executor == 1 && !ensure_queue(d, dp)) return nullptr;
+        if (c.executor == 1 && !ensure_queue(d, dp)) return nullptr;
         return &dp;
     }
-    if (!build_plan(h, d->st, dp.P, nv, d->tile_bytes, d->nf_schedule, d->executor, d->wave_bytes)) return nullptr;
+    if (!build_plan(h, d->st, dp.P, nv, c.tile_bytes, c.nf_schedule, c.executor, c.wave_bytes)) return nullptr;
     const Plan &P = dp.P;
     auto alloc_copy = [&](void **dst, const void *src, size_t bytes) -> bool {
         if (bytes == 0) bytes = 64;
@@ -1932,7 +1941,7 @@ static DevPlan *get_plan(H2ERI_p h, int nv) {
     if (!alloc_copy((void **)&dp.d_rsrc, P.rsrc.empty() ? nullptr : P.rsrc.data(), P.rsrc.size() * sizeof(int))) return nullptr;
     if (!alloc_copy((void **)&dp.d_vsrc, P.vsrc.empty() ? nullptr : P.vsrc.data(), P.vsrc.size() * sizeof(int))) return nullptr;
     dp.ready = true;
-    if (d->executor == 1 && !ensure_queue(d, dp)) return nullptr;
+    if (c.executor == 1 && !ensure_queue(d, dp)) return nullptr;
     return &dp;
 }
 
@@ -1997,7 +2006,7 @@ static bool run_queue(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, doub
 // Enqueue one batched matvec of nv vectors: X (nv x N, vector-major) -> Y.
 // In phase-timing mode an event pair brackets every launch.
 static bool run_matvec(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, double *Y, hipStream_t s) {
-    if (d->executor == 1) return run_queue(h, d, dp, X, Y, s);
+    if (d->cfg[nv_index(dp.P.nv)].executor == 1) return run_queue(h, d, dp, X, Y, s);
     const Plan &P = dp.P;
     const long long ld = h->krnl_mat_size;
     const bool tim = d->phase_timing && !d->capturing;
@@ -2333,7 +2342,8 @@ void H2ERI_matvec_bytes(H2ERI_p h, double *out6) {
 void H2ERI_set_schedule(H2ERI_p h, const int mode) {
     if (!h || mode < 0 || mode > 17) { set_error(ERR_ARG, "H2ERI_set_schedule: 0 to 17"); return; }
     DevState *d = dstate_get(h);
-    if (d->nf_schedule != mode) { d->nf_schedule = mode; release_plans(d); }
+    for (int i = 0; i < 4; i++)
+        if (d->cfg[i].nf_schedule != mode) { d->cfg[i].nf_schedule = mode; release_plan(d, i); }
 }
 
 // Executor of the matvec (re-plans): 0 launches every stage separately (the
@@ -2346,10 +2356,11 @@ void H2ERI_set_schedule(H2ERI_p h, const int mode) {
 void H2ERI_set_executor(H2ERI_p h, const int mode) {
     if (!h || (mode != 0 && mode != 1)) { set_error(ERR_ARG, "H2ERI_set_executor: 0 or 1"); return; }
     DevState *d = dstate_get(h);
-    if (d->executor != mode) { d->executor = mode; release_plans(d); }
+    for (int i = 0; i < 4; i++)
+        if (d->cfg[i].executor != mode) { d->cfg[i].executor = mode; release_plan(d, i); }
 }
 
-int H2ERI_get_executor(H2ERI_p h) { return h && h->dev ? dstate(h)->executor : 0; }
+int H2ERI_get_executor(H2ERI_p h) { return h && h->dev ? dstate(h)->cfg[0].executor : 0; }
 
 // Waits for the device and reports whether a wait of the persistent executor
 // exceeded its bound since the last call (which would mean a planning fault:
@@ -2396,14 +2407,15 @@ int H2ERI_queue_stats(H2ERI_p h, const int nv, double *out4) {
 void H2ERI_set_wave_tile_bytes(H2ERI_p h, const double bytes) {
     if (!h || bytes < 0) { set_error(ERR_ARG, "H2ERI_set_wave_tile_bytes: bytes >= 0"); return; }
     DevState *d = dstate_get(h);
-    if (d->wave_bytes != bytes) { d->wave_bytes = bytes; release_plans(d); }
+    for (int i = 0; i < 4; i++)
+        if (d->cfg[i].wave_bytes != bytes) { d->cfg[i].wave_bytes = bytes; release_plan(d, i); }
 }
 
 // Tile size of the bi-GEMV tasks in bytes (tuning knob; re-plans).
 void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
     if (!h || bytes < 1024) return;
     DevState *d = dstate_get(h);
-    d->tile_bytes = bytes;
+    for (int i = 0; i < 4; i++) d->cfg[i].tile_bytes = bytes;
     release_plans(d);
 }
 
@@ -2426,8 +2438,10 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
     const int scheds[] = {0, 2, 3, 4};
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { set_error(ERR_DEVICE, "autotune: events"); return -1; }
-    double best = 1e300, best_tile = d->tile_bytes, best_wave = d->wave_bytes;
-    int best_sched = d->nf_schedule, best_ex = d->executor;
+    const int vi = nv_index(nv);
+    Tune &c = d->cfg[vi];
+    Tune best_cfg = c;
+    double best = 1e300;
     const double waves[] = {0.0, 32768.0};
     const bool was_timing = d->phase_timing;
     d->phase_timing = false;
@@ -2436,11 +2450,11 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
     for (int wi = 0; ok && wi < (ex == 0 ? 2 : 1); wi++)
     for (double tb : tiles) {
         for (int sc : scheds) {
-            release_plans(d);
-            d->executor = ex;
-            d->wave_bytes = ex == 0 ? waves[wi] : 0.0;
-            d->tile_bytes = tb;
-            d->nf_schedule = sc;
+            release_plan(d, vi);
+            c.executor = ex;
+            c.wave_bytes = ex == 0 ? waves[wi] : 0.0;
+            c.tile_bytes = tb;
+            c.nf_schedule = sc;
             ok = matvec_multi_dev_impl(h, nv, d->d_x, d->d_y, nullptr);   // plans, uploads, warms up
             ok = ok && hipEventRecord(e0, nullptr) == hipSuccess;
             for (int r = 0; ok && r < reps; r++) ok = matvec_multi_dev_impl(h, nv, d->d_x, d->d_y, nullptr);
@@ -2448,23 +2462,22 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
             float ms = 0.f;
             ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
             if (!ok) break;
-            if (ms / reps < best) { best = ms / reps; best_tile = tb; best_sched = sc; best_ex = ex; best_wave = d->wave_bytes; }
+            if (ms / reps < best) { best = ms / reps; best_cfg = c; }
         }
         if (!ok) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     d->phase_timing = was_timing;
-    release_plans(d);
-    d->tile_bytes = best_tile;
-    d->nf_schedule = best_sched;
-    d->executor = best_ex;
-    d->wave_bytes = best_wave;
+    release_plan(d, vi);
+    c = best_cfg;
     if (!ok) {
         if (!has_error()) set_error(ERR_DEVICE, "autotune: timing failed");
         return -1;
     }
-    if (out5) { out5[0] = best; out5[1] = best_tile; out5[2] = best_sched; out5[3] = best_ex; out5[4] = best_wave; }
+    if (out5) {
+        out5[0] = best; out5[1] = c.tile_bytes; out5[2] = c.nf_schedule; out5[3] = c.executor; out5[4] = c.wave_bytes;
+    }
     return 0;
 }
 
@@ -2520,7 +2533,7 @@ void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_
         if (!upload_storage(h)) { group_free(g); return nullptr; }
         DevState *d = dstate(h);
         DevPlan &dp = g->plans[k];
-        if (!build_plan(h, d->st, dp.P, 1, d->tile_bytes, d->nf_schedule, 1, 0.0)) { group_free(g); return nullptr; }
+        if (!build_plan(h, d->st, dp.P, 1, d->cfg[0].tile_bytes, d->cfg[0].nf_schedule, 1, 0.0)) { group_free(g); return nullptr; }
         if (!queue_graph(dp.P, 1, G, graphs[k])) { group_free(g); return nullptr; }
         const Plan &P = dp.P;
         auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
@@ -2620,7 +2633,8 @@ int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double 
     S.build(h, ub_bits == 32);
     Plan P;
     DevState *ds = dstate_get(h);
-    if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, 0, ds->wave_bytes)) return -1;
+    const Tune &c = ds->cfg[nv_index(nv)];
+    if (!build_plan(h, S, P, nv, c.tile_bytes, c.nf_schedule, 0, c.wave_bytes)) return -1;
     const long long N = h->krnl_mat_size;
     std::vector<double> Wv((size_t)P.nW * nv, std::nan(""));
     std::vector<int> wmark((size_t)P.nW, -1), rmark((size_t)P.nW, -1);
@@ -2739,7 +2753,8 @@ int H2ERI_queue_emulate(H2ERI_p h, const int ub_bits, const int nv, const int wo
     S.build(h, ub_bits == 32);
     Plan P;
     DevState *ds = dstate_get(h);
-    if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, 1, 0.0)) return -1;
+    const Tune &c = ds->cfg[nv_index(nv)];
+    if (!build_plan(h, S, P, nv, c.tile_bytes, c.nf_schedule, 1, 0.0)) return -1;
     QPlan Q;
     if (!build_queue(P, nv, workgroups, Q)) return -1;
     const long long N = h->krnl_mat_size;
@@ -2846,7 +2861,8 @@ int H2ERI_plan_check(H2ERI_p h, const int ub_bits) {
     for (int nv : NV_VARIANTS)
         for (int ex = 0; ex < 2; ex++) {
             Plan P;
-            if (!build_plan(h, S, P, nv, ds->tile_bytes, ds->nf_schedule, ex, ds->wave_bytes)) return -1;
+            const Tune &c = ds->cfg[nv_index(nv)];
+            if (!build_plan(h, S, P, nv, c.tile_bytes, c.nf_schedule, ex, c.wave_bytes)) return -1;
             if (nv == 1 && ex == 0) tasks = (int)(P.t64.size() + P.t32.size() + P.w64.size() + P.w32.size());
         }
     return tasks;

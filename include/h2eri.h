@@ -351,7 +351,7 @@ void H2ERI_set_schedule(H2ERI_p h2eri, const int mode);
  * timing sees one launch per pass: its time goes to the near-field slot of
  * H2ERI_get_phase_ms and of the reference's timers. */
 void H2ERI_set_executor(H2ERI_p h2eri, const int mode);
-int  H2ERI_get_executor(H2ERI_p h2eri);
+int  H2ERI_get_executor(H2ERI_p h2eri);   /* of single-vector passes */
 
 /* Synchronises the device and reports whether a dependency wait of the
  * persistent executor exceeded its bound (about a second) since the last
@@ -381,9 +381,11 @@ void  H2ERI_group_destroy(void *group);
 /* Autotuner: times nv-vector device matvecs (reps calls after a warm-up)
  * for every executor (H2ERI_set_executor) x tile size {64, 128, 256} KB x
  * near-field schedule {0, 2, 3, 4} x (staged executor) wave-tile limit {0,
- * 32} KB, and keeps the fastest combination for later calls.  out5 = {ms per
- * call, tile bytes, schedule, executor, wave-tile bytes} of the combination
- * kept.  Returns 0, -1 on error. */
+ * 32} KB, and keeps the fastest combination for later passes of nv vectors
+ * (each batch width 1, 2, 4, 8 has a configuration of its own; the tuning
+ * setters above change all four).  out5 = {ms per call, tile bytes,
+ * schedule, executor, wave-tile bytes} of the combination kept.  Returns 0,
+ * -1 on error. */
 int H2ERI_autotune(H2ERI_p h2eri, const int nv, const int reps, double *out5);
 
 /* Staged executor: matrix parts up to `bytes` (default 32768) run as wave

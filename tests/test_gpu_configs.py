@@ -132,4 +132,13 @@ def test_autotune_keeps_results(tmp_path):
     assert res["ms"] > 0 and res["tile_bytes"] in (65536, 131072, 262144) and res["schedule"] in (0, 2, 3, 4)
     assert res["executor"] in (0, 1) and h.executor() == res["executor"] and res["wave_tile_bytes"] in (0, 32768)
     x = np.random.default_rng(2).standard_normal(h.n)
-    assert relerr(h.matvec(x), oracle_bind.Oracle(str(tmp_path / "a.rep")).matvec(x)) < 1e-13
+    o = oracle_bind.Oracle(str(tmp_path / "a.rep"))
+    assert relerr(h.matvec(x), o.matvec(x)) < 1e-13
+    # every batch width keeps a configuration of its own: tuning 8 vectors per
+    # pass leaves the single-vector configuration alone
+    res8 = h.autotune(nv=8, reps=3)
+    assert res8["executor"] in (0, 1) and h.executor() == res["executor"]
+    X = np.random.default_rng(3).standard_normal((9, h.n))   # one pass of 8, one of 1
+    Y = h.matvec_multi(X)
+    for v in range(9):
+        assert relerr(Y[v], o.matvec(X[v])) < 1e-13
