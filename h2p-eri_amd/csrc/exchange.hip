@@ -100,9 +100,10 @@ static const int KSPLIT = 256;  // local m rows per bra task (LDS block KSPLIT x
 // Ket on MFMA: G[n][t][q] = sum over the entries (c, p) of q of R[c][t] D[n][p].
 // Per q this is a GEMM (rows t, columns n, inner dimension the entries of q).
 // Task = (term, t tile of 16, 64 functions n (16 per wave), KQC functions q).
-// A wave computes four consecutive q together and stores each (n, t) row of
-// the four as one aligned 32-byte write (single 8-byte writes scattered over
-// 16 rows cost four times their bytes in HBM write traffic).
+// A wave computes four consecutive q together, walking their entry lists in
+// step, and stores each (n, t) row of the four as one aligned 32-byte write
+// (single 8-byte writes scattered over 16 rows cost four times their bytes in
+// HBM write traffic).
 __global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
                                                const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
                                                const int *__restrict__ qptr, const int2 *__restrict__ qent,
@@ -117,7 +118,10 @@ __global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, c
     const double *R = (t.Rsrc == KS_PB ? PB : (t.Rsrc == KS_CB ? CB : SC)) + t.Roff;
     const int nl = nl0 + (lane & 15);
     const bool nv = nl < ni.nbf;
-    const double *Drow = D + (size_t)(nv ? bfidx[ni.bf_off + nl] : 0) * nbf;
+    // D is symmetric (the entry point's contract), so D[n][p] is read as
+    // D[p][n]: the 16 lanes of an entry (consecutive functions n of the node)
+    // read along one row of D instead of down a column
+    const int bfn = nv ? bfidx[ni.bf_off + nl] : 0;
     const int ta = tk.t0 + (lane & 15);   // A operand row (t)
     const bool tv = ta < t.kk;
     const int kl = lane >> 4;
@@ -125,32 +129,42 @@ __global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, c
     double *G = SC + t.Goff;
     const int qend = min(tk.pad_ + KQC, nj.nbf);
     for (int qg = tk.pad_; qg < qend; qg += 4) {
-        d4 acc[4];
+        // The entry lists of the four q are walked together (entries past a
+        // list are zeros), so each iteration has the gathers of four
+        // independent accumulations in flight.
+        int qa[4], qb[4], len = 0;
         #pragma unroll
         for (int j = 0; j < 4; j++) {
-            acc[j] = d4{0.0, 0.0, 0.0, 0.0};
             const int ql = qg + j;
-            if (ql >= qend) continue;
-            const int q0 = qptr[nj.q_ptr + ql], q1 = qptr[nj.q_ptr + ql + 1];
-            // four MFMA steps per iteration, every index and operand load
-            // issued before the first of them (entries past the list: zeros)
-            for (int e0 = q0; e0 < q1; e0 += 16) {
-                int2 cp[4];
+            qa[j] = ql < qend ? qptr[nj.q_ptr + ql] : 0;
+            qb[j] = ql < qend ? qptr[nj.q_ptr + ql + 1] : 0;
+            len = max(len, qb[j] - qa[j]);
+        }
+        d4 acc[4];
+        #pragma unroll
+        for (int j = 0; j < 4; j++) acc[j] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int e0 = 0; e0 < len; e0 += 8) {
+            int2 cp[4][2];
+            #pragma unroll
+            for (int j = 0; j < 4; j++)
                 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int e = e0 + 4 * u + kl;
-                    cp[u] = e < q1 ? qent[e] : make_int2(-1, 0);
+                for (int u = 0; u < 2; u++) {
+                    const int e = qa[j] + e0 + 4 * u + kl;
+                    cp[j][u] = e < qb[j] ? qent[e] : make_int2(-1, 0);
                 }
-                double a[4], b[4];
+            double a[4][2], b[4][2];
+            #pragma unroll
+            for (int j = 0; j < 4; j++)
                 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const bool ev = cp[u].x >= 0;
-                    a[u] = (ev && tv) ? R[(long long)cp[u].x * t.Rrs + (long long)ta * t.Rcs] : 0.0;
-                    b[u] = (ev && nv) ? Drow[cp[u].y] : 0.0;
+                for (int u = 0; u < 2; u++) {
+                    const bool ev = cp[j][u].x >= 0;
+                    a[j][u] = (ev && tv) ? R[(long long)cp[j][u].x * t.Rrs + (long long)ta * t.Rcs] : 0.0;
+                    b[j][u] = (ev && nv) ? D[(size_t)cp[j][u].y * nbf + bfn] : 0.0;
                 }
+            #pragma unroll
+            for (int u = 0; u < 2; u++)
                 #pragma unroll
-                for (int u = 0; u < 4; u++) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc[j], 0, 0, 0);
-            }
+                for (int j = 0; j < 4; j++) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j][u], b[j][u], acc[j], 0, 0, 0);
         }
         #pragma unroll
         for (int r = 0; r < 4; r++) {
