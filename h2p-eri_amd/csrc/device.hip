@@ -305,10 +305,10 @@ __device__ void rowdot_tile(const BTask &t, const T *__restrict__ S, double *__r
 // (the plan's lanes per row cover the tile's columns at two per lane) and
 // sums over the rows of its row group; it loads RC rows per block, and two
 // register buffers take the blocks alternately, so two blocks' loads are in
-// flight while a block is consumed.  Rows past the
-// tile are clamped to its last row and meet a zero matrix value.  The column
-// sums are combined across the row groups of a wave by shuffles and across
-// the four waves through LDS, in a fixed order.
+// flight while a block is consumed.  Rows past the tile are clamped to its
+// last row and meet a zero matrix value.  The column sums are combined across
+// the row groups of a wave by shuffles and across the four waves through LDS,
+// in a fixed order.
 template <typename T, int NV, bool COH, int RC>
 __device__ void coldot_tile(const BTask &t, const T *__restrict__ S, double *__restrict__ Wv, const double *yin,
                             double *lds) {
@@ -382,8 +382,10 @@ __device__ void coldot_tile(const BTask &t, const T *__restrict__ S, double *__r
     }
 }
 
-// NV = 8 paths: rows per block of the column tile, and the stride (doubles
-// per element) of the row-dot tile's staged x.
+// NV = 8 paths: rows per block of the column tile (RC); the stride in
+// doubles per element of the row-dot tile's staged x (XS, padded so that the
+// four lanes of a row read different LDS banks) and its columns per lane and
+// step (RW).
 template <typename T> struct Dot8 { static const int RC = 4, XS = 10, RW = 4; };
 
 template <typename T, int NV, bool COH, int NT = WG>
