@@ -96,6 +96,7 @@ static const int KWG = 256;
 __host__ __device__ inline int gld(int nbf_j) { return (nbf_j + 3) & ~3; }
 static const int KQC = 16;      // column functions q per ket task
 static const int KSPLIT = 256;  // local m rows per bra task (LDS block KSPLIT x 16)
+static const int KEB = 4;       // entry blocks of 16 sharing one walk over t in the bra
 
 // Ket on MFMA: G[n][t][q] = sum over the entries (c, p) of q of R[c][t] D[n][p].
 // Per q this is a GEMM (rows t, columns n, inner dimension the entries of q).
@@ -181,7 +182,9 @@ __global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, c
 // KSPLIT functions m); wave w takes the functions n = w, w + 4, ... and adds
 // its 16 x 16 output tiles into an LDS block of the split with FP64 LDS
 // atomics (tiles of different n may share an m); the block is added into K
-// at the end.  The t loop issues the operands of four MFMA steps at once.
+// at the end.  The t loop issues the operands of four MFMA steps at once,
+// and one walk over t serves up to KEB blocks of 16 entries of n, which share
+// its G operands.
 __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
                                                const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
                                                const int *__restrict__ nspl, const int2 *__restrict__ nent,
@@ -205,28 +208,49 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
     for (int nl = w; nl < ni.nbf; nl += KWG / 64) {
         const int eb = nspl[ni.s_ptr + nl * ns1 + split], ee = nspl[ni.s_ptr + nl * ns1 + split + 1];
         const double *Gn = G + (size_t)nl * t.kk * gld(nj.nbf) + ql;
-        for (int e0 = eb; e0 < ee; e0 += 16) {
-            const int e = e0 + (lane & 15);
-            const bool ev = e < ee;
-            const int row = ev ? nent[e].x : 0;
-            const double *Lr = L + (long long)row * t.Lld;
-            d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+        // up to KEB blocks of 16 entries per walk over t, so that each G
+        // operand load feeds KEB accumulations
+        for (int e0 = eb; e0 < ee; e0 += 16 * KEB) {
+            const double *Lr[KEB];
+            bool evj[KEB];
+            #pragma unroll
+            for (int j = 0; j < KEB; j++) {
+                const int e = e0 + 16 * j + (lane & 15);
+                evj[j] = e < ee;
+                Lr[j] = L + (long long)(evj[j] ? nent[e].x : 0) * t.Lld;
+            }
+            const int nblk = min(KEB, (ee - e0 + 15) / 16);   // blocks with entries (wave-uniform)
+            d4 acc[KEB];
+            #pragma unroll
+            for (int j = 0; j < KEB; j++) acc[j] = d4{0.0, 0.0, 0.0, 0.0};
             for (int t0 = 0; t0 < t.kk; t0 += 16) {
-                double a[4], b[4];
+                double b[4];
                 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int tt = t0 + 4 * u + kl;
-                    const bool tv = tt < t.kk;
-                    a[u] = (ev && tv) ? Lr[tt] : 0.0;
-                    b[u] = (qv && tv) ? Gn[(size_t)tt * gld(nj.nbf)] : 0.0;
+                    b[u] = (qv && tt < t.kk) ? Gn[(size_t)tt * gld(nj.nbf)] : 0.0;
                 }
                 #pragma unroll
-                for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+                for (int j = 0; j < KEB; j++) {
+                    if (j >= nblk) break;
+                    double a[4];
+                    #pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int tt = t0 + 4 * u + kl;
+                        a[u] = (evj[j] && tt < t.kk) ? Lr[j][tt] : 0.0;
+                    }
+                    #pragma unroll
+                    for (int u = 0; u < 4; u++) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc[j], 0, 0, 0);
+                }
             }
             #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int er = e0 + kl + 4 * r;
-                if (er < ee && qv) atomicAdd(&Kb[(nent[er].y - ml0) * 16 + (lane & 15)], acc[r]);
+            for (int j = 0; j < KEB; j++) {
+                if (j >= nblk) break;
+                #pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int er = e0 + 16 * j + kl + 4 * r;
+                    if (er < ee && qv) atomicAdd(&Kb[(nent[er].y - ml0) * 16 + (lane & 15)], acc[j][r]);
+                }
             }
         }
     }
