@@ -100,12 +100,15 @@ static const int KEB = 4;       // entry blocks of 16 sharing one walk over t in
 
 // Ket on MFMA: G[n][t][q] = sum over the entries (c, p) of q of R[c][t] D[n][p].
 // Per q this is a GEMM (rows t, columns n, inner dimension the entries of q).
-// Task = (term, t tile of 16, 64 functions n (16 per wave), KQC functions q).
-// A wave computes four consecutive q together, walking their entry lists in
-// step, and stores each (n, t) row of the four as one aligned 32-byte write
-// (single 8-byte writes scattered over 16 rows cost four times their bytes in
-// HBM write traffic).
-__global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
+// Task = (term, t tile of 16, 64 functions n, KQC functions q); the four
+// waves split it 2 x 2 into 32 functions n (two MFMA tiles that share every
+// gathered R operand and entry index) by KQC / 2 functions q.  A wave computes
+// four consecutive q together, walking their entry lists in step, and stores
+// each (n, t) row of the four as one aligned 32-byte write (single 8-byte
+// writes scattered over 16 rows cost four times their bytes in HBM write
+// traffic).  Registers are capped for four waves per SIMD: uncapped, the
+// eight accumulators take 200 and halve the waves that hide the gathers.
+__global__ __launch_bounds__(KWG) __attribute__((amdgpu_waves_per_eu(4))) void kx_ket(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
                                                const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
                                                const int *__restrict__ qptr, const int2 *__restrict__ qent,
                                                const double *__restrict__ PB, const double *__restrict__ CB,
@@ -114,24 +117,32 @@ __global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, c
     const KTerm t = terms[tk.term];
     const KNode ni = nodes[t.ni], nj = nodes[t.nj];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int nl0 = tk.e0 + 16 * w;
-    if (nl0 >= ni.nbf) return;   // whole wave
+    // wave w: functions n [nl0, nl0 + 32) as two MFMA tiles that share every
+    // R operand, and half of the task's functions q
+    const int nl0 = tk.e0 + 32 * (w & 1);
+    const int qbeg = tk.pad_ + (KQC / 2) * (w >> 1);
+    const int qend = min(qbeg + KQC / 2, nj.nbf);
+    if (nl0 >= ni.nbf || qbeg >= qend) return;   // whole wave
     const double *R = (t.Rsrc == KS_PB ? PB : (t.Rsrc == KS_CB ? CB : SC)) + t.Roff;
-    const int nl = nl0 + (lane & 15);
-    const bool nv = nl < ni.nbf;
-    // D is symmetric (the entry point's contract), so D[n][p] is read as
-    // D[p][n]: the 16 lanes of an entry (consecutive functions n of the node)
-    // read along one row of D instead of down a column
-    const int bfn = nv ? bfidx[ni.bf_off + nl] : 0;
+    int nl[2], bfn[2];
+    bool nv[2];
+    #pragma unroll
+    for (int h = 0; h < 2; h++) {
+        nl[h] = nl0 + 16 * h + (lane & 15);
+        nv[h] = nl[h] < ni.nbf;
+        // D is symmetric (the entry point's contract), so D[n][p] is read as
+        // D[p][n]: the 16 lanes of an entry (consecutive functions n of the
+        // node) read along one row of D instead of down a column
+        bfn[h] = nv[h] ? bfidx[ni.bf_off + nl[h]] : 0;
+    }
     const int ta = tk.t0 + (lane & 15);   // A operand row (t)
     const bool tv = ta < t.kk;
     const int kl = lane >> 4;
     const int ldq = gld(nj.nbf);
     double *G = SC + t.Goff;
-    const int qend = min(tk.pad_ + KQC, nj.nbf);
-    for (int qg = tk.pad_; qg < qend; qg += 4) {
+    for (int qg = qbeg; qg < qend; qg += 4) {
         // The entry lists of the four q are walked together (entries past a
-        // list are zeros), so each iteration has the gathers of four
+        // list are zeros), so each iteration has the gathers of eight
         // independent accumulations in flight.
         int qa[4], qb[4], len = 0;
         #pragma unroll
@@ -141,38 +152,40 @@ __global__ __launch_bounds__(KWG) void kx_ket(const KTask *__restrict__ tasks, c
             qb[j] = ql < qend ? qptr[nj.q_ptr + ql + 1] : 0;
             len = max(len, qb[j] - qa[j]);
         }
-        d4 acc[4];
+        d4 acc[2][4];
         #pragma unroll
-        for (int j = 0; j < 4; j++) acc[j] = d4{0.0, 0.0, 0.0, 0.0};
-        for (int e0 = 0; e0 < len; e0 += 8) {
-            int2 cp[4][2];
+        for (int h = 0; h < 2; h++)
+            #pragma unroll
+            for (int j = 0; j < 4; j++) acc[h][j] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int e0 = 0; e0 < len; e0 += 4) {
+            int2 cp[4];
+            #pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int e = qa[j] + e0 + kl;
+                cp[j] = e < qb[j] ? qent[e] : make_int2(-1, 0);
+            }
+            double a[4], b[2][4];
+            #pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const bool ev = cp[j].x >= 0;
+                a[j] = (ev && tv) ? R[(long long)cp[j].x * t.Rrs + (long long)ta * t.Rcs] : 0.0;
+                #pragma unroll
+                for (int h = 0; h < 2; h++) b[h][j] = (ev && nv[h]) ? D[(size_t)cp[j].y * nbf + bfn[h]] : 0.0;
+            }
             #pragma unroll
             for (int j = 0; j < 4; j++)
                 #pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    const int e = qa[j] + e0 + 4 * u + kl;
-                    cp[j][u] = e < qb[j] ? qent[e] : make_int2(-1, 0);
-                }
-            double a[4][2], b[4][2];
-            #pragma unroll
-            for (int j = 0; j < 4; j++)
-                #pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    const bool ev = cp[j][u].x >= 0;
-                    a[j][u] = (ev && tv) ? R[(long long)cp[j][u].x * t.Rrs + (long long)ta * t.Rcs] : 0.0;
-                    b[j][u] = (ev && nv) ? D[(size_t)cp[j][u].y * nbf + bfn] : 0.0;
-                }
-            #pragma unroll
-            for (int u = 0; u < 2; u++)
-                #pragma unroll
-                for (int j = 0; j < 4; j++) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j][u], b[j][u], acc[j], 0, 0, 0);
+                for (int h = 0; h < 2; h++) acc[h][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[h][j], acc[h][j], 0, 0, 0);
         }
         #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int tt = tk.t0 + kl + 4 * r;
-            if (tt < t.kk && nv)
-                *reinterpret_cast<d4 *>(G + ((size_t)nl * t.kk + tt) * ldq + qg) = d4{acc[0][r], acc[1][r], acc[2][r], acc[3][r]};
-        }
+        for (int h = 0; h < 2; h++)
+            #pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int tt = tk.t0 + kl + 4 * r;
+                if (tt < t.kk && nv[h])
+                    *reinterpret_cast<d4 *>(G + ((size_t)nl[h] * t.kk + tt) * ldq + qg) =
+                        d4{acc[h][0][r], acc[h][1][r], acc[h][2][r], acc[h][3][r]};
+            }
     }
 }
 
