@@ -94,7 +94,7 @@ static const int KWG = 256;
 // Row length of G: |B(j)| rounded up to 4, so that four consecutive q of a
 // (n, t) row form one aligned 32-byte store.
 __host__ __device__ inline int gld(int nbf_j) { return (nbf_j + 3) & ~3; }
-static const int KQC = 16;      // column functions q per ket task
+static const int KQC = 8;       // column functions q per ket task (one group of four per wave)
 static const int KSPLIT = 256;  // local m rows per bra task (LDS block KSPLIT x 16)
 static const int KEB = 4;       // entry blocks of 16 sharing one walk over t in the bra
 
@@ -120,6 +120,7 @@ __global__ __launch_bounds__(KWG) __attribute__((amdgpu_waves_per_eu(4))) void k
     // wave w: functions n [nl0, nl0 + 32) as two MFMA tiles that share every
     // R operand, and half of the task's functions q
     const int nl0 = tk.e0 + 32 * (w & 1);
+    static_assert(KQC % 8 == 0, "each wave stores whole groups of four q");
     const int qbeg = tk.pad_ + (KQC / 2) * (w >> 1);
     const int qend = min(qbeg + KQC / 2, nj.nbf);
     if (nl0 >= ni.nbf || qbeg >= qend) return;   // whole wave
