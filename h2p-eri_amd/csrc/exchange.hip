@@ -158,25 +158,35 @@ __global__ __launch_bounds__(KWG) __attribute__((amdgpu_waves_per_eu(4))) void k
         for (int h = 0; h < 2; h++)
             #pragma unroll
             for (int j = 0; j < 4; j++) acc[h][j] = d4{0.0, 0.0, 0.0, 0.0};
-        for (int e0 = 0; e0 < len; e0 += 4) {
-            int2 cp[4];
-            #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int e = qa[j] + e0 + kl;
-                cp[j] = e < qb[j] ? qent[e] : make_int2(-1, 0);
-            }
-            double a[4], b[2][4];
-            #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const bool ev = cp[j].x >= 0;
-                a[j] = (ev && tv) ? R[(long long)cp[j].x * t.Rrs + (long long)ta * t.Rcs] : 0.0;
+        // entry indices come in windows of 16 per q: lane l loads entry
+        // (l & 15) of q (l >> 4) and the lanes of each step take theirs by
+        // a cross-lane shuffle, which keeps them off the vector memory path
+        const int jq = lane >> 4;
+        const int wa = jq == 0 ? qa[0] : jq == 1 ? qa[1] : jq == 2 ? qa[2] : qa[3];
+        const int wb = jq == 0 ? qb[0] : jq == 1 ? qb[1] : jq == 2 ? qb[2] : qb[3];
+        for (int w0 = 0; w0 < len; w0 += 16) {
+            const int ew = wa + w0 + (lane & 15);
+            const int2 mine = ew < wb ? qent[ew] : make_int2(-1, 0);
+            for (int s4 = 0; s4 < 16 && w0 + s4 < len; s4 += 4) {
+                int2 cp[4];
                 #pragma unroll
-                for (int h = 0; h < 2; h++) b[h][j] = (ev && nv[h]) ? D[(size_t)cp[j].y * nbf + bfn[h]] : 0.0;
-            }
-            #pragma unroll
-            for (int j = 0; j < 4; j++)
+                for (int j = 0; j < 4; j++) {
+                    const int src = 16 * j + s4 + kl;
+                    cp[j] = make_int2(__shfl(mine.x, src), __shfl(mine.y, src));
+                }
+                double a[4], b[2][4];
                 #pragma unroll
-                for (int h = 0; h < 2; h++) acc[h][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[h][j], acc[h][j], 0, 0, 0);
+                for (int j = 0; j < 4; j++) {
+                    const bool ev = cp[j].x >= 0;
+                    a[j] = (ev && tv) ? R[(long long)cp[j].x * t.Rrs + (long long)ta * t.Rcs] : 0.0;
+                    #pragma unroll
+                    for (int h = 0; h < 2; h++) b[h][j] = (ev && nv[h]) ? D[(size_t)cp[j].y * nbf + bfn[h]] : 0.0;
+                }
+                #pragma unroll
+                for (int j = 0; j < 4; j++)
+                    #pragma unroll
+                    for (int h = 0; h < 2; h++) acc[h][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[h][j], acc[h][j], 0, 0, 0);
+            }
         }
         #pragma unroll
         for (int h = 0; h < 2; h++)
