@@ -62,7 +62,7 @@ struct KTerm {
     long long Loff;
     int Rsrc, Rrs, Rcs;   // R[c][t] = src[Roff + c*Rrs + t*Rcs]
     long long Roff;
-    long long Goff;       // G (|B(i)| x kk x gld(|B(j)|)) in scratch, q fastest
+    long long Goff;       // G in scratch: [q / 8][n][t][q % 8], gld(|B(j)|) / 8 blocks
     int pad_[2];
 };
 
@@ -91,9 +91,14 @@ struct KNode {           // per row/column range: basis-function set and the ori
 };
 
 static const int KWG = 256;
-// Row length of G: |B(j)| rounded up to 4, so that four consecutive q of a
-// (n, t) row form one aligned 32-byte store.
-__host__ __device__ inline int gld(int nbf_j) { return (nbf_j + 3) & ~3; }
+// G is stored in blocks of 8 functions q, [q / 8][n][t][q % 8]: a ket task
+// (8 q) then writes whole 64-byte (n, t) records, two per 128-byte line, and
+// four consecutive q form one aligned 32-byte store.  gld(|B(j)|) is the
+// padded q count.
+__host__ __device__ inline int gld(int nbf_j) { return (nbf_j + 7) & ~7; }
+__device__ inline size_t gidx(int q, int n, int t, int nbf_i, int kk) {
+    return (((size_t)(q >> 3) * nbf_i + n) * kk + t) * 8 + (q & 7);
+}
 // Workgroups are dispatched to the 8 XCDs round-robin by id, and each XCD has
 // its own L2.  This bijection of [0, n) hands out runs of C consecutive tasks
 // of the list, run j to XCD j mod 8 (the tail of fewer than 8 runs is split
@@ -155,7 +160,6 @@ __global__ __launch_bounds__(KWG) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int ta = tk.t0 + (lane & 15);   // A operand row (t)
     const bool tv = ta < t.kk;
     const int kl = lane >> 4;
-    const int ldq = gld(nj.nbf);
     double *G = SC + t.Goff;
     for (int qg = qbeg; qg < qend; qg += 4) {
         // The entry lists of the four q are walked together (entries past a
@@ -210,7 +214,7 @@ __global__ __launch_bounds__(KWG) __attribute__((amdgpu_waves_per_eu(4))) void k
             for (int r = 0; r < 4; r++) {
                 const int tt = tk.t0 + kl + 4 * r;
                 if (tt < t.kk && nv[h])
-                    *reinterpret_cast<d4 *>(G + ((size_t)nl[h] * t.kk + tt) * ldq + qg) =
+                    *reinterpret_cast<d4 *>(G + gidx(qg, nl[h], tt, ni.nbf, t.kk)) =
                         d4{acc[h][0][r], acc[h][1][r], acc[h][2][r], acc[h][3][r]};
             }
     }
@@ -247,7 +251,7 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
     const int ns1 = ni.nsplit + 1;
     for (int nl = w; nl < ni.nbf; nl += KWG / 64) {
         const int eb = nspl[ni.s_ptr + nl * ns1 + split], ee = nspl[ni.s_ptr + nl * ns1 + split + 1];
-        const double *Gn = G + (size_t)nl * t.kk * gld(nj.nbf) + ql;
+        const double *Gn = G + gidx(ql, nl, 0, ni.nbf, t.kk);   // t stride 8
         // up to KEB blocks of 16 entries per walk over t, so that each G
         // operand load feeds KEB accumulations
         for (int e0 = eb; e0 < ee; e0 += 16 * KEB) {
@@ -268,7 +272,7 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
                 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int tt = t0 + 4 * u + kl;
-                    b[u] = (qv && tt < t.kk) ? Gn[(size_t)tt * gld(nj.nbf)] : 0.0;
+                    b[u] = (qv && tt < t.kk) ? Gn[(size_t)tt * 8] : 0.0;
                 }
                 #pragma unroll
                 for (int j = 0; j < KEB; j++) {
