@@ -87,7 +87,8 @@ struct KNode {           // per row/column range: basis-function set and the ori
     int q_ptr;            // byQ CSR rows: qptr[q_ptr .. q_ptr + nbf]   entries (column c, p)
     int m_ptr;            // byM CSR rows: mptr[m_ptr .. m_ptr + nbf]   entries (row r, local n)
     int s_ptr;            // byN split table: nspl[s_ptr + n * (nsplit + 1) + s] = first entry (row r, local m)
-    int nsplit;           //   of local n with local m >= s * KSPLIT (entries of each n sorted by m)
+    int nsplit;           //   of local n with local m >= s * ssz (entries of each n sorted by m)
+    int ssz;              // functions m per split: the nbf functions in nsplit = ceil(nbf / KSPLIT) even parts
 };
 
 static const int KWG = 256;
@@ -116,7 +117,9 @@ __device__ inline int xcd_task(int bid, int n, int C) {
 // cut the bra's HBM reads from 39 to 22 GB but unbalanced the XCDs)
 static const int KRUN_KET = 64, KRUN_BRA = 8;
 static const int KQC = 8;       // column functions q per ket task (one group of four per wave)
-static const int KSPLIT = 256;  // local m rows per bra task (LDS block KSPLIT x 16)
+// most local m rows per bra task (LDS block KSPLIT x 16); on C30H62 K took
+// 13.0 ms with 256, 12.5 ms with 192 and even splits, 12.9 ms with 160
+static const int KSPLIT = 192;
 static const int KEB = 4;       // entry blocks of 16 sharing one walk over t in the bra
 
 // Ket on MFMA: G[n][t][q] = sum over the entries (c, p) of q of R[c][t] D[n][p].
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
     const KTerm t = terms[tk.term];
     const KNode ni = nodes[t.ni], nj = nodes[t.nj];
     const int split = tk.t0, ql0 = tk.e0;
-    const int ml0 = split * KSPLIT;
+    const int ml0 = split * ni.ssz;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, kl = lane >> 4;
     for (int i = threadIdx.x; i < KSPLIT * 16; i += KWG) Kb[i] = 0.0;
     __syncthreads();
@@ -300,7 +303,7 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
     }
     __syncthreads();
     double *Kout = t.diag ? Kd : K1;
-    const int mlN = min(KSPLIT, ni.nbf - ml0);
+    const int mlN = min(ni.ssz, ni.nbf - ml0);
     for (int i = threadIdx.x; i < mlN * 16; i += KWG) {
         const int qq = ql0 + (i & 15);
         const double v = Kb[i];
@@ -546,17 +549,19 @@ struct KBuilder {
         for (int i = 0; i < nb; i++) { P.mptr.push_back((int)P.ment.size()); P.ment.insert(P.ment.end(), bym[i].begin(), bym[i].end()); }
         P.mptr.push_back((int)P.ment.size());
         // byN: for each local n the (row, local m) entries, sorted by m, with the
-        // first entry of every split of KSPLIT functions m
+        // first entry of every split of ssz functions m (at most KSPLIT, the
+        // node's functions in even parts so that no split task is a sliver)
         std::vector<std::vector<int2>> byn(nb);
         for (int ml = 0; ml < nb; ml++)
             for (const int2 &e : bym[ml]) byn[e.y].push_back(make_int2(e.x, ml));
         kn.nsplit = (nb + KSPLIT - 1) / KSPLIT;
+        kn.ssz = kn.nsplit > 0 ? (nb + kn.nsplit - 1) / kn.nsplit : KSPLIT;
         kn.s_ptr = (int)P.nspl.size();
         for (int nl = 0; nl < nb; nl++) {
             const int base = (int)P.nent.size();
             int k = 0;
             for (int sp = 0; sp <= kn.nsplit; sp++) {
-                while (k < (int)byn[nl].size() && byn[nl][k].y < sp * KSPLIT) k++;
+                while (k < (int)byn[nl].size() && byn[nl][k].y < sp * kn.ssz) k++;
                 P.nspl.push_back(base + k);
             }
             P.nent.insert(P.nent.end(), byn[nl].begin(), byn[nl].end());
@@ -707,14 +712,14 @@ struct KBuilder {
                 ok = P.ment[j].x >= 0 && P.ment[j].x < t.m0 && P.ment[j].y >= 0 && P.ment[j].y < ni.nbf;
             // byN splits: increasing, inside nent, entries of a split within its m range
             const int ns1 = ni.nsplit + 1;
-            ok = ok && ni.nsplit >= 1 && ni.s_ptr >= 0 && ni.s_ptr + (long long)ni.nbf * ns1 <= (long long)P.nspl.size();
+            ok = ok && ni.nsplit >= 1 && ni.ssz >= 1 && ni.ssz <= KSPLIT && ni.s_ptr >= 0 && ni.s_ptr + (long long)ni.nbf * ns1 <= (long long)P.nspl.size();
             for (int nl = 0; ok && nl < ni.nbf; nl++)
                 for (int sp = 0; ok && sp < ni.nsplit; sp++) {
                     const int eb = P.nspl[ni.s_ptr + nl * ns1 + sp], ee = P.nspl[ni.s_ptr + nl * ns1 + sp + 1];
                     ok = eb >= 0 && eb <= ee && ee <= (int)P.nent.size();
                     for (int e = eb; ok && e < ee; e++)
-                        ok = P.nent[e].x >= 0 && P.nent[e].x < t.m0 && P.nent[e].y >= sp * KSPLIT &&
-                             P.nent[e].y < std::min((sp + 1) * KSPLIT, ni.nbf);
+                        ok = P.nent[e].x >= 0 && P.nent[e].x < t.m0 && P.nent[e].y >= sp * ni.ssz &&
+                             P.nent[e].y < std::min((sp + 1) * ni.ssz, ni.nbf);
                 }
             ok = ok && t.Loff >= 0 && t.Loff + (long long)(t.m0 - 1) * t.Lld + t.kk <= size_of(t.Lsrc);
             if (!t.direct) {
