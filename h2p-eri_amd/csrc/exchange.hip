@@ -128,9 +128,9 @@ static const int KEB = 4;       // entry blocks of 16 sharing one walk over t in
 // waves split it 2 x 2 into 32 functions n (two MFMA tiles that share every
 // gathered R operand and entry index) by KQC / 2 functions q.  A wave computes
 // four consecutive q together, walking their entry lists in step, and stores
-// each (n, t) row of the four as one aligned 32-byte write (single 8-byte
-// writes scattered over 16 rows cost four times their bytes in HBM write
-// traffic).  Registers are capped for four waves per SIMD: uncapped, the
+// the four q of each (n, t) record as one aligned 32-byte write (single
+// 8-byte writes scattered over 16 rows cost four times their bytes in HBM
+// write traffic).  Registers are capped for four waves per SIMD: uncapped, the
 // eight accumulators take 200 and halve the waves that hide the gathers.
 __global__ __launch_bounds__(KWG) __attribute__((amdgpu_waves_per_eu(4))) void kx_ket(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
                                                const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(KWG) __attribute__((amdgpu_waves_per_eu(4))) void k
 // Bra on MFMA: K[m][q] += sum_n sum_{(r,m) of n} sum_t L[r][t] G[n][t][q].
 // For each n the rows r that pair with n form a GEMM (rows: those entries,
 // columns q, inner dimension t).  Task = (term, 16 functions q, split of
-// KSPLIT functions m); wave w takes the functions n = w, w + 4, ... and adds
+// ssz <= KSPLIT functions m); wave w takes the functions n = w, w + 4, ... and adds
 // its 16 x 16 output tiles into an LDS block of the split with FP64 LDS
 // atomics (tiles of different n may share an m); the block is added into K
 // at the end.  The t loop issues the operands of four MFMA steps at once,
