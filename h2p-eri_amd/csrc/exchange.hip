@@ -117,8 +117,9 @@ __device__ inline int xcd_task(int bid, int n, int C) {
 // cut the bra's HBM reads from 39 to 22 GB but unbalanced the XCDs)
 static const int KRUN_KET = 64, KRUN_BRA = 8;
 static const int KQC = 8;       // column functions q per ket task (one group of four per wave)
-// most local m rows per bra task (LDS block KSPLIT x 16); on C30H62 K took
-// 13.0 ms with 256, 12.5 ms with 192 and even splits, 12.9 ms with 160
+// most local m rows per bra task (LDS block KSPLIT x 32); on C30H62, with
+// tasks of 16 q, K took 13.0 ms with 256, 12.5 ms with 192 and even splits,
+// 12.9 ms with 160
 static const int KSPLIT = 192;
 static const int KEB = 4;       // entry blocks of 16 sharing one walk over t in the bra
 
@@ -225,11 +226,12 @@ __global__ __launch_bounds__(KWG) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // Bra on MFMA: K[m][q] += sum_n sum_{(r,m) of n} sum_t L[r][t] G[n][t][q].
 // For each n the rows r that pair with n form a GEMM (rows: those entries,
-// columns q, inner dimension t).  Task = (term, 16 functions q, split of
-// ssz <= KSPLIT functions m); wave w takes the functions n = w, w + 4, ... and adds
-// its 16 x 16 output tiles into an LDS block of the split with FP64 LDS
-// atomics (tiles of different n may share an m); the block is added into K
-// at the end.  The t loop issues the operands of four MFMA steps at once,
+// columns q, inner dimension t).  Task = (term, 32 functions q as two MFMA
+// column tiles that share every L operand, split of ssz <= KSPLIT functions
+// m); wave w takes the functions n = w, w + 4, ... and adds its 16 x 16
+// output tiles into an LDS block of the split with FP64 LDS atomics (tiles
+// of different n may share an m); the block is added into K at the end.
+// Tasks of 16 q read every L row twice as often: K 12.5 ms against 12.1.  The t loop issues the operands of four MFMA steps at once,
 // and one walk over t serves up to KEB blocks of 16 entries of n, which share
 // its G operands.
 __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, const KTerm *__restrict__ terms,
@@ -238,23 +240,26 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
                                                const double *__restrict__ PB, const double *__restrict__ CB,
                                                const double *__restrict__ SC, double *__restrict__ K1,
                                                double *__restrict__ Kd, int nbf) {
-    __shared__ double Kb[KSPLIT * 16];
+    __shared__ double Kb[KSPLIT * 32];
     const KTask tk = tasks[xcd_task(blockIdx.x, gridDim.x, KRUN_BRA)];   // term, ql0, split
     const KTerm t = terms[tk.term];
     const KNode ni = nodes[t.ni], nj = nodes[t.nj];
     const int split = tk.t0, ql0 = tk.e0;
     const int ml0 = split * ni.ssz;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, kl = lane >> 4;
-    for (int i = threadIdx.x; i < KSPLIT * 16; i += KWG) Kb[i] = 0.0;
+    for (int i = threadIdx.x; i < KSPLIT * 32; i += KWG) Kb[i] = 0.0;
     __syncthreads();
     const double *L = (t.Lsrc == KS_PB ? PB : (t.Lsrc == KS_CB ? CB : SC)) + t.Loff;
     const double *G = SC + t.Goff;
-    const int ql = ql0 + (lane & 15);
-    const bool qv = ql < nj.nbf;
+    bool qv[2];
+    #pragma unroll
+    for (int h = 0; h < 2; h++) qv[h] = ql0 + 16 * h + (lane & 15) < nj.nbf;
     const int ns1 = ni.nsplit + 1;
     for (int nl = w; nl < ni.nbf; nl += KWG / 64) {
         const int eb = nspl[ni.s_ptr + nl * ns1 + split], ee = nspl[ni.s_ptr + nl * ns1 + split + 1];
-        const double *Gn = G + gidx(ql, nl, 0, ni.nbf, t.kk);   // t stride 8
+        const double *Gn[2];
+        #pragma unroll
+        for (int h = 0; h < 2; h++) Gn[h] = G + gidx(min(ql0 + 16 * h + (lane & 15), gld(nj.nbf) - 1), nl, 0, ni.nbf, t.kk);
         // up to KEB blocks of 16 entries per walk over t, so that each G
         // operand load feeds KEB accumulations
         for (int e0 = eb; e0 < ee; e0 += 16 * KEB) {
@@ -267,16 +272,18 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
                 Lr[j] = L + (long long)(evj[j] ? nent[e].x : 0) * t.Lld;
             }
             const int nblk = min(KEB, (ee - e0 + 15) / 16);   // blocks with entries (wave-uniform)
-            d4 acc[KEB];
+            d4 acc[KEB][2];
             #pragma unroll
-            for (int j = 0; j < KEB; j++) acc[j] = d4{0.0, 0.0, 0.0, 0.0};
+            for (int j = 0; j < KEB; j++) acc[j][0] = acc[j][1] = d4{0.0, 0.0, 0.0, 0.0};
             for (int t0 = 0; t0 < t.kk; t0 += 16) {
-                double b[4];
+                double b[2][4];
                 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int tt = t0 + 4 * u + kl;
-                    b[u] = (qv && tt < t.kk) ? Gn[(size_t)tt * 8] : 0.0;
-                }
+                for (int h = 0; h < 2; h++)
+                    #pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int tt = t0 + 4 * u + kl;
+                        b[h][u] = (qv[h] && tt < t.kk) ? Gn[h][(size_t)tt * 8] : 0.0;
+                    }
                 #pragma unroll
                 for (int j = 0; j < KEB; j++) {
                     if (j >= nblk) break;
@@ -287,7 +294,9 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
                         a[u] = (evj[j] && tt < t.kk) ? Lr[j][tt] : 0.0;
                     }
                     #pragma unroll
-                    for (int u = 0; u < 4; u++) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc[j], 0, 0, 0);
+                    for (int u = 0; u < 4; u++)
+                        #pragma unroll
+                        for (int h = 0; h < 2; h++) acc[j][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[h][u], acc[j][h], 0, 0, 0);
                 }
             }
             #pragma unroll
@@ -296,7 +305,12 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
                 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int er = e0 + 16 * j + kl + 4 * r;
-                    if (er < ee && qv) atomicAdd(&Kb[(nent[er].y - ml0) * 16 + (lane & 15)], acc[j][r]);
+                    if (er < ee) {
+                        const int mrow = nent[er].y - ml0;
+                        #pragma unroll
+                        for (int h = 0; h < 2; h++)
+                            if (qv[h]) atomicAdd(&Kb[mrow * 32 + 16 * h + (lane & 15)], acc[j][h][r]);
+                    }
                 }
             }
         }
@@ -304,11 +318,11 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
     __syncthreads();
     double *Kout = t.diag ? Kd : K1;
     const int mlN = min(ni.ssz, ni.nbf - ml0);
-    for (int i = threadIdx.x; i < mlN * 16; i += KWG) {
-        const int qq = ql0 + (i & 15);
+    for (int i = threadIdx.x; i < mlN * 32; i += KWG) {
+        const int qq = ql0 + (i & 31);
         const double v = Kb[i];
         if (qq < nj.nbf && v != 0.0)
-            unsafeAtomicAdd(Kout + (size_t)bfidx[ni.bf_off + ml0 + (i >> 4)] * nbf + bfidx[nj.bf_off + qq], v);
+            unsafeAtomicAdd(Kout + (size_t)bfidx[ni.bf_off + ml0 + (i >> 5)] * nbf + bfidx[nj.bf_off + qq], v);
     }
 }
 
@@ -679,7 +693,7 @@ struct KBuilder {
             for (int t0 = 0; t0 < t.kk; t0 += 16)
                 for (int nl0 = 0; nl0 < ni.nbf; nl0 += 64)
                     for (int ql0 = 0; ql0 < nj.nbf; ql0 += KQC) P.ket.push_back(KTask{ti, nl0, t0, ql0});
-            for (int ql0 = 0; ql0 < nj.nbf; ql0 += 16)
+            for (int ql0 = 0; ql0 < nj.nbf; ql0 += 32)
                 for (int sp = 0; sp < ni.nsplit; sp++) P.bra.push_back(KTask{ti, ql0, sp, 0});
         }
         return validate();
