@@ -726,7 +726,9 @@ struct KBuilder {
                 ok = P.ment[j].x >= 0 && P.ment[j].x < t.m0 && P.ment[j].y >= 0 && P.ment[j].y < ni.nbf;
             // byN splits: increasing, inside nent, entries of a split within its m range
             const int ns1 = ni.nsplit + 1;
-            ok = ok && ni.nsplit >= 1 && ni.ssz >= 1 && ni.ssz <= KSPLIT && ni.s_ptr >= 0 && ni.s_ptr + (long long)ni.nbf * ns1 <= (long long)P.nspl.size();
+            // (the splits cover every local m: nsplit * ssz >= nbf)
+            ok = ok && ni.nsplit >= 1 && ni.ssz >= 1 && ni.ssz <= KSPLIT && (long long)ni.nsplit * ni.ssz >= ni.nbf &&
+                 ni.s_ptr >= 0 && ni.s_ptr + (long long)ni.nbf * ns1 <= (long long)P.nspl.size();
             for (int nl = 0; ok && nl < ni.nbf; nl++)
                 for (int sp = 0; ok && sp < ni.nsplit; sp++) {
                     const int eb = P.nspl[ni.s_ptr + nl * ns1 + sp], ee = P.nspl[ni.s_ptr + nl * ns1 + sp + 1];

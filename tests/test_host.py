@@ -157,3 +157,27 @@ def test_exchange_needs_molecule(tmp_path):
     h.lib.H2ERI_build_exchange(h.h, buf.ctypes.data_as(h2eri.capi.c_dbl_p), buf.ctypes.data_as(h2eri.capi.c_dbl_p))
     with pytest.raises(h2eri.H2ERIError, match="shell-pair metadata"):
         h2eri._check(h.lib)
+
+
+@pytest.mark.parametrize("kind,size", [("alkane", 4), ("alkane", 10), ("water", 6)])
+def test_exchange_plan_checks_on_host(kind, size):
+    """The exchange plan is built and every kernel access bounds-checked on
+    the host (no GPU): entry lists, the row splits of the bra (C10H22 has
+    260 basis functions, more than one split's 192), G offsets and the
+    basis-product GEMMs.  Every D and B block of these
+    representations becomes one term, and basis products exist exactly when
+    there are admissible (B) pairs (C4H10 has none)."""
+    import h2eri
+    from h2eri import molecules as M
+    rng = M.rng_for(700 + size)
+    atoms = M.alkane(size, rng) if kind == "alkane" else M.water_cluster(size, rng)
+    h = h2eri.H2ERI(1e-10, 1e-10, 1e-6)
+    h.build_from_mol(text=M.mol_text(atoms))
+    r = h.exchange_plan_check()
+    s = h.s
+    assert r["terms"] == s.n_leaf_node + s.n_r_inadm_pair + s.n_r_adm_pair
+    assert r["terms_f"] == r["terms"]
+    assert (r["gemm_flops"] > 0) == (s.n_r_adm_pair > 0)
+    assert r["scratch_doubles"] > 0
+    # resident bytes include the density and the three K buffers (4 nbf^2 doubles)
+    assert r["resident_bytes"] > 8 * 4 * h.nbf ** 2
