@@ -11,7 +11,9 @@ and accumulation), each for 1 density and for batches of 2, 4 and 8 densities
 per pass over the matrix data.  The single-density form runs the configuration
 the project's autotuner picks for it (FP32-stored), each batch width the one it
 picks for that width and precision.  `value` is the FP32-stored single-density
-form's algorithmic HBM throughput; the batched forms are reported beside it.
+form's algorithmic HBM throughput; the batched forms are reported beside it,
+and so is the FP64 exchange (K) build of the same molecule and density
+(`exchange_fp64`: ms per H2ERI_build_exchange_dev call, plan uploaded once).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
