@@ -706,13 +706,16 @@ struct MolPtrs {
 // The tasks of every stage of a plan, in a topological order computed on the
 // host.  Dependencies are per input list: a tile's in0/in1 or a reduce task's
 // source list is ready once every task writing one of its slots has counted
-// itself into the list's counter.  Workgroup b < nstatic runs task b first (those
-// tasks have no inputs to wait for); the remaining tasks are taken in chunks
-// of QCHUNK consecutive tasks from one ticket counter, the next chunk claimed
-// while the current one runs.  A task only ever waits for tasks of lower
-// index, and those are held by running workgroups, so the launch drains
-// whatever the residency.  Every wait is bounded: past `spin_limit` polls the
-// workgroup raises the error flag and goes on, so every wave exits.
+// itself into the list's counter.  Every task is taken from one ticket
+// counter, in chunks of QCHUNK consecutive tasks, the next chunk claimed
+// while the current one runs; a workgroup holds a ticket only once it runs.
+// A task only ever waits for tasks of lower ticket, and those were claimed
+// earlier by workgroups that are running, so the lowest unfinished ticket
+// can always proceed and the launch drains whatever the residency (work of
+// other streams on the same CUs included).  Every wait is bounded: past
+// `spin_limit` polls the workgroup raises the error flag and goes on, so
+// every wave exits; the synchronous entry points read the flag and report
+// the pass as failed.
 //
 // Data between workgroups goes through agent-coherent workspace accesses
 // (wld/wst with COH); a task's counters move only after every wave's stores
@@ -752,7 +755,7 @@ __device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x
 // GROUP = false: one representation, pointers from the arguments; GROUP =
 // true: QTask::pad_[0] names the molecule whose pointers come from mols.
 template <int NV, bool GROUP>
-__global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ tasks, int ntask, int nstatic,
+__global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ tasks, int ntask,
                                                       const int *__restrict__ need, int *ctr, int spin_limit,
                                                       const MolPtrs *__restrict__ mols, const BTask *__restrict__ t64,
                                                       const BTask *__restrict__ t32, const RTask *__restrict__ rt,
@@ -762,16 +765,10 @@ __global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ 
                                                       double *__restrict__ Y, long long ldx) {
     __shared__ double lds[LDS_DOUBLES];
     __shared__ int s_next[2];
-    int t0, t1;
-    if ((int)blockIdx.x < nstatic) {
-        t0 = blockIdx.x;
-        t1 = t0 + 1;
-    } else {
-        if (threadIdx.x == 0) s_next[0] = __hip_atomic_fetch_add(ctr + QC_TICKET, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        t0 = nstatic + s_next[0] * QCHUNK;
-        t1 = min(t0 + QCHUNK, ntask);
-    }
+    if (threadIdx.x == 0) s_next[0] = __hip_atomic_fetch_add(ctr + QC_TICKET, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    int t0 = s_next[0] * QCHUNK;
+    int t1 = min(t0 + QCHUNK, ntask);
     for (int par = 1; t0 < ntask; par ^= 1) {
         for (int t = t0; t < t1; t++) {
             QTask q = tasks[t];
@@ -805,7 +802,7 @@ __global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ 
                 if (q.out1 >= 0) __hip_atomic_fetch_add(ctr + QC_LISTS + q.out1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        t0 = nstatic + s_next[par] * QCHUNK;
+        t0 = s_next[par] * QCHUNK;
         t1 = min(t0 + QCHUNK, ntask);
     }
 }
@@ -1013,6 +1010,13 @@ struct DevState {
     bool collected = true;
     double phase_ms[7] = {0, 0, 0, 0, 0, 0, 0};
     size_t resident_bytes = 0;
+    // bound of every dependency wait of the persistent executor, in polls
+    // (about a second; H2ERI_set_spin_limit)
+    int spin_limit = 1 << 22;
+    // buffers of the captured Coulomb graph: only H2ERI_graph_capture
+    // allocates them and only release_graph frees them, so no eager call
+    // that grows its own buffers can leave the graph pointing at freed memory
+    double *g_x = nullptr, *g_y = nullptr, *g_Jh = nullptr;
 };
 
 static DevState *dstate(H2ERI_p h) { return (DevState *)h->dev; }
@@ -1050,6 +1054,9 @@ static void release_plans(DevState *d) {
 
 static void release_graph(DevState *d) {
     if (d->graph) { (void)hipGraphExecDestroy(d->graph); d->graph = nullptr; }
+    free_dev((void *&)d->g_x);
+    free_dev((void *&)d->g_y);
+    free_dev((void *&)d->g_Jh);
 }
 
 static void release_buffers(DevState *d) {
@@ -1589,7 +1596,7 @@ static bool build_plan(H2ERI_p h, const Storage &S, Plan &P, int nv, double tb, 
 struct QPlan {
     std::vector<QTask> tasks;   // ticket order
     std::vector<int> need;      // producer count of every list
-    int nstatic = 0;            // leading tasks without inputs, one per workgroup
+    int nstatic = 0;            // leading tasks without inputs (at most one per workgroup; statistics)
     double est_us = 0;          // list-scheduling estimate of the pass
 };
 
@@ -1862,10 +1869,19 @@ static bool upload_storage(H2ERI_p h) {
         std::vector<int> g(N);
         for (int i = 0; i < np; i++) {
             int s0 = h->sp_shell_idx[i], s1 = h->sp_shell_idx[i + np];
+            if (s0 < 0 || s1 < 0 || s0 >= h->nshell || s1 >= h->nshell) {
+                set_error(ERR_ARG, "Coulomb index map: shell pair %d names no shell", i);
+                return false;
+            }
             int srow = h->shell_bf_sidx[s0], nrow = h->shell_bf_sidx[s0 + 1] - srow;
             int scol = h->shell_bf_sidx[s1], ncol = h->shell_bf_sidx[s1 + 1] - scol;
             int two = (s0 != s1) ? 1 : 0;
             int xo = h->sp_bfp_sidx[i];
+            if (xo < 0 || nrow < 0 || ncol < 0 || (long long)xo + (long long)nrow * ncol > N || srow < 0 || scol < 0 ||
+                scol + ncol > nbf || srow + nrow > nbf) {
+                set_error(ERR_ARG, "Coulomb index map: shell pair %d lies outside x or D", i);
+                return false;
+            }
             for (int j = 0; j < ncol; j++)
                 for (int k = 0; k < nrow; k++) g[xo + j * nrow + k] = (((scol + j) * nbf + srow + k) << 1) | two;
         }
@@ -1963,14 +1979,14 @@ static void launch_stage(const Stage &st, const DevPlan &dp, DevState *d, const 
 
 static void collect_phase_ms(H2ERI_p h, DevState *d);
 
-// Spin bound of one wait of the persistent kernel: about a second of polling,
-// far beyond any dependency of a pass.
+// Spin bound of one wait of a group's persistent kernel: about a second of
+// polling, far beyond any dependency of a pass.
 static const int QUEUE_SPIN_LIMIT = 1 << 22;
 
 template <int NV>
 static void launch_queue(const DevPlan &dp, DevState *d, const double *X, double *Y, long long ld, hipStream_t s) {
-    hipLaunchKernelGGL((queue_kernel<NV, false>), dim3(dp.q_grid), dim3(WG), 0, s, dp.d_q, dp.q_n, dp.q_static,
-                       dp.d_need, dp.d_ctr, QUEUE_SPIN_LIMIT, (const MolPtrs *)nullptr, dp.d_t64, dp.d_t32, dp.d_rt,
+    hipLaunchKernelGGL((queue_kernel<NV, false>), dim3(dp.q_grid), dim3(WG), 0, s, dp.d_q, dp.q_n,
+                       dp.d_need, dp.d_ctr, d->spin_limit, (const MolPtrs *)nullptr, dp.d_t64, dp.d_t32, dp.d_rt,
                        dp.d_vsrc, dp.d_rsrc, d->S64, d->S32, dp.Wv, X, Y, ld);
 }
 
@@ -2113,6 +2129,8 @@ struct HostTiming {
     ~HostTiming() { d->phase_timing = was; }
 };
 
+static int queue_flags(DevState *d);
+
 static bool matvec_host_impl(H2ERI_p h, int nvec, const double *x, double *y) {
     if (!upload_storage(h)) return false;
     DevState *d = dstate(h);
@@ -2123,37 +2141,74 @@ static bool matvec_host_impl(H2ERI_p h, int nvec, const double *x, double *y) {
     if (!matvec_multi_dev_impl(h, nvec, d->d_x, d->d_y, nullptr)) return false;
     HIP_OK(hipMemcpy(y, d->d_y, sizeof(double) * N * nvec, hipMemcpyDeviceToHost));
     collect_phase_ms(h, d);
+    if (queue_flags(d) != 0) return false;
     h->n_matvec += nvec;
     return true;
 }
 
-// J_k = contract(A uncontract(D_k)) for ndens densities on the device.
+// J_k = contract(A uncontract(D_k)) for ndens densities on the device, with
+// x and y in xb / yb (N * ndens doubles each) and the half matrix in Jh
+// (num_bf^2 doubles).
+static bool coulomb_run(H2ERI_p h, int ndens, const double *D_dev, double *J_dev, hipStream_t s, double *xb,
+                        double *yb, double *Jh) {
+    DevState *d = dstate(h);
+    const int N = h->krnl_mat_size, nbf = h->num_bf;
+    const size_t nn = (size_t)nbf * nbf;
+    for (int k = 0; k < ndens; k++) {
+        hipLaunchKernelGGL(uncontract_kernel, dim3((N + 255) / 256), dim3(256), 0, s, d->d_gidx, D_dev + k * nn,
+                           xb + (size_t)k * N, N);
+        HIP_OK(hipGetLastError());
+    }
+    if (!matvec_multi_dev_impl(h, ndens, xb, yb, s)) return false;
+    for (int k = 0; k < ndens; k++) {
+        hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<size_t>((nn + 255) / 256, 4096)), dim3(256), 0, s,
+                           Jh, nn);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(contract_scatter_kernel, dim3((N + 255) / 256), dim3(256), 0, s, d->d_gidx,
+                           yb + (size_t)k * N, Jh, N);
+        HIP_OK(hipGetLastError());
+        dim3 grid((nbf + 31) / 32, (nbf + 31) / 32);
+        hipLaunchKernelGGL(symmetrize_kernel, grid, dim3(256), 0, s, Jh, J_dev + k * nn, nbf);
+        HIP_OK(hipGetLastError());
+    }
+    return true;
+}
+
+// Eager Coulomb build on the handle's own buffers (grown when needed).
 static bool coulomb_dev_impl(H2ERI_p h, int ndens, const double *D_dev, double *J_dev, hipStream_t s) {
     if (!upload_storage(h)) return false;
     DevState *d = dstate(h);
     if (!d->d_gidx) { set_error(ERR_ARG, "Coulomb build needs shell-pair metadata"); return false; }
-    const int N = h->krnl_mat_size, nbf = h->num_bf;
-    const size_t nn = (size_t)nbf * nbf;
-    if (!ensure_io(d, (size_t)N * ndens)) return false;
+    const size_t nn = (size_t)h->num_bf * h->num_bf;
+    if (!ensure_io(d, (size_t)h->krnl_mat_size * ndens)) return false;
     if (!d->d_Jh) HIP_OK(hipMalloc(&d->d_Jh, sizeof(double) * nn));
-    for (int k = 0; k < ndens; k++) {
-        hipLaunchKernelGGL(uncontract_kernel, dim3((N + 255) / 256), dim3(256), 0, s, d->d_gidx, D_dev + k * nn,
-                           d->d_x + (size_t)k * N, N);
-        HIP_OK(hipGetLastError());
+    return coulomb_run(h, ndens, D_dev, J_dev, s, d->d_x, d->d_y, d->d_Jh);
+}
+
+// Reads and clears the error flag of every persistent plan's counters, after
+// the work that used them has completed.  Returns 1 with ERR_DEVICE set when
+// a dependency wait exceeded its bound (that pass's results are invalid), 0
+// when clean, -1 on a device error.
+static int queue_flags(DevState *d) {
+    int bad = 0;
+    for (DevPlan &p : d->plans) {
+        if (!p.q_ready) continue;
+        int e = 0;
+        if (hipMemcpy(&e, p.d_ctr + QC_ERROR, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error(ERR_DEVICE, "persistent executor: reading the error flag failed");
+            return -1;
+        }
+        if (e) {
+            bad = 1;
+            const int zero = 0;
+            if (hipMemcpy(p.d_ctr + QC_ERROR, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+                set_error(ERR_DEVICE, "persistent executor: clearing the error flag failed");
+                return -1;
+            }
+        }
     }
-    if (!matvec_multi_dev_impl(h, ndens, d->d_x, d->d_y, s)) return false;
-    for (int k = 0; k < ndens; k++) {
-        hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<size_t>((nn + 255) / 256, 4096)), dim3(256), 0, s,
-                           d->d_Jh, nn);
-        HIP_OK(hipGetLastError());
-        hipLaunchKernelGGL(contract_scatter_kernel, dim3((N + 255) / 256), dim3(256), 0, s, d->d_gidx,
-                           d->d_y + (size_t)k * N, d->d_Jh, N);
-        HIP_OK(hipGetLastError());
-        dim3 grid((nbf + 31) / 32, (nbf + 31) / 32);
-        hipLaunchKernelGGL(symmetrize_kernel, grid, dim3(256), 0, s, d->d_Jh, J_dev + k * nn, nbf);
-        HIP_OK(hipGetLastError());
-    }
-    return true;
+    if (bad) set_error(ERR_DEVICE, "persistent executor: a dependency wait exceeded its bound; the results are invalid");
+    return bad;
 }
 
 }  // namespace h2e
@@ -2198,6 +2253,7 @@ void H2ERI_build_Coulomb_multi(H2ERI_p h, const int ndens, const double *den_mat
     if (!coulomb_dev_impl(h, ndens, d->d_D, d->d_J, nullptr)) return;
     if (hipMemcpy(J_mats, d->d_J, nn * 8 * ndens, hipMemcpyDeviceToHost) != hipSuccess) { set_error(ERR_DEVICE, "copy J"); return; }
     collect_phase_ms(h, d);
+    if (queue_flags(d) != 0) return;
     h->n_matvec += ndens;
 }
 
@@ -2237,25 +2293,29 @@ int H2ERI_graph_capture(H2ERI_p h, const int kind, const int n, const double *in
         if (!get_plan(h, nv)) return -1;
         done += nv;
     }
+    if (!d->side && hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking) != hipSuccess) { set_error(ERR_DEVICE, "stream"); return -1; }
+    if (!d->ev_fork && hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming) != hipSuccess) { set_error(ERR_DEVICE, "event"); return -1; }
+    if (!d->ev_join && hipEventCreateWithFlags(&d->ev_join, hipEventDisableTiming) != hipSuccess) { set_error(ERR_DEVICE, "event"); return -1; }
+    release_graph(d);   // also frees the previous graph's buffers
     if (kind == 1) {
+        // the graph's own x, y and half-matrix buffers (nothing else resizes them)
         if (!d->d_gidx) { set_error(ERR_ARG, "Coulomb build needs shell-pair metadata"); return -1; }
-        if (!ensure_io(d, (size_t)h->krnl_mat_size * n)) return -1;
-        if (!d->d_Jh && hipMalloc(&d->d_Jh, sizeof(double) * (size_t)h->num_bf * h->num_bf) != hipSuccess) {
+        const size_t nx = (size_t)h->krnl_mat_size * n, nn = (size_t)h->num_bf * h->num_bf;
+        if (hipMalloc(&d->g_x, sizeof(double) * nx) != hipSuccess || hipMalloc(&d->g_y, sizeof(double) * nx) != hipSuccess ||
+            hipMalloc(&d->g_Jh, sizeof(double) * nn) != hipSuccess) {
+            release_graph(d);
             set_error(ERR_DEVICE, "H2ERI_graph_capture: allocation failed");
             return -1;
         }
     }
-    if (!d->side && hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking) != hipSuccess) { set_error(ERR_DEVICE, "stream"); return -1; }
-    if (!d->ev_fork && hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming) != hipSuccess) { set_error(ERR_DEVICE, "event"); return -1; }
-    if (!d->ev_join && hipEventCreateWithFlags(&d->ev_join, hipEventDisableTiming) != hipSuccess) { set_error(ERR_DEVICE, "event"); return -1; }
-    release_graph(d);
     hipStream_t cs;
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) { set_error(ERR_DEVICE, "capture stream"); return -1; }
     hipGraph_t g = nullptr;
     bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
         d->capturing = true;
-        ok = kind == 0 ? matvec_multi_dev_impl(h, n, in_dev, out_dev, cs) : coulomb_dev_impl(h, n, in_dev, out_dev, cs);
+        ok = kind == 0 ? matvec_multi_dev_impl(h, n, in_dev, out_dev, cs)
+                       : coulomb_run(h, n, in_dev, out_dev, cs, d->g_x, d->g_y, d->g_Jh);
         d->capturing = false;
         const bool end_ok = hipStreamEndCapture(cs, &g) == hipSuccess;
         ok = ok && end_ok && g;
@@ -2267,6 +2327,7 @@ int H2ERI_graph_capture(H2ERI_p h, const int kind, const int n, const double *in
     if (g) (void)hipGraphDestroy(g);
     (void)hipStreamDestroy(cs);
     if (!ok) {
+        release_graph(d);
         if (!has_error()) set_error(ERR_DEVICE, "H2ERI_graph_capture: capture failed");
         return -1;
     }
@@ -2372,22 +2433,14 @@ int H2ERI_device_status(H2ERI_p h) {
     if (!h) return -1;
     DevState *d = dstate_get(h);
     if (hipDeviceSynchronize() != hipSuccess) { set_error(ERR_DEVICE, "H2ERI_device_status: device error"); return -1; }
-    int bad = 0;
-    for (DevPlan &p : d->plans) {
-        if (!p.q_ready) continue;
-        int e = 0;
-        if (hipMemcpy(&e, p.d_ctr + QC_ERROR, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
-            set_error(ERR_DEVICE, "H2ERI_device_status: copy failed");
-            return -1;
-        }
-        if (e) {
-            bad = 1;
-            const int zero = 0;
-            if (hipMemcpy(p.d_ctr + QC_ERROR, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return -1;
-        }
-    }
-    if (bad) set_error(ERR_DEVICE, "persistent executor: a dependency wait exceeded its bound");
-    return bad;
+    return queue_flags(d);
+}
+
+// Bound of every dependency wait of the persistent executor, in polls
+// (tuning and tests; no re-plan).
+void H2ERI_set_spin_limit(H2ERI_p h, const int polls) {
+    if (!h || polls < 0) { set_error(ERR_ARG, "H2ERI_set_spin_limit: polls >= 0"); return; }
+    dstate_get(h)->spin_limit = polls;
 }
 
 // Queue statistics of the nv-vector plan (building it when needed):
@@ -2587,7 +2640,7 @@ int H2ERI_group_matvec(void *group, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(queue_reset_kernel, dim3((std::max(g->q_nlist, QC_LISTS) + 255) / 256), dim3(256), 0, s, g->d_ctr,
                        g->q_nlist);
-    hipLaunchKernelGGL((queue_kernel<1, true>), dim3(g->q_grid), dim3(WG), 0, s, g->d_q, g->q_n, g->q_static, g->d_need,
+    hipLaunchKernelGGL((queue_kernel<1, true>), dim3(g->q_grid), dim3(WG), 0, s, g->d_q, g->q_n, g->d_need,
                        g->d_ctr, QUEUE_SPIN_LIMIT, g->d_mols, (const BTask *)nullptr, (const BTask *)nullptr,
                        (const RTask *)nullptr, (const int *)nullptr, (const int *)nullptr, (const double *)nullptr,
                        (const float *)nullptr, (double *)nullptr, (const double *)nullptr, (double *)nullptr, 0LL);

@@ -37,10 +37,18 @@ struct R {
     int i32() { int v = 0; get(&v, 1); return v; }
     template <class T> T *arr(size_t n) {
         T *p = (T *)malloc(sizeof(T) * (n > 0 ? n : 1));
+        if (!p) { ok = false; return nullptr; }
         get(p, n);
         return p;
     }
 };
+
+// Bounds of a loaded representation: counts large enough for any molecule
+// this library builds and small enough that no size product overflows.
+const int MAX_COUNT = 1 << 28;
+const long long MAX_BLOCK_ELEMS = (1LL << 31) - 1;   // H2E_dense_mat.size is an int
+
+bool in_range(long long v, long long lo, long long hi) { return v >= lo && v < hi; }
 
 void put_blk(W &w, H2E_dense_mat_p b) {
     int nr = b ? b->nrow : 0, nc = b ? b->ncol : 0, ld = b ? b->ld : 0;
@@ -50,7 +58,9 @@ void put_blk(W &w, H2E_dense_mat_p b) {
 
 H2E_dense_mat_p get_blk(R &r) {
     int nr = r.i32(), nc = r.i32(), ld = r.i32();
-    if (!r.ok || nr < 0 || nc < 0) { r.ok = false; return nullptr; }
+    if (!r.ok || nr < 0 || nc < 0 || nr >= MAX_COUNT || nc >= MAX_COUNT || ld < -MAX_COUNT) { r.ok = false; return nullptr; }
+    const long long elems = ld >= 0 ? (long long)nr * nc : (long long)(nr + nc) * (-(long long)ld);
+    if (elems > MAX_BLOCK_ELEMS || (ld < 0 && (-ld > nr || -ld > nc))) { r.ok = false; return nullptr; }
     H2E_dense_mat_p b;
     if (ld >= 0) {
         b = dmat_new(nr, nc);
@@ -60,6 +70,105 @@ H2E_dense_mat_p get_blk(R &r) {
     }
     r.get(b->data, blk_elems(b));
     return b;
+}
+
+// Header counts of a file: every count non-negative and bounded, a tree
+// with at least one node and one leaf, at most 63 levels.
+bool header_ok(const int *hdr) {
+    for (int i = 0; i < 11; i++)
+        if (!in_range(hdr[i], 0, MAX_COUNT)) return false;
+    const int n_node = hdr[5], n_leaf = hdr[6], max_level = hdr[7], min_adm = hdr[8];
+    return n_node >= 1 && in_range(n_leaf, 1, n_node + 1) && max_level < 63 && min_adm <= max_level + 1;
+}
+
+// Index arrays of a loaded tree: every stored index inside the array it
+// addresses, shell-pair offsets monotone from 0 to krnl_mat_size, each leaf a
+// non-empty contiguous range of shell pairs, and node lists naming nodes.
+bool tree_ok(H2ERI_p h, bool chem) {
+    const int nn = h->n_node, np = h->num_sp, nl = h->n_leaf_node, L = h->max_level + 1;
+    if (h->sp_bfp_sidx[0] != 0 || h->sp_bfp_sidx[np] != h->krnl_mat_size) return false;
+    for (int i = 0; i < np; i++)
+        if (h->sp_bfp_sidx[i + 1] < h->sp_bfp_sidx[i]) return false;
+    if (chem) {
+        if (h->shell_bf_sidx[0] != 0) return false;
+        for (int i = 0; i < h->nshell; i++)
+            if (h->shell_bf_sidx[i + 1] < h->shell_bf_sidx[i]) return false;
+        for (size_t i = 0; i < 2 * (size_t)np; i++)
+            if (!in_range(h->sp_shell_idx[i], 0, h->nshell)) return false;
+        // each shell pair's functions fill exactly its x range
+        for (int i = 0; i < np; i++) {
+            const int s0 = h->sp_shell_idx[i], s1 = h->sp_shell_idx[i + np];
+            const long long nb = (long long)(h->shell_bf_sidx[s0 + 1] - h->shell_bf_sidx[s0]) *
+                                 (h->shell_bf_sidx[s1 + 1] - h->shell_bf_sidx[s1]);
+            if (nb != h->sp_bfp_sidx[i + 1] - h->sp_bfp_sidx[i]) return false;
+        }
+    }
+    int leaves = 0, next_pair = 0;
+    for (int i = 0; i < nn; i++) {
+        if (!in_range(h->parent[i], -1, nn) || !in_range(h->n_child[i], 0, 9)) return false;
+        if (!in_range(h->node_level[i], 0, L)) return false;
+        for (int c = 0; c < h->n_child[i]; c++)
+            if (!in_range(h->children[8 * (size_t)i + c], 0, i)) return false;   // post order: children first
+        const int ps = h->pt_cluster[2 * i], pe = h->pt_cluster[2 * i + 1];
+        if (!in_range(ps, 0, np) || !in_range(pe, ps, np)) return false;
+        if (h->n_child[i] == 0) {
+            // leaves in node order take consecutive shell-pair ranges that
+            // cover every pair once (their x ranges then partition x)
+            if (ps != next_pair) return false;
+            next_pair = pe + 1;
+            leaves++;
+        }
+    }
+    if (leaves != nl || next_pair != np) return false;
+    for (int l = 0; l < L; l++) {
+        if (!in_range(h->level_n_node[l], 0, nl + 1) || !in_range(h->height_n_node[l], 0, nl + 1)) return false;
+        for (int j = 0; j < h->level_n_node[l]; j++)
+            if (!in_range(h->level_nodes[(size_t)l * nl + j], 0, nn)) return false;
+        for (int j = 0; j < h->height_n_node[l]; j++)
+            if (!in_range(h->height_nodes[(size_t)l * nl + j], 0, nn)) return false;
+    }
+    if (h->height_n_node[0] != nl) return false;
+    for (int j = 0; j < nl; j++)
+        if (h->n_child[h->height_nodes[j]] != 0) return false;
+    for (size_t i = 0; i < 2 * (size_t)h->n_r_adm_pair; i++)
+        if (!in_range(h->r_adm_pairs[i], 0, nn)) return false;
+    for (size_t i = 0; i < 2 * (size_t)h->n_r_inadm_pair; i++)
+        if (!in_range(h->r_inadm_pairs[i], 0, nn)) return false;
+    for (int i = 0; i < h->n_r_inadm_pair; i++)   // near-field pairs join two leaves
+        if (h->n_child[h->r_inadm_pairs[2 * i]] != 0 || h->n_child[h->r_inadm_pairs[2 * i + 1]] != 0) return false;
+    return true;
+}
+
+int leaf_elems(H2ERI_p h, int node) {
+    return h->sp_bfp_sidx[h->pt_cluster[2 * node + 1] + 1] - h->sp_bfp_sidx[h->pt_cluster[2 * node]];
+}
+
+// Block shapes against the tree: a leaf's U has one row per basis-function
+// pair of the leaf, an inner node's U one row per column of its children's
+// U; B and D blocks have the shapes the tree gives them.
+bool blocks_ok(H2ERI_p h) {
+    auto rows = [](H2E_dense_mat_p b) { return b ? b->nrow : 0; };
+    auto cols = [](H2E_dense_mat_p b) { return b ? b->ncol : 0; };
+    for (int i = 0; i < h->n_node; i++) {
+        H2E_dense_mat_p u = h->U[i];
+        if (!u) continue;
+        if (u->ld < 0) return false;   // U blocks are dense
+        if (cols(u) == 0) continue;
+        int nr = 0;
+        if (h->n_child[i] == 0) nr = leaf_elems(h, i);
+        else
+            for (int c = 0; c < h->n_child[i]; c++) nr += cols(h->U[h->children[8 * (size_t)i + c]]);
+        if (rows(u) != nr) return false;
+    }
+    for (int i = 0; i < h->n_B; i++) {
+        H2E_dense_mat_p b = h->c_B_blks[i];
+        if (b && (b->nrow || b->ncol) && (b->nrow != h->B_nrow[i] || b->ncol != h->B_ncol[i])) return false;
+    }
+    for (int i = 0; i < h->n_D; i++) {
+        H2E_dense_mat_p b = h->c_D_blks[i];
+        if (b && (b->nrow || b->ncol) && (b->nrow != h->D_nrow[i] || b->ncol != h->D_ncol[i])) return false;
+    }
+    return true;
 }
 
 }  // namespace
@@ -121,6 +230,11 @@ int H2ERI_load_rep(H2ERI_p h, const char *fname) {
     }
     int hdr[11];
     r.get(hdr, 11);
+    if (!r.ok || !header_ok(hdr)) {
+        fclose(f);
+        set_error(ERR_IO, "load_rep: %s has invalid header counts", fname);
+        return -1;
+    }
     h->natom = hdr[0]; h->nshell = hdr[1]; h->num_bf = hdr[2]; h->num_sp = hdr[3];
     h->krnl_mat_size = h->num_sp_bfp = hdr[4]; h->n_node = hdr[5]; h->n_leaf_node = hdr[6];
     h->max_level = hdr[7]; h->min_adm_level = hdr[8]; h->n_r_adm_pair = hdr[9]; h->n_r_inadm_pair = hdr[10];
@@ -140,6 +254,11 @@ int H2ERI_load_rep(H2ERI_p h, const char *fname) {
     h->height_n_node = r.arr<int>(L); h->height_nodes = r.arr<int>(L * nl);
     h->r_adm_pairs = r.arr<int>(2 * (size_t)h->n_r_adm_pair);
     h->r_inadm_pairs = r.arr<int>(2 * (size_t)h->n_r_inadm_pair);
+    if (r.ok && !tree_ok(h, (flags & 1u) != 0)) {
+        fclose(f);
+        set_error(ERR_IO, "load_rep: %s holds an index outside its range", fname);
+        return -1;
+    }
     if (r.ok) {
         h->sp_nbfp = (int *)malloc(sizeof(int) * (h->num_sp > 0 ? h->num_sp : 1));
         for (int i = 0; i < h->num_sp; i++) h->sp_nbfp[i] = h->sp_bfp_sidx[i + 1] - h->sp_bfp_sidx[i];
@@ -159,6 +278,7 @@ int H2ERI_load_rep(H2ERI_p h, const char *fname) {
     }
     fclose(f);
     if (!r.ok) { set_error(ERR_IO, "load_rep: %s is truncated or corrupt", fname); return -1; }
+    if (!blocks_ok(h)) { set_error(ERR_IO, "load_rep: %s holds a block whose shape does not match the tree", fname); return -1; }
     if (h->shell_bf_sidx) {
         h->num_bf = h->shell_bf_sidx[h->nshell];
         int mx = 0;

@@ -231,6 +231,11 @@ class H2ERI:
         _check(self.lib)
         return r
 
+    def set_spin_limit(self, polls):
+        """Bound of the persistent executor's dependency waits, in polls."""
+        self.lib.H2ERI_set_spin_limit(self.h, int(polls))
+        _check(self.lib)
+
     def queue_stats(self, nv=1):
         out = np.zeros(4)
         self.lib.H2ERI_queue_stats(self.h, nv, _dptr(out))
@@ -322,6 +327,16 @@ class H2ERI:
 
     def build_exchange_dev(self, d_ptr, k_ptr, stream=None):
         self.lib.H2ERI_build_exchange_dev(self.h, C.c_void_p(d_ptr), C.c_void_p(k_ptr), C.c_void_p(stream or 0))
+        _check(self.lib)
+
+    def exchange_workbuf_init(self):
+        """Builds and uploads the exchange plan now (H2ERI_exchange_workbuf_init)."""
+        self.lib.H2ERI_exchange_workbuf_init(self.h)
+        _check(self.lib)
+
+    def exchange_workbuf_free(self):
+        """Releases the exchange plan's device memory (H2ERI_exchange_workbuf_free)."""
+        self.lib.H2ERI_exchange_workbuf_free(self.h)
         _check(self.lib)
 
     def exchange_stats(self):

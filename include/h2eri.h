@@ -282,7 +282,10 @@ void H2ERI_build_Coulomb_multi_dev(H2ERI_p h2eri, const int ndens, const double 
  * with n vectors, kind 1 H2ERI_build_Coulomb_multi_dev with n densities, on
  * the device buffers in_dev/out_dev; nothing runs during the capture.
  * H2ERI_graph_replay enqueues the captured sequence on `stream`.  One graph
- * per handle; changing precision, tile size or schedule releases it.
+ * per handle; changing precision, tile size, schedule or executor, running
+ * the autotuner, or capturing again releases it (a replay after that fails
+ * with the error set).  A Coulomb graph owns its x, y and half-matrix
+ * buffers, so eager calls of any size between replays leave it valid.
  * Return 0, or -1 with the error set. */
 int H2ERI_graph_capture(H2ERI_p h2eri, const int kind, const int n, const double *in_dev, double *out_dev);
 int H2ERI_graph_replay(H2ERI_p h2eri, void *stream);
@@ -356,8 +359,16 @@ int  H2ERI_get_executor(H2ERI_p h2eri);   /* of single-vector passes */
 /* Synchronises the device and reports whether a dependency wait of the
  * persistent executor exceeded its bound (about a second) since the last
  * call; such a pass's results are invalid.  0 clean, 1 with the error set,
- * -1 on a device error.  Clears the flag. */
+ * -1 on a device error.  Clears the flag.  The synchronous entry points
+ * (H2ERI_matvec[_multi], H2ERI_build_Coulomb[_multi]) read the flag
+ * themselves and set ERR_DEVICE for such a pass; device-pointer forms and
+ * graph replays leave it to this call. */
 int H2ERI_device_status(H2ERI_p h2eri);
+/* Bound of every dependency wait of the persistent executor, in polls
+ * (default 2^22, about a second; additive, for tests of the error path: a
+ * tiny bound makes waits give up, every wave still exits and the pass is
+ * reported as failed). */
+void H2ERI_set_spin_limit(H2ERI_p h2eri, const int polls);
 
 /* Queue of the nv-vector plan for the persistent executor (built when
  * needed): out4 = {tasks, tasks started without a ticket, workgroups,

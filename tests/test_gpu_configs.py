@@ -49,7 +49,7 @@ def test_water_cluster_j_and_k(water8):
         J32 = h.build_coulomb(D)
     finally:
         h.set_precision(64)
-    assert relerr(J32, J) < 1e-6
+    assert relerr(J32, J) < 1e-7   # precision gate of the FP32-stored form
     K = h.build_exchange(D)
     assert relerr(K, o.exchange(D)) < 1e-12
 
@@ -66,7 +66,7 @@ def test_tight_tolerance_chain(tmp_path_factory):
     assert relerr(J, Jo) < 1e-13
     h.set_precision(32)
     try:
-        assert relerr(h.build_coulomb(D), Jo) < 1e-6
+        assert relerr(h.build_coulomb(D), Jo) < 1e-7   # precision gate of the FP32-stored form
     finally:
         h.set_precision(64)
 

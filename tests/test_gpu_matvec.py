@@ -226,3 +226,42 @@ def test_wave_tile_limits(reps, wave_bytes):
         h.set_wave_tile_bytes(32768)
     for k in range(3):
         assert relerr(Y[k], o.matvec(X[k])) < 1e-13
+
+
+def test_persistent_wait_bound_reports_error(tmp_path):
+    """The persistent executor's dependency waits given up at once (a bound
+    of zero polls, set from the host): every kernel still exits, and each
+    synchronous call either reports the failed pass with an error or, when
+    no wait had to give up, returns the correct product; on a tree of
+    thousands of dependent tasks at least one call reports it.  With the
+    default bound the same handle is correct again."""
+    import h2eri
+    h = make_synthetic(tmp_path / "w.rep", 8000, 31, leaf=60, max_rank=30, lr_frac=0.4, diag_lr_frac=0.3,
+                       shape="cube")
+    o = oracle_bind.Oracle(str(tmp_path / "w.rep"))
+    X = np.random.default_rng(60).standard_normal((3, h.n))
+    yo = o.matvec(X[0])
+    h.set_executor(1)
+    failed = 0
+    try:
+        h.set_spin_limit(0)
+        for _ in range(3):
+            try:
+                y = h.matvec(X[0])
+            except h2eri.H2ERIError as e:
+                failed += 1
+                assert "bound" in str(e)
+            else:
+                assert relerr(y, yo) < 1e-13
+        try:
+            Y = h.matvec_multi(X)
+        except h2eri.H2ERIError:
+            failed += 1
+        else:
+            assert relerr(Y[0], yo) < 1e-13
+    finally:
+        h.set_spin_limit(1 << 22)
+    assert failed >= 1
+    assert relerr(h.matvec(X[0]), yo) < 1e-13
+    assert h.device_status() == 0
+    h.set_executor(0)

@@ -75,6 +75,62 @@ def test_errors_are_reported_not_aborting(tmp_path):
         h3.build_h2(bd_jit=1)
 
 
+def _rep_offsets(data):
+    """Byte offsets of the sections of a representation file without
+    chemistry metadata (the layout written by H2ERI_save_rep)."""
+    hdr = np.frombuffer(data, dtype="<i4", count=11, offset=16)
+    num_sp, n_node, n_leaf, max_level = int(hdr[3]), int(hdr[5]), int(hdr[6]), int(hdr[7])
+    n_adm, n_inadm = int(hdr[9]), int(hdr[10])
+    L = max_level + 1
+    off = {"hdr": 16, "sp_bfp_sidx": 60}
+    o = 60 + 4 * (num_sp + 1)
+    for name, cnt in (("parent", n_node), ("children", 8 * n_node), ("n_child", n_node), ("node_level", n_node),
+                      ("node_height", n_node), ("pt_cluster", 2 * n_node), ("mat_cluster", 2 * n_node),
+                      ("level_n_node", L), ("level_nodes", L * n_leaf), ("height_n_node", L),
+                      ("height_nodes", L * n_leaf), ("r_adm_pairs", 2 * n_adm), ("r_inadm_pairs", 2 * n_inadm)):
+        off[name] = o
+        o += 4 * cnt
+    off["blocks"] = o
+    return off, hdr
+
+
+def test_load_rejects_corrupt_files(tmp_path):
+    """A representation file with counts, indices or block shapes outside
+    what its tree allows is refused with an error, before anything reads
+    through them."""
+    import h2eri
+    make_synthetic(tmp_path / "ok.rep", 600, 3, leaf=25, max_rank=10, lr_frac=0.5, diag_lr_frac=0.5)
+    data = bytearray(open(tmp_path / "ok.rep", "rb").read())
+    off, hdr = _rep_offsets(data)
+    n_node = int(hdr[5])
+    root_children = off["children"] + 4 * 8 * (n_node - 1)
+    cases = {
+        "negative n_node": (off["hdr"] + 4 * 5, -5),
+        "huge n_node": (off["hdr"] + 4 * 5, 1 << 30),
+        "negative num_sp": (off["hdr"] + 4 * 3, -1),
+        "x offset past N": (off["sp_bfp_sidx"] + 4, 1 << 27),
+        "child index past the tree": (root_children, n_node + 10),
+        "admissible pair names no node": (off["r_adm_pairs"], n_node),
+        "leaf range past the pairs": (off["pt_cluster"] + 4, int(hdr[3]) + 5),
+        "first block taller than its leaf": (off["blocks"], 100000),
+    }
+    for name, (pos, val) in cases.items():
+        bad = bytearray(data)
+        bad[pos:pos + 4] = int(val).to_bytes(4, "little", signed=True)
+        p = tmp_path / "bad.rep"
+        p.write_bytes(bytes(bad))
+        with pytest.raises(h2eri.H2ERIError):
+            h2eri.H2ERI.load(str(p))
+            pytest.fail(name)
+    for cut in (20, off["parent"] + 7, off["blocks"] + 10, len(data) - 4):
+        p = tmp_path / "cut.rep"
+        p.write_bytes(bytes(data[:cut]))
+        with pytest.raises(h2eri.H2ERIError):
+            h2eri.H2ERI.load(str(p))
+    # the untouched file still loads
+    h2eri.H2ERI.load(str(tmp_path / "ok.rep"))
+
+
 @pytest.mark.parametrize("seed,n,leaf,mr,shape", [(1, 900, 30, 12, "chain"), (2, 1200, 150, 40, "chain"),
                                                   (4, 1500, 40, 20, "cube"), (5, 40, 300, 8, "chain"),
                                                   (6, 2000, 120, 600, "chain")])
