@@ -1,25 +1,41 @@
 #!/usr/bin/env python3
 """Benchmark of the H2 J-build matvec on one MI355X.
 
-Workload (default): linear alkane C30H62, cc-pVDZ (760 Cartesian basis
-functions), H2 relative tolerance 1e-6, seeded damped density with 121
-occupied columns.  The H2 representation is constructed on the host and
-uploaded once.  A step is one H2 matvec y = A_H2 x on the uncontracted
-density, replayed from a captured HIP graph, with x and y resident in HBM.
-Timed forms: FP64 storage (the drop-in) and FP32-stored U/B (FP64 near field
-and accumulation), each for 1 density and for batches of 2, 4 and 8 densities
-per pass over the matrix data.  The single-density form runs the configuration
-the project's autotuner picks for it (FP32-stored), each batch width the one it
-picks for that width and precision.  `value` is the FP32-stored single-density
-form's algorithmic HBM throughput; the batched forms are reported beside it,
-and so is the FP64 exchange (K) build of the same molecule and density
-(`exchange_fp64`: ms per H2ERI_build_exchange_dev call, plan uploaded once).
+Workload (default, --config 2): linear alkane C100H202, cc-pVDZ (2510
+Cartesian basis functions), H2 relative tolerance 1e-8, the largest
+representation of the benchmark configurations on one GPU (6.7 GB of FP64
+matrix data).  The representation is constructed on the host and uploaded
+once.  A step is one H2 matvec y = A_H2 x on the uncontracted density (seeded
+damped density, 401 occupied columns), replayed from a captured HIP graph,
+with x and y resident in HBM.  Timed in the same run:
+
+* one density per pass: FP64 storage (the drop-in) and FP32-stored U/B (FP64
+  near field and accumulation); the executor, tile size and near-field
+  schedule are the project's autotuner's choice for the FP32-stored form;
+* the batching workload on the same representation: 16 seeded densities in
+  one graph replay (two passes of 8 over the matrix data), FP64 and
+  FP32-stored, with the autotuner's choice for 8 densities per pass.
+
+`value` is the FP32-stored single-density form's algorithmic HBM throughput
+(bytes of the representation and vectors one matvec must move, divided by
+the graph replay's time).  Beside it: the batched forms, the precision error
+(FP32-stored J against FP64 J), the FP64 J against the oracle, the
+compression error of both against directly evaluated ERI rows (256 sampled
+bra shell pairs), the roofline of the dominant kernel from the union of its
+launch intervals, per-phase times, and the oracle's matvec on the host cores
+(median of 10 after a warm-up, all cores and one core).
+
+--config 1 and --config 4 run the same measurement on C30H62 (reltol 1e-6)
+and (H2O)64 (reltol 1e-6, with the exchange build K beside J); --config 3
+runs the molecule group of the batching workload (C20..C160 in one
+persistent launch).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -29,13 +45,18 @@ sys.path.insert(0, os.path.join(ROOT, "h2p-eri_amd"))
 
 METRIC = "J-build H2 matvec ms & HBM GB/s, FP32-stored vs FP64 on 1 GPU, rel.err ≤1e-7"
 HBM_PEAK_GBS = 8000.0
+MFMA_FP64_PEAK_TFLOPS = 78.6
 MALL_BYTES = 256e6
+EPS_PREC_GATE = 1e-7
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# ---------------------------------------------------------------------------
+# timing helpers
+# ---------------------------------------------------------------------------
 def time_graph(h, torch, stream, steps, warmup):
     """ms per replay of the captured graph, HIP events on the replay stream."""
     for _ in range(warmup):
@@ -51,15 +72,35 @@ def time_graph(h, torch, stream, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
-def launch_profile(h, torch, x, y, stream, reps):
-    """Per-launch durations (HIP events around each launch, on the stream the
-    launch runs on), averaged over `reps` eager matvecs."""
+def interval_union(recs):
+    """Length of the union of the launch intervals [start, start + ms]."""
+    iv = sorted((r["start"], r["start"] + r["ms"]) for r in recs)
+    tot, cur0, cur1 = 0.0, None, None
+    for a, b in iv:
+        if cur1 is None or a > cur1:
+            if cur1 is not None:
+                tot += cur1 - cur0
+            cur0, cur1 = a, b
+        else:
+            cur1 = max(cur1, b)
+    if cur1 is not None:
+        tot += cur1 - cur0
+    return tot
+
+
+def launch_profile(h, torch, x_ptr, y_ptr, stream, reps):
+    """Per-launch durations of one eager single-density matvec (HIP events
+    around each launch, on the stream the launch runs on, plus a base event
+    before the pass), averaged over `reps` matvecs: the records, the mean
+    union of the launch intervals and the mean sum of the launch durations."""
     h.set_phase_timing(True)
-    acc = None
+    acc, unions, sums = None, [], []
     for _ in range(reps):
-        h.matvec_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+        h.matvec_dev(x_ptr, y_ptr, stream.cuda_stream)
         torch.cuda.synchronize()
         st = h.launch_stats()
+        unions.append(interval_union(st))
+        sums.append(sum(r["ms"] for r in st))
         if acc is None:
             acc = [dict(r) for r in st]
         else:
@@ -68,31 +109,7 @@ def launch_profile(h, torch, x, y, stream, reps):
     h.set_phase_timing(False)
     for a in acc:
         a["ms"] /= reps
-    return acc
-
-
-def cpu_baseline(rep_path, x, seconds):
-    """The oracle (plain C + OpenMP restatement of the reference matvec) on
-    the host cores, on the same representation."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_bind
-    nthr = os.cpu_count() or 1
-    try:
-        nthr = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
-    nthr = min(nthr, int(os.environ.get("OMP_NUM_THREADS", nthr)))
-    lib = oracle_bind.load()
-    lib.orc_set_threads(nthr)
-    o = oracle_bind.Oracle(rep_path)
-    o.matvec(x)   # first touch
-    t0 = time.perf_counter()
-    n = 0
-    while n < 5 or (time.perf_counter() - t0 < seconds and n < 200):
-        o.matvec(x)
-        n += 1
-    ms = (time.perf_counter() - t0) / n * 1e3
-    return ms, nthr, n, o
+    return acc, sum(unions) / reps, sum(sums) / reps
 
 
 def uncontract(tr, D, n):
@@ -117,7 +134,7 @@ def build_molecule(h2eri, M, atoms, reltol, label):
     h.read_mol_text(M.mol_text(atoms))
     h.process_shells()
     h.partition()
-    log(f"[bench] {label}: {h.nbf} basis functions, N={h.n}; building H2 ...")
+    log(f"[bench] {label}: {h.nbf} basis functions, {h.s.num_sp} shell pairs, N={h.n}; building H2 ...")
     h.s.print_dbginfo = 1
     h.build_h2()
     h.s.print_dbginfo = 0
@@ -125,162 +142,124 @@ def build_molecule(h2eri, M, atoms, reltol, label):
     return h
 
 
-def batching_config(args):
-    """Batching workload: 16 densities on C100H202 per graph replay (passes of
-    8 densities over the matrix data, tile shape from the autotuner), and 8
-    alkanes C20..C160 of uneven size as one molecule group (one persistent
-    launch over all their tasks), replayed 20 times."""
+# ---------------------------------------------------------------------------
+# CPU baseline: the oracle's OpenMP matvec in a child process, so that the
+# OpenMP binding variables take effect before its runtime starts
+# ---------------------------------------------------------------------------
+def host_info():
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)", "NUMA node(s)") \
+                    or (k.startswith("NUMA node") and "CPU(s)" in k):
+                info[k] = v
+    except OSError:
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return info
+
+
+def cpu_threads():
+    n = os.cpu_count() or 1
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
+
+
+def cpu_child(rep_path, x_path, threads, reps):
+    """Runs in the child: one warm-up, then `reps` timed oracle matvecs."""
     import numpy as np
-    import torch
-
-    import h2eri
-    from h2eri import molecules as M
-
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
-    rng = M.rng_for(3)
-    atoms = M.alkane(100, rng)
-    Ds = np.stack([M.density(atoms, 401, rng) for _ in range(16)])
-    h = build_molecule(h2eri, M, atoms, 1e-8, "C100H202 reltol 1e-8")
-    h.set_precision(32)
-    tune = h.autotune(nv=8, reps=5)
-    log(f"[bench] autotune (8 densities per pass): {tune}")
-    nb = h.matvec_bytes()
-    mat = nb["U"] + nb["B"] + nb["D"]
-    Dd = torch.from_numpy(Ds).to(dev)
-    Jd = torch.empty_like(Dd)
-    h.graph_capture("coulomb", 16, Dd.data_ptr(), Jd.data_ptr())
-    ms16 = time_graph(h, torch, stream, 20, 1)
-    J16 = Jd.cpu().numpy()
-    h.graph_capture("coulomb", 1, Dd.data_ptr(), Jd.data_ptr())
-    ms1 = time_graph(h, torch, stream, 20, 1)
-    J1 = Jd[0].cpu().numpy()
-    dev_single = float(np.linalg.norm(J16[0] - J1) / np.linalg.norm(J1))
-    h.set_precision(64)
-    h.graph_capture("coulomb", 1, Dd.data_ptr(), Jd.data_ptr())
-    ms1_64 = time_graph(h, torch, stream, 20, 1)
-    J64 = Jd[0].cpu().numpy()
-    eps32 = float(np.linalg.norm(J1 - J64) / np.linalg.norm(J64))
-    log(f"[bench] 16 densities: {ms16:.3f} ms per replay ({ms16 / 16:.4f} ms/density); "
-        f"1 density FP32 {ms1:.3f} ms, FP64 {ms1_64:.3f} ms")
-    alg16 = 2 * mat + 16 * nb["vec"]   # two passes of 8 densities
-    part1 = {"molecule": "C100H202", "reltol": 1e-8, "num_bf": h.nbf, "krnl_mat_size": h.n,
-             "H2_MB": {k: round(h.s.mat_size[i] * 8 / 1e6, 1) for i, k in enumerate(["U", "B", "D"])},
-             "autotune": tune, "ms_16_densities": round(ms16, 4), "ms_per_density": round(ms16 / 16, 5),
-             "ms_1_density_fp32": round(ms1, 4), "ms_1_density_fp64": round(ms1_64, 4),
-             "speedup_per_density_vs_fp64_single": round(ms1_64 / (ms16 / 16), 3),
-             "GBs_algorithmic": round(alg16 / ms16 / 1e6, 1),
-             "rel_dev_batched_vs_single_J": dev_single, "rel_err_J_fp32_vs_fp64": eps32}
-    del h, Dd, Jd
-    # part 2: uneven molecules as one group
-    hs, X, Y, ns = [], [], [], []
-    for n in range(20, 161, 20):
-        r = M.rng_for(300 + n)
-        at = M.alkane(n, r)
-        D = M.density(at, 4 * n + 1, r)
-        hh = build_molecule(h2eri, M, at, 1e-6, f"C{n}H{2 * n + 2}")
-        hh.set_precision(32)
-        hh.upload()
-        x = uncontract(hh.tree(), D, hh.n)
-        hs.append(hh)
-        X.append(torch.from_numpy(x).to(dev))
-        Y.append(torch.empty_like(X[-1]))
-        ns.append(hh.nbf)
-    g = h2eri.Group(hs, [x.data_ptr() for x in X], [y.data_ptr() for y in Y])
-    for _ in range(2):
-        g.matvec(stream.cuda_stream)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(20):
-        g.matvec(stream.cuda_stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms_group = e0.elapsed_time(e1) / 20
-    g.status()   # raises if a dependency wait of the persistent launch exceeded its bound
-    Yg = [y.cpu().numpy() for y in Y]
-    e0.record(stream)
-    for _ in range(20):
-        for hh, x, y in zip(hs, X, Y):
-            hh.matvec_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms_seq = e0.elapsed_time(e1) / 20
-    dev_group = max(float(np.linalg.norm(a - y.cpu().numpy()) / np.linalg.norm(a)) for a, y in zip(Yg, Y))
-    alg_group = sum(hh.matvec_bytes()["alg"] for hh in hs)
-    log(f"[bench] 8 molecules: group {ms_group:.3f} ms vs one after another {ms_seq:.3f} ms")
-    g.close()
-    part2 = {"molecules": [f"C{n}H{2 * n + 2}" for n in range(20, 161, 20)], "num_bf": ns, "reltol": 1e-6,
-             "ms_group_matvec": round(ms_group, 4), "ms_sequential_matvecs": round(ms_seq, 4),
-             "GBs_algorithmic_group": round(alg_group / ms_group / 1e6, 1),
-             "rel_dev_group_vs_single": dev_group}
-    out = {"metric": METRIC, "value": part1["GBs_algorithmic"], "unit": "GB/s", "n_gpus": 1, "steps": 20,
-           "warmup": 1, "ms_per_step": part1["ms_16_densities"], "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32-stored U/B, f64 D and accumulation",
-           "data": "synthetic (seeded alkane geometries and densities)",
-           "config": {"workload": "batching: 16 densities on C100H202 per graph replay; 8 uneven alkanes as a group",
-                      "timing": "hipGraph replays / back-to-back group launches"},
-           "batched_densities": part1, "molecule_group": part2}
-    print(json.dumps(out), flush=True)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind
+    lib = oracle_bind.load()
+    lib.orc_set_threads(threads)
+    o = oracle_bind.Oracle(rep_path)
+    x = np.load(x_path)
+    o.matvec(x)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        o.matvec(x)
+        ts.append((time.perf_counter() - t0) * 1e3)
+    o.close()
+    ts.sort()
+    print(json.dumps({"median_ms": ts[len(ts) // 2] if len(ts) % 2 else 0.5 * (ts[len(ts) // 2 - 1] + ts[len(ts) // 2]),
+                      "ms": ts, "threads": threads}), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batches", default="1,2,4,8", help="densities per pass to time")
-    ap.add_argument("--tuned", default="", help="EXECUTOR,TILE_BYTES,SCHEDULE[,WAVE_TILE_BYTES] instead of running the autotuner "
-                    "(profiling runs, where the profiler distorts the tuner's timings)")
-    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
-                    help="run a plain device copy of this many MB first (HBM counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on exactly one GPU (got --gpus {args.gpus}); the single-GPU "
-              "matvec is the unit of work and no multi-GPU path is provided.", file=sys.stderr)
-        sys.exit(2)
-    if args.config == 3:
-        batching_config(args)
-        return
-    batches = [int(b) for b in args.batches.split(",")]
-    assert batches[0] == 1 and all(b in (1, 2, 4, 8) for b in batches)
-
+def cpu_baseline(rep_path, x, reps=10, one_core=True):
+    """Median ms of the oracle matvec on all host cores given to this process
+    and on one core, each in a child process with OMP_PROC_BIND=close and
+    OMP_PLACES=cores."""
     import numpy as np
-    import torch
+    res = {}
+    with tempfile.TemporaryDirectory() as d:
+        xp = os.path.join(d, "x.npy")
+        np.save(xp, x)
+        for label, nthr in (("all", cpu_threads()), ("one", 1)):
+            if label == "one" and not one_core:
+                continue
+            env = dict(os.environ, OMP_NUM_THREADS=str(nthr), OMP_PROC_BIND="close", OMP_PLACES="cores")
+            t0 = time.perf_counter()
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", rep_path, xp, str(nthr), str(reps)],
+                               stdout=subprocess.PIPE, env=env, text=True, check=True)
+            res[label] = json.loads(r.stdout.strip().splitlines()[-1])
+            log(f"[bench] CPU oracle matvec, {nthr} thread(s): median {res[label]['median_ms']:.2f} ms "
+                f"({time.perf_counter() - t0:.1f} s)")
+    return res
 
-    import h2eri
-    from h2eri import molecules as M
 
+def direct_rows_error(h, x, ys, npair=256, seed=7):
+    """Relative error of each y in `ys` against directly evaluated rows of the
+    ERI matrix times x, over the rows of `npair` seeded random shell pairs."""
+    import numpy as np
+    pairs = np.sort(np.random.default_rng(seed).choice(h.s.num_sp, min(npair, h.s.num_sp), replace=False))
     t0 = time.perf_counter()
-    atoms, mol, D, reltol = M.workload(args.config)
-    wname = M.workload_name(args.config)
-    h = h2eri.H2ERI(1e-10, 1e-10, reltol)
-    h.read_mol_text(mol)
-    h.process_shells()
-    h.partition()
-    log(f"[bench] config {args.config}: {len(atoms)} atoms, {h.nbf} basis functions, "
-        f"{h.s.num_sp} shell pairs, N={h.n}; building H2 ...")
-    h.s.print_dbginfo = 1   # construction prints a line per U level and after B and D
-    h.build_h2()
-    h.s.print_dbginfo = 0
+    rows, yd = h.direct_matvec_rows(pairs, x)
+    log(f"[bench] direct ERI rows of {len(pairs)} shell pairs ({len(rows)} rows) in {time.perf_counter() - t0:.1f} s")
+    nrm = float(np.linalg.norm(yd))
+    return {k: float(np.linalg.norm(np.asarray(y)[rows] - yd) / nrm) for k, y in ys.items()}, len(pairs), len(rows)
+
+
+# ---------------------------------------------------------------------------
+# one representation: single density and the 16-density batch
+# ---------------------------------------------------------------------------
+def single_workload(args):
+    import numpy as np
+    import torch
+
+    import h2eri
+    from h2eri import molecules as M
+
+    cfg = args.config
+    atoms, _, D, reltol = M.workload(cfg)
+    wname = M.workload_name(cfg)
+    nocc = M.CONFIGS[cfg]["nocc"]
+    t0 = time.perf_counter()
+    h = build_molecule(h2eri, M, atoms, reltol, f"{wname} reltol {reltol:g}")
     t_build = time.perf_counter() - t0
     sizes = {k: round(h.s.mat_size[i] * 8 / 1e6, 2) for i, k in enumerate(["U_MB", "B_MB", "D_MB"])}
-    log(f"[bench] H2 built in {t_build:.1f} s: {sizes}")
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
-    D_dev = torch.from_numpy(D).to(dev)
-    J_dev = torch.empty_like(D_dev)
-    x = uncontract(h.tree(), D, h.n)
-    nb_max = max(batches)
-    # batch of densities: scaled copies of x (the matvec is linear; inputs are synthetic)
-    X = np.stack([x * (1.0 + 0.05 * k) for k in range(nb_max)])
+    tr = h.tree()
+    x = uncontract(tr, D, h.n)
+    Ds = M.batch_densities(atoms, nocc, args.batch)
+    X = np.stack([uncontract(tr, Dk, h.n) for Dk in Ds])
+    x_dev = torch.from_numpy(x).to(dev)
+    y_dev = torch.empty_like(x_dev)
     X_dev = torch.from_numpy(X).to(dev)
     Y_dev = torch.empty_like(X_dev)
+    D_dev = torch.from_numpy(D).to(dev)
+    J_dev = torch.empty_like(D_dev)
 
     if args.calibrate_copy_mb > 0:
         n = args.calibrate_copy_mb * (1 << 20) // 8
@@ -292,8 +271,8 @@ def main():
         log(f"[bench] calibration copy: {n * 8} bytes read and written")
         del a, b
 
-    # executor, tile size and near-field schedule: the project's autotuner,
-    # on the headline form (FP32-stored U/B, one density per pass)
+    # executor, tile size and near-field schedule: the project's autotuner on
+    # the headline form (FP32-stored U/B, one density per pass)
     h.set_precision(32)
     h.upload()
     if args.tuned:
@@ -311,61 +290,173 @@ def main():
         log(f"[bench] autotune (FP32-stored, 1 density per pass): {tune}")
     kernel_name = "queue_kernel<1>" if tune["executor"] == 1 else "stage_kernel<1>"
 
-    def apply_config(cfg):
-        h.set_executor(cfg["executor"])
-        h.set_tile_bytes(cfg["tile_bytes"])
-        h.set_schedule(cfg["schedule"])
-        h.set_wave_tile_bytes(cfg["wave_tile_bytes"])
-
-    def config_record(cfg):
-        return {"executor": ["per stage", "persistent"][cfg["executor"]], "tile_bytes": int(cfg["tile_bytes"]),
-                "schedule": cfg["schedule"], "wave_tile_bytes": int(cfg["wave_tile_bytes"])}
+    def config_record(c):
+        return {"executor": ["per stage", "persistent"][c["executor"]], "tile_bytes": int(c["tile_bytes"]),
+                "schedule": c["schedule"], "wave_tile_bytes": int(c["wave_tile_bytes"])}
 
     res = {}
     for bits in (64, 32):
         h.set_precision(bits)
-        resident = h.upload()
+        h.upload()
         nb = h.matvec_bytes()
         mat = nb["U"] + nb["B"] + nb["D"]
-        r = dict(resident=resident, nb=nb, batch={})
-        for nv in batches:
-            # one density: the headline configuration (set for every width);
-            # a batch: the autotuner's choice for that width and storage
-            # precision (the library keeps a configuration per width)
-            if nv == 1 or args.tuned:
-                cfg = tune
-                if nv == 1:
-                    apply_config(cfg)
-            else:
-                cfg = h.autotune(nv=nv, reps=5)
-            h.graph_capture("matvec", nv, X_dev.data_ptr(), Y_dev.data_ptr())
-            steps = args.steps if nv == 1 else max(10, args.steps // 2)
-            ms = time_graph(h, torch, stream, steps, args.warmup)
-            alg = mat + nv * nb["vec"]
-            r["batch"][nv] = dict(ms=ms, ms_per_density=ms / nv, alg=alg, GBs=alg / ms / 1e6,
-                                  config=config_record(cfg))
-            if nv == 1:
-                Y1 = Y_dev[0].cpu().numpy()
-            else:   # every density of the batch equals the single-density product (linearity)
-                Yb = Y_dev[:nv].cpu().numpy()
-                r["batch"][nv]["max_rel_dev_vs_single"] = max(
-                    float(np.linalg.norm(Yb[k] - (1.0 + 0.05 * k) * Y1) / np.linalg.norm((1.0 + 0.05 * k) * Y1))
-                    for k in range(nv))
-            log(f"[bench] FP{bits}-stored, {nv} densities/pass: {ms:.4f} ms/pass, {ms / nv:.4f} ms/density, "
-                f"{alg / ms / 1e6:.0f} GB/s algorithmic")
-        r["y"] = Y1
-        r["prof"] = launch_profile(h, torch, X_dev[0], Y_dev[0], stream, 10)
+        r = {"nb": nb, "mat": mat}
+        # one density per pass
+        h.graph_capture("matvec", 1, x_dev.data_ptr(), y_dev.data_ptr())
+        ms1 = time_graph(h, torch, stream, args.steps, args.warmup)
+        r["ms1"], r["alg1"] = ms1, mat + nb["vec"]
+        r["y"] = y_dev.cpu().numpy()
+        log(f"[bench] FP{bits}-stored, 1 density: {ms1:.4f} ms, {r['alg1'] / ms1 / 1e6:.0f} GB/s algorithmic")
+        r["prof"], r["union_ms"], r["sum_ms"] = launch_profile(h, torch, x_dev.data_ptr(), y_dev.data_ptr(), stream, 10)
         h.build_coulomb_dev(D_dev.data_ptr(), J_dev.data_ptr(), stream.cuda_stream)
         torch.cuda.synchronize()
         r["J"] = J_dev.cpu().numpy()
+        # the batch: the autotuner's choice for 8 densities per pass
+        if args.tuned:
+            tune8 = tune
+        else:
+            tune8 = h.autotune(nv=8, reps=5)
+            log(f"[bench] autotune (FP{bits}-stored, 8 densities per pass): {tune8}")
+        h.graph_capture("matvec", args.batch, X_dev.data_ptr(), Y_dev.data_ptr())
+        msb = time_graph(h, torch, stream, max(5, args.steps // 2), max(1, args.warmup // 2))
+        passes = (args.batch + 7) // 8
+        algb = passes * mat + args.batch * nb["vec"]
+        Yb = Y_dev.cpu().numpy()
+        dev_single = 0.0
+        for k in sorted({0, args.batch - 1}):
+            h.matvec_dev(X_dev[k].data_ptr(), y_dev.data_ptr(), stream.cuda_stream)
+            torch.cuda.synchronize()
+            y1 = y_dev.cpu().numpy()
+            dev_single = max(dev_single, float(np.linalg.norm(Yb[k] - y1) / np.linalg.norm(y1)))
+        r["batch"] = {"densities": args.batch, "passes": passes, "ms": msb, "ms_per_density": msb / args.batch,
+                      "alg": algb, "GBs": algb / msb / 1e6, "config_8_per_pass": config_record(tune8),
+                      "max_rel_dev_vs_single": dev_single}
+        log(f"[bench] FP{bits}-stored, {args.batch} densities: {msb:.4f} ms per replay, {msb / args.batch:.4f} ms/density, "
+            f"{algb / msb / 1e6:.0f} GB/s algorithmic")
+        # the single-density configuration back for the next precision
+        h.set_executor(tune["executor"])
+        h.set_tile_bytes(tune["tile_bytes"])
+        h.set_schedule(tune["schedule"])
+        h.set_wave_tile_bytes(tune["wave_tile_bytes"])
         res[bits] = r
     r64, r32 = res[64], res[32]
     eps_prec = float(np.linalg.norm(r32["J"] - r64["J"]) / np.linalg.norm(r64["J"]))
+    log(f"[bench] J FP32-stored vs FP64: {eps_prec:.3e}")
 
-    # exchange build (H2ERI_build_exchange) on the same representation, FP64
-    t0k = time.perf_counter()
+    # compression error against directly evaluated ERI rows
+    eps_direct, npair_s, nrow_s = direct_rows_error(h, x, {"fp64": r64["y"], "fp32": r32["y"]}, npair=args.direct_pairs)
+
+    # exchange build (configuration 4)
+    exch = None
+    if args.exchange:
+        exch = exchange_measure(h, torch, D, D_dev, stream)
+
+    # dominant kernel of the headline form
+    prof = r32["prof"]
+    alg1 = r32["alg1"]
+    dom_gbs = alg1 / r32["union_ms"] / 1e6
+    phases = {}
+    for p in ("up", "B", "y1", "down", "D", "gather", "pass"):
+        q = [a for a in prof if a["phase"] == p]
+        if q:
+            ms = sum(a["ms"] for a in q)
+            mb = sum(a["mat_bytes"] for a in q)
+            phases[p] = {"ms_sum": round(ms, 5), "launches": len(q), "matrix_GB": round(mb / 1e9, 4),
+                         "GBs_matrix": round(mb / ms / 1e6, 1) if ms > 0 else None}
+
+    cpu = None
+    eps_oracle = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_bind
+        with tempfile.TemporaryDirectory() as d:
+            rp = os.path.join(d, "rep.h2")
+            h.save(rp)
+            c = cpu_baseline(rp, x, reps=10, one_core=not args.no_one_core)
+            o = oracle_bind.Oracle(rp)
+            J_cpu = o.coulomb(D)
+            o.close()
+        eps_oracle = {"fp64": float(np.linalg.norm(r64["J"] - J_cpu) / np.linalg.norm(J_cpu)),
+                      "fp32": float(np.linalg.norm(r32["J"] - J_cpu) / np.linalg.norm(J_cpu))}
+        alg64 = r64["alg1"]
+        cpu = {"value": round(alg64 / c["all"]["median_ms"] / 1e6, 2), "unit": "GB/s", "cores": c["all"]["threads"],
+               "kind": "port",
+               "sample": f"oracle FP64 matvec of the same {wname} representation (C restatement of "
+                         f"src/H2ERI_matvec.c), 1 warm-up + 10 timed, median {c['all']['median_ms']:.2f} ms; "
+                         f"OMP_PROC_BIND=close, OMP_PLACES=cores",
+               "median_ms": round(c["all"]["median_ms"], 3),
+               "host": host_info()}
+        if "one" in c:
+            cpu["one_core"] = {"median_ms": round(c["one"]["median_ms"], 3),
+                               "value": round(alg64 / c["one"]["median_ms"] / 1e6, 2)}
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{wname.lower().replace('(', '').replace(')', '')}_j_matvec.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            rec = json.load(f)
+        # a recording of another launch structure does not describe this plan
+        if rec.get("launches_per_matvec") == len(prof) and rec.get("kernel") == kernel_name:
+            traffic = rec["traffic_bytes_per_matvec"]
+    b64, b32 = r64["batch"], r32["batch"]
+    fp64_plain = r64["ms1"]
+    out = {
+        "metric": METRIC,
+        "value": round(alg1 / r32["ms1"] / 1e6, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms1"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-stored U/B, f64 D and accumulation",
+        "data": f"synthetic (seeded {wname} geometry and densities)",
+        "config": {"workload": f"{wname} cc-pVDZ J matvec, reltol {reltol:g}, 1 density per step; "
+                               f"{args.batch} densities per graph replay beside it",
+                   "num_bf": h.nbf, "num_shell_pairs": h.s.num_sp, "krnl_mat_size": h.n, "H2_MB": sizes,
+                   "alg_MB_fp32": round(alg1 / 1e6, 2), "alg_MB_fp64": round(r64["alg1"] / 1e6, 2),
+                   "cache_resident": alg1 < MALL_BYTES, "timing": "hipGraph replays, HIP events",
+                   "autotune": config_record(tune)},
+        "fp64": {"ms": round(fp64_plain, 5), "GBs": round(r64["alg1"] / fp64_plain / 1e6, 2)},
+        "fp32_speedup_vs_fp64": round(fp64_plain / r32["ms1"], 4),
+        "batched": {f"fp{bits}": {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
+                                  for k, v in res[bits]["batch"].items() if k != "alg"} for bits in (64, 32)},
+        "fp32_batched": {"densities": b32["densities"], "ms_per_density": round(b32["ms_per_density"], 5),
+                         "speedup_vs_fp64_plain": round(fp64_plain / b32["ms_per_density"], 3),
+                         "GBs_algorithmic": round(b32["GBs"], 2),
+                         "roofline_frac": round(b32["GBs"] / HBM_PEAK_GBS, 4)},
+        "rel_err_J_fp32_vs_fp64": eps_prec,
+        "precision_gate_met": eps_prec <= EPS_PREC_GATE,
+        "rel_err_J_vs_oracle": eps_oracle,
+        "rel_err_vs_direct_eri": {"fp64": eps_direct["fp64"], "fp32": eps_direct["fp32"],
+                                  "sample": f"rows of {npair_s} seeded random bra shell pairs ({nrow_s} rows of y)"},
+        "build_s": round(t_build, 2),
+        "exchange_fp64": exch,
+        "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": kernel_name, "launches_per_matvec": len(prof),
+                     "method": "algorithmic bytes of one FP32-stored matvec / union of its launch intervals "
+                               "(HIP events around every launch on its own stream, both streams)",
+                     "alg_bytes_per_matvec": round(alg1, 1),
+                     "kernel_ms_union": round(r32["union_ms"], 6),
+                     "kernel_ms_sum": round(r32["sum_ms"], 6),
+                     "avg_launch_ms": round(r32["sum_ms"] / len(prof), 6),
+                     "streamed_matrix_bytes_per_matvec": sum(p["mat_bytes"] for p in prof)},
+        "phases_fp32": phases,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def exchange_measure(h, torch, D, D_dev, stream):
+    """K build (H2ERI_build_exchange_dev) on the representation, FP64: ms per
+    build (plan uploaded once), the MFMA rate of the whole build, and K
+    against the oracle."""
+    import numpy as np
+    t0 = time.perf_counter()
     kst = h.exchange_stats()   # plans and uploads the exchange data, runs the basis products
-    k_upload_s = time.perf_counter() - t0k
+    k_upload_s = time.perf_counter() - t0
     K_dev = torch.empty_like(D_dev)
     h.build_exchange_dev(D_dev.data_ptr(), K_dev.data_ptr(), stream.cuda_stream)
     torch.cuda.synchronize()
@@ -379,90 +470,134 @@ def main():
     k_ms = e0.elapsed_time(e1) / 3
     Kh = K_dev.cpu().numpy()
     exch = {"ms": round(k_ms, 3), "upload_s": round(k_upload_s, 2), "resident_GB": round(kst["resident_bytes"] / 1e9, 2),
-            "basis_gemm_gflop": round(kst["gemm_flops"] / 1e9, 2),
+            "basis_gemm_gflop": round(kst["gemm_flops"] / 1e9, 2), "terms": int(kst["terms"]),
             "rel_asym": float(np.linalg.norm(Kh - Kh.T) / np.linalg.norm(Kh))}
     log(f"[bench] exchange K: {k_ms:.2f} ms per build (plan + basis products {k_upload_s:.2f} s once)")
-    h.lib.H2ERI_exchange_workbuf_free(h.h)
+    h.exchange_workbuf_free()
+    return exch
 
-    # dominant kernel: stage_kernel (every launch of the staged matvec is one)
-    # or queue_kernel (the persistent pass), FP32-stored single density
-    prof = r32["prof"]
-    kern_ms = sum(p["ms"] for p in prof)
-    alg1 = r32["batch"][1]["alg"]
-    dom_gbs = alg1 / kern_ms / 1e6
+
+# ---------------------------------------------------------------------------
+# molecule group of the batching workload
+# ---------------------------------------------------------------------------
+def group_workload(args):
+    """The batching workload's uneven molecules: 8 alkanes C20..C160 (reltol
+    1e-6, FP32-stored U/B) as one molecule group, one persistent launch over
+    all their tasks per step, against the same matvecs one after another."""
+    import numpy as np
+    import torch
+
+    import h2eri
+    from h2eri import molecules as M
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    hs, X, Y, ns, xs = [], [], [], [], []
+    t0 = time.perf_counter()
+    for n in range(20, 161, 20):
+        r = M.rng_for(300 + n)
+        at = M.alkane(n, r)
+        D = M.density(at, 4 * n + 1, r)
+        hh = build_molecule(h2eri, M, at, 1e-6, f"C{n}H{2 * n + 2}")
+        hh.set_precision(32)
+        hh.upload()
+        x = uncontract(hh.tree(), D, hh.n)
+        hs.append(hh)
+        xs.append(x)
+        X.append(torch.from_numpy(x).to(dev))
+        Y.append(torch.empty_like(X[-1]))
+        ns.append(hh.nbf)
+    t_build = time.perf_counter() - t0
+    g = h2eri.Group(hs, [x.data_ptr() for x in X], [y.data_ptr() for y in Y])
+    for _ in range(args.warmup):
+        g.matvec(stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        g.matvec(stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_group = e0.elapsed_time(e1) / args.steps
+    g.status()   # raises if a dependency wait of the persistent launch exceeded its bound
+    Yg = [y.cpu().numpy() for y in Y]
+    e0.record(stream)
+    for _ in range(args.steps):
+        for hh, x, y in zip(hs, X, Y):
+            hh.matvec_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_seq = e0.elapsed_time(e1) / args.steps
+    dev_group = max(float(np.linalg.norm(a - y.cpu().numpy()) / np.linalg.norm(a)) for a, y in zip(Yg, Y))
+    alg_group = sum(hh.matvec_bytes()["alg"] for hh in hs)
+    log(f"[bench] 8 molecules: group {ms_group:.3f} ms vs one after another {ms_seq:.3f} ms")
+    g.close()
     cpu = None
-    eps_parity = None
-    eps_parity32 = None
     if not args.no_cpu:
+        tot, per = 0.0, []
         with tempfile.TemporaryDirectory() as d:
-            rp = os.path.join(d, "rep.h2")
-            h.save(rp)
-            cpu_ms, nthr, nrep, o = cpu_baseline(rp, x, args.cpu_seconds)
-            J_cpu = o.coulomb(D)
-            o.close()
-        eps_parity = float(np.linalg.norm(r64["J"] - J_cpu) / np.linalg.norm(J_cpu))
-        eps_parity32 = float(np.linalg.norm(r32["J"] - J_cpu) / np.linalg.norm(J_cpu))
-        cpu = {"value": round(r64["batch"][1]["alg"] / cpu_ms / 1e6, 2), "unit": "GB/s", "cores": nthr,
-               "kind": "port",
-               "sample": f"{nrep} oracle FP64 matvecs of the same {wname} representation, {cpu_ms:.2f} ms each"}
-        log(f"[bench] CPU oracle: {cpu_ms:.2f} ms/matvec on {nthr} threads")
-    # HBM traffic per launch of the same command, from the rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE runs recorded next to the kernel-trace summary
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_c30h62_j_matvec.json")
-    if args.config == 1 and os.path.exists(tpath):
-        with open(tpath) as f:
-            rec = json.load(f)
-        # a recording of another launch structure does not describe this plan
-        if rec.get("launches_per_matvec") == len(r32["prof"]) and rec.get("kernel") == kernel_name:
-            traffic = rec["traffic_bytes_per_launch"]
-    best32 = min(r32["batch"].values(), key=lambda b: b["ms_per_density"])
-    best32_nv = [k for k, b in r32["batch"].items() if b is best32][0]
-    fp64_plain = r64["batch"][1]["ms"]
-    out = {
-        "metric": METRIC,
-        "value": round(r32["batch"][1]["GBs"], 2),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(r32["batch"][1]["ms"], 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32-stored U/B, f64 D and accumulation",
-        "data": f"synthetic (seeded {wname} geometry and density)",
-        "config": {"workload": f"{wname} cc-pVDZ J matvec, reltol {reltol:g}, 1 density per step",
-                   "num_bf": h.nbf, "krnl_mat_size": h.n, "H2_MB": sizes,
-                   "alg_MB_fp32": round(r32["batch"][1]["alg"] / 1e6, 2),
-                   "alg_MB_fp64": round(r64["batch"][1]["alg"] / 1e6, 2),
-                   "cache_resident": r32["batch"][1]["alg"] < MALL_BYTES, "timing": "hipGraph replays",
-                   "autotune": config_record(tune)},
-        "fp64": {"ms": round(fp64_plain, 5), "GBs": round(r64["batch"][1]["GBs"], 2)},
-        "fp32_speedup_vs_fp64": round(fp64_plain / r32["batch"][1]["ms"], 4),
-        "batched": {f"fp{bits}": {str(nv): {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
-                                            for k, v in b.items() if k != "alg"}
-                                  for nv, b in res[bits]["batch"].items()} for bits in (64, 32)},
-        "fp32_batched_best": {"densities_per_pass": best32_nv, "ms_per_density": round(best32["ms_per_density"], 5),
-                              "speedup_vs_fp64_plain": round(fp64_plain / best32["ms_per_density"], 3),
-                              "GBs_algorithmic": round(best32["GBs"], 2),
-                              "roofline_frac": round(best32["GBs"] / HBM_PEAK_GBS, 4)},
-        "rel_err_J_fp32_vs_fp64": eps_prec,
-        "rel_err_J_fp64_vs_oracle": eps_parity,
-        "rel_err_J_fp32_vs_oracle": eps_parity32,
-        "build_s": round(t_build, 2),
-        "exchange_fp64": exch,
-        "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": kernel_name, "launches_per_matvec": len(prof),
-                     "alg_bytes_per_launch": round(alg1 / len(prof), 1),
-                     "avg_launch_ms": round(kern_ms / len(prof), 6),
-                     "streamed_matrix_bytes_per_matvec": sum(p["mat_bytes"] for p in prof)},
-        "phases_ms_fp32": {p: round(sum(q["ms"] for q in prof if q["phase"] == p), 5)
-                           for p in ("up", "B", "y1", "down", "D", "gather", "pass")},
-        "cpu_baseline": cpu,
-    }
+            for k, hh in enumerate(hs):
+                rp = os.path.join(d, f"m{k}.h2")
+                hh.save(rp)
+                c = cpu_baseline(rp, xs[k], reps=10, one_core=False)
+                per.append(round(c["all"]["median_ms"], 3))
+                tot += c["all"]["median_ms"]
+        alg64 = 0.0
+        for hh in hs:
+            hh.set_precision(64)
+            alg64 += hh.matvec_bytes()["alg"]
+        cpu = {"value": round(alg64 / tot / 1e6, 2), "unit": "GB/s", "cores": cpu_threads(), "kind": "port",
+               "sample": "oracle FP64 matvec of each of the 8 molecules, 1 warm-up + 10 timed each, medians summed "
+                         f"({tot:.2f} ms); OMP_PROC_BIND=close, OMP_PLACES=cores",
+               "median_ms_per_molecule": per, "host": host_info()}
+    out = {"metric": METRIC, "value": round(alg_group / ms_group / 1e6, 2), "unit": "GB/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_group, 5), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32-stored U/B, f64 D and accumulation",
+           "data": "synthetic (seeded alkane geometries and densities)",
+           "config": {"workload": "8 alkanes C20H42..C160H322 (reltol 1e-6) as one molecule group, one matvec each "
+                                  "per step", "num_bf": ns, "timing": "back-to-back group launches, HIP events"},
+           "ms_sequential_matvecs": round(ms_seq, 4), "rel_dev_group_vs_single": dev_group,
+           "build_s": round(t_build, 1),
+           "roofline": {"bound": "hbm", "achieved": round(alg_group / ms_group / 1e6, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(alg_group / ms_group / 1e6 / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "queue_kernel<1, group>", "launches_per_matvec": 1,
+                        "method": "algorithmic bytes of the 8 matvecs / time per group launch (with its counter reset)"},
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4])
+    ap.add_argument("--batch", type=int, default=16, help="densities per graph replay of the batched form")
+    ap.add_argument("--direct-pairs", type=int, default=256, help="shell pairs whose rows are evaluated directly")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-one-core", action="store_true")
+    ap.add_argument("--exchange", action="store_true", help="time the exchange build too (default with --config 4)")
+    ap.add_argument("--tuned", default="", help="EXECUTOR,TILE_BYTES,SCHEDULE[,WAVE_TILE_BYTES] instead of running the "
+                    "autotuner (profiling runs, where the profiler distorts the tuner's timings)")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="run a plain device copy of this many MB first (HBM counter calibration)")
+    ap.add_argument("--cpu-child", nargs=4, metavar=("REP", "X", "THREADS", "REPS"), help=argparse.SUPPRESS)
+    args = ap.parse_args()
+    if args.cpu_child:
+        rep, xp, thr, reps = args.cpu_child
+        cpu_child(rep, xp, int(thr), int(reps))
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on exactly one GPU (got --gpus {args.gpus}); the single-GPU "
+              "matvec is the unit of work and no multi-GPU path is provided.", file=sys.stderr)
+        sys.exit(2)
+    if args.config == 4:
+        args.exchange = True
+    if args.config == 3:
+        group_workload(args)
+    else:
+        single_workload(args)
 
 
 if __name__ == "__main__":

@@ -710,4 +710,60 @@ void H2ERI_build_H2_synthetic(H2ERI_p h, const unsigned long long seed, const in
     h->timers[U_BUILD_TIMER_IDX] = wtime() - t0;
 }
 
+// Exact rows of the ERI matrix times x for the listed shell pairs (tree
+// order): for pair i, y[rows of i] = sum over every kept shell pair j of
+// (i|j) x_j, every integral evaluated directly, nothing compressed.  The
+// rows of the listed pairs are written one pair after another into y.  A
+// measurement aid for the compression error of the H2 representation
+// (src/H2ERI_build_H2.c builds the same integrals block by block).
+void H2ERI_direct_matvec_rows(H2ERI_p h, const int npair, const int *pair_idx, const double *x, double *y) {
+    if (!h || npair < 0 || (npair > 0 && (!pair_idx || !x || !y))) { set_error(ERR_ARG, "H2ERI_direct_matvec_rows: bad argument"); return; }
+    HostState *hs = host_state(h);
+    const int np = h->num_sp;
+    if ((int)hs->pairs.size() != np || np == 0 || hs->synthetic) {
+        set_error(ERR_ARG, "H2ERI_direct_matvec_rows: needs a molecule processed in this process (shell-pair data)");
+        return;
+    }
+    std::vector<size_t> yoff(npair + 1, 0);
+    for (int k = 0; k < npair; k++) {
+        if (pair_idx[k] < 0 || pair_idx[k] >= np) { set_error(ERR_ARG, "H2ERI_direct_matvec_rows: pair %d out of range", pair_idx[k]); return; }
+        yoff[k + 1] = yoff[k] + h->sp_nbfp[pair_idx[k]];
+    }
+    const int mx = h->max_shell_nbf * h->max_shell_nbf;
+    // one (bra pair, ket range) item per task; partial row sums per task
+    const int KCH = 512;
+    const int nch = (np + KCH - 1) / KCH;
+    std::vector<double> part((size_t)npair * nch * mx, 0.0);
+    #pragma omp parallel
+    {
+        EriWork *w = eri_work_new(hs->max_am);
+        std::vector<double> blk((size_t)mx * mx);
+        #pragma omp for schedule(dynamic, 1)
+        for (long long t = 0; t < (long long)npair * nch; t++) {
+            const int k = (int)(t / nch), c = (int)(t % nch);
+            const int i = pair_idx[k], ni = h->sp_nbfp[i];
+            double *acc = part.data() + (size_t)t * mx;
+            for (int j = c * KCH; j < std::min(np, (c + 1) * KCH); j++) {
+                const int nj = h->sp_nbfp[j];
+                eri_pair_block(hs->pairs[i], hs->pairs[j], w, blk.data(), nj);
+                const double *xj = x + h->sp_bfp_sidx[j];
+                for (int r = 0; r < ni; r++) {
+                    double s = 0.0;
+                    for (int q = 0; q < nj; q++) s += blk[(size_t)r * nj + q] * xj[q];
+                    acc[r] += s;
+                }
+            }
+        }
+        eri_work_free(w);
+    }
+    for (int k = 0; k < npair; k++) {
+        const int ni = h->sp_nbfp[pair_idx[k]];
+        for (int r = 0; r < ni; r++) {
+            double s = 0.0;
+            for (int c = 0; c < nch; c++) s += part[((size_t)k * nch + c) * mx + r];
+            y[yoff[k] + r] = s;
+        }
+    }
+}
+
 }  // extern "C"

@@ -959,6 +959,7 @@ struct LaunchRec {
     int phase, blocks;
     double mat_bytes;
     float ms;
+    float start;   // ms after the pass's base event (recorded before any of its launches)
 };
 
 struct DevPlan {
@@ -1006,6 +1007,7 @@ struct DevState {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool phase_timing = false;
     std::vector<hipEvent_t> evs;
+    hipEvent_t ev_base = nullptr;   // timing base of the last timed pass
     std::vector<LaunchRec> launches;
     bool collected = true;
     double phase_ms[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1079,6 +1081,7 @@ void device_release(H2ERI_p h) {
     for (hipEvent_t e : d->evs) (void)hipEventDestroy(e);
     if (d->ev_fork) (void)hipEventDestroy(d->ev_fork);
     if (d->ev_join) (void)hipEventDestroy(d->ev_join);
+    if (d->ev_base) (void)hipEventDestroy(d->ev_base);
     if (d->side) (void)hipStreamDestroy(d->side);
     delete d;
     h->dev = nullptr;
@@ -2003,6 +2006,8 @@ static bool run_queue(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, doub
         if (!ensure_events(d, 2)) return false;
         d->launches.clear();
         d->collected = false;
+        if (!d->ev_base) HIP_OK(hipEventCreate(&d->ev_base));
+        HIP_OK(hipEventRecord(d->ev_base, s));
         HIP_OK(hipEventRecord(d->evs[0], s));
     }
     switch (dp.P.nv) {
@@ -2016,7 +2021,7 @@ static bool run_queue(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, doub
         HIP_OK(hipEventRecord(d->evs[1], s));
         double mb = 0;
         for (const Stage &st : dp.P.stages) mb += st.mat_bytes;
-        d->launches.push_back(LaunchRec{PH_N, dp.q_grid, mb, 0.f});
+        d->launches.push_back(LaunchRec{PH_N, dp.q_grid, mb, 0.f, 0.f});
     }
     return true;
 }
@@ -2035,6 +2040,8 @@ static bool run_matvec(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, dou
         if (!ensure_events(d, 2 * P.stages.size() + 2)) return false;
         d->launches.clear();
         d->collected = false;
+        if (!d->ev_base) HIP_OK(hipEventCreate(&d->ev_base));
+        HIP_OK(hipEventRecord(d->ev_base, s));   // before the fork: every launch starts after it
     }
     auto launch = [&](const Stage &st, hipStream_t q) -> bool {
         if (tim) HIP_OK(hipEventRecord(d->evs[2 * d->launches.size()], q));
@@ -2047,7 +2054,7 @@ static bool run_matvec(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, dou
         HIP_OK(hipGetLastError());
         if (tim) {
             HIP_OK(hipEventRecord(d->evs[2 * d->launches.size() + 1], q));
-            d->launches.push_back(LaunchRec{st.phase, stage_blocks(st), st.mat_bytes, 0.f});
+            d->launches.push_back(LaunchRec{st.phase, stage_blocks(st), st.mat_bytes, 0.f, 0.f});
         }
         return true;
     };
@@ -2082,6 +2089,9 @@ static void collect_phase_ms(H2ERI_p h, DevState *d) {
             hipEventElapsedTime(&ms, d->evs[2 * i], d->evs[2 * i + 1]) != hipSuccess)
             ms = 0.f;
         d->launches[i].ms = ms;
+        float t0 = 0.f;
+        if (!d->ev_base || hipEventElapsedTime(&t0, d->ev_base, d->evs[2 * i]) != hipSuccess) t0 = 0.f;
+        d->launches[i].start = t0;
         if (d->launches[i].phase < PH_N) d->phase_ms[d->launches[i].phase] += ms;
         else d->phase_ms[PH_D] += ms;   // one launch for the whole pass (persistent executor)
         d->phase_ms[6] += ms;
@@ -2373,7 +2383,7 @@ int H2ERI_get_launch_stats(H2ERI_p h, const int max_launch, double *out5) {
     for (int i = 0; i < n && i < max_launch && out5; i++) {
         const LaunchRec &r = d->launches[i];
         out5[5 * i + 0] = r.phase;
-        out5[5 * i + 1] = 0;
+        out5[5 * i + 1] = r.start;
         out5[5 * i + 2] = r.blocks;
         out5[5 * i + 3] = r.mat_bytes;
         out5[5 * i + 4] = r.ms;

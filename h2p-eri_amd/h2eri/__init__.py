@@ -231,6 +231,21 @@ class H2ERI:
         _check(self.lib)
         return r
 
+    def direct_matvec_rows(self, pair_idx, x):
+        """Exact rows (every integral evaluated directly) of the ERI matrix
+        times x for the listed shell pairs; returns (rows, y_rows) with rows
+        the indices into y of the values."""
+        pair_idx = np.ascontiguousarray(pair_idx, dtype=np.int32)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.shape == (self.n,)
+        sidx = self.tree()["sp_bfp_sidx"]
+        rows = np.concatenate([np.arange(sidx[i], sidx[i + 1]) for i in pair_idx]) if len(pair_idx) else np.zeros(0, int)
+        y = np.empty(len(rows))
+        self.lib.H2ERI_direct_matvec_rows(self.h, len(pair_idx), pair_idx.ctypes.data_as(C.POINTER(C.c_int)),
+                                          _dptr(x), _dptr(y))
+        _check(self.lib)
+        return rows, y
+
     def set_spin_limit(self, polls):
         """Bound of the persistent executor's dependency waits, in polls."""
         self.lib.H2ERI_set_spin_limit(self.h, int(polls))
@@ -375,8 +390,8 @@ class H2ERI:
         out = np.zeros(5 * max(n, 1))
         n = self.lib.H2ERI_get_launch_stats(self.h, n, _dptr(out))
         names = ["up", "B", "y1", "down", "D", "gather", "pass"]
-        return [dict(phase=names[int(out[5 * i])], blocks=int(out[5 * i + 2]), mat_bytes=out[5 * i + 3],
-                     ms=out[5 * i + 4]) for i in range(n)]
+        return [dict(phase=names[int(out[5 * i])], start=out[5 * i + 1], blocks=int(out[5 * i + 2]),
+                     mat_bytes=out[5 * i + 3], ms=out[5 * i + 4]) for i in range(n)]
 
     def matvec_bytes(self):
         out = np.zeros(6)

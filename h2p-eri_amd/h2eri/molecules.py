@@ -166,6 +166,14 @@ CONFIGS = {
 }
 
 
+def batch_densities(atoms, nocc, count=16):
+    """The batching workload's densities on a molecule: `count` damped
+    densities of nocc occupied columns, seeded with the batching
+    configuration's generator (20260003)."""
+    rng = rng_for(3)
+    return np.stack([density(atoms, nocc, rng) for _ in range(count)])
+
+
 def workload_name(config):
     """Short name of a configuration's molecule (C30H62, (H2O)64, ...)."""
     cfg = CONFIGS[config]

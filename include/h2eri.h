@@ -313,6 +313,15 @@ void H2ERI_build_H2_synthetic(H2ERI_p h2eri, const unsigned long long seed,
 void H2ERI_partition_points_synthetic(H2ERI_p h2eri, const int npts, const double *coord,
                                       const int *nbfp, const int max_leaf_points);
 
+/* Additive measurement aid (host, OpenMP): exact rows of the ERI matrix
+ * times x for npair shell pairs (indices in tree order, x in tree order):
+ * the rows of pair_idx[k] (sp_nbfp of it) are written one pair after another
+ * into y, each the sum over every kept shell pair j of (i|j) x_j with every
+ * integral evaluated directly.  Compared with the same rows of an H2 matvec
+ * it gives the compression error.  Needs a molecule processed in this
+ * process (H2ERI_process_shells + H2ERI_partition). */
+void H2ERI_direct_matvec_rows(H2ERI_p h2eri, const int npair, const int *pair_idx, const double *x, double *y);
+
 /* Per-phase device timing of the last hot-path call, milliseconds, by the
  * phase each launch belongs to: [0] upward U^T sweep, [1] B coupling (first
  * two launches), [2] B coupling (last launch), [3] downward U sweep, [4]
@@ -321,9 +330,12 @@ void H2ERI_partition_points_synthetic(H2ERI_p h2eri, const int npts, const doubl
  * H2ERI_set_phase_timing(h2eri, 1). */
 void H2ERI_set_phase_timing(H2ERI_p h2eri, const int on);
 void H2ERI_get_phase_ms(H2ERI_p h2eri, double *ms7);
-/* Per-launch records of the last timed call: for launch i, out5[5i..5i+4] =
- * phase (numbered as above), 0 (reserved), workgroups, matrix bytes
- * streamed, ms.  Returns the number of launches (fills at most max_launch). */
+/* Per-launch records of the last timed pass: for launch i, out5[5i..5i+4] =
+ * phase (numbered as above), start in ms after an event recorded before the
+ * pass's first launch (launches on the two streams overlap; the union of
+ * the intervals [start, start + ms] is the pass's kernel time), workgroups,
+ * matrix bytes streamed, ms.  Returns the number of launches (fills at most
+ * max_launch). */
 int H2ERI_get_launch_stats(H2ERI_p h2eri, const int max_launch, double *out5);
 
 /* Algorithmic bytes moved by one single-vector matvec on the device with

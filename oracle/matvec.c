@@ -28,15 +28,22 @@ static int g_threads = 0;
 void orc_set_threads(int n) { g_threads = n; }
 static int nthreads(void) { return g_threads > 0 ? g_threads : omp_get_max_threads(); }
 
-/* o0 += M i0 ; o1 += M^T i1  (M row-major nrow x ncol, one pass) */
+/* o0 += M i0 ; o1 += M^T i1  (M row-major nrow x ncol, one pass).  The row
+ * dots are SIMD reductions (several partial sums), as a BLAS dgemv's are. */
 static void bi_gemv(int nrow, int ncol, const double *M, const double *i0, const double *i1, double *o0, double *o1) {
     for (int r = 0; r < nrow; r++) {
         const double *row = M + (size_t)r * ncol;
         double s = 0.0;
-        const double a = i1 ? i1[r] : 0.0;
-        for (int c = 0; c < ncol; c++) {
-            s += row[c] * i0[c];
-            if (o1) o1[c] += a * row[c];
+        if (o1) {
+            const double a = i1[r];
+            #pragma omp simd reduction(+:s)
+            for (int c = 0; c < ncol; c++) {
+                s += row[c] * i0[c];
+                o1[c] += a * row[c];
+            }
+        } else {
+            #pragma omp simd reduction(+:s)
+            for (int c = 0; c < ncol; c++) s += row[c] * i0[c];
         }
         o0[r] += s;
     }
@@ -57,6 +64,7 @@ static void gemv_n(int nrow, int ncol, const double *M, const double *x, double 
     for (int r = 0; r < nrow; r++) {
         const double *row = M + (size_t)r * ncol;
         double s = 0.0;
+        #pragma omp simd reduction(+:s)
         for (int c = 0; c < ncol; c++) s += row[c] * x[c];
         y[r] = beta1 ? y[r] + s : s;
     }

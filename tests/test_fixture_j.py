@@ -45,3 +45,40 @@ def test_alk32_statistics(alk32):
     t = h.tree()
     assert h.nbf == 260 and t["n_leaf_node"] > 8 and t["max_level"] >= 3
     assert t["min_adm_level"] >= 1
+
+
+def _uncontract(tr, D, n):
+    """x of src/H2ERI_build_Coulomb.c:26-62, in tree order."""
+    sp, sb, bf = tr["sp_shell_idx"], tr["sp_bfp_sidx"], tr["shell_bf_sidx"]
+    npair = tr["num_sp"]
+    x = np.empty(n)
+    for i in range(npair):
+        s0, s1 = sp[i], sp[i + npair]
+        r0, nr = bf[s0], bf[s0 + 1] - bf[s0]
+        c0, nc = bf[s1], bf[s1 + 1] - bf[s1]
+        x[sb[i]:sb[i + 1]] = (D[c0:c0 + nc, r0:r0 + nr] * (1.0 if s0 == s1 else 2.0)).reshape(-1)
+    return x
+
+
+def test_direct_rows_match_fixture(alk32):
+    """Rows of the ERI matrix times x with every integral evaluated directly
+    (H2ERI_direct_matvec_rows, the reference point of the compression error)
+    reproduce the fixture's J entries of 40 sampled shell pairs M != N, whose
+    J entries are exactly those rows (src/H2ERI_build_Coulomb.c:90-132), an
+    order of magnitude closer than the H2 matvec does."""
+    import oracle_bind
+    h, path, D, Jref = alk32
+    tr = h.tree()
+    x = _uncontract(tr, D, h.n)
+    sp, bf = tr["sp_shell_idx"], tr["shell_bf_sidx"]
+    npair = tr["num_sp"]
+    cand = [i for i in range(npair) if sp[i] != sp[i + npair]]
+    pairs = np.random.default_rng(0).choice(cand, 40, replace=False)
+    rows, yd = h.direct_matvec_rows(pairs, x)
+    jr = np.concatenate([Jref[bf[sp[i + npair]]:bf[sp[i + npair] + 1], bf[sp[i]]:bf[sp[i] + 1]].reshape(-1)
+                         for i in pairs])
+    yh = oracle_bind.Oracle(path).matvec(x)[rows]
+    e_direct, e_h2 = relerr(yd, jr), relerr(yh, jr)
+    assert e_direct < 3e-8, e_direct       # measured 1.1e-8
+    assert e_h2 > 3 * e_direct, (e_h2, e_direct)
+    assert abs(relerr(yh, yd) - e_h2) < 0.3 * e_h2   # the H2 error measured against either reference

@@ -53,6 +53,7 @@ def _workload(tmp_path_factory, config):
     from h2eri import molecules as M
     atoms, _, D, reltol = M.workload(config)
     h, p = _build(tmp_path_factory, M.workload_name(config).replace("(", "").replace(")", ""), atoms, reltol)
+    h.atoms = atoms
     return h, p, D
 
 
@@ -112,9 +113,7 @@ def test_c100h202_sixteen_densities(c100):
     in one FP32-stored batched call: two passes of 8 over the matrix data."""
     from h2eri import molecules as M
     h, p, _ = c100
-    rng = M.rng_for(3)
-    atoms = M.alkane(100, rng)
-    Ds = np.stack([M.density(atoms, 401, rng) for _ in range(16)])
+    Ds = M.batch_densities(h.atoms, 401, 16)
     h.set_precision(32)
     try:
         Js = h.build_coulomb_multi(Ds)
