@@ -13,8 +13,9 @@ with x and y resident in HBM.  Timed in the same run:
   near field and accumulation); the executor, tile size and near-field
   schedule are the project's autotuner's choice for the FP32-stored form;
 * the batching workload on the same representation: 16 seeded densities in
-  one graph replay (two passes of 8 over the matrix data), FP64 and
-  FP32-stored, with the autotuner's choice for 8 densities per pass.
+  one graph replay (one pass over the matrix data, the tiles as FP64 MFMA
+  products), FP64 and FP32-stored, with the autotuner's choice for 16
+  densities per pass.
 
 `value` is the FP32-stored single-density form's algorithmic HBM throughput
 (bytes of the representation and vectors one matvec must move, divided by
@@ -57,6 +58,17 @@ def log(*a):
 # ---------------------------------------------------------------------------
 # timing helpers
 # ---------------------------------------------------------------------------
+def pass_widths(n):
+    """Vectors per pass of a batched call of n vectors (16, 8, 4, 2, 1 as the
+    library splits it)."""
+    out = []
+    while n > 0:
+        w = 16 if n >= 16 else 8 if n >= 8 else 4 if n >= 4 else 2 if n >= 2 else 1
+        out.append(w)
+        n -= w
+    return out
+
+
 def time_graph(h, torch, stream, steps, warmup):
     """ms per replay of the captured graph, HIP events on the replay stream."""
     for _ in range(warmup):
@@ -311,15 +323,16 @@ def single_workload(args):
         h.build_coulomb_dev(D_dev.data_ptr(), J_dev.data_ptr(), stream.cuda_stream)
         torch.cuda.synchronize()
         r["J"] = J_dev.cpu().numpy()
-        # the batch: the autotuner's choice for 8 densities per pass
+        # the batch: the autotuner's choice for the width of its first pass
+        widths = pass_widths(args.batch)
         if args.tuned:
-            tune8 = tune
+            tuneb = tune
         else:
-            tune8 = h.autotune(nv=8, reps=5)
-            log(f"[bench] autotune (FP{bits}-stored, 8 densities per pass): {tune8}")
+            tuneb = h.autotune(nv=widths[0], reps=5)
+            log(f"[bench] autotune (FP{bits}-stored, {widths[0]} densities per pass): {tuneb}")
         h.graph_capture("matvec", args.batch, X_dev.data_ptr(), Y_dev.data_ptr())
         msb = time_graph(h, torch, stream, max(5, args.steps // 2), max(1, args.warmup // 2))
-        passes = (args.batch + 7) // 8
+        passes = len(widths)
         algb = passes * mat + args.batch * nb["vec"]
         Yb = Y_dev.cpu().numpy()
         dev_single = 0.0
@@ -328,8 +341,8 @@ def single_workload(args):
             torch.cuda.synchronize()
             y1 = y_dev.cpu().numpy()
             dev_single = max(dev_single, float(np.linalg.norm(Yb[k] - y1) / np.linalg.norm(y1)))
-        r["batch"] = {"densities": args.batch, "passes": passes, "ms": msb, "ms_per_density": msb / args.batch,
-                      "alg": algb, "GBs": algb / msb / 1e6, "config_8_per_pass": config_record(tune8),
+        r["batch"] = {"densities": args.batch, "passes": widths, "ms": msb, "ms_per_density": msb / args.batch,
+                      "alg": algb, "GBs": algb / msb / 1e6, "config_first_pass": config_record(tuneb),
                       "max_rel_dev_vs_single": dev_single}
         log(f"[bench] FP{bits}-stored, {args.batch} densities: {msb:.4f} ms per replay, {msb / args.batch:.4f} ms/density, "
             f"{algb / msb / 1e6:.0f} GB/s algorithmic")

@@ -30,7 +30,8 @@
 //     of the sweeps' critical path;
 //   * low-rank blocks take 1.5 passes: the smaller half twice, the other half
 //     once with a fused bi-GEMV;
-//   * NV vectors (1, 2, 4, 8) are processed per pass over the matrix data;
+//   * NV vectors (1, 2, 4, 8, 16) are processed per pass over the matrix data
+//     (at 16 the tiles are FP64 MFMA products);
 //   * U and B may be stored in FP32 (accumulation stays FP64), D stays FP64.
 #include <hip/hip_runtime.h>
 
@@ -128,7 +129,7 @@ template <> struct Cfg<float, 4> { typedef float4 V; static const int W = 4, KP 
 template <> struct Cfg<float, 8> { typedef float2 V; static const int W = 2, KP = 1, RU = 2; };
 
 // columns per lane for a batch width (same for both storage types)
-static inline int cpl_of(int nv) { return nv <= 2 ? 8 : (nv == 4 ? 4 : 2); }
+static inline int cpl_of(int nv) { return nv <= 2 ? 8 : (nv == 4 ? 4 : (nv == 8 ? 2 : 1)); }
 
 __device__ __forceinline__ double vget(const double2 &v, int i) { return i == 0 ? v.x : v.y; }
 __device__ __forceinline__ double vget(const float2 &v, int i) { return (double)(i == 0 ? v.x : v.y); }
@@ -389,8 +390,8 @@ __device__ void coldot_tile(const BTask &t, const T *__restrict__ S, double *__r
 template <typename T> struct Dot8 { static const int RC = 4, XS = 10, RW = 4; };
 
 template <typename T, int NV, bool COH, int NT = WG>
-__device__ void bgemv_tile(const BTask &task, const T *__restrict__ S, const int *__restrict__ vsrc,
-                           double *__restrict__ Wv, const double *__restrict__ X, long long ldx, double *lds) {
+__device__ void bgemv_tile_valu(const BTask &task, const T *__restrict__ S, const int *__restrict__ vsrc,
+                                double *__restrict__ Wv, const double *__restrict__ X, long long ldx, double *lds) {
     // At NV = 8 the descriptor stays as loaded: the register allocation of
     // the two tile paths together fits 128 VGPRs only that way (measured on
     // the compiler's output), and the path choice is made uniform below.
@@ -588,6 +589,133 @@ __device__ void bgemv_tile(const BTask &task, const T *__restrict__ S, const int
             const double s = (red[q] + red[STRIDE + q]) + (red[2 * STRIDE + q] + red[3 * STRIDE + q]);
             wst<COH>(Wv + (long long)t.out1 * NV + q, s);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 16 vectors per pass: the tile as FP64 MFMA products
+// ---------------------------------------------------------------------------
+// At 16 vectors a tile is a small matrix product, out0 (R x 16) = M X and
+// out1 (C x 16) = M^T Y, every matrix element used 16 times per side.  Both
+// run on v_mfma_f64_16x16x4f64 (lane l holds A[l%16][l/16] and B[l/16][l%16];
+// output register r of lane l is C[l/16 + 4r][l%16]) with FP64 accumulation;
+// FP32-stored elements are widened to FP64 in registers.  C <= 64.
+//  * Row products: a wave takes 16 rows.  For each group of 16 columns, lane
+//    l loads columns 4(l/16) .. 4(l/16)+3 of row l%16 (one 16-byte load for
+//    FP32, two for FP64): the A operands of four MFMAs j whose inner index
+//    runs over the columns 4k + j of the group (a permutation of the inner
+//    index, which leaves the sum alone).  B = the rows of X of the same
+//    columns, from LDS.
+//  * Column products, out1^T = Y^T M: lane l loads columns 4(l%16) ..
+//    4(l%16)+3 of row l/16 of a group of 4 rows: the B operands of four MFMAs
+//    j, MFMA j covering the 16 columns 4n + j of the tile.  A = those rows
+//    of Y, from LDS.  The waves take blocks of 16 rows in turn, and their
+//    column sums meet in LDS in a fixed order.
+// A tile with both outputs reads its rows in both lane layouts; the second
+// read of a block follows the first at once and comes from the caches.
+// Columns and rows past the tile are clamped to its last load position and
+// meet zero inputs (LDS staging padded with zeros); their outputs are not
+// stored.
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+// Four consecutive elements from c0 (clamped so no load leaves the row's
+// padded stride): FP32 one 16-byte load at min(c0, cmax), FP64 two at
+// min(c0, cmax) and min(c0 + 2, cmax).
+__device__ __forceinline__ void load_quad(const float *row, int c0, int cmax, double (&o)[4]) {
+    const float4 v = *reinterpret_cast<const float4 *>(row + min(c0, cmax));
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ __forceinline__ void load_quad(const double *row, int c0, int cmax, double (&o)[4]) {
+    const double2 a = *reinterpret_cast<const double2 *>(row + min(c0, cmax));
+    const double2 b = *reinterpret_cast<const double2 *>(row + min(c0 + 2, cmax));
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+
+template <typename T, bool COH>
+__device__ void mfma_tile16(const BTask &task, const T *__restrict__ S, const int *__restrict__ vsrc,
+                            double *__restrict__ Wv, const double *__restrict__ X, long long ldx, double *lds) {
+    constexpr int NV = 16;
+    constexpr int VW = sizeof(T) == 4 ? 4 : 2;   // elements per vector load
+    const BTask t = uniform(task);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int R = t.nrow, C = t.ncol;
+    const bool do0 = t.out0 >= 0, do1 = t.out1 >= 0;
+    double *xin = lds;                 // C x 16, padded to whole groups of 16 columns
+    double *yin = lds + XIN_DOUBLES;   // R x 16, padded to whole blocks of 16 rows
+    const int Cp = (C + 15) & ~15, Rp = (R + 15) & ~15;
+    if (do0) {
+        stage_vec<NV, COH, WG>(vsrc, t.v0b, t.v0e, t.v0off, C, Wv, X, ldx, xin, tid);
+        for (int q = C * NV + tid; q < Cp * NV; q += WG) xin[q] = 0.0;
+    }
+    if (do1) {
+        stage_vec<NV, COH, WG>(vsrc, t.v1b, t.v1e, t.v1off, R, Wv, X, ldx, yin, tid);
+        for (int q = R * NV + tid; q < Rp * NV; q += WG) yin[q] = 0.0;
+    }
+    __syncthreads();
+    const T *base = S + t.mat;
+    const int cmax = ((C - 1) / VW) * VW;
+    const int ng = Cp >> 4;
+    d4v acc1[4];
+    #pragma unroll
+    for (int j = 0; j < 4; j++) acc1[j] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int rb = 16 * w; rb < R; rb += 64) {
+        if (do0) {
+            const T *row = base + (long long)min(rb + l16, R - 1) * t.ld;
+            double a[4][4];
+            #pragma unroll
+            for (int g = 0; g < 4; g++)
+                if (g < ng) load_quad(row, 16 * g + 4 * lq, cmax, a[g]);
+            d4v acc0 = d4v{0.0, 0.0, 0.0, 0.0};
+            #pragma unroll
+            for (int g = 0; g < 4; g++) {
+                if (g >= ng) break;
+                #pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[g][j], xin[(16 * g + 4 * lq + j) * NV + l16], acc0, 0, 0, 0);
+            }
+            #pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rr = rb + lq + 4 * r;
+                if (rr < R) wst<COH>(Wv + (long long)(t.out0 + rr) * NV + l16, acc0[r]);
+            }
+        }
+        if (do1) {
+            #pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int rr = rb + 4 * q + lq;
+                double b[4];
+                load_quad(base + (long long)min(rr, R - 1) * t.ld, 4 * l16, cmax, b);
+                const double y = yin[rr * NV + l16];   // rr < Rp: zero past the tile
+                #pragma unroll
+                for (int j = 0; j < 4; j++) acc1[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(y, b[j], acc1[j], 0, 0, 0);
+            }
+        }
+    }
+    if (do1) {
+        __syncthreads();   // the staged vectors are dead from here on
+        double *red = lds;   // 4 waves x 64 columns x 16 vectors
+        #pragma unroll
+        for (int j = 0; j < 4; j++)
+            #pragma unroll
+            for (int r = 0; r < 4; r++) red[w * 1024 + (4 * l16 + j) * NV + lq + 4 * r] = acc1[j][r];
+        __syncthreads();
+        for (int q = tid; q < C * NV; q += WG) {
+            const double v = (red[q] + red[1024 + q]) + (red[2048 + q] + red[3072 + q]);
+            wst<COH>(Wv + (long long)t.out1 * NV + q, v);
+        }
+    }
+}
+
+// A tile task: the MFMA products at 16 vectors per pass, the VALU tile below.
+template <typename T, int NV, bool COH, int NT = WG>
+__device__ __forceinline__ void bgemv_tile(const BTask &task, const T *__restrict__ S, const int *__restrict__ vsrc,
+                                           double *__restrict__ Wv, const double *__restrict__ X, long long ldx,
+                                           double *lds) {
+    if constexpr (NV == 16) {
+        if constexpr (NT == WG) mfma_tile16<T, COH>(task, S, vsrc, Wv, X, ldx, lds);   // no wave tiles at 16
+    } else {
+        bgemv_tile_valu<T, NV, COH, NT>(task, S, vsrc, Wv, X, ldx, lds);
     }
 }
 
@@ -977,8 +1105,12 @@ struct DevPlan {
     int *d_need = nullptr, *d_ctr = nullptr;
 };
 
-static const int NV_VARIANTS[4] = {1, 2, 4, 8};
-static inline int nv_index(int nv) { return nv == 1 ? 0 : (nv == 2 ? 1 : (nv == 4 ? 2 : 3)); }
+static const int NWIDTH = 5;   // batch widths: vectors per pass over the matrix data
+static const int NV_VARIANTS[NWIDTH] = {1, 2, 4, 8, 16};
+static inline int nv_index(int nv) { return nv == 1 ? 0 : (nv == 2 ? 1 : (nv == 4 ? 2 : (nv == 8 ? 3 : 4))); }
+static inline bool nv_valid(int nv) { return nv == 1 || nv == 2 || nv == 4 || nv == 8 || nv == 16; }
+// widest pass for the vectors left: 16, 8, 4, 2 or 1
+static inline int nv_pass(int rem) { return rem >= 16 ? 16 : (rem >= 8 ? 8 : (rem >= 4 ? 4 : (rem >= 2 ? 2 : 1))); }
 
 // Tuning configuration of one batch width.
 struct Tune {
@@ -991,18 +1123,19 @@ struct Tune {
 struct DevState {
     int dev = -1;
     int ub_bits = 64;
-    Tune cfg[4];              // per batch width (nv_index); the setters change all four, the autotuner one
+    Tune cfg[NWIDTH];         // per batch width (nv_index); the setters change all of them, the autotuner one
     Storage st;
     bool st_ready = false;
     double *S64 = nullptr;
     float *S32 = nullptr;
-    DevPlan plans[4];
+    DevPlan plans[NWIDTH];
     // Coulomb and host-pointer staging
     int *d_gidx = nullptr;
     double *d_D = nullptr, *d_J = nullptr, *d_Jh = nullptr, *d_y = nullptr, *d_x = nullptr;
     size_t d_io_len = 0;
     hipStream_t side = nullptr;         // near-field stream (schedule 2)
     hipGraphExec_t graph = nullptr;     // captured hot-path sequence (H2ERI_graph_capture)
+    int graph_kind = -1;                // its kind (0 matvec, 1 Coulomb, 2 exchange)
     bool capturing = false;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool phase_timing = false;
@@ -1051,11 +1184,12 @@ static void release_plan(DevState *d, int i) {
 }
 
 static void release_plans(DevState *d) {
-    for (int i = 0; i < 4; i++) release_plan(d, i);
+    for (int i = 0; i < NWIDTH; i++) release_plan(d, i);
 }
 
 static void release_graph(DevState *d) {
     if (d->graph) { (void)hipGraphExecDestroy(d->graph); d->graph = nullptr; }
+    d->graph_kind = -1;
     free_dev((void *&)d->g_x);
     free_dev((void *&)d->g_y);
     free_dev((void *&)d->g_Jh);
@@ -1085,6 +1219,11 @@ void device_release(H2ERI_p h) {
     if (d->side) (void)hipStreamDestroy(d->side);
     delete d;
     h->dev = nullptr;
+}
+
+void device_release_graph_kind(H2ERI_p h, int kind) {
+    DevState *d = h ? dstate(h) : nullptr;
+    if (d && d->graph && d->graph_kind == kind) release_graph(d);
 }
 
 void device_invalidate(H2ERI_p h) {
@@ -1142,7 +1281,7 @@ struct Planner {
     int nst = 0, sG = 0, sS = 0;   // stages: main 0..sG, near-field stream sS..nst-1 (schedule 2)
 
     Planner(H2ERI_p h_, const Storage &s_, Plan &p_, int nv_, double tb, int nfs, int rch, double wb, bool sp)
-        : h(h_), S(s_), P(p_), nv(nv_), tile_bytes(tb), nf_schedule(nfs), rchunk(rch), wave_bytes(wb),
+        : h(h_), S(s_), P(p_), nv(nv_), tile_bytes(tb), nf_schedule(nfs), rchunk(rch), wave_bytes(nv_ == 16 ? 0.0 : wb),
           split2(sp && nv_ == 8), maxc(64 * cpl_of(nv_)) {}
 
     long long walloc(long long n) {
@@ -1563,6 +1702,7 @@ struct Planner {
             }
             return true;
         };
+        if (nv == 16 && (!P.w64.empty() || !P.w32.empty())) { set_error(ERR_ARG, "device plan check failed: wave tiles at 16 vectors"); return false; }
         if (!chk(P.t64, false, XIN_DOUBLES, YIN_DOUBLES) || !chk(P.t32, true, XIN_DOUBLES, YIN_DOUBLES) ||
             !chk(P.w64, false, TileLDS<64>::X, TileLDS<64>::Y) || !chk(P.w32, true, TileLDS<64>::X, TileLDS<64>::Y))
             return false;
@@ -1904,7 +2044,8 @@ static int queue_grid(int nv) {
     case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<1, false>, WG, 0); break;
     case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<2, false>, WG, 0); break;
     case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<4, false>, WG, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<8, false>, WG, 0); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<8, false>, WG, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, queue_kernel<16, false>, WG, 0); break;
     }
     return e == hipSuccess ? ncu * std::max(per, 1) : 0;
 }
@@ -1976,8 +2117,8 @@ static void launch_stage(const Stage &st, const DevPlan &dp, DevState *d, const 
                          hipStream_t s) {
     hipLaunchKernelGGL(stage_kernel<NV>, dim3(stage_blocks(st)), dim3(WG), 0, s, dp.d_t64 + st.t64_beg, st.t64_cnt,
                        dp.d_t32 + st.t32_beg, st.t32_cnt, dp.d_rt + st.r_beg, st.r_cnt, dp.d_w64 + st.w64_beg,
-                       st.w64_cnt, dp.d_w32 + st.w32_beg, st.w32_cnt, dp.d_vsrc, dp.d_rsrc, d->S64, d->S32, dp.Wv, X, Y,
-                       ld);
+                       st.w64_cnt, dp.d_w32 + st.w32_beg, st.w32_cnt, dp.d_vsrc, dp.d_rsrc, d->S64, d->S32,
+                       dp.Wv, X, Y, ld);
 }
 
 static void collect_phase_ms(H2ERI_p h, DevState *d);
@@ -1998,8 +2139,8 @@ static bool run_queue(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, doub
     if (!dp.q_ready) { set_error(ERR_DEVICE, "persistent executor: queue not built"); return false; }
     const long long ld = h->krnl_mat_size;
     const bool tim = d->phase_timing && !d->capturing;
-    hipLaunchKernelGGL(queue_reset_kernel, dim3((std::max(dp.q_nlist, QC_LISTS) + 255) / 256), dim3(256), 0, s, dp.d_ctr,
-                       dp.q_nlist);
+    hipLaunchKernelGGL(queue_reset_kernel, dim3((std::max(dp.q_nlist, QC_LISTS) + 255) / 256), dim3(256), 0, s,
+                       dp.d_ctr, dp.q_nlist);
     HIP_OK(hipGetLastError());
     if (tim) {   // the event pair brackets the persistent launch only
         collect_phase_ms(h, d);
@@ -2014,7 +2155,8 @@ static bool run_queue(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, doub
     case 1: launch_queue<1>(dp, d, X, Y, ld, s); break;
     case 2: launch_queue<2>(dp, d, X, Y, ld, s); break;
     case 4: launch_queue<4>(dp, d, X, Y, ld, s); break;
-    default: launch_queue<8>(dp, d, X, Y, ld, s); break;
+    case 8: launch_queue<8>(dp, d, X, Y, ld, s); break;
+    default: launch_queue<16>(dp, d, X, Y, ld, s); break;
     }
     HIP_OK(hipGetLastError());
     if (tim) {
@@ -2049,7 +2191,8 @@ static bool run_matvec(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, dou
         case 1: launch_stage<1>(st, dp, d, X, Y, ld, q); break;
         case 2: launch_stage<2>(st, dp, d, X, Y, ld, q); break;
         case 4: launch_stage<4>(st, dp, d, X, Y, ld, q); break;
-        default: launch_stage<8>(st, dp, d, X, Y, ld, q); break;
+        case 8: launch_stage<8>(st, dp, d, X, Y, ld, q); break;
+        default: launch_stage<16>(st, dp, d, X, Y, ld, q); break;
         }
         HIP_OK(hipGetLastError());
         if (tim) {
@@ -2104,13 +2247,13 @@ static void collect_phase_ms(H2ERI_p h, DevState *d) {
     h->timers[MV_VOP_TIMER_IDX] += d->phase_ms[PH_GATHER] * 1e-3;
 }
 
-// Batched device matvec: nvec vectors, processed in passes of 8, 4, 2, 1.
+// Batched device matvec: nvec vectors, processed in passes of 16, 8, 4, 2, 1.
 static bool matvec_multi_dev_impl(H2ERI_p h, int nvec, const double *X, double *Y, hipStream_t s) {
     const size_t N = (size_t)h->krnl_mat_size;
     int done = 0;
     while (done < nvec) {
         const int rem = nvec - done;
-        const int nv = rem >= 8 ? 8 : (rem >= 4 ? 4 : (rem >= 2 ? 2 : 1));
+        const int nv = nv_pass(rem);
         DevPlan *dp = get_plan(h, nv);
         if (!dp) return false;
         if (!run_matvec(h, dstate(h), *dp, X + done * N, Y + done * N, s)) return false;
@@ -2285,23 +2428,28 @@ void H2ERI_build_Coulomb_multi_dev(H2ERI_p h, const int ndens, const double *den
 // Captures the device hot path for fixed device buffers into a HIP graph held
 // by the handle (replacing a previous one): kind 0 is the batched matvec
 // (H2ERI_matvec_multi_dev with n vectors), kind 1 the Coulomb build
-// (H2ERI_build_Coulomb_multi_dev with n densities).  Nothing runs during the
+// (H2ERI_build_Coulomb_multi_dev with n densities), kind 2 the exchange build
+// (H2ERI_build_exchange_dev, n = 1; the density screening included).  Nothing runs during the
 // capture; H2ERI_graph_replay enqueues the captured sequence.  The graph
 // refers to in_dev, out_dev and the handle's buffers: changing the precision,
 // tile size or schedule, or destroying the handle, releases it.
 int H2ERI_graph_capture(H2ERI_p h, const int kind, const int n, const double *in_dev, double *out_dev) {
-    if (!h || (kind != 0 && kind != 1) || n < 1 || !in_dev || !out_dev) {
+    if (!h || kind < 0 || kind > 2 || n < 1 || (kind == 2 && n != 1) || !in_dev || !out_dev) {
         set_error(ERR_ARG, "H2ERI_graph_capture: bad argument");
         return -1;
     }
     // build every plan and buffer first: allocation is not allowed while capturing
-    if (!upload_storage(h)) return -1;
-    DevState *d = dstate(h);
-    for (int done = 0; done < n;) {
-        const int rem = n - done;
-        const int nv = rem >= 8 ? 8 : (rem >= 4 ? 4 : (rem >= 2 ? 2 : 1));
-        if (!get_plan(h, nv)) return -1;
-        done += nv;
+    DevState *d = dstate_get(h);
+    if (kind == 2) {
+        if (!exchange_prepare(h)) return -1;
+    } else {
+        if (!upload_storage(h)) return -1;
+        for (int done = 0; done < n;) {
+            const int rem = n - done;
+            const int nv = nv_pass(rem);
+            if (!get_plan(h, nv)) return -1;
+            done += nv;
+        }
     }
     if (!d->side && hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking) != hipSuccess) { set_error(ERR_DEVICE, "stream"); return -1; }
     if (!d->ev_fork && hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming) != hipSuccess) { set_error(ERR_DEVICE, "event"); return -1; }
@@ -2324,8 +2472,9 @@ int H2ERI_graph_capture(H2ERI_p h, const int kind, const int n, const double *in
     bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
         d->capturing = true;
-        ok = kind == 0 ? matvec_multi_dev_impl(h, n, in_dev, out_dev, cs)
-                       : coulomb_run(h, n, in_dev, out_dev, cs, d->g_x, d->g_y, d->g_Jh);
+        ok = kind == 0   ? matvec_multi_dev_impl(h, n, in_dev, out_dev, cs)
+             : kind == 1 ? coulomb_run(h, n, in_dev, out_dev, cs, d->g_x, d->g_y, d->g_Jh)
+                         : exchange_enqueue(h, in_dev, out_dev, cs);
         d->capturing = false;
         const bool end_ok = hipStreamEndCapture(cs, &g) == hipSuccess;
         ok = ok && end_ok && g;
@@ -2341,6 +2490,7 @@ int H2ERI_graph_capture(H2ERI_p h, const int kind, const int n, const double *in
         if (!has_error()) set_error(ERR_DEVICE, "H2ERI_graph_capture: capture failed");
         return -1;
     }
+    d->graph_kind = kind;
     return 0;
 }
 
@@ -2360,7 +2510,12 @@ void H2ERI_set_precision(H2ERI_p h, const int ub_bits) {
 
 size_t H2ERI_upload(H2ERI_p h) {
     if (!h || !get_plan(h, 1)) return 0;
-    return dstate(h)->resident_bytes;
+    DevState *d = dstate(h);
+    // everything a single-vector pass needs besides its vectors
+    if (!d->side && hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking) != hipSuccess) { set_error(ERR_DEVICE, "stream"); return 0; }
+    if (!d->ev_fork && hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming) != hipSuccess) { set_error(ERR_DEVICE, "event"); return 0; }
+    if (!d->ev_join && hipEventCreateWithFlags(&d->ev_join, hipEventDisableTiming) != hipSuccess) { set_error(ERR_DEVICE, "event"); return 0; }
+    return d->resident_bytes;
 }
 
 void H2ERI_set_phase_timing(H2ERI_p h, const int on) {
@@ -2415,7 +2570,7 @@ void H2ERI_matvec_bytes(H2ERI_p h, double *out6) {
 void H2ERI_set_schedule(H2ERI_p h, const int mode) {
     if (!h || mode < 0 || mode > 17) { set_error(ERR_ARG, "H2ERI_set_schedule: 0 to 17"); return; }
     DevState *d = dstate_get(h);
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < NWIDTH; i++)
         if (d->cfg[i].nf_schedule != mode) { d->cfg[i].nf_schedule = mode; release_plan(d, i); }
 }
 
@@ -2429,7 +2584,7 @@ void H2ERI_set_schedule(H2ERI_p h, const int mode) {
 void H2ERI_set_executor(H2ERI_p h, const int mode) {
     if (!h || (mode != 0 && mode != 1)) { set_error(ERR_ARG, "H2ERI_set_executor: 0 or 1"); return; }
     DevState *d = dstate_get(h);
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < NWIDTH; i++)
         if (d->cfg[i].executor != mode) { d->cfg[i].executor = mode; release_plan(d, i); }
 }
 
@@ -2457,7 +2612,7 @@ void H2ERI_set_spin_limit(H2ERI_p h, const int polls) {
 // out4 = {tasks, tasks started without a ticket, workgroups, list-scheduling
 // estimate of the pass in microseconds}.
 int H2ERI_queue_stats(H2ERI_p h, const int nv, double *out4) {
-    if (!h || !out4 || (nv != 1 && nv != 2 && nv != 4 && nv != 8)) { set_error(ERR_ARG, "H2ERI_queue_stats: bad argument"); return -1; }
+    if (!h || !out4 || !nv_valid(nv)) { set_error(ERR_ARG, "H2ERI_queue_stats: bad argument"); return -1; }
     DevPlan *dp = get_plan(h, nv);
     if (!dp || !ensure_queue(dstate(h), *dp)) return -1;
     out4[0] = dp->q_n;
@@ -2472,7 +2627,7 @@ int H2ERI_queue_stats(H2ERI_p h, const int nv, double *out4) {
 void H2ERI_set_wave_tile_bytes(H2ERI_p h, const double bytes) {
     if (!h || bytes < 0) { set_error(ERR_ARG, "H2ERI_set_wave_tile_bytes: bytes >= 0"); return; }
     DevState *d = dstate_get(h);
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < NWIDTH; i++)
         if (d->cfg[i].wave_bytes != bytes) { d->cfg[i].wave_bytes = bytes; release_plan(d, i); }
 }
 
@@ -2480,7 +2635,7 @@ void H2ERI_set_wave_tile_bytes(H2ERI_p h, const double bytes) {
 void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
     if (!h || bytes < 1024) return;
     DevState *d = dstate_get(h);
-    for (int i = 0; i < 4; i++) d->cfg[i].tile_bytes = bytes;
+    for (int i = 0; i < NWIDTH; i++) d->cfg[i].tile_bytes = bytes;
     release_plans(d);
 }
 
@@ -2491,7 +2646,7 @@ void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
 // bytes, schedule, executor, wave-tile bytes}.  Returns 0, or -1 with the
 // error set.
 int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
-    if (!h || (nv != 1 && nv != 2 && nv != 4 && nv != 8) || reps < 1) { set_error(ERR_ARG, "H2ERI_autotune: bad argument"); return -1; }
+    if (!h || !nv_valid(nv) || reps < 1) { set_error(ERR_ARG, "H2ERI_autotune: bad argument"); return -1; }
     if (!upload_storage(h)) return -1;
     DevState *d = dstate(h);
     const size_t N = (size_t)h->krnl_mat_size;
@@ -2500,7 +2655,10 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
     hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(256), 0, 0, d->d_x, N * nv);
     if (hipGetLastError() != hipSuccess) { set_error(ERR_DEVICE, "autotune: zero"); return -1; }
     const double tiles[] = {65536.0, 131072.0, 262144.0};
-    const int scheds[] = {0, 2, 3, 4};
+    // schedules: near field in the sweep launches (0), or on a second stream in
+    // 1, 2, 3, 5 or 9 groups of three launches (more groups bring each low-rank
+    // block's second read of its smaller half closer to the first)
+    const int scheds[] = {0, 2, 3, 4, 6, 10};
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { set_error(ERR_DEVICE, "autotune: events"); return -1; }
     const int vi = nv_index(nv);
@@ -2512,7 +2670,7 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
     d->phase_timing = false;
     bool ok = true;
     for (int ex = 0; ok && ex < 2; ex++)
-    for (int wi = 0; ok && wi < (ex == 0 ? 2 : 1); wi++)
+    for (int wi = 0; ok && wi < (ex == 0 && nv != 16 ? 2 : 1); wi++)   // no wave tiles at 16 vectors per pass
     for (double tb : tiles) {
         for (int sc : scheds) {
             release_plan(d, vi);
@@ -2690,7 +2848,7 @@ size_t h2eri_struct_size(void) { return sizeof(struct H2ERI); }
 // element is stored twice.  For tests of the planner; the device entry points
 // never call it.  Returns the number of conflicts, -1 on failure.
 int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double *X, double *Y) {
-    if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !X || !Y || (nv != 1 && nv != 2 && nv != 4 && nv != 8)) {
+    if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !X || !Y || !nv_valid(nv)) {
         set_error(ERR_ARG, "plan_emulate: bad args");
         return -1;
     }
@@ -2809,7 +2967,7 @@ int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double 
 // unwritten element, and every output element not written exactly once.
 // Returns that count, or -1 on failure.
 int H2ERI_queue_emulate(H2ERI_p h, const int ub_bits, const int nv, const int workgroups, const double *X, double *Y) {
-    if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !X || !Y || (nv != 1 && nv != 2 && nv != 4 && nv != 8) ||
+    if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !X || !Y || !nv_valid(nv) ||
         workgroups < 1) {
         set_error(ERR_ARG, "queue_emulate: bad args");
         return -1;

@@ -395,6 +395,26 @@ __global__ __launch_bounds__(KWG) void kx_gemm(const KGemmTask *__restrict__ tas
     }
 }
 
+// Density screening (src/H2ERI_build_exchange.c:774-838): block (N,P) of D
+// takes part in K only when some |D_np|, n in N and p in P, exceeds
+// DTOL = 1e-10 (:790).  The reference restricts every P list to dlist[N]
+// (:410-417,1397-1398), which sums over the kept blocks only; here every
+// other block is written as zeros into Ds, the density the kernels read.
+// One thread per block; the threads of a row of blocks read consecutive
+// columns of the same rows of D.
+__global__ void kx_dlist(const int *__restrict__ sbf, int nshell, const double *__restrict__ D, double *__restrict__ Ds,
+                         int nbf) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nshell * nshell) return;
+    const int n = (int)(idx / nshell), p = (int)(idx - (long long)n * nshell);
+    const int r0 = sbf[n], r1 = sbf[n + 1], c0 = sbf[p], c1 = sbf[p + 1];
+    bool keep = false;
+    for (int i = r0; i < r1; i++)
+        for (int j = c0; j < c1; j++) keep |= fabs(D[(size_t)i * nbf + j]) > 1e-10;
+    for (int i = r0; i < r1; i++)
+        for (int j = c0; j < c1; j++) Ds[(size_t)i * nbf + j] = keep ? D[(size_t)i * nbf + j] : 0.0;
+}
+
 __global__ void kx_zero(double *__restrict__ p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t s = (size_t)gridDim.x * blockDim.x;
@@ -432,6 +452,8 @@ struct KState {
     KPlan P;
     bool ready = false;
     double *PB = nullptr, *CB = nullptr, *SC = nullptr, *K1 = nullptr, *Kd = nullptr, *D = nullptr, *K = nullptr;
+    double *Ds = nullptr;   // the density after screening (the kernels' D)
+    int *sbf = nullptr;     // shell_bf_sidx
     KTerm *terms = nullptr;
     KGemm *gemms = nullptr;
     KNode *nodes = nullptr;
@@ -451,7 +473,7 @@ static void kstate_release(KState *k) {
                    (void **)&k->K, (void **)&k->terms, (void **)&k->gemms, (void **)&k->nodes, (void **)&k->bfidx,
                    (void **)&k->qptr, (void **)&k->mptr, (void **)&k->qent, (void **)&k->ment, (void **)&k->nspl,
                    (void **)&k->nent, (void **)&k->ket,
-                   (void **)&k->bra, (void **)&k->direct, (void **)&k->gtasks};
+                   (void **)&k->bra, (void **)&k->direct, (void **)&k->gtasks, (void **)&k->Ds, (void **)&k->sbf};
     for (void **p : ps) kfree(*p);
     k->ready = false;
     k->P = KPlan();
@@ -756,6 +778,7 @@ static KState *kstate(H2ERI_p h) { return (KState *)h->thread_Kmat_workbuf; }
 void exchange_release(H2ERI_p h) {
     KState *k = kstate(h);
     if (!k) return;
+    device_release_graph_kind(h, 2);   // a captured exchange graph refers to the buffers freed here
     kstate_release(k);
     delete k;
     h->thread_Kmat_workbuf = nullptr;
@@ -772,6 +795,7 @@ void exchange_release(H2ERI_p h) {
 
 static bool k_upload(H2ERI_p h) {
     if (!h->U || !h->c_B_blks || !h->c_D_blks) { set_error(ERR_ARG, "no H2 representation built or loaded"); return false; }
+    if (!h->shell_bf_sidx || h->nshell <= 0) { set_error(ERR_ARG, "the exchange build needs shell metadata"); return false; }
     if (!h->thread_Kmat_workbuf) h->thread_Kmat_workbuf = (void **)new KState();
     KState *k = kstate(h);
     if (k->ready) return true;
@@ -792,7 +816,8 @@ static bool k_upload(H2ERI_p h) {
         !up((void **)&k->CB, P.CB.empty() ? nullptr : P.CB.data(), P.CB.size() * 8) ||
         !up((void **)&k->SC, nullptr, (size_t)P.nSC * 8) || !up((void **)&k->K1, nullptr, nn2 * 8) ||
         !up((void **)&k->Kd, nullptr, nn2 * 8) || !up((void **)&k->D, nullptr, nn2 * 8) ||
-        !up((void **)&k->K, nullptr, nn2 * 8) ||
+        !up((void **)&k->K, nullptr, nn2 * 8) || !up((void **)&k->Ds, nullptr, nn2 * 8) ||
+        !up((void **)&k->sbf, h->shell_bf_sidx, (size_t)(h->nshell + 1) * 4) ||
         !up((void **)&k->terms, P.terms.empty() ? nullptr : P.terms.data(), P.terms.size() * sizeof(KTerm)) ||
         !up((void **)&k->gemms, P.gemms.empty() ? nullptr : P.gemms.data(), P.gemms.size() * sizeof(KGemm)) ||
         !up((void **)&k->nodes, P.nodes.empty() ? nullptr : P.nodes.data(), P.nodes.size() * sizeof(KNode)) ||
@@ -824,6 +849,9 @@ static bool k_run(H2ERI_p h, const double *D_dev, double *K_dev, hipStream_t s) 
     const int nbf = h->num_bf;
     const size_t nn2 = (size_t)nbf * nbf;
     const unsigned zb = (unsigned)std::min<size_t>((nn2 + 255) / 256, 2048);
+    const long long nblk = (long long)h->nshell * h->nshell;
+    hipLaunchKernelGGL(kx_dlist, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, s, k->sbf, h->nshell, D_dev, k->Ds, nbf);
+    D_dev = k->Ds;
     hipLaunchKernelGGL(kx_zero, dim3(zb), dim3(256), 0, s, k->K1, nn2);
     hipLaunchKernelGGL(kx_zero, dim3(zb), dim3(256), 0, s, k->Kd, nn2);
     KHIP(hipGetLastError());
@@ -842,14 +870,55 @@ static bool k_run(H2ERI_p h, const double *D_dev, double *K_dev, hipStream_t s) 
     return true;
 }
 
+bool exchange_prepare(H2ERI_p h) { return k_upload(h); }
+
+bool exchange_enqueue(H2ERI_p h, const double *D_dev, double *K_dev, void *stream) {
+    return k_run(h, D_dev, K_dev, (hipStream_t)stream);
+}
+
 }  // namespace h2e
 
 using namespace h2e;
 
 extern "C" {
 
+// Density screening list on the host (src/H2ERI_build_exchange.c:774-838):
+// dlist_sidx[N]..dlist_sidx[N+1] lists, in increasing order, the shells P
+// whose block (N,P) of D has some |D_np| > 1e-10 (:790).  The device build
+// applies the same rule to its copy of D.
+void H2ERI_exchange_dlist(H2ERI_p h, const double *den_mat) {
+    if (!h || !den_mat) { set_error(ERR_ARG, "H2ERI_exchange_dlist: null argument"); return; }
+    if (!h->shell_bf_sidx || h->nshell <= 0) { set_error(ERR_ARG, "H2ERI_exchange_dlist: needs shell-pair metadata (a molecule)"); return; }
+    const int ns = h->nshell, nbf = h->num_bf;
+    const int *sbf = h->shell_bf_sidx;
+    std::vector<std::vector<int>> rows(ns);
+    #pragma omp parallel for schedule(dynamic)
+    for (int n = 0; n < ns; n++)
+        for (int p = 0; p < ns; p++) {
+            bool keep = false;
+            for (int i = sbf[n]; i < sbf[n + 1] && !keep; i++)
+                for (int j = sbf[p]; j < sbf[p + 1]; j++)
+                    if (std::fabs(den_mat[(size_t)i * nbf + j]) > 1e-10) { keep = true; break; }
+            if (keep) rows[n].push_back(p);
+        }
+    free(h->dlist_sidx);
+    free(h->dlist);
+    h->dlist_sidx = (int *)malloc(sizeof(int) * (ns + 1));
+    size_t len = 0;
+    for (int n = 0; n < ns; n++) len += rows[n].size();
+    h->dlist = (int *)malloc(sizeof(int) * (len > 0 ? len : 1));
+    if (!h->dlist_sidx || !h->dlist) { set_error(ERR_ALLOC, "H2ERI_exchange_dlist: allocation failed"); return; }
+    h->dlist_sidx[0] = 0;
+    for (int n = 0; n < ns; n++) {
+        std::copy(rows[n].begin(), rows[n].end(), h->dlist + h->dlist_sidx[n]);
+        h->dlist_sidx[n + 1] = h->dlist_sidx[n] + (int)rows[n].size();
+    }
+}
+
 void H2ERI_build_exchange(H2ERI_p h, const double *den_mat, double *K_mat) {
     if (!h || !den_mat || !K_mat) { set_error(ERR_ARG, "H2ERI_build_exchange: null argument"); return; }
+    H2ERI_exchange_dlist(h, den_mat);   // the reference's side effect on the handle
+    if (has_error()) return;
     if (!k_upload(h)) return;
     KState *k = kstate(h);
     const size_t nn2 = (size_t)h->num_bf * h->num_bf;

@@ -95,7 +95,12 @@ void update_mat_sizes(H2ERI_p h);
 // device layer (device.hip)
 void device_release(H2ERI_p h);
 void device_invalidate(H2ERI_p h);
+// releases the handle's captured graph if it is of the given kind (2: exchange)
+void device_release_graph_kind(H2ERI_p h, int kind);
 // exchange layer (exchange.hip): device state held in h->thread_Kmat_workbuf
 void exchange_release(H2ERI_p h);
+bool exchange_prepare(H2ERI_p h);   // plan, upload and basis products (allocates)
+// enqueues one K build on `stream` (a hipStream_t) without allocating
+bool exchange_enqueue(H2ERI_p h, const double *D_dev, double *K_dev, void *stream);
 
 }  // namespace h2e

@@ -200,7 +200,7 @@ class H2ERI:
 
     def plan_emulate(self, x, bits=64):
         """Host execution of the device plan (tests only): (y, conflicts).
-        x is one vector (N,) or a batch (nv, N) with nv in {1, 2, 4, 8}."""
+        x is one vector (N,) or a batch (nv, N) with nv in {1, 2, 4, 8, 16}."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         nv = 1 if x.ndim == 1 else x.shape[0]
         y = np.zeros_like(x)
@@ -322,9 +322,10 @@ class H2ERI:
         _check(self.lib)
 
     def graph_capture(self, kind, n, in_ptr, out_ptr):
-        """Capture the device matvec (kind "matvec") or Coulomb build (kind
-        "coulomb") on fixed device buffers into a HIP graph."""
-        k = {"matvec": 0, "coulomb": 1}[kind]
+        """Capture the device matvec (kind "matvec"), Coulomb build (kind
+        "coulomb") or exchange build (kind "exchange", n = 1) on fixed device
+        buffers into a HIP graph."""
+        k = {"matvec": 0, "coulomb": 1, "exchange": 2}[kind]
         self.lib.H2ERI_graph_capture(self.h, k, n, C.c_void_p(in_ptr), C.c_void_p(out_ptr))
         _check(self.lib)
 
@@ -343,6 +344,16 @@ class H2ERI:
     def build_exchange_dev(self, d_ptr, k_ptr, stream=None):
         self.lib.H2ERI_build_exchange_dev(self.h, C.c_void_p(d_ptr), C.c_void_p(k_ptr), C.c_void_p(stream or 0))
         _check(self.lib)
+
+    def exchange_dlist(self, D):
+        """The density screening list (H2ERI_exchange_dlist): (dlist_sidx, dlist)."""
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        assert D.shape == (self.nbf, self.nbf)
+        self.lib.H2ERI_exchange_dlist(self.h, _dptr(D))
+        _check(self.lib)
+        ns = self.s.nshell
+        sidx = _iarr(self.s.dlist_sidx, ns + 1)
+        return sidx, _iarr(self.s.dlist, int(sidx[-1]))
 
     def exchange_workbuf_init(self):
         """Builds and uploads the exchange plan now (H2ERI_exchange_workbuf_init)."""

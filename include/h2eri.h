@@ -235,6 +235,13 @@ void H2ERI_build_exchange(H2ERI_p h2eri, const double *den_mat, double *K_mat);
  * now (otherwise the first K build does); free releases its device memory. */
 void H2ERI_exchange_workbuf_init(H2ERI_p h2eri);
 void H2ERI_exchange_workbuf_free(H2ERI_p h2eri);
+/* The density screening list of src/H2ERI_build_exchange.c:774-838 on the
+ * host: fills h2eri->dlist_sidx (nshell + 1) and h2eri->dlist with, for each
+ * shell N, the shells P whose block (N,P) of D has some |D_np| > 1e-10.
+ * H2ERI_build_exchange calls it (the reference's side effect on the
+ * handle); every exchange build, host or device pointers, sums over those
+ * blocks of D only (the device applies the same rule to its copy of D). */
+void H2ERI_exchange_dlist(H2ERI_p h2eri, const double *den_mat);
 
 /* Additive: device-resident exchange build (den_dev, K_dev: num_bf^2 doubles
  * on the library's device; enqueued on `stream`, no synchronisation), and the
@@ -279,8 +286,11 @@ void H2ERI_build_Coulomb_multi_dev(H2ERI_p h2eri, const int ndens, const double 
 
 /* HIP-graph form of the device hot path, for loops that repeat it on the
  * same buffers (SCF iterations).  kind 0 captures H2ERI_matvec_multi_dev
- * with n vectors, kind 1 H2ERI_build_Coulomb_multi_dev with n densities, on
- * the device buffers in_dev/out_dev; nothing runs during the capture.
+ * with n vectors, kind 1 H2ERI_build_Coulomb_multi_dev with n densities,
+ * kind 2 H2ERI_build_exchange_dev (n = 1, the density screening included),
+ * on the device buffers in_dev/out_dev; nothing runs during the capture
+ * (plans and uploads happen before it).  H2ERI_exchange_workbuf_free
+ * releases an exchange graph.
  * H2ERI_graph_replay enqueues the captured sequence on `stream`.  One graph
  * per handle; changing precision, tile size, schedule or executor, running
  * the autotuner, or capturing again releases it (a replay after that fails
@@ -291,7 +301,9 @@ int H2ERI_graph_capture(H2ERI_p h2eri, const int kind, const int n, const double
 int H2ERI_graph_replay(H2ERI_p h2eri, void *stream);
 
 /* Upload the stored representation to the device now (otherwise done by the
- * first hot-path call).  Returns bytes resident on the device. */
+ * first hot-path call) and prepare everything a single-vector pass needs
+ * besides its vectors (plan, streams, events).  Returns bytes resident on
+ * the device. */
 size_t H2ERI_upload(H2ERI_p h2eri);
 
 /* Versioned binary serialisation of a built representation (the
@@ -403,9 +415,9 @@ void  H2ERI_group_destroy(void *group);
 
 /* Autotuner: times nv-vector device matvecs (reps calls after a warm-up)
  * for every executor (H2ERI_set_executor) x tile size {64, 128, 256} KB x
- * near-field schedule {0, 2, 3, 4} x (staged executor) wave-tile limit {0,
+ * near-field schedule {0, 2, 3, 4, 6, 10} x (staged executor) wave-tile limit {0,
  * 32} KB, and keeps the fastest combination for later passes of nv vectors
- * (each batch width 1, 2, 4, 8 has a configuration of its own; the tuning
+ * (each batch width 1, 2, 4, 8, 16 has a configuration of its own; the tuning
  * setters above change all four).  out5 = {ms per call, tile bytes,
  * schedule, executor, wave-tile bytes} of the combination kept.  Returns 0,
  * -1 on error. */
@@ -423,7 +435,7 @@ void H2ERI_set_wave_tile_bytes(H2ERI_p h2eri, const double bytes);
 int H2ERI_plan_check(H2ERI_p h2eri, const int ub_bits);
 
 /* Host verification of the device plan (no HIP call, tests only): executes
- * every launch of the plan for nv (1, 2, 4 or 8) vectors X (nv x N,
+ * every launch of the plan for nv (1, 2, 4, 8 or 16) vectors X (nv x N,
  * vector-major) with the kernels' semantics into Y and returns the number of
  * intra-launch read/write conflicts plus output elements not written exactly
  * once (0 expected), -1 on failure.  The hot-path entry points never call it. */

@@ -19,11 +19,16 @@
  * orientations of its rows and columns.  For symmetric D the F^T terms
  * contribute K1^T, where K1 is the sum over the F terms, so
  *   K = K1 + K1^T + K_diag.
+ * The density is screened first as the reference screens it (orc_dlist):
+ * blocks (N,P) of D with every |D_np| <= 1e-10 take no part.
  * Parity pins (tests/test_oracle_exchange.py): equality with the contraction
  * of the dense brute-force matrix orc_dense_matrix; the reference fixture
  * tests/K_alk26_ccpvdz.bin (within the H2 tolerance); symmetry and linearity
- * in D.
+ * in D for densities without sub-threshold blocks; the screening list
+ * against a numpy restatement and K of a density with sub-threshold blocks
+ * against the contraction with those blocks zeroed.
  */
+#include <math.h>
 #include <omp.h>
 #include <stdlib.h>
 #include <string.h>
@@ -95,8 +100,57 @@ static void core_rows(const orc_blk *C, int r0, int h, double *out) {
         }
 }
 
-void orc_build_exchange(orc_rep *r, const double *D, double *K) {
+/* Density screening list (src/H2ERI_build_exchange.c:774-838): shell P is in
+ * dlist[N] when some |D_np|, n in N and p in P, exceeds DTOL = 1e-10 (:790).
+ * Fills sidx[nshell + 1] and, when list is not NULL, the lists (sorted P per
+ * N); returns the total length. */
+int orc_dlist(const orc_rep *r, const double *D, int *sidx, int *list) {
+    const int nbf = r->num_bf, ns = r->nshell;
+    const double DTOL = 1e-10;
+    int pos = 0;
+    sidx[0] = 0;
+    for (int n = 0; n < ns; n++) {
+        for (int p = 0; p < ns; p++) {
+            int flag = 0;
+            for (int i = r->shell_bf_sidx[n]; i < r->shell_bf_sidx[n + 1] && !flag; i++)
+                for (int j = r->shell_bf_sidx[p]; j < r->shell_bf_sidx[p + 1]; j++)
+                    if (fabs(D[(size_t)i * nbf + j]) > DTOL) { flag = 1; break; }
+            if (flag) {
+                if (list) list[pos] = p;
+                pos++;
+            }
+        }
+        sidx[n + 1] = pos;
+    }
+    return pos;
+}
+
+/* The reference takes, for each (N,Q), only the shells P of plist[Q] that
+ * are also in dlist[N] (src/H2ERI_build_exchange.c:410-417,1397-1398): the
+ * sum over (n,p) then runs over the blocks (N,P) of dlist only, which is the
+ * sum with every other block of D set to zero. */
+static double *screened_density(const orc_rep *r, const double *D) {
+    const int nbf = r->num_bf, ns = r->nshell;
+    int *sidx = (int *)malloc(sizeof(int) * (ns + 1));
+    const int len = orc_dlist(r, D, sidx, NULL);
+    int *list = (int *)malloc(sizeof(int) * (len > 0 ? len : 1));
+    orc_dlist(r, D, sidx, list);
+    double *Ds = (double *)calloc((size_t)nbf * nbf, sizeof(double));
+    for (int n = 0; n < ns; n++)
+        for (int k = sidx[n]; k < sidx[n + 1]; k++) {
+            const int p = list[k];
+            for (int i = r->shell_bf_sidx[n]; i < r->shell_bf_sidx[n + 1]; i++)
+                for (int j = r->shell_bf_sidx[p]; j < r->shell_bf_sidx[p + 1]; j++)
+                    Ds[(size_t)i * nbf + j] = D[(size_t)i * nbf + j];
+        }
+    free(sidx);
+    free(list);
+    return Ds;
+}
+
+void orc_build_exchange(orc_rep *r, const double *D_in, double *K) {
     const int nbf = r->num_bf, N = r->N, nn = r->n_node, nl = r->n_leaf, np = r->num_sp;
+    double *D = screened_density(r, D_in);
     /* basis-function pair of every row: (a, b), and whether the shells differ */
     int *ra = (int *)malloc(sizeof(int) * N), *rb = (int *)malloc(sizeof(int) * N);
     char *sw = (char *)malloc(N);
@@ -246,4 +300,5 @@ void orc_build_exchange(orc_rep *r, const double *D, double *K) {
     free(K1t); free(Kdt); free(T);
     for (int i = 0; i < nn; i++) free(P[i]);
     free(P); free(ra); free(rb); free(sw);
+    free(D);
 }

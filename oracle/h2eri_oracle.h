@@ -10,6 +10,7 @@
  *   orc_matvec          src/H2ERI_matvec.c:24-63, 93-407, 572-757, 882-957
  *   orc_build_coulomb   src/H2ERI_build_Coulomb.c:26-152
  *   orc_build_exchange  src/H2ERI_build_exchange.c:1356-1450 (same quantity, block by block)
+ *   orc_dlist           src/H2ERI_build_exchange.c:774-838 (density screening list)
  *   orc_dense_matrix    (brute force) A_H2 assembled block by block from the
  *                       expanded nested bases, independent of the sweeps
  * Parity pins: see the header comment of each source file.
@@ -27,6 +28,7 @@ orc_rep *orc_load(const char *path);
 void orc_free(orc_rep *r);
 int orc_size(const orc_rep *r);      /* krnl_mat_size */
 int orc_nbf(const orc_rep *r);       /* num_bf (0 for synthetic representations) */
+int orc_nshell(const orc_rep *r);
 void orc_set_threads(int n);
 
 /* y := A_H2 x (restatement of H2ERI_matvec, thread-private partial sums). */
@@ -37,6 +39,9 @@ void orc_build_coulomb(orc_rep *r, const double *D, double *J);
 
 /* K := sum_{n,p} (mn|pq) D_np through the H2 representation. */
 void orc_build_exchange(orc_rep *r, const double *D, double *K);
+/* density screening list of src/H2ERI_build_exchange.c:774-838 (DTOL 1e-10):
+ * sidx[nshell + 1], list (may be NULL: count only); returns its length */
+int orc_dlist(const orc_rep *r, const double *D, int *sidx, int *list);
 
 /* Dense brute force: writes A_H2 (krnl_mat_size^2, row-major) by assembling
  * every admissible block Ubig_i B Ubig_j^T from the expanded nested bases

@@ -89,6 +89,7 @@ void orc_free(orc_rep *r) {
 
 int orc_size(const orc_rep *r) { return r->N; }
 int orc_nbf(const orc_rep *r) { return r->num_bf; }
+int orc_nshell(const orc_rep *r) { return r->nshell; }
 
 /* COO -> CSR with column-sorted rows (src/H2ERI_utils.c:121-146). */
 static void coo_csr(int nrow, int nnz, const int *row, const int *col, const int *val, int **rp, int **ci, int **vv) {

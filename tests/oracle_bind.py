@@ -29,12 +29,16 @@ def load():
     lib.orc_size.argtypes = [C.c_void_p]
     lib.orc_nbf.restype = C.c_int
     lib.orc_nbf.argtypes = [C.c_void_p]
+    lib.orc_nshell.restype = C.c_int
+    lib.orc_nshell.argtypes = [C.c_void_p]
     lib.orc_set_threads.argtypes = [C.c_int]
     lib.orc_matvec.argtypes = [C.c_void_p, dp, dp]
     lib.orc_build_coulomb.argtypes = [C.c_void_p, dp, dp]
     lib.orc_dense_matrix.argtypes = [C.c_void_p, dp]
     if hasattr(lib, "orc_build_exchange"):
         lib.orc_build_exchange.argtypes = [C.c_void_p, dp, dp]
+        lib.orc_dlist.restype = C.c_int
+        lib.orc_dlist.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     _lib = lib
     return lib
 
@@ -51,6 +55,7 @@ class Oracle:
             raise RuntimeError(f"oracle could not read {rep_path}")
         self.n = self.lib.orc_size(self.r)
         self.nbf = self.lib.orc_nbf(self.r)
+        self.nshell = self.lib.orc_nshell(self.r)
 
     def close(self):
         if self.r:
@@ -79,6 +84,17 @@ class Oracle:
         J = np.empty_like(D)
         self.lib.orc_build_coulomb(self.r, _p(D), _p(J))
         return J
+
+    def dlist(self, D):
+        """Density screening list: (sidx, list) as the reference's dlist_sidx / dlist."""
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        ns = self.nshell
+        sidx = np.zeros(ns + 1, dtype=np.int32)
+        ip = C.POINTER(C.c_int)
+        n = self.lib.orc_dlist(self.r, _p(D), sidx.ctypes.data_as(ip), None)
+        lst = np.zeros(max(n, 1), dtype=np.int32)
+        self.lib.orc_dlist(self.r, _p(D), sidx.ctypes.data_as(ip), lst.ctypes.data_as(ip))
+        return sidx, lst[:n]
 
     def exchange(self, D):
         D = np.ascontiguousarray(D, dtype=np.float64)

@@ -99,6 +99,31 @@ def test_matvec_multi(reps, name, nvec):
         assert relerr(Y[k], o.matvec(X[k])) < 1e-13
 
 
+@pytest.mark.parametrize("nvec", [16, 17, 33])
+@pytest.mark.parametrize("name", ["cube", "chain_lrdiag", "big_rank", "wide_leaves", "tiny"])
+def test_matvec_sixteen_per_pass(reps, name, nvec):
+    """16 vectors per pass (FP64 MFMA tile products), with the rest in
+    narrower passes; both executors, FP64 and FP32-stored U/B."""
+    h, path = reps[name]
+    o = oracle_bind.Oracle(path)
+    X = np.random.default_rng(70 + nvec).standard_normal((nvec, h.n))
+    Yo = [o.matvec(X[k]) for k in range(nvec)]
+    try:
+        for ex in (0, 1):
+            h.set_executor(ex)
+            for bits in (64, 32):
+                h.set_precision(bits)
+                Y = h.matvec_multi(X)
+                for k in range(nvec):
+                    e = relerr(Y[k], Yo[k])
+                    assert e < (1e-13 if bits == 64 else 2e-6), (ex, bits, k, e)
+                if ex == 1:
+                    assert h.device_status() == 0
+    finally:
+        h.set_executor(0)
+        h.set_precision(64)
+
+
 @pytest.mark.parametrize("nvec", [1, 8])
 def test_matvec_multi_fp32_storage(reps, nvec):
     h, path = reps["cube"]

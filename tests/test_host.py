@@ -157,7 +157,7 @@ def test_device_plan_semantics_match_oracle(tmp_path, seed, n, leaf, mr, shape):
         assert relerr(y1, yo) < 1e-13
 
 
-@pytest.mark.parametrize("nv", [2, 4, 8])
+@pytest.mark.parametrize("nv", [2, 4, 8, 16])
 @pytest.mark.parametrize("seed,n,leaf,mr,shape", [(2, 1200, 150, 40, "chain"), (4, 1500, 40, 20, "cube"),
                                                   (6, 2000, 120, 600, "chain")])
 def test_device_plan_semantics_batched(tmp_path, nv, seed, n, leaf, mr, shape):
@@ -196,7 +196,7 @@ def test_queue_order_covers_every_dependency(tmp_path, seed, n, leaf, mr, shape)
             y, bad = h.queue_emulate(x, 64, workgroups=wg)
             assert bad == 0
             assert relerr(y, yo) < 1e-13
-    for nv in (4, 8):
+    for nv in (4, 8, 16):
         X = rng.standard_normal((nv, h.n))
         Y, bad = h.queue_emulate(X, 32, workgroups=256)
         assert bad == 0

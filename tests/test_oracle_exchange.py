@@ -102,3 +102,76 @@ def test_oracle_exchange_alk26_fixture(tmp_path):
     Kref = np.fromfile(os.path.join(GOLDEN, "K_alk26_ccpvdz.bin")).reshape(nbf, nbf)
     K = oracle_bind.Oracle(p).exchange(D)
     assert relerr(K, Kref) < 5e-6
+
+
+def np_dlist(D, bf):
+    """The reference's density screening rule (src/H2ERI_build_exchange.c:
+    774-838): block (N,P) is kept when some |D_np| > 1e-10."""
+    ns = len(bf) - 1
+    keep = np.zeros((ns, ns), bool)
+    for n in range(ns):
+        for p in range(ns):
+            keep[n, p] = np.abs(D[bf[n]:bf[n + 1], bf[p]:bf[p + 1]]).max() > 1e-10
+    return keep
+
+
+def screened_test_density(D, bf, seed):
+    """A symmetric density whose shell blocks sit just above or below the
+    screening threshold: 70% of the blocks scaled to a maximum of 9e-11
+    (dropped), the rest to 2e-9 (kept), plus one block exactly at 1e-10
+    (dropped: the rule is strict) and one at 1.0000001e-10 (kept)."""
+    rng = np.random.default_rng(seed)
+    ns = len(bf) - 1
+    out = np.zeros_like(D)
+    special = [(0, 1, 1e-10), (1, 2, 1.0000001e-10)]
+    for n in range(ns):
+        for p in range(n, ns):
+            blk = D[bf[n]:bf[n + 1], bf[p]:bf[p + 1]]
+            mx = np.abs(blk).max()
+            target = 9e-11 if rng.random() < 0.7 else 2e-9
+            for a, b, v in special:
+                if (n, p) == (a, b):
+                    target = v
+            scaled = blk * (target / mx) if mx > 0 else blk
+            out[bf[n]:bf[n + 1], bf[p]:bf[p + 1]] = scaled
+            out[bf[p]:bf[p + 1], bf[n]:bf[n + 1]] = scaled.T
+    return out
+
+
+def test_dlist_matches_restatement(ethane):
+    """Oracle and library density screening lists equal the numpy
+    restatement, on a damped density and on one with blocks around the
+    threshold (boundary cases included)."""
+    h, p, D = ethane
+    o = oracle_bind.Oracle(p)
+    bf = h.tree()["shell_bf_sidx"]
+    for Dx in (D, screened_test_density(D, bf, 3)):
+        keep = np_dlist(Dx, bf)
+        sidx_ref = np.concatenate([[0], np.cumsum(keep.sum(axis=1))])
+        list_ref = np.concatenate([np.nonzero(keep[n])[0] for n in range(len(bf) - 1)])
+        for sidx, lst in (o.dlist(Dx), h.exchange_dlist(Dx)):
+            assert np.array_equal(sidx, sidx_ref)
+            assert np.array_equal(lst, list_ref)
+    Dx = screened_test_density(D, bf, 3)
+    keep = np_dlist(Dx, bf)
+    assert not keep[0, 1] and keep[1, 2] and 0.2 < keep.mean() < 0.5
+
+
+def test_oracle_exchange_screens_density(ethane):
+    """K of a density with sub-threshold blocks equals the dense contraction
+    with those blocks zeroed, and differs from the unscreened contraction."""
+    h, p, D = ethane
+    o = oracle_bind.Oracle(p)
+    bf = h.tree()["shell_bf_sidx"]
+    Dx = screened_test_density(D, bf, 4)
+    keep = np_dlist(Dx, bf)
+    Ds = Dx.copy()
+    for n in range(len(bf) - 1):
+        for q in range(len(bf) - 1):
+            if not keep[n, q]:
+                Ds[bf[n]:bf[n + 1], bf[q]:bf[q + 1]] = 0.0
+    A = o.dense()
+    ra, rb, same = pair_rows(h.tree(), h.n)
+    K = o.exchange(Dx)
+    assert relerr(K, dense_exchange(A, Ds, ra, rb, same)) < 1e-13
+    assert relerr(K, dense_exchange(A, Dx, ra, rb, same)) > 1e-2
