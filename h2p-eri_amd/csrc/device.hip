@@ -39,6 +39,7 @@
 #include <climits>
 #include <cmath>
 #include <numeric>
+#include <mutex>
 #include <queue>
 #include <type_traits>
 
@@ -935,6 +936,12 @@ __global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ 
     }
 }
 
+// Clears n counters (every control word included).
+__global__ void counters_clear_kernel(int *ctr, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ctr[i] = 0;
+}
+
 // Clears the ticket and list counters before a queue launch (the error flag
 // stays until the host reads it).
 __global__ void queue_reset_kernel(int *ctr, int nlist) {
@@ -1152,6 +1159,12 @@ struct DevState {
     // allocates them and only release_graph frees them, so no eager call
     // that grows its own buffers can leave the graph pointing at freed memory
     double *g_x = nullptr, *g_y = nullptr, *g_Jh = nullptr;
+    // workspace of the pass being enqueued when it is not the plan's own
+    // (the allocation-free entry point puts it in the file-scope scratch
+    // array); null: the plan's Wv and counters
+    double *ws_Wv = nullptr;
+    int *ws_ctr = nullptr;
+    int *last_noalloc_ctr = nullptr;   // counters of the last allocation-free persistent pass
 };
 
 static DevState *dstate(H2ERI_p h) { return (DevState *)h->dev; }
@@ -2112,13 +2125,17 @@ static inline int stage_blocks(const Stage &st) {
     return st.t64_cnt + st.t32_cnt + st.r_cnt + (st.w64_cnt + 3) / 4 + (st.w32_cnt + 3) / 4;
 }
 
+// Workspace and counters of the pass being enqueued.
+static inline double *pass_Wv(const DevState *d, const DevPlan &dp) { return d->ws_Wv ? d->ws_Wv : dp.Wv; }
+static inline int *pass_ctr(const DevState *d, const DevPlan &dp) { return d->ws_ctr ? d->ws_ctr : dp.d_ctr; }
+
 template <int NV>
 static void launch_stage(const Stage &st, const DevPlan &dp, DevState *d, const double *X, double *Y, long long ld,
                          hipStream_t s) {
     hipLaunchKernelGGL(stage_kernel<NV>, dim3(stage_blocks(st)), dim3(WG), 0, s, dp.d_t64 + st.t64_beg, st.t64_cnt,
                        dp.d_t32 + st.t32_beg, st.t32_cnt, dp.d_rt + st.r_beg, st.r_cnt, dp.d_w64 + st.w64_beg,
                        st.w64_cnt, dp.d_w32 + st.w32_beg, st.w32_cnt, dp.d_vsrc, dp.d_rsrc, d->S64, d->S32,
-                       dp.Wv, X, Y, ld);
+                       pass_Wv(d, dp), X, Y, ld);
 }
 
 static void collect_phase_ms(H2ERI_p h, DevState *d);
@@ -2130,8 +2147,8 @@ static const int QUEUE_SPIN_LIMIT = 1 << 22;
 template <int NV>
 static void launch_queue(const DevPlan &dp, DevState *d, const double *X, double *Y, long long ld, hipStream_t s) {
     hipLaunchKernelGGL((queue_kernel<NV, false>), dim3(dp.q_grid), dim3(WG), 0, s, dp.d_q, dp.q_n,
-                       dp.d_need, dp.d_ctr, d->spin_limit, (const MolPtrs *)nullptr, dp.d_t64, dp.d_t32, dp.d_rt,
-                       dp.d_vsrc, dp.d_rsrc, d->S64, d->S32, dp.Wv, X, Y, ld);
+                       dp.d_need, pass_ctr(d, dp), d->spin_limit, (const MolPtrs *)nullptr, dp.d_t64, dp.d_t32, dp.d_rt,
+                       dp.d_vsrc, dp.d_rsrc, d->S64, d->S32, pass_Wv(d, dp), X, Y, ld);
 }
 
 // One pass of the persistent executor on stream s (reset + queue launch).
@@ -2140,7 +2157,7 @@ static bool run_queue(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, doub
     const long long ld = h->krnl_mat_size;
     const bool tim = d->phase_timing && !d->capturing;
     hipLaunchKernelGGL(queue_reset_kernel, dim3((std::max(dp.q_nlist, QC_LISTS) + 255) / 256), dim3(256), 0, s,
-                       dp.d_ctr, dp.q_nlist);
+                       pass_ctr(d, dp), dp.q_nlist);
     HIP_OK(hipGetLastError());
     if (tim) {   // the event pair brackets the persistent launch only
         collect_phase_ms(h, d);
@@ -2360,8 +2377,94 @@ static int queue_flags(DevState *d) {
             }
         }
     }
+    if (d->last_noalloc_ctr) {   // the last allocation-free persistent pass (counters in the scratch array)
+        int e = 0;
+        if (hipMemcpy(&e, d->last_noalloc_ctr + QC_ERROR, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error(ERR_DEVICE, "persistent executor: reading the error flag failed");
+            return -1;
+        }
+        d->last_noalloc_ctr = nullptr;
+        if (e) bad = 1;
+    }
     if (bad) set_error(ERR_DEVICE, "persistent executor: a dependency wait exceeded its bound; the results are invalid");
     return bad;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free matvec
+// ---------------------------------------------------------------------------
+// The persistent executor's counters (when that executor runs), then the
+// workspace of every intermediate vector of one single-vector pass.  Each GPU
+// the library runs on holds its own copy until the process ends; every
+// allocation-free call on a GPU shares it, so calls that overlap on one GPU
+// race.  Host code never passes `scratch` itself (its name is a host-side
+// stand-in): it passes the device address that hipGetSymbolAddress returns
+// for the current device.
+static const size_t NOALLOC_SCRATCH_BYTES = (size_t)128 << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+static std::mutex g_scratch_mu;
+static std::vector<double *> g_scratch_ptr;   // per device, filled once
+
+// Device address of the scratch array on device `dev`, which must be current.
+static double *noalloc_scratch(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev < 0) { set_error(ERR_DEVICE, "scratch: no current device"); return nullptr; }
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(ERR_DEVICE, "scratch: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (double *)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// One single-vector pass of a prepared handle with the workspace (and the
+// persistent executor's counters) in the scratch array; allocates nothing.
+static bool matvec_noalloc_impl(H2ERI_p h, const double *X, double *Y, hipStream_t s) {
+    DevState *d = h->dev ? dstate(h) : nullptr;
+    DevPlan *dp = d ? &d->plans[0] : nullptr;
+    if (!d || !d->st_ready || !dp->ready || !d->side || !d->ev_fork || !d->ev_join ||
+        (d->cfg[0].executor == 1 && !dp->q_ready)) {
+        set_error(ERR_ARG, "H2ERI_matvec_dev_noalloc: the handle is not prepared; call H2ERI_upload after the last "
+                           "change of precision, executor, tile size or schedule");
+        return false;
+    }
+    int dev = -1;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev != d->dev) { set_error(ERR_ARG, "H2ERI_matvec_dev_noalloc: the handle's device %d is not current", d->dev); return false; }
+    double *base = noalloc_scratch(dev);
+    if (!base) return false;
+    const bool queue = d->cfg[0].executor == 1;
+    const size_t nctr = queue ? (size_t)QC_LISTS + dp->q_nlist : 0;
+    const size_t ctr_bytes = (nctr * sizeof(int) + 255) & ~(size_t)255;
+    const size_t need = ctr_bytes + (size_t)dp->P.nW * sizeof(double);
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        set_error(ERR_ARG, "H2ERI_matvec_dev_noalloc: this representation needs %zu bytes of scratch, the maximum is %zu",
+                  need, NOALLOC_SCRATCH_BYTES);
+        return false;
+    }
+    int *ctr = (int *)base;
+    if (queue) {
+        // every counter, the error flag included, starts at zero (a kernel,
+        // so that the call can be captured in a graph)
+        hipLaunchKernelGGL(counters_clear_kernel, dim3((unsigned)((nctr + 255) / 256)), dim3(256), 0, s, ctr, (int)nctr);
+        HIP_OK(hipGetLastError());
+    }
+    const bool was_timing = d->phase_timing;   // timing would create events
+    d->phase_timing = false;
+    d->ws_Wv = (double *)((char *)base + ctr_bytes);
+    d->ws_ctr = queue ? ctr : nullptr;
+    const bool ok = run_matvec(h, d, *dp, X, Y, s);
+    d->ws_Wv = nullptr;
+    d->ws_ctr = nullptr;
+    d->phase_timing = was_timing;
+    if (queue) d->last_noalloc_ctr = ctr;
+    return ok;
 }
 
 }  // namespace h2e
@@ -2378,6 +2481,13 @@ void H2ERI_matvec(H2ERI_p h, const double *x, double *y) {
 void H2ERI_matvec_dev(H2ERI_p h, const double *x_dev, double *y_dev, void *stream) {
     if (!h || !x_dev || !y_dev) { set_error(ERR_ARG, "H2ERI_matvec_dev: null argument"); return; }
     matvec_multi_dev_impl(h, 1, x_dev, y_dev, (hipStream_t)stream);
+}
+
+// No counterpart in the reference: H2ERI_matvec_dev for callers that cannot
+// allocate (graph capture, tight serving loops).
+void H2ERI_matvec_dev_noalloc(H2ERI_p h, const double *x_dev, double *y_dev, void *stream) {
+    if (!h || !x_dev || !y_dev) { set_error(ERR_ARG, "H2ERI_matvec_dev_noalloc: null argument"); return; }
+    matvec_noalloc_impl(h, x_dev, y_dev, (hipStream_t)stream);
 }
 
 void H2ERI_matvec_multi(H2ERI_p h, const int nvec, const double *X, double *Y) {
@@ -2511,10 +2621,12 @@ void H2ERI_set_precision(H2ERI_p h, const int ub_bits) {
 size_t H2ERI_upload(H2ERI_p h) {
     if (!h || !get_plan(h, 1)) return 0;
     DevState *d = dstate(h);
-    // everything a single-vector pass needs besides its vectors
+    // everything a single-vector pass needs besides its vectors, so that the
+    // allocation-free entry point finds it in place
     if (!d->side && hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking) != hipSuccess) { set_error(ERR_DEVICE, "stream"); return 0; }
     if (!d->ev_fork && hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming) != hipSuccess) { set_error(ERR_DEVICE, "event"); return 0; }
     if (!d->ev_join && hipEventCreateWithFlags(&d->ev_join, hipEventDisableTiming) != hipSuccess) { set_error(ERR_DEVICE, "event"); return 0; }
+    if (!noalloc_scratch(d->dev)) return 0;
     return d->resident_bytes;
 }
 

@@ -246,6 +246,12 @@ class H2ERI:
         _check(self.lib)
         return rows, y
 
+    def matvec_dev_noalloc(self, x_ptr, y_ptr, stream=None):
+        """H2ERI_matvec_dev without any allocation (workspace in the
+        library's scratch array); the handle must be prepared by upload()."""
+        self.lib.H2ERI_matvec_dev_noalloc(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), C.c_void_p(stream or 0))
+        _check(self.lib)
+
     def set_spin_limit(self, polls):
         """Bound of the persistent executor's dependency waits, in polls."""
         self.lib.H2ERI_set_spin_limit(self.h, int(polls))

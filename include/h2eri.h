@@ -284,6 +284,20 @@ void H2ERI_build_Coulomb_dev(H2ERI_p h2eri, const double *den_dev, double *J_dev
 void H2ERI_build_Coulomb_multi_dev(H2ERI_p h2eri, const int ndens, const double *den_dev, double *J_dev,
                                    void *stream);
 
+/* No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): H2ERI_matvec_dev that allocates nothing.  The
+ * workspace of the pass (and the persistent executor's counters) lives in a
+ * file-scope device array of 128 MiB that each GPU the library runs on holds
+ * until the process ends; a representation whose single-vector workspace
+ * needs more is rejected with an error.  The handle must be prepared by
+ * H2ERI_upload after its last change of precision, executor, tile size or
+ * schedule (the call fails with an error otherwise), and its device must be
+ * current.  Every allocation-free call on one GPU shares that array: calls
+ * that overlap on one GPU race.  Results equal H2ERI_matvec_dev's bitwise;
+ * H2ERI_device_status reports a failed wait of the last such pass. */
+void H2ERI_matvec_dev_noalloc(H2ERI_p h2eri, const double *x_dev, double *y_dev, void *stream);
+
 /* HIP-graph form of the device hot path, for loops that repeat it on the
  * same buffers (SCF iterations).  kind 0 captures H2ERI_matvec_multi_dev
  * with n vectors, kind 1 H2ERI_build_Coulomb_multi_dev with n densities,
@@ -302,8 +316,8 @@ int H2ERI_graph_replay(H2ERI_p h2eri, void *stream);
 
 /* Upload the stored representation to the device now (otherwise done by the
  * first hot-path call) and prepare everything a single-vector pass needs
- * besides its vectors (plan, streams, events).  Returns bytes resident on
- * the device. */
+ * besides its vectors (plan, streams, events, the allocation-free entry
+ * point's scratch address).  Returns bytes resident on the device. */
 size_t H2ERI_upload(H2ERI_p h2eri);
 
 /* Versioned binary serialisation of a built representation (the
