@@ -298,6 +298,7 @@ def single_workload(args):
         tune = dict(ms=0.0, tile_bytes=tb, schedule=sc, executor=ex, wave_tile_bytes=wb)
         log(f"[bench] configuration given: {tune}")
     else:
+        h.s.print_dbginfo = 1   # the autotuner prints a line per configuration it times
         tune = h.autotune(nv=1, reps=10)
         log(f"[bench] autotune (FP32-stored, 1 density per pass): {tune}")
     kernel_name = "queue_kernel<1>" if tune["executor"] == 1 else "stage_kernel<1>"
@@ -305,6 +306,29 @@ def single_workload(args):
     def config_record(c):
         return {"executor": ["per stage", "persistent"][c["executor"]], "tile_bytes": int(c["tile_bytes"]),
                 "schedule": c["schedule"], "wave_tile_bytes": int(c["wave_tile_bytes"])}
+
+    if args.passes_only > 0:
+        # profiling runs: only the headline form, a fixed number of eager
+        # passes after two warm-ups, so its dispatches are easy to pick out
+        h.set_precision(32)
+        h.upload()
+        for _ in range(2):
+            h.matvec_dev(x_dev.data_ptr(), y_dev.data_ptr(), stream.cuda_stream)
+        torch.cuda.synchronize()
+        log(f"[bench] profiling: {args.passes_only} FP32-stored single-density passes follow")
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.passes_only):
+            h.matvec_dev(x_dev.data_ptr(), y_dev.data_ptr(), stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.passes_only
+        nb = h.matvec_bytes()
+        print(json.dumps({"workload": wname, "passes": args.passes_only, "ms_per_pass": round(ms, 5),
+                          "alg_MB": round((nb["U"] + nb["B"] + nb["D"] + nb["vec"]) / 1e6, 2),
+                          "tuned": config_record(tune)}), flush=True)
+        return
 
     res = {}
     for bits in (64, 32):
@@ -341,7 +365,21 @@ def single_workload(args):
             torch.cuda.synchronize()
             y1 = y_dev.cpu().numpy()
             dev_single = max(dev_single, float(np.linalg.norm(Yb[k] - y1) / np.linalg.norm(y1)))
+        # per-launch record of one eager batched call (the last pass of it)
+        h.set_phase_timing(True)
+        h.matvec_multi_dev(args.batch, X_dev.data_ptr(), Y_dev.data_ptr(), stream.cuda_stream)
+        torch.cuda.synchronize()
+        bprof = h.launch_stats()
+        h.set_phase_timing(False)
+        bphase = {}
+        for p in ("up", "B", "y1", "down", "D", "gather", "pass"):
+            q = [a for a in bprof if a["phase"] == p]
+            if q:
+                bphase[p] = {"ms_sum": round(sum(a["ms"] for a in q), 5), "launches": len(q),
+                             "matrix_GB": round(sum(a["mat_bytes"] for a in q) / 1e9, 4)}
+        bphase["union_ms"] = round(interval_union(bprof), 5)
         r["batch"] = {"densities": args.batch, "passes": widths, "ms": msb, "ms_per_density": msb / args.batch,
+                      "phases_last_pass": bphase,
                       "alg": algb, "GBs": algb / msb / 1e6, "config_first_pass": config_record(tuneb),
                       "max_rel_dev_vs_single": dev_single}
         log(f"[bench] FP{bits}-stored, {args.batch} densities: {msb:.4f} ms per replay, {msb / args.batch:.4f} ms/density, "
@@ -377,6 +415,7 @@ def single_workload(args):
             phases[p] = {"ms_sum": round(ms, 5), "launches": len(q), "matrix_GB": round(mb / 1e9, 4),
                          "GBs_matrix": round(mb / ms / 1e6, 1) if ms > 0 else None}
 
+    h.s.print_dbginfo = 0
     cpu = None
     eps_oracle = None
     if not args.no_cpu:
@@ -593,6 +632,8 @@ def main():
     ap.add_argument("--exchange", action="store_true", help="time the exchange build too (default with --config 4)")
     ap.add_argument("--tuned", default="", help="EXECUTOR,TILE_BYTES,SCHEDULE[,WAVE_TILE_BYTES] instead of running the "
                     "autotuner (profiling runs, where the profiler distorts the tuner's timings)")
+    ap.add_argument("--passes-only", type=int, default=0,
+                    help="profiling: run only this many eager FP32-stored single-density passes (needs --tuned)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MB first (HBM counter calibration)")
     ap.add_argument("--cpu-child", nargs=4, metavar=("REP", "X", "THREADS", "REPS"), help=argparse.SUPPRESS)

@@ -600,24 +600,36 @@ __device__ void bgemv_tile_valu(const BTask &task, const T *__restrict__ S, cons
 // out1 (C x 16) = M^T Y, every matrix element used 16 times per side.  Both
 // run on v_mfma_f64_16x16x4f64 (lane l holds A[l%16][l/16] and B[l/16][l%16];
 // output register r of lane l is C[l/16 + 4r][l%16]) with FP64 accumulation;
-// FP32-stored elements are widened to FP64 in registers.  C <= 64.
-//  * Row products: a wave takes 16 rows.  For each group of 16 columns, lane
-//    l loads columns 4(l/16) .. 4(l/16)+3 of row l%16 (one 16-byte load for
+// FP32-stored elements are widened to FP64 in registers.
+//
+// The inputs are read from memory as MFMA operands, not staged in LDS: an
+// element's 16 values are contiguous (the workspace is element-major, and at
+// 16 vectors the external input is an element-major copy made at the start
+// of the pass), so the 16 lanes that need them read one 128-byte run, and
+// the rows of a tile reuse them from the caches.  Without an LDS bound a tile
+// spans whole rows (row products) or whole columns (column products): every
+// output element of a block side is complete in one tile, where a tile that
+// stages its inputs in LDS leaves a partial sum of 16 values per output
+// element and column split for the consumers to read back.
+//  * Row products: a wave takes 64 rows (four blocks of 16) at a time and
+//    walks the columns in groups of 16.  For a group, lane l loads columns
+//    4(l/16) .. 4(l/16)+3 of row l%16 of each block (one 16-byte load for
 //    FP32, two for FP64): the A operands of four MFMAs j whose inner index
 //    runs over the columns 4k + j of the group (a permutation of the inner
-//    index, which leaves the sum alone).  B = the rows of X of the same
-//    columns, from LDS.
-//  * Column products, out1^T = Y^T M: lane l loads columns 4(l%16) ..
-//    4(l%16)+3 of row l/16 of a group of 4 rows: the B operands of four MFMAs
-//    j, MFMA j covering the 16 columns 4n + j of the tile.  A = those rows
-//    of Y, from LDS.  The waves take blocks of 16 rows in turn, and their
-//    column sums meet in LDS in a fixed order.
-// A tile with both outputs reads its rows in both lane layouts; the second
-// read of a block follows the first at once and comes from the caches.
-// Columns and rows past the tile are clamped to its last load position and
-// meet zero inputs (LDS staging padded with zeros); their outputs are not
-// stored.
+//    index, which leaves the sum alone).  The B operands (X at those
+//    columns) serve all four blocks.
+//  * Column products, out1^T = Y^T M, at most 256 columns: a span of 64
+//    columns belongs to one wave or is shared by 2 or 4 waves, each taking
+//    blocks of 16 rows in turn.  For a group of 4 rows, lane l loads columns
+//    4(l%16) .. 4(l%16)+3 of row l/16 of the span: the B operands of four
+//    MFMAs j, MFMA j covering the 16 columns 4n + j of the span; A = those
+//    rows of Y.  Waves sharing a span add their sums in LDS in a fixed order.
+// A tile with both outputs runs the row product, then the column product; the
+// second read of its rows comes from the caches.  Rows and columns past the
+// tile are clamped to its last load position and meet zero operands; their
+// outputs are not stored.
 typedef double d4v __attribute__((ext_vector_type(4)));
+static const int T16_MAXC = 256;   // columns of a tile with a column product (four spans of 64)
 
 // Four consecutive elements from c0 (clamped so no load leaves the row's
 // padded stride): FP32 one 16-byte load at min(c0, cmax), FP64 two at
@@ -632,78 +644,131 @@ __device__ __forceinline__ void load_quad(const double *row, int c0, int cmax, d
     o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
 }
 
+// A tile input at 16 vectors: vsrc[b..e) at element offset off.  Element i,
+// vector v is the external input's element-major copy, or the sum of the
+// listed workspace slots in list order.  The first slots' offsets are held
+// in registers (uniform across the workgroup).
+struct In16 {
+    const int *vsrc;
+    int b, e, off;
+    int s[4];
+    int ns;   // slots held in s (all of them when e - b <= 4), 0: external
+};
+__device__ __forceinline__ In16 in16_make(const int *__restrict__ vsrc, int b, int e, int off) {
+    In16 r;
+    r.vsrc = vsrc; r.b = b; r.e = e; r.off = off;
+    const int s0 = ufl(vsrc[b]);
+    r.ns = s0 < 0 ? 0 : min(e - b, 4);
+    #pragma unroll
+    for (int k = 0; k < 4; k++) r.s[k] = (k < r.ns) ? ufl(vsrc[b + k]) : s0;
+    return r;
+}
+template <bool COH>
+__device__ __forceinline__ double in16_get(const In16 &in, int i, int v, const double *__restrict__ Wv,
+                                           const double *__restrict__ Xt) {
+    if (in.ns == 0) return Xt[(long long)(-in.s[0] - 1 + in.off + i) * 16 + v];
+    double sum = wld<COH>(Wv + (long long)(in.s[0] + in.off + i) * 16 + v);
+    #pragma unroll
+    for (int k = 1; k < 4; k++)
+        if (k < in.ns) sum += wld<COH>(Wv + (long long)(in.s[k] + in.off + i) * 16 + v);
+    for (int j = in.b + 4; j < in.e; j++) sum += wld<COH>(Wv + (long long)(in.vsrc[j] + in.off + i) * 16 + v);
+    return sum;
+}
+
 template <typename T, bool COH>
 __device__ void mfma_tile16(const BTask &task, const T *__restrict__ S, const int *__restrict__ vsrc,
-                            double *__restrict__ Wv, const double *__restrict__ X, long long ldx, double *lds) {
+                            double *__restrict__ Wv, const double *__restrict__ Xt, double *lds) {
     constexpr int NV = 16;
     constexpr int VW = sizeof(T) == 4 ? 4 : 2;   // elements per vector load
     const BTask t = uniform(task);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int R = t.nrow, C = t.ncol;
-    const bool do0 = t.out0 >= 0, do1 = t.out1 >= 0;
-    double *xin = lds;                 // C x 16, padded to whole groups of 16 columns
-    double *yin = lds + XIN_DOUBLES;   // R x 16, padded to whole blocks of 16 rows
-    const int Cp = (C + 15) & ~15, Rp = (R + 15) & ~15;
-    if (do0) {
-        stage_vec<NV, COH, WG>(vsrc, t.v0b, t.v0e, t.v0off, C, Wv, X, ldx, xin, tid);
-        for (int q = C * NV + tid; q < Cp * NV; q += WG) xin[q] = 0.0;
-    }
-    if (do1) {
-        stage_vec<NV, COH, WG>(vsrc, t.v1b, t.v1e, t.v1off, R, Wv, X, ldx, yin, tid);
-        for (int q = R * NV + tid; q < Rp * NV; q += WG) yin[q] = 0.0;
-    }
-    __syncthreads();
     const T *base = S + t.mat;
     const int cmax = ((C - 1) / VW) * VW;
-    const int ng = Cp >> 4;
-    d4v acc1[4];
-    #pragma unroll
-    for (int j = 0; j < 4; j++) acc1[j] = d4v{0.0, 0.0, 0.0, 0.0};
-    for (int rb = 16 * w; rb < R; rb += 64) {
-        if (do0) {
-            const T *row = base + (long long)min(rb + l16, R - 1) * t.ld;
-            double a[4][4];
+    if (t.out0 >= 0) {
+        const In16 x = in16_make(vsrc, t.v0b, t.v0e, t.v0off);
+        const int ng = (C + 15) >> 4;   // groups of 16 columns
+        for (int rb0 = 64 * w; rb0 < R; rb0 += 256) {
+            const int nb = min(4, (R - rb0 + 15) >> 4);   // blocks of 16 rows in this step (wave-uniform)
+            const T *row[4];
             #pragma unroll
-            for (int g = 0; g < 4; g++)
-                if (g < ng) load_quad(row, 16 * g + 4 * lq, cmax, a[g]);
-            d4v acc0 = d4v{0.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < 4; k++) row[k] = base + (long long)min(rb0 + 16 * k + l16, R - 1) * t.ld;
+            d4v acc[4];
             #pragma unroll
-            for (int g = 0; g < 4; g++) {
-                if (g >= ng) break;
+            for (int k = 0; k < 4; k++) acc[k] = d4v{0.0, 0.0, 0.0, 0.0};
+            for (int g = 0; g < ng; g++) {
+                const int c0 = 16 * g + 4 * lq;
+                double a[4][4], bx[4];
                 #pragma unroll
-                for (int j = 0; j < 4; j++)
-                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[g][j], xin[(16 * g + 4 * lq + j) * NV + l16], acc0, 0, 0, 0);
+                for (int k = 0; k < 4; k++)
+                    if (k < nb) load_quad(row[k], c0, cmax, a[k]);
+                #pragma unroll
+                for (int j = 0; j < 4; j++) bx[j] = c0 + j < C ? in16_get<COH>(x, c0 + j, l16, Wv, Xt) : 0.0;
+                #pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (k >= nb) break;
+                    #pragma unroll
+                    for (int j = 0; j < 4; j++) acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[k][j], bx[j], acc[k], 0, 0, 0);
+                }
             }
             #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int rr = rb + lq + 4 * r;
-                if (rr < R) wst<COH>(Wv + (long long)(t.out0 + rr) * NV + l16, acc0[r]);
-            }
-        }
-        if (do1) {
-            #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int rr = rb + 4 * q + lq;
-                double b[4];
-                load_quad(base + (long long)min(rr, R - 1) * t.ld, 4 * l16, cmax, b);
-                const double y = yin[rr * NV + l16];   // rr < Rp: zero past the tile
+            for (int k = 0; k < 4; k++) {
+                if (k >= nb) break;
                 #pragma unroll
-                for (int j = 0; j < 4; j++) acc1[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(y, b[j], acc1[j], 0, 0, 0);
+                for (int r = 0; r < 4; r++) {
+                    const int rr = rb0 + 16 * k + lq + 4 * r;
+                    if (rr < R) wst<COH>(Wv + (long long)(t.out0 + rr) * NV + l16, acc[k][r]);
+                }
             }
         }
     }
-    if (do1) {
-        __syncthreads();   // the staged vectors are dead from here on
-        double *red = lds;   // 4 waves x 64 columns x 16 vectors
+    if (t.out1 >= 0) {
+        const In16 y = in16_make(vsrc, t.v1b, t.v1e, t.v1off);
+        const int nspan = (C + 63) >> 6;                          // spans of 64 columns (1..4)
+        const int wps = nspan == 1 ? 4 : (nspan == 2 ? 2 : 1);   // waves per span
+        const int span = w / wps, part = w - span * wps;
+        const int c0 = 64 * span;
+        d4v acc1[4];
         #pragma unroll
-        for (int j = 0; j < 4; j++)
+        for (int j = 0; j < 4; j++) acc1[j] = d4v{0.0, 0.0, 0.0, 0.0};
+        if (span < nspan) {
+            for (int rb = 16 * part; rb < R; rb += 16 * wps) {
+                #pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int rr = rb + 4 * q + lq;
+                    double b[4];
+                    load_quad(base + (long long)min(rr, R - 1) * t.ld, c0 + 4 * l16, cmax, b);
+                    const double ya = rr < R ? in16_get<COH>(y, rr, l16, Wv, Xt) : 0.0;
+                    #pragma unroll
+                    for (int j = 0; j < 4; j++) acc1[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ya, b[j], acc1[j], 0, 0, 0);
+                }
+            }
+        }
+        if (wps == 1) {   // each span summed by one wave: straight to the slot
+            if (span < nspan) {
+                #pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int col = c0 + 4 * l16 + j;
+                    if (col < C) {
+                        #pragma unroll
+                        for (int r = 0; r < 4; r++) wst<COH>(Wv + (long long)(t.out1 + col) * NV + lq + 4 * r, acc1[j][r]);
+                    }
+                }
+            }
+        } else {
+            double *red = lds;   // per wave: 64 columns x 16 vectors
             #pragma unroll
-            for (int r = 0; r < 4; r++) red[w * 1024 + (4 * l16 + j) * NV + lq + 4 * r] = acc1[j][r];
-        __syncthreads();
-        for (int q = tid; q < C * NV; q += WG) {
-            const double v = (red[q] + red[1024 + q]) + (red[2048 + q] + red[3072 + q]);
-            wst<COH>(Wv + (long long)t.out1 * NV + q, v);
+            for (int j = 0; j < 4; j++)
+                #pragma unroll
+                for (int r = 0; r < 4; r++) red[w * 1024 + (4 * l16 + j) * NV + lq + 4 * r] = acc1[j][r];
+            __syncthreads();
+            for (int q = tid; q < C * NV; q += WG) {
+                const int sp = q >> 10, loc = q & 1023;
+                const double *rs = red + sp * wps * 1024 + loc;
+                const double v = wps == 4 ? (rs[0] + rs[1024]) + (rs[2048] + rs[3072]) : rs[0] + rs[1024];
+                wst<COH>(Wv + (long long)t.out1 * NV + q, v);
+            }
         }
     }
 }
@@ -714,7 +779,9 @@ __device__ __forceinline__ void bgemv_tile(const BTask &task, const T *__restric
                                            double *__restrict__ Wv, const double *__restrict__ X, long long ldx,
                                            double *lds) {
     if constexpr (NV == 16) {
-        if constexpr (NT == WG) mfma_tile16<T, COH>(task, S, vsrc, Wv, X, ldx, lds);   // no wave tiles at 16
+        // X is the element-major copy of the input (the pass's first launch
+        // makes it); no wave tiles at 16
+        if constexpr (NT == WG) mfma_tile16<T, COH>(task, S, vsrc, Wv, X, lds);
     } else {
         bgemv_tile_valu<T, NV, COH, NT>(task, S, vsrc, Wv, X, ldx, lds);
     }
@@ -989,6 +1056,22 @@ __global__ void symmetrize_kernel(const double *__restrict__ Jh, double *__restr
     }
 }
 
+// Element-major copy of 16 vectors (Xt[i * 16 + v] = X[v * ld + i]), the
+// first launch of a 16-vector pass: both sides coalesced through a 16 x 256
+// LDS tile.
+__global__ void transpose16_kernel(const double *__restrict__ X, long long ld, double *__restrict__ Xt, int n) {
+    __shared__ double tile[16][257];
+    const int i0 = blockIdx.x * 256, tid = threadIdx.x;
+    #pragma unroll
+    for (int v = 0; v < 16; v++) tile[v][tid] = i0 + tid < n ? X[(long long)v * ld + i0 + tid] : 0.0;
+    __syncthreads();
+    #pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int q = k * 256 + tid, il = q >> 4, v = q & 15;
+        if (i0 + il < n) Xt[(long long)(i0 + il) * 16 + v] = tile[v][il];
+    }
+}
+
 __global__ void zero_kernel(double *__restrict__ p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -1110,6 +1193,8 @@ struct DevPlan {
     double q_est_us = 0;
     QTask *d_q = nullptr;
     int *d_need = nullptr, *d_ctr = nullptr;
+    // 16 vectors per pass: element-major copy of the pass's input (N x 16)
+    double *Xt = nullptr;
 };
 
 static const int NWIDTH = 5;   // batch widths: vectors per pass over the matrix data
@@ -1189,7 +1274,7 @@ static void release_plan(DevState *d, int i) {
     DevPlan &p = d->plans[i];
     void **ps[] = {(void **)&p.Wv, (void **)&p.d_t64, (void **)&p.d_t32, (void **)&p.d_w64, (void **)&p.d_w32,
                    (void **)&p.d_rt, (void **)&p.d_rsrc, (void **)&p.d_vsrc, (void **)&p.d_q, (void **)&p.d_need,
-                   (void **)&p.d_ctr};
+                   (void **)&p.d_ctr, (void **)&p.Xt};
     for (void **q : ps) free_dev(*q);
     p.ready = false;
     p.q_ready = false;
@@ -1325,10 +1410,47 @@ struct Planner {
         // a small part is one wave tile: one wave of a workgroup, a quarter of its LDS
         const bool wave = esz * nrow * ncol <= wave_bytes && ncol <= maxc &&
                           ncol * nv <= TileLDS<64>::X && (d1 < 0 || nrow * nv <= TileLDS<64>::Y);
+        if (nv == 16) { add_matrix16(st, m, r0_, nrow, v0, o0, v1, o1, d0, d1); return; }
         const int nct = wave ? 1 : (ncol + maxc - 1) / maxc;
         const int ctw = wave ? ncol : ((ncol + nct - 1) / nct + 7) / 8 * 8;   // multiple of 8 (vector alignment), <= maxc
         int rows = wave ? nrow : (int)std::max(1.0, tile_bytes / (esz * ctw));
         if (d1 >= 0 && !wave) rows = std::min(rows, YIN_DOUBLES / nv);
+        emit_tiles(st, m, r0_, nrow, nct, ctw, rows, wave, v0, o0, v1, o1, d0, d1);
+    }
+
+    // 16 vectors per pass: the MFMA tile reads its inputs from memory, so a
+    // tile spans whole rows (a row product leaves no partial sums over column
+    // tiles) or whole columns.  Row products: row strips of the full width,
+    // about tile_bytes each (at least 64 rows).  Column products: strips of
+    // at most 256 columns (four spans of 64), full height up to 16 tile_bytes.
+    // A block with both outputs is one tile when at most 256 columns wide;
+    // a wider one is a set of row strips plus a set of column strips (read
+    // twice, no partial sums).
+    void add_matrix16(int st, const MatRef &m, int r0_, int nrow, int v0, int o0, int v1, int o1, int d0, int d1) {
+        const int ncol = m.ncol;
+        const double esz = m.f32 ? 4.0 : 8.0;
+        if (d0 >= 0 && d1 >= 0 && ncol > T16_MAXC) {
+            add_matrix16(st, m, r0_, nrow, v0, o0, -1, 0, d0, -1);
+            add_matrix16(st, m, r0_, nrow, -1, 0, v1, o1, -1, d1);
+            return;
+        }
+        int nct = 1, ctw = ncol, rows;
+        if (d1 < 0) {
+            rows = std::max(64, (int)(tile_bytes / (esz * ncol)));
+        } else {
+            nct = (ncol + T16_MAXC - 1) / T16_MAXC;
+            ctw = std::min(T16_MAXC, ((ncol + nct - 1) / nct + 7) / 8 * 8);
+            rows = std::max(64, (int)(16.0 * tile_bytes / (esz * ctw)));
+        }
+        emit_tiles(st, m, r0_, nrow, nct, ctw, rows, false, v0, o0, v1, o1, d0, d1);
+    }
+
+    // Tiles of nct column strips of ctw columns (the last narrower) by row
+    // strips of about `rows` rows (balanced), with their output slots.
+    void emit_tiles(int st, const MatRef &m, int r0_, int nrow, int nct, int ctw, int rows, bool wave, int v0, int o0,
+                    int v1, int o1, int d0, int d1) {
+        const int ncol = m.ncol;
+        const double esz = m.f32 ? 4.0 : 8.0;
         const int nrt = (nrow + rows - 1) / rows;
         rows = (nrow + nrt - 1) / nrt;
         std::vector<long long> s0(nct, -1), s1(nrt, -1);
@@ -1695,15 +1817,18 @@ struct Planner {
             const long long nstore = f32 ? (long long)S.h32.size() : (long long)S.h64.size();
             const int W = !f32 ? 2 : 4;   // widest vector load of any tile path (FP32: float4)
             for (const BTask &t : tl) {
-                bool ok = t.nrow > 0 && t.ncol > 0 && t.ncol <= maxc && t.ncol * nv <= xcap && t.ld >= t.ncol &&
-                          t.ld % W == 0 && t.mat % W == 0 && t.mat >= 0 && (t.lpr & (t.lpr - 1)) == 0 && t.lpr >= 4 &&
-                          t.lpr <= 64 && t.lpr * cpl_of(nv) >= t.ncol && (t.out0 >= 0 || t.out1 >= 0);
+                bool ok = t.nrow > 0 && t.ncol > 0 && t.ld >= t.ncol && t.ld % W == 0 && t.mat % W == 0 && t.mat >= 0 &&
+                          (t.out0 >= 0 || t.out1 >= 0);
+                if (nv == 16) {   // the MFMA tile: four spans of 64 columns for a column product
+                    ok = ok && (t.out1 < 0 || t.ncol <= T16_MAXC);
+                } else {
+                    ok = ok && t.ncol <= maxc && t.ncol * nv <= xcap && (t.lpr & (t.lpr - 1)) == 0 && t.lpr >= 4 &&
+                         t.lpr <= 64 && t.lpr * cpl_of(nv) >= t.ncol && (t.out1 < 0 || t.nrow * nv <= ycap);
+                }
                 const long long last = t.mat + (long long)(t.nrow - 1) * t.ld + ((t.ncol - 1) / W) * W + W;
                 ok = ok && last <= nstore;
                 if (t.out0 >= 0) ok = ok && src_ok(t.v0b, t.v0e, t.v0off, t.ncol) && t.out0 + (long long)t.nrow <= P.nW;
-                if (t.out1 >= 0)
-                    ok = ok && t.nrow * nv <= ycap && src_ok(t.v1b, t.v1e, t.v1off, t.nrow) &&
-                         t.out1 + (long long)t.ncol <= P.nW;
+                if (t.out1 >= 0) ok = ok && src_ok(t.v1b, t.v1e, t.v1off, t.nrow) && t.out1 + (long long)t.ncol <= P.nW;
                 if (!ok) {
                     set_error(ERR_ARG,
                               "device plan check failed (nv %d): tile mat=%lld %dx%d ld=%d lpr=%d in0 [%d,%d)+%d "
@@ -2115,6 +2240,7 @@ static DevPlan *get_plan(H2ERI_p h, int nv) {
     if (!alloc_copy((void **)&dp.d_rt, P.rt.empty() ? nullptr : P.rt.data(), P.rt.size() * sizeof(RTask))) return nullptr;
     if (!alloc_copy((void **)&dp.d_rsrc, P.rsrc.empty() ? nullptr : P.rsrc.data(), P.rsrc.size() * sizeof(int))) return nullptr;
     if (!alloc_copy((void **)&dp.d_vsrc, P.vsrc.empty() ? nullptr : P.vsrc.data(), P.vsrc.size() * sizeof(int))) return nullptr;
+    if (nv == 16 && !alloc_copy((void **)&dp.Xt, nullptr, (size_t)h->krnl_mat_size * 16 * sizeof(double))) return nullptr;
     dp.ready = true;
     if (c.executor == 1 && !ensure_queue(d, dp)) return nullptr;
     return &dp;
@@ -2135,7 +2261,7 @@ static void launch_stage(const Stage &st, const DevPlan &dp, DevState *d, const 
     hipLaunchKernelGGL(stage_kernel<NV>, dim3(stage_blocks(st)), dim3(WG), 0, s, dp.d_t64 + st.t64_beg, st.t64_cnt,
                        dp.d_t32 + st.t32_beg, st.t32_cnt, dp.d_rt + st.r_beg, st.r_cnt, dp.d_w64 + st.w64_beg,
                        st.w64_cnt, dp.d_w32 + st.w32_beg, st.w32_cnt, dp.d_vsrc, dp.d_rsrc, d->S64, d->S32,
-                       pass_Wv(d, dp), X, Y, ld);
+                       pass_Wv(d, dp), NV == 16 ? dp.Xt : X, Y, ld);
 }
 
 static void collect_phase_ms(H2ERI_p h, DevState *d);
@@ -2148,7 +2274,7 @@ template <int NV>
 static void launch_queue(const DevPlan &dp, DevState *d, const double *X, double *Y, long long ld, hipStream_t s) {
     hipLaunchKernelGGL((queue_kernel<NV, false>), dim3(dp.q_grid), dim3(WG), 0, s, dp.d_q, dp.q_n,
                        dp.d_need, pass_ctr(d, dp), d->spin_limit, (const MolPtrs *)nullptr, dp.d_t64, dp.d_t32, dp.d_rt,
-                       dp.d_vsrc, dp.d_rsrc, d->S64, d->S32, pass_Wv(d, dp), X, Y, ld);
+                       dp.d_vsrc, dp.d_rsrc, d->S64, d->S32, pass_Wv(d, dp), NV == 16 ? dp.Xt : X, Y, ld);
 }
 
 // One pass of the persistent executor on stream s (reset + queue launch).
@@ -2156,6 +2282,10 @@ static bool run_queue(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, doub
     if (!dp.q_ready) { set_error(ERR_DEVICE, "persistent executor: queue not built"); return false; }
     const long long ld = h->krnl_mat_size;
     const bool tim = d->phase_timing && !d->capturing;
+    if (dp.P.nv == 16) {
+        hipLaunchKernelGGL(transpose16_kernel, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, s, X, ld, dp.Xt, (int)ld);
+        HIP_OK(hipGetLastError());
+    }
     hipLaunchKernelGGL(queue_reset_kernel, dim3((std::max(dp.q_nlist, QC_LISTS) + 255) / 256), dim3(256), 0, s,
                        pass_ctr(d, dp), dp.q_nlist);
     HIP_OK(hipGetLastError());
@@ -2194,6 +2324,10 @@ static bool run_matvec(H2ERI_p h, DevState *d, DevPlan &dp, const double *X, dou
     const bool tim = d->phase_timing && !d->capturing;
     bool has_side = false;
     for (const Stage &st : P.stages) has_side |= st.stream == 1;
+    if (P.nv == 16) {   // element-major copy of the input, before the fork
+        hipLaunchKernelGGL(transpose16_kernel, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, s, X, ld, dp.Xt, (int)ld);
+        HIP_OK(hipGetLastError());
+    }
     if (tim) {
         collect_phase_ms(h, d);   // a previous pass of the same call goes into the timers first
         if (!ensure_events(d, 2 * P.stages.size() + 2)) return false;
@@ -2798,6 +2932,11 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
             ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
             if (!ok) break;
             if (ms / reps < best) { best = ms / reps; best_cfg = c; }
+            if (h->print_dbginfo) {
+                fprintf(stderr, "[H2ERI_autotune] nv %d: executor %d, tile %.0f KB, schedule %d, wave tiles %.0f KB: %.4f ms\n",
+                        nv, ex, tb / 1024.0, sc, c.wave_bytes / 1024.0, ms / reps);
+                fflush(stderr);
+            }
         }
         if (!ok) break;
     }
@@ -3187,6 +3326,48 @@ int H2ERI_queue_emulate(H2ERI_p h, const int ub_bits, const int nv, const int wo
 // Builds the device plans of every batch width on the host only and runs
 // their bounds checks; no HIP call is made.  Returns the number of tile
 // tasks of the single-vector plan, or -1 on failure.
+// Work figures of the nv-vector plan with the current configuration, built
+// on the host only (no HIP call): out8 = {tile tasks, reduce tasks, stages,
+// workspace elements per vector, matrix bytes streamed per pass, slot
+// elements written by tiles per vector, slot elements read by tiles and
+// reduce tasks per vector, external output elements}.  0, -1 on failure.
+int H2ERI_plan_stats(H2ERI_p h, const int ub_bits, const int nv, double *out8) {
+    if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !out8 || !nv_valid(nv)) { set_error(ERR_ARG, "plan_stats: bad argument"); return -1; }
+    Storage S;
+    S.build(h, ub_bits == 32);
+    DevState *ds = dstate_get(h);
+    const Tune &c = ds->cfg[nv_index(nv)];
+    Plan P;
+    if (!build_plan(h, S, P, nv, c.tile_bytes, c.nf_schedule, c.executor, c.wave_bytes)) return -1;
+    double mat = 0, wr = 0, rd = 0, rd_r = 0, wr_r = 0;
+    for (const Stage &st : P.stages) mat += st.mat_bytes;
+    for (const auto *L : {&P.t64, &P.t32, &P.w64, &P.w32})
+        for (const BTask &t : *L) {
+            if (t.out0 >= 0) { wr += t.nrow; if (P.vsrc[t.v0b] >= 0) rd += (double)t.ncol * (t.v0e - t.v0b); }
+            if (t.out1 >= 0) { wr += t.ncol; if (P.vsrc[t.v1b] >= 0) rd += (double)t.nrow * (t.v1e - t.v1b); }
+        }
+    double ext = 0;
+    for (const RTask &r : P.rt) {
+        rd_r += (double)r.len * (r.src_end - r.src_beg);
+        if (r.flags & 1) ext += r.len;
+        else wr_r += r.len;
+    }
+    if (getenv("H2ERI_PLAN_STATS_DETAIL"))
+        fprintf(stderr, "plan nv %d: tiles write %.3g read %.3g, reduce tasks write %.3g read %.3g elements per vector\n",
+                nv, wr, rd, wr_r, rd_r);
+    wr += wr_r;
+    rd += rd_r;
+    out8[0] = (double)(P.t64.size() + P.t32.size() + P.w64.size() + P.w32.size());
+    out8[1] = (double)P.rt.size();
+    out8[2] = (double)P.stages.size();
+    out8[3] = (double)P.nW;
+    out8[4] = mat;
+    out8[5] = wr;
+    out8[6] = rd;
+    out8[7] = ext;
+    return 0;
+}
+
 int H2ERI_plan_check(H2ERI_p h, const int ub_bits) {
     if (!h || !h->U || !h->c_B_blks || !h->c_D_blks) { set_error(ERR_ARG, "plan_check: nothing built"); return -1; }
     Storage S;

@@ -252,6 +252,14 @@ class H2ERI:
         self.lib.H2ERI_matvec_dev_noalloc(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), C.c_void_p(stream or 0))
         _check(self.lib)
 
+    def plan_stats(self, bits=64, nv=1):
+        """Work figures of the device plan (host only)."""
+        out = np.zeros(8)
+        self.lib.H2ERI_plan_stats(self.h, bits, nv, _dptr(out))
+        _check(self.lib)
+        keys = ["tiles", "reduces", "stages", "nW", "mat_bytes", "ws_written", "ws_read", "out_elems"]
+        return dict(zip(keys, out.tolist()))
+
     def set_spin_limit(self, polls):
         """Bound of the persistent executor's dependency waits, in polls."""
         self.lib.H2ERI_set_spin_limit(self.h, int(polls))

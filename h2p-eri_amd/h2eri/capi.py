@@ -119,6 +119,7 @@ SIGNATURES = {
     "H2ERI_get_executor": (C.c_int, [H2ERI_p]),
     "H2ERI_device_status": (C.c_int, [H2ERI_p]),
     "H2ERI_set_spin_limit": (None, [H2ERI_p, C.c_int]),
+    "H2ERI_plan_stats": (C.c_int, [H2ERI_p, C.c_int, C.c_int, c_dbl_p]),
     "H2ERI_exchange_dlist": (None, [H2ERI_p, c_dbl_p]),
     "H2ERI_matvec_dev_noalloc": (None, [H2ERI_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "H2ERI_direct_matvec_rows": (None, [H2ERI_p, C.c_int, C.POINTER(C.c_int), c_dbl_p, c_dbl_p]),

@@ -448,6 +448,13 @@ void H2ERI_set_wave_tile_bytes(H2ERI_p h2eri, const double bytes);
  * number of tile tasks, -1 on failure. */
 int H2ERI_plan_check(H2ERI_p h2eri, const int ub_bits);
 
+/* Work figures of the nv-vector plan with the current configuration, built
+ * on the host only (no HIP call): out8 = {tile tasks, reduce tasks,
+ * stages, workspace elements per vector, matrix bytes streamed per pass,
+ * workspace elements written per vector, workspace elements read per
+ * vector, output elements}.  Returns 0, -1 on failure. */
+int H2ERI_plan_stats(H2ERI_p h2eri, const int ub_bits, const int nv, double *out8);
+
 /* Host verification of the device plan (no HIP call, tests only): executes
  * every launch of the plan for nv (1, 2, 4, 8 or 16) vectors X (nv x N,
  * vector-major) with the kernels' semantics into Y and returns the number of

@@ -11,14 +11,15 @@ sizes (h2eri.molecules), saved, and read by the oracle:
 * the batching workload on that representation: 16 seeded densities (nocc
   401) in one batched FP32-stored call, each equal to its single-density
   call <= 1e-13 and two of them against the oracle's FP64 J within the
-  precision gate; and the eight alkanes C20..C160 (reltol 1e-6) as one
-  molecule group, each against its oracle matvec <= 1e-13;
+  precision gate; and the eight alkanes C20..C160 (reltol 1e-6,
+  FP32-stored) as one molecule group, each against its oracle FP64 matvec
+  within the precision gate;
 * (H2O)64, reltol 1e-6: J in FP64 against the oracle <= 1e-13, FP32-stored
   J within 1e-7, and K (nocc 320) against the oracle <= 1e-12.
 
-These take minutes (host construction of the five representations, oracle
-products on the host cores); the small analogues in the other files cover
-the same code paths in seconds.
+Together these take minutes (host construction of the representations,
+oracle products on the host cores), each test well under two; the small
+analogues in the other files cover the same code paths in seconds.
 """
 import time
 
@@ -132,41 +133,57 @@ def test_c100h202_sixteen_densities(c100):
     o.close()
 
 
-def test_alkane_group_c20_to_c160(tmp_path_factory):
+ALKANES = list(range(20, 161, 20))
+_group = {}   # C count -> (handle, representation path, x): built one test at a time
+
+
+@pytest.mark.parametrize("n", ALKANES)
+def test_alkane_group_member(tmp_path_factory, n):
+    """Builds one molecule of the batching workload's group (one test each,
+    so that no single test runs for minutes) and checks its FP32-stored
+    matvec against the oracle's FP64 matvec within the precision gate."""
+    from h2eri import molecules as M
+    rng = M.rng_for(300 + n)
+    atoms = M.alkane(n, rng)
+    h, p = _build(tmp_path_factory, f"C{n}", atoms, 1e-6)
+    h.set_precision(32)
+    x = np.random.default_rng(n).standard_normal(h.n)
+    o = oracle_bind.Oracle(p)
+    e = relerr(h.matvec(x), o.matvec(x))
+    o.close()
+    assert e < EPS_PREC
+    _group[n] = (h, p, x)
+
+
+def test_alkane_group_c20_to_c160():
     """The eight alkanes C20H42..C160H322 (reltol 1e-6, FP32-stored) as one
     molecule group, one persistent launch; each molecule's y against its
     oracle FP64 matvec within the precision gate, and bitwise equal to the
     FP32-stored persistent matvec of the molecule alone."""
     import torch
     import h2eri
-    from h2eri import molecules as M
-    hs, paths, xs = [], [], []
-    for n in range(20, 161, 20):
-        rng = M.rng_for(300 + n)
-        atoms = M.alkane(n, rng)
-        h, p = _build(tmp_path_factory, f"C{n}", atoms, 1e-6)
-        h.set_precision(32)
-        hs.append(h)
-        paths.append(p)
-        xs.append(np.random.default_rng(n).standard_normal(h.n))
-    X = [torch.from_numpy(x).to("cuda:0") for x in xs]
+    assert sorted(_group) == ALKANES, "the member tests build the molecules"
+    hs = [_group[n][0] for n in ALKANES]
+    X = [torch.from_numpy(_group[n][2]).to("cuda:0") for n in ALKANES]
     Y = [torch.zeros_like(x) for x in X]
     g = h2eri.Group(hs, [x.data_ptr() for x in X], [y.data_ptr() for y in Y])
     try:
         g.matvec(torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         assert g.status() == 0
-        for k, n in enumerate(range(20, 161, 20)):
-            o = oracle_bind.Oracle(paths[k])
-            e = relerr(Y[k].cpu().numpy(), o.matvec(xs[k]))
+        for k, n in enumerate(ALKANES):
+            h, p, x = _group[n]
+            o = oracle_bind.Oracle(p)
+            e = relerr(Y[k].cpu().numpy(), o.matvec(x))
             o.close()
             _log(f"C{n}: group (FP32-stored) vs oracle {e:.2e}")
             assert e < EPS_PREC
-            hs[k].set_executor(1)
-            assert np.array_equal(Y[k].cpu().numpy(), hs[k].matvec(xs[k]))
-            hs[k].set_executor(0)
+            h.set_executor(1)
+            assert np.array_equal(Y[k].cpu().numpy(), h.matvec(x))
+            h.set_executor(0)
     finally:
         g.close()
+        _group.clear()
 
 
 def test_water64_j_and_k(water64):
