@@ -291,11 +291,13 @@ def single_workload(args):
         vals = [int(v) for v in args.tuned.split(",")]
         ex, tb, sc = vals[:3]
         wb = vals[3] if len(vals) > 3 else 32768
+        ng = vals[4] if len(vals) > 4 else 0
         h.set_executor(ex)
         h.set_tile_bytes(tb)
         h.set_schedule(sc)
         h.set_wave_tile_bytes(wb)
-        tune = dict(ms=0.0, tile_bytes=tb, schedule=sc, executor=ex, wave_tile_bytes=wb)
+        h.set_near_field_grid(ng)
+        tune = dict(ms=0.0, tile_bytes=tb, schedule=sc, executor=ex, wave_tile_bytes=wb, near_field_grid=ng)
         log(f"[bench] configuration given: {tune}")
     else:
         h.s.print_dbginfo = 1   # the autotuner prints a line per configuration it times
@@ -305,22 +307,31 @@ def single_workload(args):
 
     def config_record(c):
         return {"executor": ["per stage", "persistent"][c["executor"]], "tile_bytes": int(c["tile_bytes"]),
-                "schedule": c["schedule"], "wave_tile_bytes": int(c["wave_tile_bytes"])}
+                "schedule": c["schedule"], "wave_tile_bytes": int(c["wave_tile_bytes"]),
+                "near_field_grid": int(c.get("near_field_grid", 0))}
 
     if args.passes_only > 0:
-        # profiling runs: only the headline form, a fixed number of eager
-        # passes after two warm-ups, so its dispatches are easy to pick out
+        # profiling runs: only the headline form (or the batch of
+        # --passes-width densities), a fixed number of eager passes after two
+        # warm-ups, so its dispatches are easy to pick out
         h.set_precision(32)
         h.upload()
+        pw = args.passes_width
+
+        def one_pass():
+            if pw == 1:
+                h.matvec_dev(x_dev.data_ptr(), y_dev.data_ptr(), stream.cuda_stream)
+            else:
+                h.matvec_multi_dev(pw, X_dev.data_ptr(), Y_dev.data_ptr(), stream.cuda_stream)
         for _ in range(2):
-            h.matvec_dev(x_dev.data_ptr(), y_dev.data_ptr(), stream.cuda_stream)
+            one_pass()
         torch.cuda.synchronize()
-        log(f"[bench] profiling: {args.passes_only} FP32-stored single-density passes follow")
+        log(f"[bench] profiling: {args.passes_only} FP32-stored passes of {pw} densities follow")
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(args.passes_only):
-            h.matvec_dev(x_dev.data_ptr(), y_dev.data_ptr(), stream.cuda_stream)
+            one_pass()
         e1.record(stream)
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / args.passes_only
@@ -389,6 +400,7 @@ def single_workload(args):
         h.set_tile_bytes(tune["tile_bytes"])
         h.set_schedule(tune["schedule"])
         h.set_wave_tile_bytes(tune["wave_tile_bytes"])
+        h.set_near_field_grid(tune["near_field_grid"])
         res[bits] = r
     r64, r32 = res[64], res[32]
     eps_prec = float(np.linalg.norm(r32["J"] - r64["J"]) / np.linalg.norm(r64["J"]))
@@ -630,10 +642,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-one-core", action="store_true")
     ap.add_argument("--exchange", action="store_true", help="time the exchange build too (default with --config 4)")
-    ap.add_argument("--tuned", default="", help="EXECUTOR,TILE_BYTES,SCHEDULE[,WAVE_TILE_BYTES] instead of running the "
+    ap.add_argument("--tuned", default="", help="EXECUTOR,TILE_BYTES,SCHEDULE[,WAVE_TILE_BYTES[,NEAR_FIELD_GRID]] instead of running the "
                     "autotuner (profiling runs, where the profiler distorts the tuner's timings)")
     ap.add_argument("--passes-only", type=int, default=0,
                     help="profiling: run only this many eager FP32-stored single-density passes (needs --tuned)")
+    ap.add_argument("--passes-width", type=int, default=1, choices=[1, 16],
+                    help="profiling: densities per pass of the --passes-only passes")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MB first (HBM counter calibration)")
     ap.add_argument("--cpu-child", nargs=4, metavar=("REP", "X", "THREADS", "REPS"), help=argparse.SUPPRESS)

@@ -165,6 +165,11 @@ __device__ __forceinline__ void stage_vec(const int *__restrict__ vsrc, int b, i
                                           const double *__restrict__ Wv, const double *__restrict__ X, long long ldx,
                                           double *dst, int tid) {
     const int s0 = vsrc[b];
+    if (s0 < 0 && ldx < 0) {   // the external input as an element-major copy (NV values per element)
+        const double *p = X + (long long)(-s0 - 1 + off) * NV;
+        for (int q = tid; q < n * NV; q += NT) dst[XS == NV ? q : (q / NV) * XS + q % NV] = p[q];
+        return;
+    }
     if (s0 < 0) {
         const double *p = X + (long long)(-s0 - 1 + off);
         for (int i = tid; i < n; i += NT) {
@@ -602,34 +607,39 @@ __device__ void bgemv_tile_valu(const BTask &task, const T *__restrict__ S, cons
 // output register r of lane l is C[l/16 + 4r][l%16]) with FP64 accumulation;
 // FP32-stored elements are widened to FP64 in registers.
 //
-// The inputs are read from memory as MFMA operands, not staged in LDS: an
-// element's 16 values are contiguous (the workspace is element-major, and at
-// 16 vectors the external input is an element-major copy made at the start
-// of the pass), so the 16 lanes that need them read one 128-byte run, and
-// the rows of a tile reuse them from the caches.  Without an LDS bound a tile
-// spans whole rows (row products) or whole columns (column products): every
-// output element of a block side is complete in one tile, where a tile that
-// stages its inputs in LDS leaves a partial sum of 16 values per output
-// element and column split for the consumers to read back.
-//  * Row products: a wave takes 64 rows (four blocks of 16) at a time and
+// Inputs are staged in LDS (an element's 16 values contiguous, the slots of
+// a virtual vector summed once) when X and Y fit beside each other in the
+// 4096 doubles of a workgroup.  A row product spans the whole width of its
+// block, so that no partial sums over column tiles are left (each would be
+// 16 values per output row for the consumers to read back); when its X does
+// not fit the LDS it reads the operands from memory: the workspace is
+// element-major, and at 16 vectors the pass's first launch makes an
+// element-major copy of the external input, so the 16 lanes that need an
+// element read one 128-byte run, and the tile's rows reuse it from the
+// caches.
+//  * Row products: a wave takes up to four blocks of 16 rows at a time (the
+//    waves' blocks interleave, so a tile of 64 rows keeps all four busy) and
 //    walks the columns in groups of 16.  For a group, lane l loads columns
 //    4(l/16) .. 4(l/16)+3 of row l%16 of each block (one 16-byte load for
 //    FP32, two for FP64): the A operands of four MFMAs j whose inner index
 //    runs over the columns 4k + j of the group (a permutation of the inner
 //    index, which leaves the sum alone).  The B operands (X at those
-//    columns) serve all four blocks.
-//  * Column products, out1^T = Y^T M, at most 256 columns: a span of 64
-//    columns belongs to one wave or is shared by 2 or 4 waves, each taking
-//    blocks of 16 rows in turn.  For a group of 4 rows, lane l loads columns
+//    columns) serve all of the wave's blocks.
+//  * Column products, out1^T = Y^T M, at most 128 columns: a span of 64
+//    columns is shared by 2 or 4 waves, each taking blocks of 16 rows in
+//    turn.  For a group of 4 rows, lane l loads columns
 //    4(l%16) .. 4(l%16)+3 of row l/16 of the span: the B operands of four
 //    MFMAs j, MFMA j covering the 16 columns 4n + j of the span; A = those
 //    rows of Y.  Waves sharing a span add their sums in LDS in a fixed order.
-// A tile with both outputs runs the row product, then the column product; the
-// second read of its rows comes from the caches.  Rows and columns past the
-// tile are clamped to its last load position and meet zero operands; their
-// outputs are not stored.
+//  * A tile with both outputs (mfma_tile16_2s) reads its rows once: the two
+//    products need the matrix in transposed register layouts, and a wave
+//    moves each 16 x 16 block from one to the other through 2 KB of LDS of
+//    its own.
+// Rows and columns past the tile are clamped to its last load position and
+// meet zero operands; their outputs are not stored.
 typedef double d4v __attribute__((ext_vector_type(4)));
-static const int T16_MAXC = 256;   // columns of a tile with a column product (four spans of 64)
+static const int T16_MAXC = 128;   // columns of a tile with a column product (two spans of 64)
+static const int T16_LDS = LDS_DOUBLES;   // staging doubles of a 16-vector tile (X and Y)
 
 // Four consecutive elements from c0 (clamped so no load leaves the row's
 // padded stride): FP32 one 16-byte load at min(c0, cmax), FP64 two at
@@ -675,6 +685,89 @@ __device__ __forceinline__ double in16_get(const In16 &in, int i, int v, const d
     return sum;
 }
 
+// A tile with both products (at most 128 columns, any number of rows), in one
+// walk over its rows.  Waves take blocks of 16 rows (block b of the tile on
+// wave b % 4).  For a group of 16 columns, lane l loads columns
+// 4(l/16) .. 4(l/16)+3 of row l%16 of the block: the row product's A
+// operands (X staged in LDS supplies B, as in the row product above).  The
+// lane writes them to the wave's 16 x 16 LDS tile and reads back row
+// 4q + l/16, column l%16: the column product's B operand of MFMA q (rows
+// 4q .. 4q+3 of the block), whose A operand is Y at those rows, read from
+// memory once per block (element-major: 16 lanes read one 128-byte run).
+// Each wave keeps its column sums in registers, one accumulator per group of
+// 16 columns, until the end of the tile; then the four waves add them in LDS
+// in a fixed order, 64 columns at a time.
+template <typename T, bool COH>
+__device__ void mfma_tile16_2s(const BTask &t, const T *__restrict__ base, const int *__restrict__ vsrc,
+                               double *__restrict__ Wv, const double *__restrict__ Xt, double *lds) {
+    constexpr int NV = 16;
+    constexpr int VW = sizeof(T) == 4 ? 4 : 2;   // elements per vector load
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int R = t.nrow, C = t.ncol;
+    const int cmax = ((C - 1) / VW) * VW;
+    const int Cp = (C + 15) & ~15, ng = Cp >> 4;   // groups of 16 columns, at most 8
+    double *xin = lds;                                // Cp x 16, zero past C
+    double *tr = lds + T16_MAXC * NV + 256 * w;       // this wave's 16 x 16 transposition tile
+    stage_vec<NV, COH, WG>(vsrc, t.v0b, t.v0e, t.v0off, C, Wv, Xt, -1, xin, tid);
+    for (int q = C * NV + tid; q < Cp * NV; q += WG) xin[q] = 0.0;
+    __syncthreads();
+    const In16 y = in16_make(vsrc, t.v1b, t.v1e, t.v1off);
+    d4v acc1[8];
+    #pragma unroll
+    for (int g = 0; g < 8; g++) acc1[g] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int rb = 16 * w; rb < R; rb += 64) {
+        double ya[4];   // Y at rows rb + 4q + l/16 (zero past the tile), vector l%16
+        #pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int rr = rb + 4 * q + lq;
+            ya[q] = rr < R ? in16_get<COH>(y, rr, l16, Wv, Xt) : 0.0;
+        }
+        const T *row = base + (long long)min(rb + l16, R - 1) * t.ld;
+        d4v acc0 = d4v{0.0, 0.0, 0.0, 0.0};
+        #pragma unroll
+        for (int g = 0; g < 8; g++) {
+            if (g < ng) {
+                const int c0 = 16 * g + 4 * lq;
+                double a[4];
+                load_quad(row, c0, cmax, a);
+                #pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], xin[(c0 + j) * NV + l16], acc0, 0, 0, 0);
+                double2 *tw = reinterpret_cast<double2 *>(tr + l16 * 16 + 4 * lq);
+                tw[0] = double2{a[0], a[1]};
+                tw[1] = double2{a[2], a[3]};
+                #pragma unroll
+                for (int q = 0; q < 4; q++)
+                    acc1[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[q], tr[(4 * q + lq) * 16 + l16], acc1[g], 0, 0, 0);
+            }
+        }
+        #pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int rr = rb + lq + 4 * r;
+            if (rr < R) wst<COH>(Wv + (long long)(t.out0 + rr) * NV + l16, acc0[r]);
+        }
+    }
+    __syncthreads();   // X and the transposition tiles are dead from here on
+    double *red = lds;  // per wave: 64 columns x 16 vectors
+    #pragma unroll
+    for (int h = 0; h < 2; h++) {
+        if (64 * h < C) {   // uniform
+            #pragma unroll
+            for (int g4 = 0; g4 < 4; g4++)
+                #pragma unroll
+                for (int r = 0; r < 4; r++) red[w * 1024 + (16 * g4 + l16) * NV + lq + 4 * r] = acc1[4 * h + g4][r];
+            __syncthreads();
+            const int nq = min(64, C - 64 * h) * NV;
+            for (int q = tid; q < nq; q += WG) {
+                const double v = (red[q] + red[1024 + q]) + (red[2048 + q] + red[3072 + q]);
+                wst<COH>(Wv + (long long)(t.out1 + 64 * h) * NV + q, v);
+            }
+            __syncthreads();
+        }
+    }
+}
+
 template <typename T, bool COH>
 __device__ void mfma_tile16(const BTask &task, const T *__restrict__ S, const int *__restrict__ vsrc,
                             double *__restrict__ Wv, const double *__restrict__ Xt, double *lds) {
@@ -684,16 +777,39 @@ __device__ void mfma_tile16(const BTask &task, const T *__restrict__ S, const in
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int R = t.nrow, C = t.ncol;
+    const bool do0 = t.out0 >= 0, do1 = t.out1 >= 0;
     const T *base = S + t.mat;
+    if (do0 && do1) { mfma_tile16_2s<T, COH>(t, base, vsrc, Wv, Xt, lds); return; }
     const int cmax = ((C - 1) / VW) * VW;
-    if (t.out0 >= 0) {
+    // inputs staged in LDS (their slots summed once) when they fit beside
+    // each other, padded with zeros to whole groups of 16; otherwise (a row
+    // product wider than the LDS) the row product reads X from memory
+    const int Cp = (C + 15) & ~15, Rp = (R + 15) & ~15;
+    const int need1 = do1 ? Rp * NV : 0;
+    const bool st0 = do0 && Cp * NV + need1 <= T16_LDS;
+    double *xin = lds;
+    double *yin = lds + (st0 ? Cp * NV : 0);
+    if (st0) {
+        stage_vec<NV, COH, WG>(vsrc, t.v0b, t.v0e, t.v0off, C, Wv, Xt, -1, xin, tid);
+        for (int q = C * NV + tid; q < Cp * NV; q += WG) xin[q] = 0.0;
+    }
+    if (do1) {
+        stage_vec<NV, COH, WG>(vsrc, t.v1b, t.v1e, t.v1off, R, Wv, Xt, -1, yin, tid);
+        for (int q = R * NV + tid; q < Rp * NV; q += WG) yin[q] = 0.0;
+    }
+    __syncthreads();
+    if (do0) {
         const In16 x = in16_make(vsrc, t.v0b, t.v0e, t.v0off);
-        const int ng = (C + 15) >> 4;   // groups of 16 columns
-        for (int rb0 = 64 * w; rb0 < R; rb0 += 256) {
-            const int nb = min(4, (R - rb0 + 15) >> 4);   // blocks of 16 rows in this step (wave-uniform)
+        const int ng = Cp >> 4;   // groups of 16 columns
+        // a step of the tile covers 256 rows: wave w takes its blocks of 16
+        // rows w, w + 4, w + 8, w + 12 (fewer at the end of the tile)
+        for (int rs = 0; rs < R; rs += 256) {
+            const int rb0 = rs + 16 * w;
+            const int nb = rb0 < R ? min(4, (R - rb0 + 63) >> 6) : 0;   // blocks of this wave in the step (wave-uniform)
+            if (nb == 0) break;
             const T *row[4];
             #pragma unroll
-            for (int k = 0; k < 4; k++) row[k] = base + (long long)min(rb0 + 16 * k + l16, R - 1) * t.ld;
+            for (int k = 0; k < 4; k++) row[k] = base + (long long)min(rb0 + 64 * k + l16, R - 1) * t.ld;
             d4v acc[4];
             #pragma unroll
             for (int k = 0; k < 4; k++) acc[k] = d4v{0.0, 0.0, 0.0, 0.0};
@@ -704,7 +820,8 @@ __device__ void mfma_tile16(const BTask &task, const T *__restrict__ S, const in
                 for (int k = 0; k < 4; k++)
                     if (k < nb) load_quad(row[k], c0, cmax, a[k]);
                 #pragma unroll
-                for (int j = 0; j < 4; j++) bx[j] = c0 + j < C ? in16_get<COH>(x, c0 + j, l16, Wv, Xt) : 0.0;
+                for (int j = 0; j < 4; j++)
+                    bx[j] = st0 ? xin[(c0 + j) * NV + l16] : (c0 + j < C ? in16_get<COH>(x, c0 + j, l16, Wv, Xt) : 0.0);
                 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     if (k >= nb) break;
@@ -717,58 +834,43 @@ __device__ void mfma_tile16(const BTask &task, const T *__restrict__ S, const in
                 if (k >= nb) break;
                 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int rr = rb0 + 16 * k + lq + 4 * r;
+                    const int rr = rb0 + 64 * k + lq + 4 * r;
                     if (rr < R) wst<COH>(Wv + (long long)(t.out0 + rr) * NV + l16, acc[k][r]);
                 }
             }
         }
     }
-    if (t.out1 >= 0) {
-        const In16 y = in16_make(vsrc, t.v1b, t.v1e, t.v1off);
-        const int nspan = (C + 63) >> 6;                          // spans of 64 columns (1..4)
-        const int wps = nspan == 1 ? 4 : (nspan == 2 ? 2 : 1);   // waves per span
+    if (do1) {
+        const int nspan = (C + 63) >> 6;   // spans of 64 columns (1 or 2)
+        const int wps = 4 / nspan;         // waves per span
         const int span = w / wps, part = w - span * wps;
         const int c0 = 64 * span;
         d4v acc1[4];
         #pragma unroll
         for (int j = 0; j < 4; j++) acc1[j] = d4v{0.0, 0.0, 0.0, 0.0};
-        if (span < nspan) {
-            for (int rb = 16 * part; rb < R; rb += 16 * wps) {
+        for (int rb = 16 * part; rb < R; rb += 16 * wps) {
+            #pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int rr = rb + 4 * q + lq;
+                double b[4];
+                load_quad(base + (long long)min(rr, R - 1) * t.ld, c0 + 4 * l16, cmax, b);
+                const double ya = yin[rr * NV + l16];   // rr < Rp: zero past the tile
                 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int rr = rb + 4 * q + lq;
-                    double b[4];
-                    load_quad(base + (long long)min(rr, R - 1) * t.ld, c0 + 4 * l16, cmax, b);
-                    const double ya = rr < R ? in16_get<COH>(y, rr, l16, Wv, Xt) : 0.0;
-                    #pragma unroll
-                    for (int j = 0; j < 4; j++) acc1[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ya, b[j], acc1[j], 0, 0, 0);
-                }
+                for (int j = 0; j < 4; j++) acc1[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ya, b[j], acc1[j], 0, 0, 0);
             }
         }
-        if (wps == 1) {   // each span summed by one wave: straight to the slot
-            if (span < nspan) {
-                #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int col = c0 + 4 * l16 + j;
-                    if (col < C) {
-                        #pragma unroll
-                        for (int r = 0; r < 4; r++) wst<COH>(Wv + (long long)(t.out1 + col) * NV + lq + 4 * r, acc1[j][r]);
-                    }
-                }
-            }
-        } else {
-            double *red = lds;   // per wave: 64 columns x 16 vectors
+        __syncthreads();   // the staged inputs are dead from here on
+        double *red = lds;   // per wave: 64 columns x 16 vectors
+        #pragma unroll
+        for (int j = 0; j < 4; j++)
             #pragma unroll
-            for (int j = 0; j < 4; j++)
-                #pragma unroll
-                for (int r = 0; r < 4; r++) red[w * 1024 + (4 * l16 + j) * NV + lq + 4 * r] = acc1[j][r];
-            __syncthreads();
-            for (int q = tid; q < C * NV; q += WG) {
-                const int sp = q >> 10, loc = q & 1023;
-                const double *rs = red + sp * wps * 1024 + loc;
-                const double v = wps == 4 ? (rs[0] + rs[1024]) + (rs[2048] + rs[3072]) : rs[0] + rs[1024];
-                wst<COH>(Wv + (long long)t.out1 * NV + q, v);
-            }
+            for (int r = 0; r < 4; r++) red[w * 1024 + (4 * l16 + j) * NV + lq + 4 * r] = acc1[j][r];
+        __syncthreads();
+        for (int q = tid; q < C * NV; q += WG) {
+            const int sp = q >> 10, loc = q & 1023;
+            const double *rs = red + sp * wps * 1024 + loc;
+            const double v = wps == 4 ? (rs[0] + rs[1024]) + (rs[2048] + rs[3072]) : rs[0] + rs[1024];
+            wst<COH>(Wv + (long long)t.out1 * NV + q, v);
         }
     }
 }
@@ -850,16 +952,21 @@ __device__ void reduce_task(const RTask &task, const int *__restrict__ src, doub
 // One launch per stage: n64 FP64 tiles, then n32 FP32 tiles, then nr reduce
 // tasks, then the wave tiles, four per workgroup (FP64 first), all
 // independent of each other within the launch.
+// Waves per SIMD the kernels are built for: 4 (at most 128 VGPRs); at 16
+// vectors per pass 3, for the column sums a two-sided MFMA tile keeps in
+// registers.
+template <int NV> constexpr int waves_per_simd() { return NV == 16 ? 3 : 4; }
+
+// Task b of a staged launch: tiles (FP64, FP32), reduce tasks, then groups of
+// four wave tiles.
 template <int NV>
-__global__ __launch_bounds__(WG, 4) void stage_kernel(const BTask *__restrict__ t64, int n64, const BTask *__restrict__ t32,
-                                                   int n32, const RTask *__restrict__ rt, int nr,
-                                                   const BTask *__restrict__ w64, int nw64, const BTask *__restrict__ w32,
-                                                   int nw32, const int *__restrict__ vsrc, const int *__restrict__ rsrc,
-                                                   const double *__restrict__ S64, const float *__restrict__ S32,
-                                                   double *__restrict__ Wv, const double *__restrict__ X,
-                                                   double *__restrict__ Y, long long ldx) {
-    __shared__ double lds[LDS_DOUBLES];
-    const int b = blockIdx.x;
+__device__ __forceinline__ void stage_task(int b, const BTask *__restrict__ t64, int n64, const BTask *__restrict__ t32,
+                                           int n32, const RTask *__restrict__ rt, int nr, const BTask *__restrict__ w64,
+                                           int nw64, const BTask *__restrict__ w32, int nw32,
+                                           const int *__restrict__ vsrc, const int *__restrict__ rsrc,
+                                           const double *__restrict__ S64, const float *__restrict__ S32,
+                                           double *__restrict__ Wv, const double *__restrict__ X,
+                                           double *__restrict__ Y, long long ldx, double *lds) {
     if (b < n64) {
         const BTask t = t64[b];
         bgemv_tile<double, NV, false>(t, S64, vsrc, Wv, X, ldx, lds);
@@ -881,6 +988,39 @@ __global__ __launch_bounds__(WG, 4) void stage_kernel(const BTask *__restrict__ 
             const BTask t = w32[j];
             bgemv_tile<float, NV, false, 64>(t, S32, vsrc, Wv, X, ldx, wl);
         }
+    }
+}
+
+// One launch per stage: one task per workgroup.
+template <int NV>
+__global__ __launch_bounds__(WG, waves_per_simd<NV>()) void stage_kernel(const BTask *__restrict__ t64, int n64, const BTask *__restrict__ t32,
+                                                   int n32, const RTask *__restrict__ rt, int nr,
+                                                   const BTask *__restrict__ w64, int nw64, const BTask *__restrict__ w32,
+                                                   int nw32, const int *__restrict__ vsrc, const int *__restrict__ rsrc,
+                                                   const double *__restrict__ S64, const float *__restrict__ S32,
+                                                   double *__restrict__ Wv, const double *__restrict__ X,
+                                                   double *__restrict__ Y, long long ldx) {
+    __shared__ double lds[LDS_DOUBLES];
+    stage_task<NV>(blockIdx.x, t64, n64, t32, n32, rt, nr, w64, nw64, w32, nw32, vsrc, rsrc, S64, S32, Wv, X, Y, ldx,
+                   lds);
+}
+
+// A near-field launch with fewer workgroups than tasks (the sweeps' launches
+// on the other stream keep CUs of their own): workgroup g takes tasks
+// g, g + gridDim.x, ...
+template <int NV>
+__global__ __launch_bounds__(WG, waves_per_simd<NV>()) void stage_loop_kernel(const BTask *__restrict__ t64, int n64, const BTask *__restrict__ t32,
+                                                        int n32, const RTask *__restrict__ rt, int nr,
+                                                        const BTask *__restrict__ w64, int nw64, const BTask *__restrict__ w32,
+                                                        int nw32, const int *__restrict__ vsrc, const int *__restrict__ rsrc,
+                                                        const double *__restrict__ S64, const float *__restrict__ S32,
+                                                        double *__restrict__ Wv, const double *__restrict__ X,
+                                                        double *__restrict__ Y, long long ldx) {
+    __shared__ double lds[LDS_DOUBLES];
+    const int total = n64 + n32 + nr + (nw64 + 3) / 4 + (nw32 + 3) / 4;
+    for (int b = blockIdx.x; b < total; b += gridDim.x) {
+        if (b != (int)blockIdx.x) __syncthreads();   // the previous task's LDS
+        stage_task<NV>(b, t64, n64, t32, n32, rt, nr, w64, nw64, w32, nw32, vsrc, rsrc, S64, S32, Wv, X, Y, ldx, lds);
     }
 }
 
@@ -951,7 +1091,7 @@ __device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x
 // GROUP = false: one representation, pointers from the arguments; GROUP =
 // true: QTask::pad_[0] names the molecule whose pointers come from mols.
 template <int NV, bool GROUP>
-__global__ __launch_bounds__(WG, 4) void queue_kernel(const QTask *__restrict__ tasks, int ntask,
+__global__ __launch_bounds__(WG, waves_per_simd<NV>()) void queue_kernel(const QTask *__restrict__ tasks, int ntask,
                                                       const int *__restrict__ need, int *ctr, int spin_limit,
                                                       const MolPtrs *__restrict__ mols, const BTask *__restrict__ t64,
                                                       const BTask *__restrict__ t32, const RTask *__restrict__ rt,
@@ -1210,6 +1350,7 @@ struct Tune {
     int nf_schedule = 3;      // near-field placement, see H2ERI_set_schedule (two groups on a second stream)
     int executor = 0;         // 0: one launch per stage, 1: one persistent launch per pass (H2ERI_set_executor)
     double wave_bytes = 32768.0;   // staged executor: matrix parts up to this size run as wave tiles
+    int nf_grid = 0;          // staged executor: workgroups of a near-field launch (0: one per task)
 };
 
 struct DevState {
@@ -1418,30 +1559,35 @@ struct Planner {
         emit_tiles(st, m, r0_, nrow, nct, ctw, rows, wave, v0, o0, v1, o1, d0, d1);
     }
 
-    // 16 vectors per pass: the MFMA tile reads its inputs from memory, so a
-    // tile spans whole rows (a row product leaves no partial sums over column
-    // tiles) or whole columns.  Row products: row strips of the full width,
-    // about tile_bytes each (at least 64 rows).  Column products: strips of
-    // at most 256 columns (four spans of 64), full height up to 16 tile_bytes.
-    // A block with both outputs is one tile when at most 256 columns wide;
-    // a wider one is a set of row strips plus a set of column strips (read
-    // twice, no partial sums).
+    // 16 vectors per pass: a row product spans whole rows and leaves no
+    // partial sums over column tiles (a partial sum is 16 values per output
+    // row); its X is staged in LDS when it fits and read from memory when
+    // not.  Row products: row strips of the full width, about tile_bytes each
+    // (at least 64 rows, one block of 16 per wave).  Column products: strips
+    // of at most 128 columns (one or two spans of 64, so all four waves share
+    // the work) by row strips of at most 256 rows (Y staged in LDS); their
+    // partial sums over row strips are short (the strip's width).  A block
+    // with both outputs wider than 256 columns is a set of row strips plus a
+    // set of column tiles (the matrix read twice, no partial row sums); a
+    // narrower one takes two-sided tiles of at most 128 columns by row strips
+    // of about tile_bytes, which read their rows once.
     void add_matrix16(int st, const MatRef &m, int r0_, int nrow, int v0, int o0, int v1, int o1, int d0, int d1) {
         const int ncol = m.ncol;
         const double esz = m.f32 ? 4.0 : 8.0;
-        if (d0 >= 0 && d1 >= 0 && ncol > T16_MAXC) {
+        if (d0 >= 0 && d1 >= 0 && ncol > 2 * T16_MAXC) {
             add_matrix16(st, m, r0_, nrow, v0, o0, -1, 0, d0, -1);
             add_matrix16(st, m, r0_, nrow, -1, 0, v1, o1, -1, d1);
             return;
         }
-        int nct = 1, ctw = ncol, rows;
-        if (d1 < 0) {
-            rows = std::max(64, (int)(tile_bytes / (esz * ncol)));
-        } else {
+        int nct = 1, ctw = ncol;
+        if (d1 >= 0) {
             nct = (ncol + T16_MAXC - 1) / T16_MAXC;
             ctw = std::min(T16_MAXC, ((ncol + nct - 1) / nct + 7) / 8 * 8);
-            rows = std::max(64, (int)(16.0 * tile_bytes / (esz * ctw)));
         }
+        int rows = std::max(64, (int)(tile_bytes / (esz * ctw)));
+        // a column product alone stages Y in LDS (256 rows); a tile with both
+        // products reads Y from memory
+        if (d1 >= 0 && d0 < 0) rows = std::min(rows, T16_LDS / 16);
         emit_tiles(st, m, r0_, nrow, nct, ctw, rows, false, v0, o0, v1, o1, d0, d1);
     }
 
@@ -1590,12 +1736,16 @@ struct Planner {
         const int sD0 = sBA + 3;
         sG = sD0 + nU;
         sS = sG + 1;
-        // schedule >= 2: near field on its own stream in (schedule - 1) groups
-        // of three launches each, balanced by bytes, so that a low-rank block's
-        // re-read half is read again shortly after its first pass
+        // schedule >= 2: near field on its own stream in g = schedule - 1
+        // byte-balanced groups of blocks, software-pipelined over g + 2
+        // launches: launch k runs the first pass of group k, the second of
+        // group k - 1 and the third of group k - 2.  A low-rank block's
+        // smaller half is then read again about two groups of bytes after
+        // its first read, which a small enough group keeps within reach of
+        // the last-level cache.
         const int ngroup = nf_schedule >= 2 ? nf_schedule - 1 : 0;
         std::vector<double> gload(std::max(ngroup, 1), 0.0);
-        nst = ngroup > 0 ? sS + 3 * ngroup : sG + 1;
+        nst = ngroup > 0 ? sS + ngroup + 2 : sG + 1;
         SR.assign(nst, {});
         sload.assign(nst, 0.0);
         sphase.assign(nst, PH_GATHER);
@@ -1691,7 +1841,7 @@ struct Planner {
             if (ngroup > 0) {
                 const int g = (int)(std::min_element(gload.begin(), gload.end()) - gload.begin());
                 gload[g] += bytes;
-                best = sS + 3 * g;
+                best = sS + g;
             } else if (nf_schedule == 0) {
                 double bestv = 1e300;
                 for (int a = 0; a + span - 1 < sG; a++) {
@@ -1819,8 +1969,9 @@ struct Planner {
             for (const BTask &t : tl) {
                 bool ok = t.nrow > 0 && t.ncol > 0 && t.ld >= t.ncol && t.ld % W == 0 && t.mat % W == 0 && t.mat >= 0 &&
                           (t.out0 >= 0 || t.out1 >= 0);
-                if (nv == 16) {   // the MFMA tile: four spans of 64 columns for a column product
-                    ok = ok && (t.out1 < 0 || t.ncol <= T16_MAXC);
+                if (nv == 16) {   // the MFMA tile: two spans of 64 columns, Y staged for a column product alone
+                    const int Rp = (t.nrow + 15) & ~15;
+                    ok = ok && (t.out1 < 0 || (t.ncol <= T16_MAXC && (t.out0 >= 0 || Rp * 16 <= T16_LDS)));
                 } else {
                     ok = ok && t.ncol <= maxc && t.ncol * nv <= xcap && (t.lpr & (t.lpr - 1)) == 0 && t.lpr >= 4 &&
                          t.lpr <= 64 && t.lpr * cpl_of(nv) >= t.ncol && (t.out1 < 0 || t.nrow * nv <= ycap);
@@ -2255,13 +2406,23 @@ static inline int stage_blocks(const Stage &st) {
 static inline double *pass_Wv(const DevState *d, const DevPlan &dp) { return d->ws_Wv ? d->ws_Wv : dp.Wv; }
 static inline int *pass_ctr(const DevState *d, const DevPlan &dp) { return d->ws_ctr ? d->ws_ctr : dp.d_ctr; }
 
+// Workgroups of a launch: one per task; a near-field launch on the second
+// stream takes at most the configured number, looping over its tasks, so the
+// sweeps' launches beside it find free CUs.
+static inline int launch_blocks(const Stage &st, const DevState *d, int nv) {
+    const int nb = stage_blocks(st);
+    const int cap = d->cfg[nv_index(nv)].nf_grid;
+    return (st.stream == 1 && cap > 0) ? std::min(nb, cap) : nb;
+}
+
 template <int NV>
 static void launch_stage(const Stage &st, const DevPlan &dp, DevState *d, const double *X, double *Y, long long ld,
                          hipStream_t s) {
-    hipLaunchKernelGGL(stage_kernel<NV>, dim3(stage_blocks(st)), dim3(WG), 0, s, dp.d_t64 + st.t64_beg, st.t64_cnt,
-                       dp.d_t32 + st.t32_beg, st.t32_cnt, dp.d_rt + st.r_beg, st.r_cnt, dp.d_w64 + st.w64_beg,
-                       st.w64_cnt, dp.d_w32 + st.w32_beg, st.w32_cnt, dp.d_vsrc, dp.d_rsrc, d->S64, d->S32,
-                       pass_Wv(d, dp), NV == 16 ? dp.Xt : X, Y, ld);
+    const int nb = stage_blocks(st), grid = launch_blocks(st, d, NV);
+    hipLaunchKernelGGL(grid < nb ? stage_loop_kernel<NV> : stage_kernel<NV>, dim3(grid), dim3(WG), 0, s,
+                       dp.d_t64 + st.t64_beg, st.t64_cnt, dp.d_t32 + st.t32_beg, st.t32_cnt, dp.d_rt + st.r_beg,
+                       st.r_cnt, dp.d_w64 + st.w64_beg, st.w64_cnt, dp.d_w32 + st.w32_beg, st.w32_cnt, dp.d_vsrc,
+                       dp.d_rsrc, d->S64, d->S32, pass_Wv(d, dp), NV == 16 ? dp.Xt : X, Y, ld);
 }
 
 static void collect_phase_ms(H2ERI_p h, DevState *d);
@@ -2810,11 +2971,11 @@ void H2ERI_matvec_bytes(H2ERI_p h, double *out6) {
 
 // Near-field schedule (tuning knob; re-plans): 0 balances the near-field
 // tiles over the sweep launches, 1 runs them in three launches of their own
-// ahead of the sweeps, 2 + g (default 3) runs them on a second stream, concurrently with
-// the sweeps and joined before the gather, in g + 1 byte-balanced groups of
-// three launches.
+// ahead of the sweeps, 2 + g (default 3) runs them on a second stream,
+// concurrently with the sweeps and joined before the gather, as g + 1
+// byte-balanced groups of blocks pipelined over g + 3 launches.
 void H2ERI_set_schedule(H2ERI_p h, const int mode) {
-    if (!h || mode < 0 || mode > 17) { set_error(ERR_ARG, "H2ERI_set_schedule: 0 to 17"); return; }
+    if (!h || mode < 0 || mode > 257) { set_error(ERR_ARG, "H2ERI_set_schedule: 0 to 257"); return; }
     DevState *d = dstate_get(h);
     for (int i = 0; i < NWIDTH; i++)
         if (d->cfg[i].nf_schedule != mode) { d->cfg[i].nf_schedule = mode; release_plan(d, i); }
@@ -2877,6 +3038,18 @@ void H2ERI_set_wave_tile_bytes(H2ERI_p h, const double bytes) {
         if (d->cfg[i].wave_bytes != bytes) { d->cfg[i].wave_bytes = bytes; release_plan(d, i); }
 }
 
+// Workgroups of every near-field launch of the staged executor on the second
+// stream (tuning knob; no re-plan, releases a captured graph): 0 gives each
+// task a workgroup of its own; n > 0 at most n, each taking tasks
+// blockIdx.x, blockIdx.x + n, ..., so the sweeps' launches beside them keep
+// CUs of their own.
+void H2ERI_set_near_field_grid(H2ERI_p h, const int n) {
+    if (!h || n < 0) { set_error(ERR_ARG, "H2ERI_set_near_field_grid: n >= 0"); return; }
+    DevState *d = dstate_get(h);
+    release_graph(d);
+    for (int i = 0; i < NWIDTH; i++) d->cfg[i].nf_grid = n;
+}
+
 // Tile size of the bi-GEMV tasks in bytes (tuning knob; re-plans).
 void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
     if (!h || bytes < 1024) return;
@@ -2891,7 +3064,7 @@ void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
 // after one warm-up) and keeps the fastest.  out5 = {best ms per call, tile
 // bytes, schedule, executor, wave-tile bytes}.  Returns 0, or -1 with the
 // error set.
-int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
+int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out6) {
     if (!h || !nv_valid(nv) || reps < 1) { set_error(ERR_ARG, "H2ERI_autotune: bad argument"); return -1; }
     if (!upload_storage(h)) return -1;
     DevState *d = dstate(h);
@@ -2900,10 +3073,11 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
     // inputs: a fixed pattern (the time does not depend on the values)
     hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(256), 0, 0, d->d_x, N * nv);
     if (hipGetLastError() != hipSuccess) { set_error(ERR_DEVICE, "autotune: zero"); return -1; }
-    const double tiles[] = {65536.0, 131072.0, 262144.0};
+    const double tiles[] = {65536.0, 131072.0, 262144.0, 524288.0};
     // schedules: near field in the sweep launches (0), or on a second stream in
-    // 1, 2, 3, 5 or 9 groups of three launches (more groups bring each low-rank
-    // block's second read of its smaller half closer to the first)
+    // 1, 2, 3, 5 or 9 pipelined groups (more groups bring each low-rank block's
+    // second read of its smaller half closer to the first, at the cost of more
+    // launches)
     const int scheds[] = {0, 2, 3, 4, 6, 10};
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { set_error(ERR_DEVICE, "autotune: events"); return -1; }
@@ -2915,6 +3089,24 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
     const bool was_timing = d->phase_timing;
     d->phase_timing = false;
     bool ok = true;
+    // times the current configuration c and keeps it when it is the fastest
+    auto time_cfg = [&]() -> bool {
+        if (!matvec_multi_dev_impl(h, nv, d->d_x, d->d_y, nullptr)) return false;   // plans, uploads, warms up
+        if (hipEventRecord(e0, nullptr) != hipSuccess) return false;
+        for (int r = 0; r < reps; r++)
+            if (!matvec_multi_dev_impl(h, nv, d->d_x, d->d_y, nullptr)) return false;
+        if (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return false;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return false;
+        if (ms / reps < best) { best = ms / reps; best_cfg = c; }
+        if (h->print_dbginfo) {
+            fprintf(stderr, "[H2ERI_autotune] nv %d: executor %d, tile %.0f KB, schedule %d, wave tiles %.0f KB, "
+                    "near-field grid %d: %.4f ms\n", nv, c.executor, c.tile_bytes / 1024.0, c.nf_schedule,
+                    c.wave_bytes / 1024.0, c.nf_grid, ms / reps);
+            fflush(stderr);
+        }
+        return true;
+    };
     for (int ex = 0; ok && ex < 2; ex++)
     for (int wi = 0; ok && wi < (ex == 0 && nv != 16 ? 2 : 1); wi++)   // no wave tiles at 16 vectors per pass
     for (double tb : tiles) {
@@ -2924,21 +3116,28 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
             c.wave_bytes = ex == 0 ? waves[wi] : 0.0;
             c.tile_bytes = tb;
             c.nf_schedule = sc;
-            ok = matvec_multi_dev_impl(h, nv, d->d_x, d->d_y, nullptr);   // plans, uploads, warms up
-            ok = ok && hipEventRecord(e0, nullptr) == hipSuccess;
-            for (int r = 0; ok && r < reps; r++) ok = matvec_multi_dev_impl(h, nv, d->d_x, d->d_y, nullptr);
-            ok = ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
-            float ms = 0.f;
-            ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            c.nf_grid = 0;
+            ok = time_cfg();
             if (!ok) break;
-            if (ms / reps < best) { best = ms / reps; best_cfg = c; }
-            if (h->print_dbginfo) {
-                fprintf(stderr, "[H2ERI_autotune] nv %d: executor %d, tile %.0f KB, schedule %d, wave tiles %.0f KB: %.4f ms\n",
-                        nv, ex, tb / 1024.0, sc, c.wave_bytes / 1024.0, ms / reps);
-                fflush(stderr);
-            }
         }
         if (!ok) break;
+    }
+    // the best staged configuration with a second stream, its near-field
+    // launches capped at 3/4 and 1/2 of the workgroups the GPU holds at once
+    if (ok && best_cfg.executor == 0 && best_cfg.nf_schedule >= 2) {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d->dev) != hipSuccess || ncu <= 0) ncu = 256;
+        const int resident = ncu * (nv == 16 ? waves_per_simd<16>() : waves_per_simd<1>());   // 4 waves each: one per SIMD
+        const Tune base = best_cfg;
+        for (int frac : {3, 2}) {
+            if (c.executor != base.executor || c.tile_bytes != base.tile_bytes || c.nf_schedule != base.nf_schedule ||
+                c.wave_bytes != base.wave_bytes)
+                release_plan(d, vi);
+            c = base;
+            c.nf_grid = resident * frac / 4;
+            ok = time_cfg();
+            if (!ok) break;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -2949,8 +3148,9 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out5) {
         if (!has_error()) set_error(ERR_DEVICE, "autotune: timing failed");
         return -1;
     }
-    if (out5) {
-        out5[0] = best; out5[1] = c.tile_bytes; out5[2] = c.nf_schedule; out5[3] = c.executor; out5[4] = c.wave_bytes;
+    if (out6) {
+        out6[0] = best; out6[1] = c.tile_bytes; out6[2] = c.nf_schedule; out6[3] = c.executor; out6[4] = c.wave_bytes;
+        out6[5] = c.nf_grid;
     }
     return 0;
 }

@@ -217,6 +217,11 @@ class H2ERI:
         self.lib.H2ERI_set_wave_tile_bytes(self.h, float(nbytes))
         _check(self.lib)
 
+    def set_near_field_grid(self, n):
+        """Staged executor: at most n workgroups per near-field launch (0: one per task)."""
+        self.lib.H2ERI_set_near_field_grid(self.h, int(n))
+        _check(self.lib)
+
     def set_executor(self, mode):
         """0: one launch per stage; 1: one persistent launch per pass."""
         self.lib.H2ERI_set_executor(self.h, mode)
@@ -395,11 +400,11 @@ class H2ERI:
     def autotune(self, nv=1, reps=10):
         """Times executors x tile sizes x near-field schedules on the device,
         keeps the best."""
-        out = np.zeros(5)
+        out = np.zeros(6)
         self.lib.H2ERI_autotune(self.h, nv, reps, _dptr(out))
         _check(self.lib)
         return dict(ms=float(out[0]), tile_bytes=int(out[1]), schedule=int(out[2]), executor=int(out[3]),
-                    wave_tile_bytes=int(out[4]))
+                    wave_tile_bytes=int(out[4]), near_field_grid=int(out[5]))
 
     def set_phase_timing(self, on=True):
         self.lib.H2ERI_set_phase_timing(self.h, 1 if on else 0)

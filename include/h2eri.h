@@ -378,9 +378,10 @@ void H2ERI_set_tile_bytes(H2ERI_p h2eri, const double bytes);
 
 /* Near-field schedule (tuning; re-plans): 0 balances the near-field tiles
  * over the launches of the sweeps, 1 gives them three launches of their own
- * ahead of the sweeps, 2 + g (g = 0..15) runs them on a second stream
- * concurrently with the sweeps in g + 1 byte-balanced groups of three
- * launches. */
+ * ahead of the sweeps, 2 + g (g = 0..255) runs them on a second stream
+ * concurrently with the sweeps as g + 1 byte-balanced groups of blocks
+ * pipelined over g + 3 launches (launch k: first pass of group k, second of
+ * group k - 1, third of group k - 2). */
 void H2ERI_set_schedule(H2ERI_p h2eri, const int mode);
 
 /* Matvec executor (re-plans; no reference counterpart): 0 launches every
@@ -428,20 +429,28 @@ int   H2ERI_group_status(void *group);
 void  H2ERI_group_destroy(void *group);
 
 /* Autotuner: times nv-vector device matvecs (reps calls after a warm-up)
- * for every executor (H2ERI_set_executor) x tile size {64, 128, 256} KB x
+ * for every executor (H2ERI_set_executor) x tile size {64, 128, 256, 512} KB x
  * near-field schedule {0, 2, 3, 4, 6, 10} x (staged executor) wave-tile limit {0,
- * 32} KB, and keeps the fastest combination for later passes of nv vectors
- * (each batch width 1, 2, 4, 8, 16 has a configuration of its own; the tuning
- * setters above change all four).  out5 = {ms per call, tile bytes,
- * schedule, executor, wave-tile bytes} of the combination kept.  Returns 0,
- * -1 on error. */
-int H2ERI_autotune(H2ERI_p h2eri, const int nv, const int reps, double *out5);
+ * 32} KB, then, when the staged executor with a second stream wins, its
+ * near-field launches capped at 3/4 and 1/2 of the workgroups the GPU holds
+ * (H2ERI_set_near_field_grid), and keeps the fastest combination for later
+ * passes of nv vectors (each batch width 1, 2, 4, 8, 16 has a configuration
+ * of its own; the tuning setters above change all five).  out6 = {ms per
+ * call, tile bytes, schedule, executor, wave-tile bytes, near-field grid} of
+ * the combination kept.  Returns 0, -1 on error. */
+int H2ERI_autotune(H2ERI_p h2eri, const int nv, const int reps, double *out6);
 
 /* Staged executor: matrix parts up to `bytes` (default 32768) run as wave
  * tiles, one wave each and four per workgroup, which pays off for the many
  * small coupling blocks of globular molecules; 0 disables them (tuning;
  * re-plans; no reference counterpart). */
 void H2ERI_set_wave_tile_bytes(H2ERI_p h2eri, const double bytes);
+
+/* Workgroups of each near-field launch of the staged executor on its second
+ * stream (tuning; no re-plan; releases a captured graph): 0 (default) gives
+ * every task a workgroup of its own, n > 0 at most n workgroups looping over
+ * the launch's tasks, so that the sweeps' launches beside them keep CUs. */
+void H2ERI_set_near_field_grid(H2ERI_p h2eri, const int n);
 
 /* Builds the device execution plan on the host only (no HIP call) and
  * checks every access of every launch against its buffer.  Returns the

@@ -129,7 +129,8 @@ def test_autotune_keeps_results(tmp_path):
     from conftest import make_synthetic
     h = make_synthetic(tmp_path / "a.rep", 2000, 21, leaf=60, max_rank=30, lr_frac=0.4, diag_lr_frac=0.3)
     res = h.autotune(nv=1, reps=3)
-    assert res["ms"] > 0 and res["tile_bytes"] in (65536, 131072, 262144) and res["schedule"] in (0, 2, 3, 4, 6, 10)
+    assert res["ms"] > 0 and res["tile_bytes"] in (65536, 131072, 262144, 524288)
+    assert res["schedule"] in (0, 2, 3, 4, 6, 10) and res["near_field_grid"] >= 0
     assert res["executor"] in (0, 1) and h.executor() == res["executor"] and res["wave_tile_bytes"] in (0, 32768)
     x = np.random.default_rng(2).standard_normal(h.n)
     o = oracle_bind.Oracle(str(tmp_path / "a.rep"))

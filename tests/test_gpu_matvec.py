@@ -253,6 +253,37 @@ def test_wave_tile_limits(reps, wave_bytes):
         assert relerr(Y[k], o.matvec(X[k])) < 1e-13
 
 
+@pytest.mark.parametrize("grid", [1, 7, 300])
+def test_near_field_grid_cap(reps, grid):
+    """Near-field launches with fewer workgroups than tasks (each workgroup
+    loops over tasks g, g + grid, ...) run the same tasks with the same sums:
+    results bitwise equal to one workgroup per task, at 1, 4 and 16 vectors
+    per pass, eager and as a graph replay."""
+    import torch
+    h, path = reps["chain_lrdiag"]
+    X = np.random.default_rng(60).standard_normal((21, h.n))   # passes of 16, 4, 1
+    h.set_executor(0)
+    h.set_schedule(3)
+    Y0 = h.matvec_multi(X)
+    try:
+        h.set_near_field_grid(grid)
+        Y1 = h.matvec_multi(X)
+        Xd = torch.from_numpy(X).to("cuda:0")
+        Yd = torch.empty_like(Xd)
+        h.graph_capture("matvec", 21, Xd.data_ptr(), Yd.data_ptr())
+        s = torch.cuda.current_stream().cuda_stream
+        for _ in range(2):
+            Yd.zero_()
+            h.graph_replay(s)
+            torch.cuda.synchronize()
+            assert np.array_equal(Yd.cpu().numpy(), Y0)
+    finally:
+        h.set_near_field_grid(0)
+    assert np.array_equal(Y1, Y0)
+    o = oracle_bind.Oracle(path)
+    assert relerr(Y1[20], o.matvec(X[20])) < 1e-13
+
+
 def test_persistent_wait_bound_reports_error(tmp_path):
     """The persistent executor's dependency waits given up at once (a bound
     of zero polls, set from the host): every kernel still exits, and each

@@ -150,7 +150,10 @@ def test_device_plan_semantics_match_oracle(tmp_path, seed, n, leaf, mr, shape):
     y32, conflicts = h.plan_emulate(x, 32)
     assert conflicts == 0
     assert relerr(y32, yo) < 2e-6
-    for sched in (0, 1, 4):   # near field in the sweep launches / own launches / 3 groups on a second stream
+    # near field in the sweep launches / own launches / 1, 3, 8 or 39 groups
+    # pipelined over launches of a second stream (more groups than blocks
+    # leave launches empty)
+    for sched in (0, 1, 2, 4, 9, 40):
         h.set_schedule(sched)
         y1, conflicts = h.plan_emulate(x, 64)
         assert conflicts == 0
@@ -169,10 +172,12 @@ def test_device_plan_semantics_batched(tmp_path, nv, seed, n, leaf, mr, shape):
                        shape=shape)
     o = oracle_bind.Oracle(str(tmp_path / "r.rep"))
     X = np.random.default_rng(seed + 10 * nv).standard_normal((nv, h.n))
-    Y, conflicts = h.plan_emulate(X, 64)
-    assert conflicts == 0
-    for v in range(nv):
-        assert relerr(Y[v], o.matvec(X[v])) < 1e-13
+    for sched in (3, 9):
+        h.set_schedule(sched)
+        Y, conflicts = h.plan_emulate(X, 64)
+        assert conflicts == 0
+        for v in range(nv):
+            assert relerr(Y[v], o.matvec(X[v])) < 1e-13
 
 
 @pytest.mark.parametrize("seed,n,leaf,mr,shape", [(1, 900, 30, 12, "chain"), (4, 1500, 40, 20, "cube"),
@@ -190,7 +195,7 @@ def test_queue_order_covers_every_dependency(tmp_path, seed, n, leaf, mr, shape)
     rng = np.random.default_rng(seed)
     x = rng.standard_normal(h.n)
     yo = o.matvec(x)
-    for sched in (3, 0):
+    for sched in (3, 0, 12):
         h.set_schedule(sched)
         for wg in (1, 7, 1024):
             y, bad = h.queue_emulate(x, 64, workgroups=wg)
