@@ -62,7 +62,7 @@ struct KTerm {
     long long Loff;
     int Rsrc, Rrs, Rcs;   // R[c][t] = src[Roff + c*Rrs + t*Rcs]
     long long Roff;
-    long long Goff;       // G in scratch: [q / 8][n][t][q % 8], gld(|B(j)|) / 8 blocks
+    long long Goff;       // G in its chunk's scratch: [q / 8][n][t][q % 8], gld(|B(j)|) / 8 blocks
     int pad_[2];
 };
 
@@ -137,7 +137,8 @@ __global__ __launch_bounds__(KWG) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
                                                const int *__restrict__ qptr, const int2 *__restrict__ qent,
                                                const double *__restrict__ PB, const double *__restrict__ CB,
-                                               double *__restrict__ SC, const double *__restrict__ D, int nbf) {
+                                               const double *__restrict__ SC, double *__restrict__ GS,
+                                               const double *__restrict__ D, int nbf) {
     const KTask tk = tasks[xcd_task(blockIdx.x, gridDim.x, KRUN_KET)];   // term, nl0, t0, ql0
     const KTerm t = terms[tk.term];
     const KNode ni = nodes[t.ni], nj = nodes[t.nj];
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(KWG) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int ta = tk.t0 + (lane & 15);   // A operand row (t)
     const bool tv = ta < t.kk;
     const int kl = lane >> 4;
-    double *G = SC + t.Goff;
+    double *G = GS + t.Goff;
     for (int qg = qbeg; qg < qend; qg += 4) {
         // The entry lists of the four q are walked together (entries past a
         // list are zeros), so each iteration has the gathers of eight
@@ -238,8 +239,8 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
                                                const KNode *__restrict__ nodes, const int *__restrict__ bfidx,
                                                const int *__restrict__ nspl, const int2 *__restrict__ nent,
                                                const double *__restrict__ PB, const double *__restrict__ CB,
-                                               const double *__restrict__ SC, double *__restrict__ K1,
-                                               double *__restrict__ Kd, int nbf) {
+                                               const double *__restrict__ SC, const double *__restrict__ GS,
+                                               double *__restrict__ K1, double *__restrict__ Kd, int nbf) {
     __shared__ double Kb[KSPLIT * 32];
     const KTask tk = tasks[xcd_task(blockIdx.x, gridDim.x, KRUN_BRA)];   // term, ql0, split
     const KTerm t = terms[tk.term];
@@ -250,7 +251,7 @@ __global__ __launch_bounds__(KWG) void kx_bra(const KTask *__restrict__ tasks, c
     for (int i = threadIdx.x; i < KSPLIT * 32; i += KWG) Kb[i] = 0.0;
     __syncthreads();
     const double *L = (t.Lsrc == KS_PB ? PB : (t.Lsrc == KS_CB ? CB : SC)) + t.Loff;
-    const double *G = SC + t.Goff;
+    const double *G = GS + t.Goff;
     bool qv[2];
     #pragma unroll
     for (int h = 0; h < 2; h++) qv[h] = ql0 + 16 * h + (lane & 15) < nj.nbf;
@@ -444,14 +445,28 @@ struct KPlan {
     std::vector<int2> qent, ment, nent;
     std::vector<KTask> ket, bra, direct;
     std::vector<KGemmTask> gtasks;
-    long long nSC = 0;
+    long long nSC = 0;      // L / R products (computed once)
+    long long nG = 0;       // G scratch of the largest chunk of terms
+    std::vector<int> ket_chunk, bra_chunk;   // first ket / bra task of every chunk, and the end
     double flops = 0;
 };
+
+// Default bound of the G scratch (bytes): the terms run in chunks whose G
+// fits it, each chunk's ket then its bra, all chunks in one scratch region.
+// Measured K per build against the bound (unbounded / 4 GiB / 1 GiB /
+// 256 MiB): C30H62 12.2 / 13.1 / 16.7 / 33.1 ms with 16.0 / 6.0 / 2.8 / 2.0 GB
+// resident; (H2O)64 75.7 ms unbounded (76 GB), 90.1 ms at 1 GiB (8.5 GB).
+// Smaller chunks give launches too few tasks to fill the GPU, and the G of a
+// chunk is not re-read from the last-level cache often enough to make up for
+// it.  8 GiB keeps every benchmark molecule near 10 GB for a few percent.
+static const double KG_BUDGET_DEFAULT = 8.0 * 1024.0 * 1024.0 * 1024.0;
 
 struct KState {
     KPlan P;
     bool ready = false;
+    double g_budget = KG_BUDGET_DEFAULT;   // bytes of the G scratch (H2ERI_set_exchange_scratch)
     double *PB = nullptr, *CB = nullptr, *SC = nullptr, *K1 = nullptr, *Kd = nullptr, *D = nullptr, *K = nullptr;
+    double *GS = nullptr;   // G of one chunk of terms
     double *Ds = nullptr;   // the density after screening (the kernels' D)
     int *sbf = nullptr;     // shell_bf_sidx
     KTerm *terms = nullptr;
@@ -469,7 +484,7 @@ static void kfree(void *&p) {
 }
 
 static void kstate_release(KState *k) {
-    void **ps[] = {(void **)&k->PB, (void **)&k->CB, (void **)&k->SC, (void **)&k->K1, (void **)&k->Kd, (void **)&k->D,
+    void **ps[] = {(void **)&k->PB, (void **)&k->CB, (void **)&k->SC, (void **)&k->GS, (void **)&k->K1, (void **)&k->Kd, (void **)&k->D,
                    (void **)&k->K, (void **)&k->terms, (void **)&k->gemms, (void **)&k->nodes, (void **)&k->bfidx,
                    (void **)&k->qptr, (void **)&k->mptr, (void **)&k->qent, (void **)&k->ment, (void **)&k->nspl,
                    (void **)&k->nent, (void **)&k->ket,
@@ -482,12 +497,14 @@ static void kstate_release(KState *k) {
 struct KBuilder {
     H2ERI_p h;
     KPlan &P;
+    double g_budget;                 // bytes of G per chunk of terms
+    std::vector<long long> gsize;    // G doubles of every term (0: direct)
     std::vector<int> ra, rb;
     std::vector<char> sw;
     std::vector<long long> pb_off;   // expanded basis of each tree node in PB (-1: none)
     std::vector<int> node_list;      // tree node -> KNode id (-1: not built)
 
-    KBuilder(H2ERI_p h_, KPlan &p_) : h(h_), P(p_) {}
+    KBuilder(H2ERI_p h_, KPlan &p_, double g_budget_) : h(h_), P(p_), g_budget(g_budget_) {}
 
     int rank_of(int n) const { return h->U[n] ? h->U[n]->ncol : 0; }
     int rows_of(int n) const { return h->mat_cluster[2 * n + 1] - h->mat_cluster[2 * n] + 1; }
@@ -681,7 +698,7 @@ struct KBuilder {
             if (bas1) { t.Rsrc = KS_SC; t.Roff = gemm(pb_off[n1], k1, m1, k1, c.VT, k, k); t.Rrs = k; t.Rcs = 1; }
             else { t.Rsrc = KS_CB; t.Roff = c.VT; t.Rrs = k; t.Rcs = 1; }
         }
-        t.Goff = salloc((long long)t.kk * P.nodes[t.ni].nbf * gld(P.nodes[t.nj].nbf));
+        t.Goff = -1;   // placed when the terms are cut into chunks
         P.terms.push_back(t);
     }
 
@@ -704,20 +721,37 @@ struct KBuilder {
         for (int g = 0; g < (int)P.gemms.size(); g++)
             for (int r0 = 0; r0 < P.gemms[g].m; r0 += 64)
                 for (int c0 = 0; c0 < P.gemms[g].n; c0 += 64) P.gtasks.push_back(KGemmTask{g, r0, c0, 0});
+        // chunks of consecutive terms whose G fits the budget (a larger term
+        // alone); G offsets restart in every chunk
+        const long long budget = std::max(1LL, (long long)(g_budget / 8.0));
+        long long used = 0;
+        bool open = false;
         for (int ti = 0; ti < (int)P.terms.size(); ti++) {
-            const KTerm &t = P.terms[ti];
+            KTerm &t = P.terms[ti];
             const KNode &ni = P.nodes[t.ni], &nj = P.nodes[t.nj];
             if (t.direct) {
                 const long long ne = (long long)ni.nbf * nj.nbf;
                 for (long long e0 = 0; e0 < ne; e0 += KWG) P.direct.push_back(KTask{ti, (int)e0, 0, 0});
                 continue;
             }
+            const long long gs = ((long long)t.kk * ni.nbf * gld(nj.nbf) + 31) & ~31LL;
+            if (!open || (used > 0 && used + gs > budget)) {
+                P.ket_chunk.push_back((int)P.ket.size());
+                P.bra_chunk.push_back((int)P.bra.size());
+                used = 0;
+                open = true;
+            }
+            t.Goff = used;
+            used += gs;
+            P.nG = std::max(P.nG, used);
             for (int t0 = 0; t0 < t.kk; t0 += 16)
                 for (int nl0 = 0; nl0 < ni.nbf; nl0 += 64)
                     for (int ql0 = 0; ql0 < nj.nbf; ql0 += KQC) P.ket.push_back(KTask{ti, nl0, t0, ql0});
             for (int ql0 = 0; ql0 < nj.nbf; ql0 += 32)
                 for (int sp = 0; sp < ni.nsplit; sp++) P.bra.push_back(KTask{ti, ql0, sp, 0});
         }
+        P.ket_chunk.push_back((int)P.ket.size());
+        P.bra_chunk.push_back((int)P.bra.size());
         return validate();
     }
 
@@ -763,10 +797,30 @@ struct KBuilder {
             if (!t.direct) {
                 const long long last = t.Roff + (long long)(t.m1 - 1) * t.Rrs + (long long)(t.kk - 1) * t.Rcs;
                 ok = ok && t.Roff >= 0 && last < size_of(t.Rsrc) && t.Goff >= 0 &&
-                     t.Goff % 4 == 0 && t.Goff + (long long)t.kk * ni.nbf * gld(nj.nbf) <= nSC;
+                     t.Goff % 4 == 0 && t.Goff + (long long)t.kk * ni.nbf * gld(nj.nbf) <= P.nG;
             }
             if (!ok) { set_error(ERR_ARG, "exchange plan check failed: term"); return false; }
         }
+        // chunks: the task ranges cover both lists in order, and all tasks of
+        // a term fall in one chunk (the chunks share one G region)
+        const size_t nc = P.ket_chunk.size();
+        bool ok = nc >= 1 && P.bra_chunk.size() == nc && P.ket_chunk.front() == 0 && P.bra_chunk.front() == 0 &&
+                  P.ket_chunk.back() == (int)P.ket.size() && P.bra_chunk.back() == (int)P.bra.size();
+        std::vector<int> tchunk(P.terms.size(), -1);
+        for (size_t c = 0; ok && c + 1 < nc; c++) {
+            ok = P.ket_chunk[c] <= P.ket_chunk[c + 1] && P.bra_chunk[c] <= P.bra_chunk[c + 1];
+            for (int side = 0; ok && side < 2; side++) {
+                const std::vector<KTask> &L = side == 0 ? P.ket : P.bra;
+                const std::vector<int> &B = side == 0 ? P.ket_chunk : P.bra_chunk;
+                for (int i = B[c]; ok && i < B[c + 1]; i++) {
+                    const int ti = L[i].term;
+                    ok = ti >= 0 && ti < (int)P.terms.size() && !P.terms[ti].direct &&
+                         (tchunk[ti] < 0 || tchunk[ti] == (int)c);
+                    if (ok) tchunk[ti] = (int)c;
+                }
+            }
+        }
+        if (!ok) { set_error(ERR_ARG, "exchange plan check failed: chunks"); return false; }
         return true;
     }
 };
@@ -799,8 +853,9 @@ static bool k_upload(H2ERI_p h) {
     if (!h->thread_Kmat_workbuf) h->thread_Kmat_workbuf = (void **)new KState();
     KState *k = kstate(h);
     if (k->ready) return true;
+    const double budget = k->g_budget;
     kstate_release(k);
-    KBuilder b(h, k->P);
+    KBuilder b(h, k->P, budget);
     if (!b.build()) return false;
     const KPlan &P = k->P;
     const int nbf = h->num_bf;
@@ -814,7 +869,8 @@ static bool k_upload(H2ERI_p h) {
     const size_t nn2 = (size_t)nbf * nbf;
     if (!up((void **)&k->PB, P.PB.empty() ? nullptr : P.PB.data(), P.PB.size() * 8) ||
         !up((void **)&k->CB, P.CB.empty() ? nullptr : P.CB.data(), P.CB.size() * 8) ||
-        !up((void **)&k->SC, nullptr, (size_t)P.nSC * 8) || !up((void **)&k->K1, nullptr, nn2 * 8) ||
+        !up((void **)&k->SC, nullptr, (size_t)P.nSC * 8) || !up((void **)&k->GS, nullptr, (size_t)P.nG * 8) ||
+        !up((void **)&k->K1, nullptr, nn2 * 8) ||
         !up((void **)&k->Kd, nullptr, nn2 * 8) || !up((void **)&k->D, nullptr, nn2 * 8) ||
         !up((void **)&k->K, nullptr, nn2 * 8) || !up((void **)&k->Ds, nullptr, nn2 * 8) ||
         !up((void **)&k->sbf, h->shell_bf_sidx, (size_t)(h->nshell + 1) * 4) ||
@@ -855,15 +911,20 @@ static bool k_run(H2ERI_p h, const double *D_dev, double *K_dev, hipStream_t s) 
     hipLaunchKernelGGL(kx_zero, dim3(zb), dim3(256), 0, s, k->K1, nn2);
     hipLaunchKernelGGL(kx_zero, dim3(zb), dim3(256), 0, s, k->Kd, nn2);
     KHIP(hipGetLastError());
-    if (!P.ket.empty())
-        hipLaunchKernelGGL(kx_ket, dim3((unsigned)P.ket.size()), dim3(KWG), 0, s, k->ket, k->terms, k->nodes, k->bfidx,
-                           k->qptr, k->qent, k->PB, k->CB, k->SC, D_dev, nbf);
     if (!P.direct.empty())
         hipLaunchKernelGGL(kx_direct, dim3((unsigned)P.direct.size()), dim3(KWG), 0, s, k->direct, k->terms, k->nodes,
                            k->bfidx, k->qptr, k->qent, k->mptr, k->ment, k->CB, D_dev, k->K1, k->Kd, nbf);
-    if (!P.bra.empty())
-        hipLaunchKernelGGL(kx_bra, dim3((unsigned)P.bra.size()), dim3(KWG), 0, s, k->bra, k->terms, k->nodes, k->bfidx,
-                           k->nspl, k->nent, k->PB, k->CB, k->SC, k->K1, k->Kd, nbf);
+    // every chunk of terms: its ket writes G into the shared scratch, its bra
+    // reads it back (same stream, so the next chunk's ket waits for it)
+    for (size_t c = 0; c + 1 < P.ket_chunk.size(); c++) {
+        const int kb = P.ket_chunk[c], ke = P.ket_chunk[c + 1], bb = P.bra_chunk[c], be = P.bra_chunk[c + 1];
+        if (ke > kb)
+            hipLaunchKernelGGL(kx_ket, dim3((unsigned)(ke - kb)), dim3(KWG), 0, s, k->ket + kb, k->terms, k->nodes,
+                               k->bfidx, k->qptr, k->qent, k->PB, k->CB, k->SC, k->GS, D_dev, nbf);
+        if (be > bb)
+            hipLaunchKernelGGL(kx_bra, dim3((unsigned)(be - bb)), dim3(KWG), 0, s, k->bra + bb, k->terms, k->nodes,
+                               k->bfidx, k->nspl, k->nent, k->PB, k->CB, k->SC, k->GS, k->K1, k->Kd, nbf);
+    }
     KHIP(hipGetLastError());
     hipLaunchKernelGGL(kx_final, dim3(zb), dim3(256), 0, s, k->K1, k->Kd, K_dev, nbf);
     KHIP(hipGetLastError());
@@ -949,15 +1010,15 @@ void H2ERI_exchange_workbuf_free(H2ERI_p h) {
 int H2ERI_exchange_plan_check(H2ERI_p h, double *out4) {
     if (!h || !h->U || !h->c_B_blks || !h->c_D_blks) { set_error(ERR_ARG, "exchange_plan_check: nothing built"); return -1; }
     KPlan P;
-    KBuilder b(h, P);
+    KBuilder b(h, P, kstate(h) ? kstate(h)->g_budget : KG_BUDGET_DEFAULT);
     if (!b.build()) return -1;
     if (out4) {
         const double nn2 = (double)h->num_bf * h->num_bf;
         out4[0] = P.flops;
         out4[1] = (double)P.terms.size();
-        out4[2] = 8.0 * ((double)P.PB.size() + (double)P.CB.size() + (double)P.nSC + 4.0 * nn2) +
+        out4[2] = 8.0 * ((double)P.PB.size() + (double)P.CB.size() + (double)P.nSC + (double)P.nG + 5.0 * nn2) +
                   4.0 * ((double)P.bfidx.size() + P.qptr.size() + P.mptr.size()) + 8.0 * ((double)P.qent.size() + P.ment.size());
-        out4[3] = (double)P.nSC;
+        out4[3] = (double)(P.nSC + P.nG);
     }
     return (int)P.terms.size();
 }
@@ -970,7 +1031,21 @@ void H2ERI_exchange_stats(H2ERI_p h, double *out4) {
     out4[0] = k->P.flops;
     out4[1] = (double)k->P.terms.size();
     out4[2] = (double)k->resident;
-    out4[3] = (double)k->P.nSC;
+    out4[3] = (double)(k->P.nSC + k->P.nG);
+}
+
+// Bound of the exchange build's G scratch in bytes (additive, tuning): the
+// block terms run in chunks whose intermediate G fits it, each chunk's ket
+// then its bra, in one scratch region reused by every chunk.  Releases the
+// exchange plan (rebuilt by the next K build).
+void H2ERI_set_exchange_scratch(H2ERI_p h, const double bytes) {
+    if (!h || !(bytes >= 1024.0)) { set_error(ERR_ARG, "H2ERI_set_exchange_scratch: at least 1024 bytes"); return; }
+    if (!h->thread_Kmat_workbuf) h->thread_Kmat_workbuf = (void **)new KState();
+    KState *k = kstate(h);
+    if (k->g_budget == bytes) return;
+    device_release_graph_kind(h, 2);
+    kstate_release(k);
+    k->g_budget = bytes;
 }
 
 }  // extern "C"

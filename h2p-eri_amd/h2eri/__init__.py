@@ -384,6 +384,11 @@ class H2ERI:
         self.lib.H2ERI_exchange_workbuf_free(self.h)
         _check(self.lib)
 
+    def set_exchange_scratch(self, nbytes):
+        """Bound of the exchange build's intermediate G (bytes); the terms run in chunks that fit it."""
+        self.lib.H2ERI_set_exchange_scratch(self.h, float(nbytes))
+        _check(self.lib)
+
     def exchange_stats(self):
         out = np.zeros(4)
         self.lib.H2ERI_exchange_stats(self.h, _dptr(out))

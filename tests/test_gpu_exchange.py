@@ -97,6 +97,27 @@ def test_exchange_density_screening(propane):
     assert relerr(h.build_exchange(Dall) * 1e-3, Ko) > 1e-2
 
 
+def test_exchange_scratch_chunks(propane):
+    """The block terms run in chunks whose intermediate G fits a bound
+    (here 64 KB, so nearly every term is a chunk of its own, one ket and one
+    bra launch each, all in one reused scratch region): K equals the
+    one-chunk build to 1e-14 (FP64 atomics order the adds differently) and
+    the oracle to 1e-12, and the resident scratch shrinks."""
+    h, p, D = propane
+    h.set_exchange_scratch(1 << 40)
+    K1 = h.build_exchange(D)
+    big = h.exchange_stats()["scratch_doubles"]
+    try:
+        h.set_exchange_scratch(65536)
+        Kc = h.build_exchange(D)
+        small = h.exchange_stats()["scratch_doubles"]
+    finally:
+        h.set_exchange_scratch(8 << 30)   # the default
+    assert relerr(Kc, K1) < 1e-14
+    assert relerr(Kc, oracle_bind.Oracle(p).exchange(D)) < 1e-12
+    assert small < big
+
+
 @pytest.mark.parametrize("name", ["propane", "water8"])
 def test_exchange_graph_replay(propane, tmp_path_factory, name):
     """The exchange build captured in a HIP graph (kind 2) and replayed three

@@ -240,5 +240,11 @@ def test_exchange_plan_checks_on_host(kind, size):
     assert r["terms_f"] == r["terms"]
     assert (r["gemm_flops"] > 0) == (s.n_r_adm_pair > 0)
     assert r["scratch_doubles"] > 0
-    # resident bytes include the density and the three K buffers (4 nbf^2 doubles)
-    assert r["resident_bytes"] > 8 * 4 * h.nbf ** 2
+    # resident bytes include the density, its screened copy and the three K buffers
+    assert r["resident_bytes"] > 8 * 5 * h.nbf ** 2
+    # G in chunks of at most 64 KB (nearly a chunk per term): the plan still
+    # passes every check (chunk ranges, one chunk per term, G offsets within
+    # the shared region) and its scratch shrinks
+    h.set_exchange_scratch(65536)
+    r2 = h.exchange_plan_check()
+    assert r2["terms"] == r["terms"] and r2["scratch_doubles"] < r["scratch_doubles"]
