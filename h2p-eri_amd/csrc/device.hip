@@ -3160,6 +3160,9 @@ int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out6) {
 // ---------------------------------------------------------------------------
 struct H2EGroup {
     std::vector<H2ERI_p> hs;
+    std::vector<const double *> xs;
+    std::vector<double *> ys;
+    int mode = 0;                 // 0: one grouped persistent launch, 1: each molecule's own pass in turn
     std::vector<DevPlan> plans;   // per molecule: workspace and task arrays (one vector)
     MolPtrs *d_mols = nullptr;
     QTask *d_q = nullptr;
@@ -3227,6 +3230,8 @@ void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_
         mols[k] = MolPtrs{d->S64, d->S32, dp.Wv, dp.d_t64, dp.d_t32, dp.d_rt, dp.d_vsrc, dp.d_rsrc, X_dev[k], Y_dev[k],
                           (long long)h->krnl_mat_size};
         g->hs.push_back(h);
+        g->xs.push_back(X_dev[k]);
+        g->ys.push_back(Y_dev[k]);
     }
     QPlan Q;
     if (!queue_schedule(graphs, G, Q)) { group_free(g); return nullptr; }
@@ -3252,11 +3257,18 @@ void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_
     return g;
 }
 
-// Enqueues one matvec of every molecule of the group on `stream`.
+// Enqueues one matvec of every molecule of the group on `stream`: one grouped
+// persistent launch (mode 0), or each molecule's own pass in turn with its
+// handle's configuration (mode 1).
 int H2ERI_group_matvec(void *group, void *stream) {
     H2EGroup *g = (H2EGroup *)group;
     if (!g) { set_error(ERR_ARG, "H2ERI_group_matvec: null group"); return -1; }
     hipStream_t s = (hipStream_t)stream;
+    if (g->mode == 1) {
+        for (size_t k = 0; k < g->hs.size(); k++)
+            if (!matvec_multi_dev_impl(g->hs[k], 1, g->xs[k], g->ys[k], s)) return -1;
+        return 0;
+    }
     hipLaunchKernelGGL(queue_reset_kernel, dim3((std::max(g->q_nlist, QC_LISTS) + 255) / 256), dim3(256), 0, s, g->d_ctr,
                        g->q_nlist);
     hipLaunchKernelGGL((queue_kernel<1, true>), dim3(g->q_grid), dim3(WG), 0, s, g->d_q, g->q_n, g->d_need,
@@ -3278,6 +3290,11 @@ int H2ERI_group_status(void *group) {
         set_error(ERR_DEVICE, "H2ERI_group_status: device error");
         return -1;
     }
+    for (H2ERI_p h : g->hs) {   // the molecules' own passes (mode 1)
+        const int r = queue_flags(dstate(h));
+        if (r < 0) return -1;
+        if (r > 0) e = 1;
+    }
     if (!e) return 0;
     const int zero = 0;
     if (hipMemcpy(g->d_ctr + QC_ERROR, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return -1;
@@ -3287,6 +3304,48 @@ int H2ERI_group_status(void *group) {
 
 void H2ERI_group_destroy(void *group) {
     if (group) group_free((H2EGroup *)group);
+}
+
+// Mode of H2ERI_group_matvec: 0 one grouped persistent launch, 1 each
+// molecule's own pass in turn.  Returns 0, -1 on a bad argument.
+int H2ERI_group_set_mode(void *group, const int mode) {
+    H2EGroup *g = (H2EGroup *)group;
+    if (!g || mode < 0 || mode > 1) { set_error(ERR_ARG, "H2ERI_group_set_mode: bad argument"); return -1; }
+    g->mode = mode;
+    return 0;
+}
+
+// Times both modes (reps group matvecs after a warm-up each, HIP events on
+// the null stream) and keeps the faster: the grouped launch wins when the
+// molecules are small and leave the GPU idle alone, the molecules' own passes
+// when each fills the GPU.  out3 = {ms of mode 0, ms of mode 1, mode kept}.
+// Returns 0, -1 on error.
+int H2ERI_group_autotune(void *group, const int reps, double *out3) {
+    H2EGroup *g = (H2EGroup *)group;
+    if (!g || reps < 1) { set_error(ERR_ARG, "H2ERI_group_autotune: bad argument"); return -1; }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { set_error(ERR_DEVICE, "group autotune: events"); return -1; }
+    double ms[2] = {0.0, 0.0};
+    bool ok = true;
+    for (int m = 0; ok && m < 2; m++) {
+        g->mode = m;
+        ok = H2ERI_group_matvec(g, nullptr) == 0 && hipEventRecord(e0, nullptr) == hipSuccess;
+        for (int r = 0; ok && r < reps; r++) ok = H2ERI_group_matvec(g, nullptr) == 0;
+        float t = 0.f;
+        ok = ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+        ms[m] = t / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!ok) {
+        if (!has_error()) set_error(ERR_DEVICE, "group autotune: timing failed");
+        return -1;
+    }
+    if (H2ERI_group_status(g) != 0) return -1;
+    g->mode = ms[1] < ms[0] ? 1 : 0;
+    if (out3) { out3[0] = ms[0]; out3[1] = ms[1]; out3[2] = g->mode; }
+    return 0;
 }
 
 size_t h2eri_struct_size(void) { return sizeof(struct H2ERI); }

@@ -69,6 +69,18 @@ class Group:
         _check(self.lib)
         return r
 
+    def set_mode(self, mode):
+        """0: one grouped persistent launch; 1: each molecule's own pass in turn."""
+        self.lib.H2ERI_group_set_mode(self.g, int(mode))
+        _check(self.lib)
+
+    def autotune(self, reps=5):
+        """Times both modes and keeps the faster: {mode0_ms, mode1_ms, mode}."""
+        out = np.zeros(3)
+        self.lib.H2ERI_group_autotune(self.g, int(reps), _dptr(out))
+        _check(self.lib)
+        return dict(mode0_ms=float(out[0]), mode1_ms=float(out[1]), mode=int(out[2]))
+
     def close(self):
         if self.g:
             self.lib.H2ERI_group_destroy(self.g)

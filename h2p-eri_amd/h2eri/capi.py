@@ -112,6 +112,8 @@ SIGNATURES = {
     "H2ERI_group_matvec": (C.c_int, [C.c_void_p, C.c_void_p]),
     "H2ERI_group_destroy": (None, [C.c_void_p]),
     "H2ERI_group_status": (C.c_int, [C.c_void_p]),
+    "H2ERI_group_set_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "H2ERI_group_autotune": (C.c_int, [C.c_void_p, C.c_int, c_dbl_p]),
     "H2ERI_plan_check": (C.c_int, [H2ERI_p, C.c_int]),
     "H2ERI_plan_emulate": (C.c_int, [H2ERI_p, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
     "H2ERI_set_executor": (None, [H2ERI_p, C.c_int]),

@@ -434,6 +434,13 @@ void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_
 int   H2ERI_group_matvec(void *group, void *stream);
 int   H2ERI_group_status(void *group);
 void  H2ERI_group_destroy(void *group);
+/* Mode of H2ERI_group_matvec (additive): 0 (default) one grouped persistent
+ * launch, 1 each molecule's own pass in turn with its handle's configuration
+ * (faster when every molecule fills the GPU alone).  H2ERI_group_autotune
+ * times both (reps calls each) and keeps the faster: out3 = {ms of mode 0,
+ * ms of mode 1, mode kept}.  Both return 0, -1 on error. */
+int   H2ERI_group_set_mode(void *group, const int mode);
+int   H2ERI_group_autotune(void *group, const int reps, double *out3);
 
 /* Autotuner: times nv-vector device matvecs (reps calls after a warm-up)
  * for every executor (H2ERI_set_executor) x tile size {64, 128, 256, 512} KB x

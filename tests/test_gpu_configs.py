@@ -122,6 +122,24 @@ def test_molecule_group(tmp_path_factory):
             hs[k].set_executor(1)
             assert np.array_equal(Y[k].cpu().numpy(), hs[k].matvec(xs[k]))
             hs[k].set_executor(0)
+    # mode 1: each molecule's own pass (its handle's configuration, here the
+    # staged executor): bitwise its own matvec; the tuner keeps either mode
+    g.set_mode(1)
+    for y in Y:
+        y.zero_()
+    g.matvec(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert g.status() == 0
+    for k in range(3):
+        assert np.array_equal(Y[k].cpu().numpy(), hs[k].matvec(xs[k]))
+    res = g.autotune(reps=2)
+    assert res["mode"] in (0, 1) and res["mode0_ms"] > 0 and res["mode1_ms"] > 0
+    for y in Y:
+        y.zero_()
+    g.matvec(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    for k in range(3):
+        assert relerr(Y[k].cpu().numpy(), oracle_bind.Oracle(paths[k]).matvec(xs[k])) < 1e-13
     g.close()
 
 
