@@ -3611,9 +3611,34 @@ int H2ERI_plan_stats(H2ERI_p h, const int ub_bits, const int nv, double *out8) {
         if (r.flags & 1) ext += r.len;
         else wr_r += r.len;
     }
-    if (getenv("H2ERI_PLAN_STATS_DETAIL"))
+    if (getenv("H2ERI_PLAN_STATS_DETAIL")) {
         fprintf(stderr, "plan nv %d: tiles write %.3g read %.3g, reduce tasks write %.3g read %.3g elements per vector\n",
                 nv, wr, rd, wr_r, rd_r);
+        // per phase: tile writes / reads, reduce writes / reads
+        double pw[PH_N + 1] = {0}, pr[PH_N + 1] = {0}, qw[PH_N + 1] = {0}, qr[PH_N + 1] = {0};
+        for (const Stage &st : P.stages) {
+            const int ph = std::min(std::max(st.phase, 0), (int)PH_N);
+            for (int pass = 0; pass < 2; pass++) {
+                const std::vector<BTask> &L = pass == 0 ? P.t64 : P.t32;
+                const int b = pass == 0 ? st.t64_beg : st.t32_beg, n = pass == 0 ? st.t64_cnt : st.t32_cnt;
+                for (int i = b; i < b + n; i++) {
+                    const BTask &t = L[i];
+                    if (t.out0 >= 0) { pw[ph] += t.nrow; if (P.vsrc[t.v0b] >= 0) pr[ph] += (double)t.ncol * (t.v0e - t.v0b); }
+                    if (t.out1 >= 0) { pw[ph] += t.ncol; if (P.vsrc[t.v1b] >= 0) pr[ph] += (double)t.nrow * (t.v1e - t.v1b); }
+                }
+            }
+            for (int i = st.r_beg; i < st.r_beg + st.r_cnt; i++) {
+                const RTask &r = P.rt[i];
+                qr[ph] += (double)r.len * (r.src_end - r.src_beg);
+                if (!(r.flags & 1)) qw[ph] += r.len;
+            }
+        }
+        static const char *nm[] = {"up", "B", "y1", "down", "D", "gather", "?"};
+        for (int ph = 0; ph <= PH_N; ph++)
+            if (pw[ph] + pr[ph] + qw[ph] + qr[ph] > 0)
+                fprintf(stderr, "  phase %s: tiles write %.3g read %.3g, reduces write %.3g read %.3g\n", nm[std::min(ph, 6)],
+                        pw[ph], pr[ph], qw[ph], qr[ph]);
+    }
     wr += wr_r;
     rd += rd_r;
     out8[0] = (double)(P.t64.size() + P.t32.size() + P.w64.size() + P.w32.size());
