@@ -460,7 +460,7 @@ def single_workload(args):
             rec = json.load(f)
         # a recording of another launch structure does not describe this plan
         if rec.get("launches_per_matvec") == len(prof) and rec.get("kernel") == kernel_name:
-            traffic = rec["traffic_bytes_per_matvec"]
+            traffic = rec.get("traffic_bytes_per_matvec")
     b64, b32 = r64["batch"], r32["batch"]
     fp64_plain = r64["ms1"]
     out = {
@@ -506,7 +506,11 @@ def single_workload(args):
                      "kernel_ms_union": round(r32["union_ms"], 6),
                      "kernel_ms_sum": round(r32["sum_ms"], 6),
                      "avg_launch_ms": round(r32["sum_ms"] / len(prof), 6),
-                     "streamed_matrix_bytes_per_matvec": sum(p["mat_bytes"] for p in prof)},
+                     "streamed_matrix_bytes_per_matvec": sum(p["mat_bytes"] for p in prof),
+                     # the bytes the plan actually streams (a low-rank block's smaller half twice)
+                     # over the same union: how close the launches come to the HBM peak
+                     "streamed_frac": round(sum(p["mat_bytes"] for p in prof) / (r32["union_ms"] * 1e6)
+                                            / HBM_PEAK_GBS, 4)},
         "phases_fp32": phases,
         "cpu_baseline": cpu,
     }
