@@ -451,22 +451,25 @@ struct KPlan {
     double flops = 0;
 };
 
-// Default bound of the G scratch (bytes): the terms run in chunks whose G
-// fits it, each chunk's ket then its bra, all chunks in one scratch region.
-// Measured K per build against the bound (unbounded / 4 GiB / 1 GiB /
-// 256 MiB): C30H62 12.2 / 13.1 / 16.7 / 33.1 ms with 16.0 / 6.0 / 2.8 / 2.0 GB
-// resident; (H2O)64 75.7 ms unbounded (76 GB), 90.1 ms at 1 GiB (8.5 GB).
-// Smaller chunks give launches too few tasks to fill the GPU, and the G of a
-// chunk is not re-read from the last-level cache often enough to make up for
-// it.  8 GiB keeps every benchmark molecule near 10 GB for a few percent.
-static const double KG_BUDGET_DEFAULT = 8.0 * 1024.0 * 1024.0 * 1024.0;
+// Default bound of the G scratch of one chunk (bytes): the terms run in
+// chunks whose G fits it, each chunk's ket then its bra; with two or more
+// chunks, chunk c + 1's ket runs beside chunk c's bra on a second stream, in
+// a second region (the scratch is twice the bound).  Measured K per build
+// against the bound (unbounded / 4 / 2 / 1 GiB / 256 MiB): C30H62 12.2 /
+// 12.7 / 13.1 / 13.9 / 22.2 ms with 16.0 / 10.3 / 6.0 / 3.9 / 2.3 GB
+// resident; (H2O)64 75.7 ms unbounded (76 GB), 80.1 ms at 4 and at 1 GiB
+// (16.0 and 9.6 GB).  On one stream the chunks cost more (C30H62 16.7 ms at
+// 1 GiB): a launch of a small chunk has too few tasks to fill the GPU.
+static const double KG_BUDGET_DEFAULT = 2.0 * 1024.0 * 1024.0 * 1024.0;
 
 struct KState {
     KPlan P;
     bool ready = false;
     double g_budget = KG_BUDGET_DEFAULT;   // bytes of the G scratch (H2ERI_set_exchange_scratch)
     double *PB = nullptr, *CB = nullptr, *SC = nullptr, *K1 = nullptr, *Kd = nullptr, *D = nullptr, *K = nullptr;
-    double *GS = nullptr;   // G of one chunk of terms
+    double *GS = nullptr;   // G of two chunks of terms (chunk c in region c % 2)
+    hipStream_t side = nullptr;                        // odd chunks, beside the caller's stream
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double *Ds = nullptr;   // the density after screening (the kernels' D)
     int *sbf = nullptr;     // shell_bf_sidx
     KTerm *terms = nullptr;
@@ -490,6 +493,9 @@ static void kstate_release(KState *k) {
                    (void **)&k->nent, (void **)&k->ket,
                    (void **)&k->bra, (void **)&k->direct, (void **)&k->gtasks, (void **)&k->Ds, (void **)&k->sbf};
     for (void **p : ps) kfree(*p);
+    if (k->side) { (void)hipStreamDestroy(k->side); k->side = nullptr; }
+    if (k->ev_fork) { (void)hipEventDestroy(k->ev_fork); k->ev_fork = nullptr; }
+    if (k->ev_join) { (void)hipEventDestroy(k->ev_join); k->ev_join = nullptr; }
     k->ready = false;
     k->P = KPlan();
 }
@@ -869,7 +875,8 @@ static bool k_upload(H2ERI_p h) {
     const size_t nn2 = (size_t)nbf * nbf;
     if (!up((void **)&k->PB, P.PB.empty() ? nullptr : P.PB.data(), P.PB.size() * 8) ||
         !up((void **)&k->CB, P.CB.empty() ? nullptr : P.CB.data(), P.CB.size() * 8) ||
-        !up((void **)&k->SC, nullptr, (size_t)P.nSC * 8) || !up((void **)&k->GS, nullptr, (size_t)P.nG * 8) ||
+        !up((void **)&k->SC, nullptr, (size_t)P.nSC * 8) ||
+        !up((void **)&k->GS, nullptr, (size_t)P.nG * 8 * (P.ket_chunk.size() > 2 ? 2 : 1)) ||
         !up((void **)&k->K1, nullptr, nn2 * 8) ||
         !up((void **)&k->Kd, nullptr, nn2 * 8) || !up((void **)&k->D, nullptr, nn2 * 8) ||
         !up((void **)&k->K, nullptr, nn2 * 8) || !up((void **)&k->Ds, nullptr, nn2 * 8) ||
@@ -888,6 +895,11 @@ static bool k_upload(H2ERI_p h) {
         !up((void **)&k->direct, P.direct.empty() ? nullptr : P.direct.data(), P.direct.size() * sizeof(KTask)) ||
         !up((void **)&k->gtasks, P.gtasks.empty() ? nullptr : P.gtasks.data(), P.gtasks.size() * sizeof(KGemmTask)))
         return false;
+    if (P.ket_chunk.size() > 2) {   // two or more chunks: odd ones run on a second stream
+        KHIP(hipStreamCreateWithFlags(&k->side, hipStreamNonBlocking));
+        KHIP(hipEventCreateWithFlags(&k->ev_fork, hipEventDisableTiming));
+        KHIP(hipEventCreateWithFlags(&k->ev_join, hipEventDisableTiming));
+    }
     // L / R products do not depend on D: computed once at upload
     if (!P.gtasks.empty()) {
         hipLaunchKernelGGL(kx_gemm, dim3((unsigned)P.gtasks.size()), dim3(KWG), 0, 0, k->gtasks, k->gemms, k->PB, k->CB, k->SC);
@@ -911,21 +923,36 @@ static bool k_run(H2ERI_p h, const double *D_dev, double *K_dev, hipStream_t s) 
     hipLaunchKernelGGL(kx_zero, dim3(zb), dim3(256), 0, s, k->K1, nn2);
     hipLaunchKernelGGL(kx_zero, dim3(zb), dim3(256), 0, s, k->Kd, nn2);
     KHIP(hipGetLastError());
+    // every chunk of terms: its ket writes G, its bra reads it back.  Chunk c
+    // uses G region c % 2 on stream c % 2 (the caller's, or the second stream
+    // forked from it after the zeroing), so chunk c + 1's ket runs beside
+    // chunk c's bra and a region is reused only after its previous bra; both
+    // streams add into K1 / Kd with atomics and join before kx_final.
+    const int nchunk = (int)P.ket_chunk.size() - 1;
+    const bool two = nchunk > 1;
+    if (two) {
+        KHIP(hipEventRecord(k->ev_fork, s));
+        KHIP(hipStreamWaitEvent(k->side, k->ev_fork, 0));
+    }
     if (!P.direct.empty())
         hipLaunchKernelGGL(kx_direct, dim3((unsigned)P.direct.size()), dim3(KWG), 0, s, k->direct, k->terms, k->nodes,
                            k->bfidx, k->qptr, k->qent, k->mptr, k->ment, k->CB, D_dev, k->K1, k->Kd, nbf);
-    // every chunk of terms: its ket writes G into the shared scratch, its bra
-    // reads it back (same stream, so the next chunk's ket waits for it)
-    for (size_t c = 0; c + 1 < P.ket_chunk.size(); c++) {
+    for (int c = 0; c < nchunk; c++) {
+        hipStream_t q = (two && (c & 1)) ? k->side : s;
+        double *G = k->GS + (two && (c & 1) ? P.nG : 0);
         const int kb = P.ket_chunk[c], ke = P.ket_chunk[c + 1], bb = P.bra_chunk[c], be = P.bra_chunk[c + 1];
         if (ke > kb)
-            hipLaunchKernelGGL(kx_ket, dim3((unsigned)(ke - kb)), dim3(KWG), 0, s, k->ket + kb, k->terms, k->nodes,
-                               k->bfidx, k->qptr, k->qent, k->PB, k->CB, k->SC, k->GS, D_dev, nbf);
+            hipLaunchKernelGGL(kx_ket, dim3((unsigned)(ke - kb)), dim3(KWG), 0, q, k->ket + kb, k->terms, k->nodes,
+                               k->bfidx, k->qptr, k->qent, k->PB, k->CB, k->SC, G, D_dev, nbf);
         if (be > bb)
-            hipLaunchKernelGGL(kx_bra, dim3((unsigned)(be - bb)), dim3(KWG), 0, s, k->bra + bb, k->terms, k->nodes,
-                               k->bfidx, k->nspl, k->nent, k->PB, k->CB, k->SC, k->GS, k->K1, k->Kd, nbf);
+            hipLaunchKernelGGL(kx_bra, dim3((unsigned)(be - bb)), dim3(KWG), 0, q, k->bra + bb, k->terms, k->nodes,
+                               k->bfidx, k->nspl, k->nent, k->PB, k->CB, k->SC, G, k->K1, k->Kd, nbf);
     }
     KHIP(hipGetLastError());
+    if (two) {
+        KHIP(hipEventRecord(k->ev_join, k->side));
+        KHIP(hipStreamWaitEvent(s, k->ev_join, 0));
+    }
     hipLaunchKernelGGL(kx_final, dim3(zb), dim3(256), 0, s, k->K1, k->Kd, K_dev, nbf);
     KHIP(hipGetLastError());
     return true;
@@ -1016,9 +1043,10 @@ int H2ERI_exchange_plan_check(H2ERI_p h, double *out4) {
         const double nn2 = (double)h->num_bf * h->num_bf;
         out4[0] = P.flops;
         out4[1] = (double)P.terms.size();
-        out4[2] = 8.0 * ((double)P.PB.size() + (double)P.CB.size() + (double)P.nSC + (double)P.nG + 5.0 * nn2) +
+        const double nG = (double)P.nG * (P.ket_chunk.size() > 2 ? 2 : 1);   // two regions with two or more chunks
+        out4[2] = 8.0 * ((double)P.PB.size() + (double)P.CB.size() + (double)P.nSC + nG + 5.0 * nn2) +
                   4.0 * ((double)P.bfidx.size() + P.qptr.size() + P.mptr.size()) + 8.0 * ((double)P.qent.size() + P.ment.size());
-        out4[3] = (double)(P.nSC + P.nG);
+        out4[3] = (double)P.nSC + nG;
     }
     return (int)P.terms.size();
 }
@@ -1031,7 +1059,7 @@ void H2ERI_exchange_stats(H2ERI_p h, double *out4) {
     out4[0] = k->P.flops;
     out4[1] = (double)k->P.terms.size();
     out4[2] = (double)k->resident;
-    out4[3] = (double)(k->P.nSC + k->P.nG);
+    out4[3] = (double)(k->P.nSC + k->P.nG * (k->P.ket_chunk.size() > 2 ? 2 : 1));
 }
 
 // Bound of the exchange build's G scratch in bytes (additive, tuning): the

@@ -251,10 +251,12 @@ void H2ERI_exchange_dlist(H2ERI_p h2eri, const double *den_mat);
 void H2ERI_build_exchange_dev(H2ERI_p h2eri, const double *den_dev, double *K_dev, void *stream);
 void H2ERI_exchange_stats(H2ERI_p h2eri, double *out4);
 /* Additive (tuning): bound in bytes of the exchange build's intermediate G
- * (default 8 GiB).  The block terms run in chunks whose G fits it, each
- * chunk's ket contraction then its bra contraction, in one scratch region
- * that every chunk reuses; a term larger than the bound is a chunk of its
- * own.  Releases the exchange plan and an exchange graph. */
+ * per chunk (default 2 GiB).  The block terms run in chunks whose G fits it,
+ * each chunk's ket contraction then its bra contraction; with two or more
+ * chunks, odd chunks run on a second stream in a second scratch region, so a
+ * chunk's ket overlaps the previous chunk's bra (scratch: twice the bound).
+ * A term larger than the bound is a chunk of its own.  Releases the exchange
+ * plan and an exchange graph. */
 void H2ERI_set_exchange_scratch(H2ERI_p h2eri, const double bytes);
 /* Additive: the exchange plan built and bounds-checked on the host only (no
  * HIP call); out4 as above with out4[2] the bytes it would make resident.

@@ -99,10 +99,12 @@ def test_exchange_density_screening(propane):
 
 def test_exchange_scratch_chunks(propane):
     """The block terms run in chunks whose intermediate G fits a bound
-    (here 64 KB, so nearly every term is a chunk of its own, one ket and one
-    bra launch each, all in one reused scratch region): K equals the
-    one-chunk build to 1e-14 (FP64 atomics order the adds differently) and
-    the oracle to 1e-12, and the resident scratch shrinks."""
+    (here 64 KB, so nearly every term is a chunk of its own; even chunks on
+    the caller's stream, odd ones on a second stream in a second region):
+    K equals the one-chunk build to 1e-14 (FP64 atomics order the adds
+    differently) and the oracle to 1e-12, from host and device pointers and
+    as a graph replayed twice, and the resident scratch shrinks."""
+    import torch
     h, p, D = propane
     h.set_exchange_scratch(1 << 40)
     K1 = h.build_exchange(D)
@@ -111,8 +113,21 @@ def test_exchange_scratch_chunks(propane):
         h.set_exchange_scratch(65536)
         Kc = h.build_exchange(D)
         small = h.exchange_stats()["scratch_doubles"]
+        Dd = torch.from_numpy(D).to("cuda:0")
+        Kd = torch.empty_like(Dd)
+        s = torch.cuda.current_stream().cuda_stream
+        h.build_exchange_dev(Dd.data_ptr(), Kd.data_ptr(), s)
+        torch.cuda.synchronize()
+        assert relerr(Kd.cpu().numpy(), Kc) < 1e-14
+        Kg = torch.empty_like(Dd)
+        h.graph_capture("exchange", 1, Dd.data_ptr(), Kg.data_ptr())
+        for _ in range(2):
+            Kg.zero_()
+            h.graph_replay(s)
+            torch.cuda.synchronize()
+            assert relerr(Kg.cpu().numpy(), Kc) < 1e-14
     finally:
-        h.set_exchange_scratch(8 << 30)   # the default
+        h.set_exchange_scratch(2 << 30)   # the default
     assert relerr(Kc, K1) < 1e-14
     assert relerr(Kc, oracle_bind.Oracle(p).exchange(D)) < 1e-12
     assert small < big

@@ -248,3 +248,20 @@ def test_exchange_plan_checks_on_host(kind, size):
     h.set_exchange_scratch(65536)
     r2 = h.exchange_plan_check()
     assert r2["terms"] == r["terms"] and r2["scratch_doubles"] < r["scratch_doubles"]
+
+
+def test_traffic_records_are_readable():
+    """Every committed HBM-traffic record carries the fields the benchmark
+    reads (kernel, launches per matvec, traffic bytes per matvec), so a
+    record never breaks the benchmark line it feeds."""
+    import glob
+    import json
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    files = glob.glob(os.path.join(root, "profiles", "traffic_*_j_matvec.json"))
+    assert files
+    for f in files:
+        with open(f) as fh:
+            rec = json.load(fh)
+        assert isinstance(rec["kernel"], str) and rec["launches_per_matvec"] >= 1, f
+        assert rec["traffic_bytes_per_matvec"] > 0, f
