@@ -639,6 +639,7 @@ __device__ void bgemv_tile_valu(const BTask &task, const T *__restrict__ S, cons
 // meet zero operands; their outputs are not stored.
 typedef double d4v __attribute__((ext_vector_type(4)));
 static const int T16_MAXC = 128;   // columns of a tile with a column product (two spans of 64)
+static const int T16_MAXC2 = 64;   // columns of a tile with both products (column sums in registers)
 static const int T16_LDS = LDS_DOUBLES;   // staging doubles of a 16-vector tile (X and Y)
 
 // Four consecutive elements from c0 (clamped so no load leaves the row's
@@ -685,7 +686,7 @@ __device__ __forceinline__ double in16_get(const In16 &in, int i, int v, const d
     return sum;
 }
 
-// A tile with both products (at most 128 columns, any number of rows), in one
+// A tile with both products (at most 64 columns, any number of rows), in one
 // walk over its rows.  Waves take blocks of 16 rows (block b of the tile on
 // wave b % 4).  For a group of 16 columns, lane l loads columns
 // 4(l/16) .. 4(l/16)+3 of row l%16 of the block: the row product's A
@@ -696,7 +697,7 @@ __device__ __forceinline__ double in16_get(const In16 &in, int i, int v, const d
 // memory once per block (element-major: 16 lanes read one 128-byte run).
 // Each wave keeps its column sums in registers, one accumulator per group of
 // 16 columns, until the end of the tile; then the four waves add them in LDS
-// in a fixed order, 64 columns at a time.
+// in a fixed order.
 template <typename T, bool COH>
 __device__ void mfma_tile16_2s(const BTask &t, const T *__restrict__ base, const int *__restrict__ vsrc,
                                double *__restrict__ Wv, const double *__restrict__ Xt, double *lds) {
@@ -706,16 +707,16 @@ __device__ void mfma_tile16_2s(const BTask &t, const T *__restrict__ base, const
     const int l16 = lane & 15, lq = lane >> 4;
     const int R = t.nrow, C = t.ncol;
     const int cmax = ((C - 1) / VW) * VW;
-    const int Cp = (C + 15) & ~15, ng = Cp >> 4;   // groups of 16 columns, at most 8
+    const int Cp = (C + 15) & ~15, ng = Cp >> 4;   // groups of 16 columns, at most 4
     double *xin = lds;                                // Cp x 16, zero past C
     double *tr = lds + T16_MAXC * NV + 256 * w;       // this wave's 16 x 16 transposition tile
     stage_vec<NV, COH, WG>(vsrc, t.v0b, t.v0e, t.v0off, C, Wv, Xt, -1, xin, tid);
     for (int q = C * NV + tid; q < Cp * NV; q += WG) xin[q] = 0.0;
     __syncthreads();
     const In16 y = in16_make(vsrc, t.v1b, t.v1e, t.v1off);
-    d4v acc1[8];
+    d4v acc1[T16_MAXC2 / 16];
     #pragma unroll
-    for (int g = 0; g < 8; g++) acc1[g] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int g = 0; g < T16_MAXC2 / 16; g++) acc1[g] = d4v{0.0, 0.0, 0.0, 0.0};
     for (int rb = 16 * w; rb < R; rb += 64) {
         double ya[4];   // Y at rows rb + 4q + l/16 (zero past the tile), vector l%16
         #pragma unroll
@@ -726,7 +727,7 @@ __device__ void mfma_tile16_2s(const BTask &t, const T *__restrict__ base, const
         const T *row = base + (long long)min(rb + l16, R - 1) * t.ld;
         d4v acc0 = d4v{0.0, 0.0, 0.0, 0.0};
         #pragma unroll
-        for (int g = 0; g < 8; g++) {
+        for (int g = 0; g < T16_MAXC2 / 16; g++) {
             if (g < ng) {
                 const int c0 = 16 * g + 4 * lq;
                 double a[4];
@@ -751,20 +752,13 @@ __device__ void mfma_tile16_2s(const BTask &t, const T *__restrict__ base, const
     __syncthreads();   // X and the transposition tiles are dead from here on
     double *red = lds;  // per wave: 64 columns x 16 vectors
     #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        if (64 * h < C) {   // uniform
-            #pragma unroll
-            for (int g4 = 0; g4 < 4; g4++)
-                #pragma unroll
-                for (int r = 0; r < 4; r++) red[w * 1024 + (16 * g4 + l16) * NV + lq + 4 * r] = acc1[4 * h + g4][r];
-            __syncthreads();
-            const int nq = min(64, C - 64 * h) * NV;
-            for (int q = tid; q < nq; q += WG) {
-                const double v = (red[q] + red[1024 + q]) + (red[2048 + q] + red[3072 + q]);
-                wst<COH>(Wv + (long long)(t.out1 + 64 * h) * NV + q, v);
-            }
-            __syncthreads();
-        }
+    for (int g4 = 0; g4 < T16_MAXC2 / 16; g4++)
+        #pragma unroll
+        for (int r = 0; r < 4; r++) red[w * 1024 + (16 * g4 + l16) * NV + lq + 4 * r] = acc1[g4][r];
+    __syncthreads();
+    for (int q = tid; q < C * NV; q += WG) {
+        const double v = (red[q] + red[1024 + q]) + (red[2048 + q] + red[3072 + q]);
+        wst<COH>(Wv + (long long)t.out1 * NV + q, v);
     }
 }
 
@@ -952,10 +946,11 @@ __device__ void reduce_task(const RTask &task, const int *__restrict__ src, doub
 // One launch per stage: n64 FP64 tiles, then n32 FP32 tiles, then nr reduce
 // tasks, then the wave tiles, four per workgroup (FP64 first), all
 // independent of each other within the launch.
-// Waves per SIMD the kernels are built for: 4 (at most 128 VGPRs); at 16
-// vectors per pass 3, for the column sums a two-sided MFMA tile keeps in
-// registers.
-template <int NV> constexpr int waves_per_simd() { return NV == 16 ? 3 : 4; }
+// Waves per SIMD the kernels are built for: 4 (at most 128 VGPRs).  A
+// two-sided 16-vector tile keeps the column sums of its (at most 64)
+// columns in registers; with 128 columns it needed 3 waves per SIMD and ran
+// 6% slower (C100H202, 16 densities: 3.60 against 3.38 ms).
+template <int NV> constexpr int waves_per_simd() { return 4; }
 
 // Task b of a staged launch: tiles (FP64, FP32), reduce tasks, then groups of
 // four wave tiles.
@@ -1569,7 +1564,7 @@ struct Planner {
     // partial sums over row strips are short (the strip's width).  A block
     // with both outputs wider than 256 columns is a set of row strips plus a
     // set of column tiles (the matrix read twice, no partial row sums); a
-    // narrower one takes two-sided tiles of at most 128 columns by row strips
+    // narrower one takes two-sided tiles of at most 64 columns by row strips
     // of about tile_bytes, which read their rows once.
     void add_matrix16(int st, const MatRef &m, int r0_, int nrow, int v0, int o0, int v1, int o1, int d0, int d1) {
         const int ncol = m.ncol;
@@ -1581,8 +1576,9 @@ struct Planner {
         }
         int nct = 1, ctw = ncol;
         if (d1 >= 0) {
-            nct = (ncol + T16_MAXC - 1) / T16_MAXC;
-            ctw = std::min(T16_MAXC, ((ncol + nct - 1) / nct + 7) / 8 * 8);
+            const int mc = d0 >= 0 ? T16_MAXC2 : T16_MAXC;
+            nct = (ncol + mc - 1) / mc;
+            ctw = std::min(mc, ((ncol + nct - 1) / nct + 7) / 8 * 8);
         }
         int rows = std::max(64, (int)(tile_bytes / (esz * ctw)));
         // a column product alone stages Y in LDS (256 rows); a tile with both
@@ -1969,9 +1965,9 @@ struct Planner {
             for (const BTask &t : tl) {
                 bool ok = t.nrow > 0 && t.ncol > 0 && t.ld >= t.ncol && t.ld % W == 0 && t.mat % W == 0 && t.mat >= 0 &&
                           (t.out0 >= 0 || t.out1 >= 0);
-                if (nv == 16) {   // the MFMA tile: two spans of 64 columns, Y staged for a column product alone
+                if (nv == 16) {   // the MFMA tile: 64 columns with both products, 128 with a column product alone
                     const int Rp = (t.nrow + 15) & ~15;
-                    ok = ok && (t.out1 < 0 || (t.ncol <= T16_MAXC && (t.out0 >= 0 || Rp * 16 <= T16_LDS)));
+                    ok = ok && (t.out1 < 0 || (t.out0 >= 0 ? t.ncol <= T16_MAXC2 : (t.ncol <= T16_MAXC && Rp * 16 <= T16_LDS)));
                 } else {
                     ok = ok && t.ncol <= maxc && t.ncol * nv <= xcap && (t.lpr & (t.lpr - 1)) == 0 && t.lpr >= 4 &&
                          t.lpr <= 64 && t.lpr * cpl_of(nv) >= t.ncol && (t.out1 < 0 || t.nrow * nv <= ycap);
