@@ -73,6 +73,18 @@ def test_errors_are_reported_not_aborting(tmp_path):
     h3 = make_synthetic(tmp_path / "x.rep", 300, 2, leaf=20)
     with pytest.raises(h2eri.H2ERIError):
         h3.build_h2(bd_jit=1)
+    # tuning setters refuse out-of-range values (no GPU involved)
+    with pytest.raises(h2eri.H2ERIError):
+        h3.set_near_field_grid(-1)
+    with pytest.raises(h2eri.H2ERIError):
+        h3.set_exchange_scratch(100)
+    with pytest.raises(h2eri.H2ERIError):
+        h3.set_schedule(258)
+    with pytest.raises(h2eri.H2ERIError):
+        h3.set_wave_tile_bytes(-1)
+    h3.set_near_field_grid(0)
+    h3.set_exchange_scratch(2 << 30)
+    h3.set_schedule(257)
 
 
 def _rep_offsets(data):
