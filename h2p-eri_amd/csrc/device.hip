@@ -735,9 +735,10 @@ __device__ void mfma_tile16_2s(const BTask &t, const T *__restrict__ base, const
                 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], xin[(c0 + j) * NV + l16], acc0, 0, 0, 0);
-                double2 *tw = reinterpret_cast<double2 *>(tr + l16 * 16 + 4 * lq);
-                tw[0] = double2{a[0], a[1]};
-                tw[1] = double2{a[2], a[3]};
+                // written and read back as doubles through the same pointer,
+                // so the compiler keeps the reads after the writes
+                #pragma unroll
+                for (int j = 0; j < 4; j++) tr[l16 * 16 + 4 * lq + j] = a[j];
                 #pragma unroll
                 for (int q = 0; q < 4; q++)
                     acc1[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[q], tr[(4 * q + lq) * 16 + l16], acc1[g], 0, 0, 0);
