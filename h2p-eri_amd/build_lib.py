@@ -19,7 +19,7 @@ BUILD = os.path.join(ROOT, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HOST_SOURCES = ["common.cpp", "eri.cpp", "idqr.cpp", "shellops.cpp", "partition.cpp", "build_h2.cpp", "rep_io.cpp"]
-DEVICE_SOURCES = ["device.hip", "exchange.hip"]
+DEVICE_SOURCES = ["device.hip", "exchange.hip", "jit.hip"]
 
 HOST_FLAGS = ["-O3", "-fPIC", "-fopenmp", "-std=c++17", "-march=x86-64-v3", "-Wall", "-Wno-unused-function"]
 DEVICE_FLAGS = [

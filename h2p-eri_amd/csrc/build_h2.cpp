@@ -104,6 +104,11 @@ void eri_pairs_to_mat(HostState *hs, const int *bra, int nbra, const int *ket, i
 
 // Compress a block into [U | V^T] when it pays (src/H2ERI_build_H2.c:843-903).
 H2E_dense_mat_p compress_blk(int nrow, int ncol, std::vector<double> &blk, double tol) {
+    if (tol < 0.0) {   // stored dense, as the on-the-fly form evaluates it
+        H2E_dense_mat_p res = dmat_new(nrow, ncol);
+        if ((long long)nrow * ncol > 0) memcpy(res->data, blk.data(), sizeof(double) * (size_t)nrow * ncol);
+        return res;
+    }
     std::vector<double> A(blk);
     std::vector<double> U;
     std::vector<int> J;
@@ -505,6 +510,7 @@ void update_mat_sizes(H2ERI_p h) {
     }
     for (int i = 0; i < h->n_D; i++) {
         H2E_dense_mat_p D = h->c_D_blks[i];
+        if (!D) continue;   // BD_JIT: evaluated during the matvec, not stored
         size_t e = blk_elems(D);
         ms[D_SIZE_IDX] += e;
         ms[MV_DEN_SIZE_IDX] += e + (size_t)(i < h->n_leaf_node ? 1 : 2) * (D->nrow + D->ncol);
@@ -517,21 +523,34 @@ using namespace h2e;
 
 extern "C" {
 
+// BD_JIT = 1 (src/H2ERI_build_H2.c:1406-1411, :1016, :1265): U, the skeleton
+// lists and the B/D shapes only; the matvec evaluates every B and D block from
+// the shell pairs when it needs it, and c_B_blks / c_D_blks hold no blocks.
 void H2ERI_build_H2(H2ERI_p h, const int BD_JIT) {
     if (!h || h->n_node <= 0) { set_error(ERR_ARG, "H2ERI_build_H2: call H2ERI_partition first"); return; }
-    if (BD_JIT != 0) {
-        set_error(ERR_UNSUPPORTED, "H2ERI_build_H2: BD_JIT=1 is not provided; B and D are stored");
-        return;
-    }
     HostState *hs = host_state(h);
     if (hs->synthetic || hs->pairs.empty()) { set_error(ERR_ARG, "H2ERI_build_H2: no shell-pair data"); return; }
+    if (BD_JIT == 1 && hs->max_am > 2) {
+        set_error(ERR_UNSUPPORTED, "H2ERI_build_H2: BD_JIT=1 evaluates shells up to d (max am %d)", hs->max_am);
+        return;
+    }
     device_invalidate(h);
-    h->BD_JIT = 0;
+    h->BD_JIT = BD_JIT == 1 ? 1 : 0;
     double t0 = wtime();
     build_U(h, 0x5EEDULL);
     double t1 = wtime();
     build_B_D_index(h);
-    const double tol = h->QR_stop_tol;
+    if (h->BD_JIT) {
+        h->c_B_blks = (H2E_dense_mat_p *)calloc(h->n_B > 0 ? h->n_B : 1, sizeof(H2E_dense_mat_p));
+        h->c_D_blks = (H2E_dense_mat_p *)calloc(h->n_D > 0 ? h->n_D : 1, sizeof(H2E_dense_mat_p));
+        update_mat_sizes(h);
+        h->timers[U_BUILD_TIMER_IDX] = t1 - t0;
+        h->timers[B_BUILD_TIMER_IDX] = 0.0;
+        h->timers[D_BUILD_TIMER_IDX] = 0.0;
+        return;
+    }
+    // a negative tolerance keeps every block dense (H2ERI_set_block_compression)
+    const double tol = hs->bd_dense ? -1.0 : h->QR_stop_tol;
     const int nB = h->n_B;
     h->c_B_blks = (H2E_dense_mat_p *)calloc(nB > 0 ? nB : 1, sizeof(H2E_dense_mat_p));
     #pragma omp parallel

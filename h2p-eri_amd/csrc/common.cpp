@@ -236,6 +236,7 @@ void H2ERI_print_statistic(H2ERI_p h) {
         if (r > 0) { sum_rank += r; n_ne += 1; if (r > max_rank) max_rank = r; }
     }
     printf("  * Max / Avg compressed rank     : %d, %.0lf \n", max_rank, n_ne > 0 ? sum_rank / n_ne : 0.0);
+    printf("  * Just-In-Time B & D build      : %s\n", h->BD_JIT ? "Yes (B & D not allocated)" : "No");
     printf("==================== H2ERI timing info =====================\n");
     double *t = h->timers;
     printf("  * H2 construction time (sec)   = %.3lf \n", t[0] + t[1] + t[2] + t[3]);
@@ -253,9 +254,11 @@ void H2ERI_print_statistic(H2ERI_p h) {
         auto gbs = [](double mb, double s) { return s > 0 ? mb / s / 1024.0 : 0.0; };
         printf("  * H2 matvec average time (sec) = %.6lf, %.2lf GB/s\n", tot, gbs(mv_MB, tot));
         printf("      |----> Forward transformation      = %.6lf, %.2lf GB/s\n", f, gbs(fwd_MB, f));
-        printf("      |----> Intermediate multiplication = %.6lf, %.2lf GB/s\n", m, gbs(mid_MB, m));
+        if (h->BD_JIT) printf("      |----> Intermediate multiplication = %.6lf\n", m);
+        else printf("      |----> Intermediate multiplication = %.6lf, %.2lf GB/s\n", m, gbs(mid_MB, m));
         printf("      |----> Backward transformation     = %.6lf, %.2lf GB/s\n", b, gbs(bwd_MB, b));
-        printf("      |----> Dense multiplication        = %.6lf, %.2lf GB/s\n", d, gbs(den_MB, d));
+        if (h->BD_JIT) printf("      |----> Dense multiplication        = %.6lf\n", d);
+        else printf("      |----> Dense multiplication        = %.6lf, %.2lf GB/s\n", d, gbs(den_MB, d));
         printf("      |----> Vector operations           = %.6lf\n", v);
     }
     printf("=============================================================\n");

@@ -1387,6 +1387,7 @@ struct DevState {
     double *ws_Wv = nullptr;
     int *ws_ctr = nullptr;
     int *last_noalloc_ctr = nullptr;   // counters of the last allocation-free persistent pass
+    JitState *jit = nullptr;           // a BD_JIT handle's on-the-fly state (jit.hip)
 };
 
 static DevState *dstate(H2ERI_p h) { return (DevState *)h->dev; }
@@ -1436,6 +1437,7 @@ static void release_buffers(DevState *d) {
     void **ps[] = {(void **)&d->S64, (void **)&d->S32, (void **)&d->d_gidx, (void **)&d->d_D, (void **)&d->d_J,
                    (void **)&d->d_Jh, (void **)&d->d_y, (void **)&d->d_x};
     for (void **p : ps) free_dev(*p);
+    jit_release(d->jit);
     d->d_io_len = 0;
     d->st = Storage();
     d->st_ready = false;
@@ -2278,10 +2280,14 @@ static bool upload_storage(H2ERI_p h) {
     DevState *d = dstate_get(h);
     if (d->st_ready) return true;
     if (!h->U || !h->c_B_blks || !h->c_D_blks) { set_error(ERR_ARG, "no H2 representation built or loaded"); return false; }
-    if (h->BD_JIT) { set_error(ERR_UNSUPPORTED, "BD_JIT representations are not supported"); return false; }
     HIP_OK(hipGetDevice(&d->dev));
     release_buffers(d);
-    d->st.build(h, d->ub_bits == 32);
+    if (h->BD_JIT) {   // U and the task lists; B and D are evaluated in each matvec
+        if (!jit_upload(h, d->jit)) return false;
+        d->resident_bytes += jit_resident_bytes(d->jit);
+    } else {
+        d->st.build(h, d->ub_bits == 32);
+    }
     auto alloc_copy = [&](void **dst, const void *src, size_t bytes) -> bool {
         if (bytes == 0) bytes = 64;
         HIP_OK(hipMalloc(dst, bytes));
@@ -2362,6 +2368,11 @@ static bool ensure_queue(DevState *d, DevPlan &dp) {
 }
 
 static DevPlan *get_plan(H2ERI_p h, int nv) {
+    if (h->BD_JIT) {
+        set_error(ERR_UNSUPPORTED, "a BD_JIT handle has no stored blocks to plan: plain and multi-vector matvec and "
+                                   "Coulomb calls only");
+        return nullptr;
+    }
     if (!upload_storage(h)) return nullptr;
     DevState *d = dstate(h);
     DevPlan &dp = d->plans[nv_index(nv)];
@@ -2558,6 +2569,7 @@ static void collect_phase_ms(H2ERI_p h, DevState *d) {
 
 // Batched device matvec: nvec vectors, processed in passes of 16, 8, 4, 2, 1.
 static bool matvec_multi_dev_impl(H2ERI_p h, int nvec, const double *X, double *Y, hipStream_t s) {
+    if (h->BD_JIT) return upload_storage(h) && jit_matvec(h, dstate(h)->jit, nvec, X, Y, s);
     const size_t N = (size_t)h->krnl_mat_size;
     int done = 0;
     while (done < nvec) {
@@ -2718,6 +2730,7 @@ static double *noalloc_scratch(int dev) {
 // One single-vector pass of a prepared handle with the workspace (and the
 // persistent executor's counters) in the scratch array; allocates nothing.
 static bool matvec_noalloc_impl(H2ERI_p h, const double *X, double *Y, hipStream_t s) {
+    if (h->BD_JIT) { set_error(ERR_UNSUPPORTED, "H2ERI_matvec_dev_noalloc: not for BD_JIT handles"); return false; }
     DevState *d = h->dev ? dstate(h) : nullptr;
     DevPlan *dp = d ? &d->plans[0] : nullptr;
     if (!d || !d->st_ready || !dp->ready || !d->side || !d->ev_fork || !d->ev_join ||
@@ -2911,6 +2924,7 @@ void H2ERI_set_precision(H2ERI_p h, const int ub_bits) {
 }
 
 size_t H2ERI_upload(H2ERI_p h) {
+    if (h && h->BD_JIT) return upload_storage(h) ? dstate(h)->resident_bytes : 0;
     if (!h || !get_plan(h, 1)) return 0;
     DevState *d = dstate(h);
     // everything a single-vector pass needs besides its vectors, so that the
@@ -3063,6 +3077,7 @@ void H2ERI_set_tile_bytes(H2ERI_p h, const double bytes) {
 // error set.
 int H2ERI_autotune(H2ERI_p h, const int nv, const int reps, double *out6) {
     if (!h || !nv_valid(nv) || reps < 1) { set_error(ERR_ARG, "H2ERI_autotune: bad argument"); return -1; }
+    if (h->BD_JIT) { set_error(ERR_UNSUPPORTED, "H2ERI_autotune: a BD_JIT handle has no stored-block plans"); return -1; }
     if (!upload_storage(h)) return -1;
     DevState *d = dstate(h);
     const size_t N = (size_t)h->krnl_mat_size;
@@ -3204,6 +3219,7 @@ void *H2ERI_group_create(const int nh, H2ERI_p *handles, const double *const *X_
     for (int k = 0; k < nh; k++) {
         H2ERI_p h = handles[k];
         if (!h || !X_dev[k] || !Y_dev[k]) { set_error(ERR_ARG, "H2ERI_group_create: null handle or buffer %d", k); group_free(g); return nullptr; }
+        if (h->BD_JIT) { set_error(ERR_UNSUPPORTED, "H2ERI_group_create: handle %d is BD_JIT", k); group_free(g); return nullptr; }
         if (!upload_storage(h)) { group_free(g); return nullptr; }
         DevState *d = dstate(h);
         DevPlan &dp = g->plans[k];

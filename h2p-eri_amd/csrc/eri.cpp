@@ -224,6 +224,23 @@ void pair_data_build(const shell_t &Ms, const shell_t &Ns, PairData *&pd) {
 }
 
 void pair_data_free(PairData *&pd) { delete pd; pd = nullptr; }
+
+PairView pair_view(const PairData *pd) {
+    PairView v;
+    v.la = pd->la; v.lb = pd->lb; v.np = pd->np;
+    v.A = pd->A; v.B = pd->B;
+    v.p = pd->p.data(); v.P = pd->P.data(); v.K = pd->K.data();
+    return v;
+}
+
+const double *boys_table(int *ngrid, int *ncol, double *step, double *tmax) {
+    std::call_once(g_boys_once, build_boys);
+    *ngrid = BOYS_NGRID;
+    *ncol = BOYS_MMAX + 1;
+    *step = BOYS_H;
+    *tmax = BOYS_TMAX;
+    return g_boys;
+}
 int pair_nbfp(const PairData *pd) { return ncart(pd->la) * ncart(pd->lb); }
 
 // ---------------------------------------------------------------------------

@@ -70,6 +70,17 @@ void nai_pair_block(const PairData *pd, int npt, const double *px, const double 
                     const double *pz, EriWork *w, double *out, int ldo);
 double boys_f(int m, double T);   // exposed for tests
 void boys_fm(int mmax, double T, double *F);
+// Read-only view of a pair's primitive data (first shell A = N, second B = M):
+// np kept primitive pairs, exponent sums p[k], centres P[3k..3k+2] and
+// prefactors K[k] = c_a c_b exp(-ab/p |AB|^2).
+struct PairView {
+    int la, lb, np;
+    const double *A, *B, *p, *P, *K;
+};
+PairView pair_view(const PairData *pd);
+// The Boys-function table of boys_fm: ngrid points of spacing step up to
+// tmax, ncol orders per point (row-major [grid][m]).
+const double *boys_table(int *ngrid, int *ncol, double *step, double *tmax);
 
 // Pivoted-QR interpolative decomposition over rows (A ~= U * A(J,:)).
 // A is nrow x ncol row-major and is destroyed.  Returns U (nrow x r,
@@ -82,6 +93,7 @@ struct HostState {
     std::vector<PairData *> pairs;   // per screened shell pair (sorted order)
     int max_am = 0;
     bool synthetic = false;
+    bool bd_dense = false;   // H2ERI_set_block_compression(h, 0): B and D stored dense
     double build_ms[4] = {0, 0, 0, 0};
 };
 HostState *host_state(H2ERI_p h);
@@ -97,6 +109,13 @@ void device_release(H2ERI_p h);
 void device_invalidate(H2ERI_p h);
 // releases the handle's captured graph if it is of the given kind (2: exchange)
 void device_release_graph_kind(H2ERI_p h, int kind);
+// on-the-fly B/D layer (jit.hip): device state of a BD_JIT handle
+struct JitState;
+bool jit_upload(H2ERI_p h, JitState *&js);   // builds the task lists and uploads them
+void jit_release(JitState *&js);
+size_t jit_resident_bytes(const JitState *js);
+// y = A x for nvec vectors of length krnl_mat_size (device pointers) on stream s
+bool jit_matvec(H2ERI_p h, JitState *js, int nvec, const double *X, double *Y, void *stream);
 // exchange layer (exchange.hip): device state held in h->thread_Kmat_workbuf
 void exchange_release(H2ERI_p h);
 bool exchange_prepare(H2ERI_p h);   // plan, upload and basis products (allocates)

@@ -1,0 +1,860 @@
+// On-the-fly B and D blocks (a BD_JIT = 1 handle) on the GPU.
+//
+// The reference's BD_JIT mode (src/H2ERI_matvec.c:411-568 intermediate
+// products, :761-879 near field, :911 and :928 the switch) stores U only and
+// evaluates every coupling block B_ij and near-field block D_ij from the
+// shell pairs at each matvec, through H2ERI_calc_ERI_pairs_to_mat
+// (src/CMS.c:387-451).  Here the same blocks are evaluated on the device:
+//
+//  * Row tasks.  Every output row has one writer: a task owns one shell pair
+//    a of one node n on one side (the node's full leaf rows, or its skeleton
+//    rows, J_row) and walks every partner block of that side, evaluating each
+//    shell quartet (a|b) and contracting it with the partner's input at once.
+//    Side rules of the reference's three coupling cases
+//    (src/H2ERI_matvec.c:455-545): node n is on its skeleton side against an
+//    admissible partner m when level(n) >= level(m), else on its full side
+//    (n is then a leaf).  Full rows are x-indexed and receive the near field
+//    (the leaf itself and its inadmissible partners, both orientations) and
+//    the coupling products of case (2)/(3) leaves; skeleton rows are y1 of
+//    the node.  A block (n, m) is so evaluated once from each side, against
+//    the reference's once with a bi-GEMV: twice the integrals, in exchange
+//    for no atomics and a fixed summation order (bit-reproducible results).
+//  * The integrals: McMurchie-Davidson (Helgaker, Jorgensen & Olsen,
+//    Molecular Electronic-Structure Theory, eqs. 9.5.x and 9.9.33),
+//    (ab|cd) = sum_{ij} 2 pi^{5/2} / (p q sqrt(p+q)) K_i K_j
+//              sum_{tuv} E^{ab}_{tuv} sum_{tau nu phi} (-1)^{tau+nu+phi}
+//              E^{cd}_{tau nu phi} R_{t+tau, u+nu, v+phi}(alpha, PQ)
+//    with Hermite coefficients E from their two-term recurrences, Hermite
+//    Coulomb integrals R from R^{(n)}_{000} = (-2 alpha)^n F_n(alpha |PQ|^2)
+//    and R^{(n)}_{t+1,u,v} = t R^{(n+1)}_{t-1,u,v} + X_PQ R^{(n+1)}_{t,u,v},
+//    and the Boys function of the host engine (the same table).  One wave
+//    evaluates one quartet: lanes over primitive quartets for the E and R
+//    tables (in LDS), then lanes over the Cartesian components, each summing
+//    its component over the primitive quartets in a fixed order.  Shells up
+//    to d (cc-pVDZ); the primitive pairs, centres and conventions are the
+//    host engine's (first shell of a pair = N, row index n * nbf(M) + m,
+//    lexicographic Cartesians).
+//  * Sweeps: the up sweep (y0 = U^T x, children first) and the down sweep
+//    (parents first) are one launch per tree level over the nodes of that
+//    level, with U in FP64.
+//
+// Sequence of one matvec: zero y1 and the row outputs, up sweep, the row
+// tasks (they need y0), down sweep (it needs y1), y = down-sweep rows + row
+// task rows.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "h2eri_internal.h"
+
+namespace h2e {
+
+static const int JIT_LMAX = 2;     // shell angular momentum (d)
+static const int JIT_LDS = 3200;   // LDS doubles of one task's wave
+static const int JIT_NAB = 36;     // Cartesian rows of a shell pair, at most
+
+// Everything a row task reads or writes; the pointers are device or, in the
+// emulator, host arrays.
+struct JitView {
+    int N;
+    const int *pinfo;     // per shell pair: la, lb, primitive pairs, first primitive
+    const double *pgeo;   // per shell pair: A (3), B (3)
+    const double *prim;   // per primitive pair: p, P (3), K
+    const double *boys;   // Boys table [grid][bcol]
+    int bgrid, bcol;
+    double bstep, btmax;
+    const int *kl_start, *kl_count;   // per partner list
+    const int *kl_pair, *kl_moff;     // per list entry: shell pair, its input map
+    const int *kmap;                  // input index per column (-1: not used)
+    const int *t_pair, *t_omap, *t_s0, *t_s1;   // per task
+    const int *segs;                  // partner lists of the tasks
+    const int *omap;                  // output index per row (-1: not written)
+    const double *X, *Y0;             // inputs: index < N in X, else Y0[index - N]
+    double *YJ, *Y1;                  // outputs: index < N in YJ, else Y1[index - N]
+};
+
+__host__ __device__ inline void jit_sync() {
+#ifdef __HIP_DEVICE_COMPILE__
+    __syncthreads();
+#endif
+}
+
+__host__ __device__ inline int jit_ncart(int L) { return (L + 1) * (L + 2) / 2; }
+__host__ __device__ inline int jit_nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+// index of the Hermite (or Cartesian) triple (t, u, v) among all triples of
+// total order <= t+u+v, lexicographic within an order
+__host__ __device__ inline int jit_herm(int t, int u, int v) {
+    const int k = t + u + v, a = k - t;
+    return k * (k + 1) * (k + 2) / 6 + a * (a + 1) / 2 + v;
+}
+// exponents of the k-th lexicographic Cartesian component of order L
+__host__ __device__ inline void jit_cart(int L, int k, int &lx, int &ly, int &lz) {
+    int a = 0;
+    while ((a + 1) * (a + 2) / 2 <= k) a++;
+    lx = L - a;
+    ly = a - (k - a * (a + 1) / 2);
+    lz = L - lx - ly;
+}
+
+// F_m(T) for m = 0..mmax into out[m * stride]: the host engine's method
+// (Taylor expansion of the tabulated grid for T <= tmax, downward
+// recursion; erf and upward recursion above).
+__host__ __device__ inline void jit_boys(const JitView &v, int mmax, double T, double *out, int stride) {
+    if (T <= v.btmax) {
+        const int gi = (int)(T / v.bstep + 0.5);
+        const double dT = gi * v.bstep - T;
+        const double *row = v.boys + (size_t)gi * v.bcol;
+        double f = 0.0, pw = 1.0, fac = 1.0;
+        for (int k = 0; k < 8; k++) {
+            f += row[mmax + k] * pw / fac;
+            pw *= dT;
+            fac *= (double)(k + 1);
+        }
+        out[mmax * stride] = f;
+        const double eT = exp(-T);
+        for (int m = mmax; m > 0; m--) out[(m - 1) * stride] = (2.0 * T * out[m * stride] + eT) / (double)(2 * m - 1);
+    } else {
+        const double eT = exp(-T), inv2T = 0.5 / T;
+        out[0] = 0.5 * sqrt(M_PI / T) * erf(sqrt(T));
+        for (int m = 0; m < mmax; m++) out[(m + 1) * stride] = ((2 * m + 1) * out[m * stride] - eT) * inv2T;
+    }
+}
+
+// Hermite expansion coefficients of one primitive pair along one direction:
+// E[(i * (lb + 1) + j) * (la + lb + 1) + t], i <= la, j <= lb, t <= i + j
+// (E^{00}_0 = 1: the exponential factor is in the pair prefactor K).
+__host__ __device__ inline void jit_hermite_e(int la, int lb, double xpa, double xpb, double i2p, double *E) {
+    const int nt = la + lb + 1;
+    auto at = [&](int i, int j, int t) -> double & { return E[(i * (lb + 1) + j) * nt + t]; };
+    auto get = [&](int i, int j, int t) -> double { return (t < 0 || t > i + j) ? 0.0 : E[(i * (lb + 1) + j) * nt + t]; };
+    at(0, 0, 0) = 1.0;
+    for (int i = 0; i < la; i++)
+        for (int t = 0; t <= i + 1; t++) at(i + 1, 0, t) = i2p * get(i, 0, t - 1) + xpa * get(i, 0, t) + (t + 1) * get(i, 0, t + 1);
+    for (int i = 0; i <= la; i++)
+        for (int j = 0; j < lb; j++)
+            for (int t = 0; t <= i + j + 1; t++)
+                at(i, j + 1, t) = i2p * get(i, j, t - 1) + xpb * get(i, j, t) + (t + 1) * get(i, j, t + 1);
+}
+
+// Input value of index k (k < 0: none).
+__host__ __device__ inline double jit_in(const JitView &v, int k) {
+    return k < 0 ? 0.0 : (k < v.N ? v.X[k] : v.Y0[k - v.N]);
+}
+
+// One row task on NL lanes (64 on the device, 1 in the emulator), lds of
+// JIT_LDS doubles.
+template <int NL>
+__host__ __device__ void jit_task(const JitView &v, int task, int lane, double *lds) {
+    const int a = v.t_pair[task];
+    const int la = v.pinfo[4 * a], lb = v.pinfo[4 * a + 1], npa = v.pinfo[4 * a + 2], pa0 = v.pinfo[4 * a + 3];
+    const double *Ag = v.pgeo + 6 * a;
+    const int na = jit_ncart(la), nb = jit_ncart(lb), nab = na * nb, Lab = la + lb;
+    const int esa_d = (la + 1) * (lb + 1) * (Lab + 1), esa = 3 * esa_d;
+    constexpr int RPL = NL >= JIT_NAB ? 1 : JIT_NAB;   // rows per lane
+    double racc[RPL];
+    for (int k = 0; k < RPL; k++) racc[k] = 0.0;
+    for (int sg = v.t_s0[task]; sg < v.t_s1[task]; sg++) {
+        const int L = v.segs[sg];
+        const int e0 = v.kl_start[L], e1 = e0 + v.kl_count[L];
+        for (int e = e0; e < e1; e++) {
+            const int b = v.kl_pair[e];
+            const int *km = v.kmap + v.kl_moff[e];
+            const int lc = v.pinfo[4 * b], ld = v.pinfo[4 * b + 1], npb = v.pinfo[4 * b + 2], pb0 = v.pinfo[4 * b + 3];
+            const double *Cg = v.pgeo + 6 * b;
+            const int nc = jit_ncart(lc), nd = jit_ncart(ld), ncd = nc * nd, Lcd = lc + ld, Lt = Lab + Lcd;
+            const int esb_d = (lc + 1) * (ld + 1) * (Lcd + 1), esb = 3 * esb_d;
+            const int nH = jit_nherm(Lt), rsz = (Lt + 1) * nH;
+            const int ncomp = nab * ncd;
+            const int perq = esa + esb + rsz;
+            double *acc = lds, *vin = lds + ncomp, *qb = vin + ncd;
+            int bq = (JIT_LDS - ncomp - ncd) / perq;
+            if (bq > NL) bq = NL;
+            for (int c = lane; c < ncomp; c += NL) acc[c] = 0.0;
+            for (int c = lane; c < ncd; c += NL) vin[c] = jit_in(v, km[c]);
+            const int nq = npa * npb;
+            for (int q0 = 0; q0 < nq; q0 += bq) {
+                const int nqb = nq - q0 < bq ? nq - q0 : bq;
+                jit_sync();
+                for (int q = lane; q < nqb; q += NL) {   // E tables and R of one primitive quartet
+                    const int i = (q0 + q) / npb, j = (q0 + q) - ((q0 + q) / npb) * npb;
+                    const double *pi = v.prim + 5 * (size_t)(pa0 + i), *pj = v.prim + 5 * (size_t)(pb0 + j);
+                    const double p = pi[0], qq = pj[0];
+                    double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
+                    for (int k = 0; k < esa + esb; k++) Ea[k] = 0.0;
+                    for (int d = 0; d < 3; d++) {
+                        jit_hermite_e(la, lb, pi[1 + d] - Ag[d], pi[1 + d] - Ag[3 + d], 0.5 / p, Ea + d * esa_d);
+                        jit_hermite_e(lc, ld, pj[1 + d] - Cg[d], pj[1 + d] - Cg[3 + d], 0.5 / qq, Eb + d * esb_d);
+                    }
+                    const double pq = p + qq, alpha = p * qq / pq;
+                    const double X = pi[1] - pj[1], Y = pi[2] - pj[2], Z = pi[3] - pj[3];
+                    const double T = alpha * (X * X + Y * Y + Z * Z);
+                    const double pre = 2.0 * M_PI * M_PI * sqrt(M_PI) / (p * qq * sqrt(pq)) * pi[4] * pj[4];
+                    jit_boys(v, Lt, T, R, nH);
+                    double s = pre;
+                    for (int n = 0; n <= Lt; n++) { R[n * nH] *= s; s *= -2.0 * alpha; }
+                    for (int k = 1; k <= Lt; k++)
+                        for (int t = k; t >= 0; t--)
+                            for (int u = k - t; u >= 0; u--) {
+                                const int w = k - t - u, h = jit_herm(t, u, w);
+                                int h1, h2 = -1, c;
+                                double xx;
+                                if (t > 0) { h1 = jit_herm(t - 1, u, w); c = t - 1; xx = X; if (c > 0) h2 = jit_herm(t - 2, u, w); }
+                                else if (u > 0) { h1 = jit_herm(t, u - 1, w); c = u - 1; xx = Y; if (c > 0) h2 = jit_herm(t, u - 2, w); }
+                                else { h1 = jit_herm(t, u, w - 1); c = w - 1; xx = Z; if (c > 0) h2 = jit_herm(t, u, w - 2); }
+                                for (int n = 0; n + k <= Lt; n++) {
+                                    double r = xx * R[(n + 1) * nH + h1];
+                                    if (h2 >= 0) r += c * R[(n + 1) * nH + h2];
+                                    R[n * nH + h] = r;
+                                }
+                            }
+                }
+                jit_sync();
+                for (int cmp = lane; cmp < ncomp; cmp += NL) {   // components over the primitive quartets
+                    const int ab = cmp / ncd, cd = cmp - ab * ncd;
+                    int ax, ay, az, bx, by, bz, cx, cy, cz, dx, dy, dz;
+                    jit_cart(la, ab / nb, ax, ay, az);
+                    jit_cart(lb, ab - (ab / nb) * nb, bx, by, bz);
+                    jit_cart(lc, cd / nd, cx, cy, cz);
+                    jit_cart(ld, cd - (cd / nd) * nd, dx, dy, dz);
+                    const int ox = (ax * (lb + 1) + bx) * (Lab + 1), oy = (ay * (lb + 1) + by) * (Lab + 1) + esa_d,
+                              oz = (az * (lb + 1) + bz) * (Lab + 1) + 2 * esa_d;
+                    const int px = (cx * (ld + 1) + dx) * (Lcd + 1), py = (cy * (ld + 1) + dy) * (Lcd + 1) + esb_d,
+                              pz = (cz * (ld + 1) + dz) * (Lcd + 1) + 2 * esb_d;
+                    double sum = acc[cmp];
+                    for (int q = 0; q < nqb; q++) {
+                        const double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
+                        double val = 0.0;
+                        for (int t = 0; t <= ax + bx; t++)
+                            for (int u = 0; u <= ay + by; u++)
+                                for (int w = 0; w <= az + bz; w++) {
+                                    const double eab = Ea[ox + t] * Ea[oy + u] * Ea[oz + w];
+                                    double inner = 0.0;
+                                    for (int tt = 0; tt <= cx + dx; tt++)
+                                        for (int uu = 0; uu <= cy + dy; uu++)
+                                            for (int ww = 0; ww <= cz + dz; ww++) {
+                                                const double ecd = Eb[px + tt] * Eb[py + uu] * Eb[pz + ww];
+                                                const double r = R[jit_herm(t + tt, u + uu, w + ww)];
+                                                inner += ((tt + uu + ww) & 1) ? -ecd * r : ecd * r;
+                                            }
+                                    val += eab * inner;
+                                }
+                        sum += val;
+                    }
+                    acc[cmp] = sum;
+                }
+            }
+            jit_sync();
+            for (int k = 0; k < RPL; k++) {   // contract the quartet with the partner's input
+                const int r = lane + k * NL;
+                if (r < nab) {
+                    double s = 0.0;
+                    for (int c = 0; c < ncd; c++) s += acc[r * ncd + c] * vin[c];
+                    racc[k] += s;
+                }
+            }
+            jit_sync();
+        }
+    }
+    const int *om = v.omap + v.t_omap[task];
+    for (int k = 0; k < RPL; k++) {
+        const int r = lane + k * NL;
+        if (r < nab && om[r] >= 0) {
+            const int o = om[r];
+            if (o < v.N) v.YJ[o] = racc[k];
+            else v.Y1[o - v.N] = racc[k];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) jit_task_kernel(JitView v, int ntask) {
+    __shared__ double lds[JIT_LDS];
+    const int task = blockIdx.x;
+    if (task >= ntask) return;
+    jit_task<64>(v, task, threadIdx.x, lds);
+}
+
+// Up sweep of one level: y0[n] = U_n^T in_n for the listed nodes, where the
+// input rows come from x (leaves) or from the children's y0 (index map uin).
+// Four waves split the rows, lanes the columns; partial sums added in a fixed
+// order.
+__global__ void __launch_bounds__(256) jit_up_kernel(const int *__restrict__ nodes, const long long *__restrict__ uoff,
+                                                     const int *__restrict__ unr, const int *__restrict__ unc,
+                                                     const int *__restrict__ uin0, const int *__restrict__ y0off,
+                                                     const int *__restrict__ uin, const double *__restrict__ U,
+                                                     const double *__restrict__ X, double *__restrict__ Y0, int N) {
+    __shared__ double red[4][64];
+    const int n = nodes[blockIdx.x];
+    const int nr = unr[n], ncl = unc[n], w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const double *Un = U + uoff[n];
+    const int *in = uin + uin0[n];
+    for (int c0 = 0; c0 < ncl; c0 += 64) {
+        const int c = c0 + lane;
+        double s = 0.0;
+        if (c < ncl)
+            for (int r = w; r < nr; r += 4) {
+                const int k = in[r];
+                const double xv = k < N ? X[k] : Y0[k - N];
+                s += Un[(size_t)r * ncl + c] * xv;
+            }
+        red[w][lane] = s;
+        __syncthreads();
+        if (w == 0 && c < ncl) Y0[y0off[n] + c] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+        __syncthreads();
+    }
+}
+
+// Down sweep of one level: t = U_n y1[n]; row r goes to its leaf row of yd
+// (index < N) or is added to the child's y1 (index - N).
+__global__ void __launch_bounds__(256) jit_down_kernel(const int *__restrict__ nodes, const long long *__restrict__ uoff,
+                                                       const int *__restrict__ unr, const int *__restrict__ unc,
+                                                       const int *__restrict__ uin0, const int *__restrict__ y0off,
+                                                       const int *__restrict__ uin, const double *__restrict__ U,
+                                                       double *__restrict__ YD, double *__restrict__ Y1, int N) {
+    const int n = nodes[blockIdx.x];
+    const int nr = unr[n], ncl = unc[n];
+    const double *Un = U + uoff[n];
+    const double *y1 = Y1 + y0off[n];
+    const int *out = uin + uin0[n];
+    for (int r = threadIdx.x; r < nr; r += 256) {
+        double s = 0.0;
+        for (int c = 0; c < ncl; c++) s += Un[(size_t)r * ncl + c] * y1[c];
+        const int k = out[r];
+        if (k < N) YD[k] = s;
+        else Y1[k - N] += s;
+    }
+}
+
+__global__ void jit_zero_kernel(double *__restrict__ a, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        a[i] = 0.0;
+}
+
+__global__ void jit_add_kernel(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ y, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a[i] + b[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host plan
+// ---------------------------------------------------------------------------
+struct JitPlan {
+    int N = 0, nY = 0, num_sp = 0, max_level = 0;
+    std::vector<int> pinfo;
+    std::vector<double> pgeo, prim;
+    std::vector<int> kl_start, kl_count, kl_pair, kl_moff, kmap;
+    std::vector<int> t_pair, t_omap, t_s0, t_s1, segs, omap;
+    // sweeps
+    std::vector<long long> uoff;
+    std::vector<int> unr, unc, uin0, y0off, uin;
+    std::vector<double> U;
+    std::vector<std::vector<int>> level_nodes;   // nodes with a non-empty U, per level
+    // work per matvec
+    double quartets = 0, prim_quartets = 0, eri_values = 0;
+    const double *boys = nullptr;
+    int bgrid = 0, bcol = 0;
+    double bstep = 0, btmax = 0;
+};
+
+static bool jit_plan(H2ERI_p h, JitPlan &J) {
+    HostState *hs = host_state(h);
+    const int np = h->num_sp, n_node = h->n_node, N = h->krnl_mat_size;
+    if (!h->BD_JIT || !h->U || (int)hs->pairs.size() != np || np <= 0) {
+        set_error(ERR_ARG, "on-the-fly B/D: no BD_JIT representation with shell-pair data");
+        return false;
+    }
+    if (!h->J_pair || !h->J_row || !h->pt_cluster || !h->mat_cluster || !h->sp_bfp_sidx || !h->node_level) {
+        set_error(ERR_ARG, "on-the-fly B/D: tree or skeleton lists missing");
+        return false;
+    }
+    J.N = N;
+    J.num_sp = np;
+    J.max_level = h->max_level;
+    // ---- shell pairs
+    J.pinfo.resize(4 * (size_t)np);
+    J.pgeo.resize(6 * (size_t)np);
+    for (int i = 0; i < np; i++) {
+        const PairView pv = pair_view(hs->pairs[i]);
+        if (pv.la > JIT_LMAX || pv.lb > JIT_LMAX || pv.la < 0 || pv.lb < 0) {
+            set_error(ERR_UNSUPPORTED, "on-the-fly B/D: shell pair %d has angular momentum above d", i);
+            return false;
+        }
+        if (jit_ncart(pv.la) * jit_ncart(pv.lb) != h->sp_nbfp[i] ||
+            h->sp_bfp_sidx[i + 1] - h->sp_bfp_sidx[i] != h->sp_nbfp[i]) {
+            set_error(ERR_ARG, "on-the-fly B/D: shell pair %d rows disagree with sp_nbfp", i);
+            return false;
+        }
+        J.pinfo[4 * i] = pv.la;
+        J.pinfo[4 * i + 1] = pv.lb;
+        J.pinfo[4 * i + 2] = pv.np;
+        J.pinfo[4 * i + 3] = (int)(J.prim.size() / 5);
+        for (int d = 0; d < 3; d++) { J.pgeo[6 * i + d] = pv.A[d]; J.pgeo[6 * i + 3 + d] = pv.B[d]; }
+        for (int k = 0; k < pv.np; k++) {
+            J.prim.push_back(pv.p[k]);
+            for (int d = 0; d < 3; d++) J.prim.push_back(pv.P[3 * k + d]);
+            J.prim.push_back(pv.K[k]);
+        }
+    }
+    if (J.prim.size() / 5 >= (size_t)INT32_MAX) { set_error(ERR_UNSUPPORTED, "on-the-fly B/D: too many primitive pairs"); return false; }
+    // ---- nodes: ranks, y0 offsets, U and the sweep index maps
+    auto rank = [&](int n) { return h->U[n] ? h->U[n]->ncol : 0; };
+    auto urows = [&](int n) { return h->U[n] ? h->U[n]->nrow : 0; };
+    J.y0off.assign(n_node, 0);
+    J.unr.assign(n_node, 0);
+    J.unc.assign(n_node, 0);
+    J.uin0.assign(n_node, 0);
+    J.uoff.assign(n_node, 0);
+    long long ny = 0;
+    for (int n = 0; n < n_node; n++) { J.y0off[n] = (int)ny; ny += rank(n); }
+    if (ny + N >= INT32_MAX) { set_error(ERR_UNSUPPORTED, "on-the-fly B/D: index space too large"); return false; }
+    J.nY = (int)ny;
+    J.level_nodes.assign(h->max_level + 1, {});
+    for (int n = 0; n < n_node; n++) {
+        const int nr = urows(n), nc = rank(n);
+        J.uin0[n] = (int)J.uin.size();
+        if (nr == 0 || nc == 0) continue;
+        const H2E_dense_mat_p Un = h->U[n];
+        J.unr[n] = nr;
+        J.unc[n] = nc;
+        J.uoff[n] = (long long)J.U.size();
+        for (int r = 0; r < nr; r++)
+            for (int c = 0; c < nc; c++) J.U.push_back(Un->data[(size_t)r * Un->ld + c]);
+        if (h->n_child[n] == 0) {
+            const int s0 = h->mat_cluster[2 * n], s1 = h->mat_cluster[2 * n + 1];
+            if (s1 - s0 + 1 != nr || s0 < 0 || s1 >= N) {
+                set_error(ERR_ARG, "on-the-fly B/D: leaf %d has %d U rows for %d x rows", n, nr, s1 - s0 + 1);
+                return false;
+            }
+            for (int r = 0; r < nr; r++) J.uin.push_back(s0 + r);
+        } else {
+            int tot = 0;
+            for (int k = 0; k < h->n_child[n]; k++) {
+                const int ch = h->children[(size_t)n * 8 + k];   // 8 child slots per node
+                for (int r = 0; r < rank(ch); r++) J.uin.push_back(N + J.y0off[ch] + r);
+                tot += rank(ch);
+            }
+            if (tot != nr) {
+                set_error(ERR_ARG, "on-the-fly B/D: node %d has %d U rows for %d child ranks", n, nr, tot);
+                return false;
+            }
+        }
+        const int lv = h->node_level[n];
+        if (lv < 0 || lv > h->max_level) { set_error(ERR_ARG, "on-the-fly B/D: node %d level %d", n, lv); return false; }
+        J.level_nodes[lv].push_back(n);
+    }
+    // ---- partner lists: 2n = full rows of leaf n, 2n + 1 = skeleton rows of n
+    auto is_leaf = [&](int n) { return h->n_child[n] == 0; };
+    J.kl_start.assign(2 * (size_t)n_node, 0);
+    J.kl_count.assign(2 * (size_t)n_node, 0);
+    std::vector<std::vector<int>> skel_pos(n_node);   // row of the concatenated J_pair rows -> skeleton index
+    for (int n = 0; n < n_node; n++) {
+        if (is_leaf(n)) {
+            J.kl_start[2 * n] = (int)J.kl_pair.size();
+            for (int b = h->pt_cluster[2 * n]; b <= h->pt_cluster[2 * n + 1]; b++) {
+                J.kl_pair.push_back(b);
+                J.kl_moff.push_back((int)J.kmap.size());
+                for (int c = 0; c < h->sp_nbfp[b]; c++) J.kmap.push_back(h->sp_bfp_sidx[b] + c);
+            }
+            J.kl_count[2 * n] = (int)J.kl_pair.size() - J.kl_start[2 * n];
+        }
+        const H2E_int_vec_p jp = h->J_pair[n], jr = h->J_row[n];
+        if (!jp || !jr || rank(n) == 0) continue;
+        if (jr->length != rank(n)) {
+            set_error(ERR_ARG, "on-the-fly B/D: node %d has %d skeleton rows for rank %d", n, jr->length, rank(n));
+            return false;
+        }
+        int tot = 0;
+        for (int k = 0; k < jp->length; k++) tot += h->sp_nbfp[jp->data[k]];
+        std::vector<int> &pos = skel_pos[n];
+        pos.assign(tot, -1);
+        for (int j = 0; j < jr->length; j++) {
+            if (jr->data[j] < 0 || jr->data[j] >= tot) { set_error(ERR_ARG, "on-the-fly B/D: node %d skeleton row", n); return false; }
+            pos[jr->data[j]] = j;
+        }
+        J.kl_start[2 * n + 1] = (int)J.kl_pair.size();
+        int rr = 0;
+        for (int k = 0; k < jp->length; k++) {
+            const int b = jp->data[k];
+            bool any = false;
+            for (int c = 0; c < h->sp_nbfp[b]; c++) any |= pos[rr + c] >= 0;
+            if (any) {
+                J.kl_pair.push_back(b);
+                J.kl_moff.push_back((int)J.kmap.size());
+                for (int c = 0; c < h->sp_nbfp[b]; c++)
+                    J.kmap.push_back(pos[rr + c] >= 0 ? N + J.y0off[n] + pos[rr + c] : -1);
+            }
+            rr += h->sp_nbfp[b];
+        }
+        J.kl_count[2 * n + 1] = (int)J.kl_pair.size() - J.kl_start[2 * n + 1];
+    }
+    // ---- partners
+    std::vector<std::vector<int>> adm(n_node), nearf(n_node);
+    for (int i = 0; i < h->n_r_adm_pair; i++) {
+        const int n0 = h->r_adm_pairs[2 * i], n1 = h->r_adm_pairs[2 * i + 1];
+        adm[n0].push_back(n1);
+        adm[n1].push_back(n0);
+    }
+    for (int i = 0; i < h->n_leaf_node; i++) nearf[h->height_nodes[i]].push_back(h->height_nodes[i]);
+    for (int i = 0; i < h->n_r_inadm_pair; i++) {
+        const int n0 = h->r_inadm_pairs[2 * i], n1 = h->r_inadm_pairs[2 * i + 1];
+        nearf[n0].push_back(n1);
+        nearf[n1].push_back(n0);
+    }
+    auto lv = [&](int n) { return h->node_level[n]; };
+    struct TaskRec { int pair, omap, s0, s1; double cost; };
+    std::vector<TaskRec> tasks;
+    auto list_cost = [&](int L, int a) {
+        double c = 0;
+        const int na = J.pinfo[4 * a + 2] * h->sp_nbfp[a];
+        for (int e = J.kl_start[L]; e < J.kl_start[L] + J.kl_count[L]; e++) {
+            const int b = J.kl_pair[e];
+            c += (double)na * J.pinfo[4 * b + 2] * h->sp_nbfp[b] + 64.0;
+        }
+        return c;
+    };
+    auto count_work = [&](int L, int a) {
+        for (int e = J.kl_start[L]; e < J.kl_start[L] + J.kl_count[L]; e++) {
+            const int b = J.kl_pair[e];
+            J.quartets += 1;
+            J.prim_quartets += (double)J.pinfo[4 * a + 2] * J.pinfo[4 * b + 2];
+            J.eri_values += (double)h->sp_nbfp[a] * h->sp_nbfp[b];
+        }
+    };
+    auto add_tasks = [&](const std::vector<int> &lists, int a, const std::vector<int> &om) {
+        TaskRec t;
+        t.pair = a;
+        t.omap = (int)J.omap.size();
+        J.omap.insert(J.omap.end(), om.begin(), om.end());
+        t.s0 = (int)J.segs.size();
+        t.cost = 0;
+        for (int L : lists) { J.segs.push_back(L); t.cost += list_cost(L, a); count_work(L, a); }
+        t.s1 = (int)J.segs.size();
+        tasks.push_back(t);
+    };
+    for (int n = 0; n < n_node; n++) {
+        if (is_leaf(n)) {   // full rows: near field and case (2)/(3) coupling with n uncompressed
+            std::vector<int> lists;
+            for (int m : nearf[n]) {
+                if (!is_leaf(m)) { set_error(ERR_ARG, "on-the-fly B/D: near-field partner %d is not a leaf", m); return false; }
+                lists.push_back(2 * m);
+            }
+            for (int m : adm[n])
+                if (lv(n) < lv(m)) lists.push_back(2 * m + 1);
+            if (!lists.empty())
+                for (int a = h->pt_cluster[2 * n]; a <= h->pt_cluster[2 * n + 1]; a++) {
+                    std::vector<int> om(h->sp_nbfp[a]);
+                    for (int r = 0; r < h->sp_nbfp[a]; r++) om[r] = h->sp_bfp_sidx[a] + r;
+                    add_tasks(lists, a, om);
+                }
+        }
+        std::vector<int> lists;   // skeleton rows: partners at the same or a coarser level
+        for (int m : adm[n]) {
+            if (lv(n) < lv(m)) continue;
+            if (lv(m) < lv(n) && !is_leaf(m)) {
+                set_error(ERR_ARG, "on-the-fly B/D: coarser admissible partner %d of %d is not a leaf", m, n);
+                return false;
+            }
+            lists.push_back(lv(m) >= lv(n) ? 2 * m + 1 : 2 * m);
+        }
+        if (lists.empty()) continue;
+        const H2E_int_vec_p jp = h->J_pair[n];
+        if (!jp || rank(n) == 0) { set_error(ERR_ARG, "on-the-fly B/D: node %d has coupling blocks and no skeleton", n); return false; }
+        int rr = 0;
+        for (int k = 0; k < jp->length; k++) {
+            const int a = jp->data[k];
+            std::vector<int> om(h->sp_nbfp[a], -1);
+            bool any = false;
+            for (int r = 0; r < h->sp_nbfp[a]; r++) {
+                const int j = skel_pos[n][rr + r];
+                if (j >= 0) { om[r] = N + J.y0off[n] + j; any = true; }
+            }
+            if (any) add_tasks(lists, a, om);
+            rr += h->sp_nbfp[a];
+        }
+    }
+    // longest tasks first
+    std::vector<int> order(tasks.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tasks[x].cost > tasks[y].cost; });
+    for (int i : order) {
+        J.t_pair.push_back(tasks[i].pair);
+        J.t_omap.push_back(tasks[i].omap);
+        J.t_s0.push_back(tasks[i].s0);
+        J.t_s1.push_back(tasks[i].s1);
+    }
+    // ---- every index within its array
+    const long long space = (long long)N + J.nY;
+    for (int k : J.kmap)
+        if (k < -1 || k >= space) { set_error(ERR_ARG, "on-the-fly B/D: input index %d out of range", k); return false; }
+    for (int k : J.omap)
+        if (k < -1 || k >= space) { set_error(ERR_ARG, "on-the-fly B/D: output index %d out of range", k); return false; }
+    for (int k : J.uin)
+        if (k < 0 || k >= space) { set_error(ERR_ARG, "on-the-fly B/D: sweep index %d out of range", k); return false; }
+    for (size_t e = 0; e < J.kl_pair.size(); e++)
+        if (J.kl_pair[e] < 0 || J.kl_pair[e] >= np || J.kl_moff[e] + h->sp_nbfp[J.kl_pair[e]] > (long long)J.kmap.size()) {
+            set_error(ERR_ARG, "on-the-fly B/D: partner entry %zu out of range", e);
+            return false;
+        }
+    for (size_t t = 0; t < J.t_pair.size(); t++)
+        if (J.t_omap[t] + h->sp_nbfp[J.t_pair[t]] > (long long)J.omap.size() || J.t_s1[t] > (int)J.segs.size()) {
+            set_error(ERR_ARG, "on-the-fly B/D: task %zu out of range", t);
+            return false;
+        }
+    for (int i = 0; i < np; i++)
+        if ((size_t)(J.pinfo[4 * i + 3] + J.pinfo[4 * i + 2]) * 5 > J.prim.size()) {
+            set_error(ERR_ARG, "on-the-fly B/D: primitives of pair %d out of range", i);
+            return false;
+        }
+    // every class fits one wave's LDS (shells up to d: at most 3,087 doubles)
+    {
+        const int esz = 3 * 3 * 3 * 5, ncomp = JIT_NAB * JIT_NAB, rsz = 9 * jit_nherm(8);
+        if (ncomp + JIT_NAB + 2 * esz + rsz > JIT_LDS) { set_error(ERR_ARG, "on-the-fly B/D: LDS budget"); return false; }
+    }
+    J.boys = boys_table(&J.bgrid, &J.bcol, &J.bstep, &J.btmax);
+    if (J.bcol < 2 * JIT_LMAX * 2 + 8) { set_error(ERR_ARG, "on-the-fly B/D: Boys table has too few orders"); return false; }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Device state
+// ---------------------------------------------------------------------------
+struct JitState {
+    JitPlan P;
+    bool ready = false;
+    int *d_ints = nullptr;      // every int array, one allocation
+    double *d_dbls = nullptr;   // pgeo, prim, boys, U
+    long long *d_uoff = nullptr;
+    double *Y0 = nullptr, *Y1 = nullptr, *YJ = nullptr, *YD = nullptr;
+    JitView view{};
+    const int *d_unr = nullptr, *d_unc = nullptr, *d_uin0 = nullptr, *d_y0off = nullptr, *d_uin = nullptr;
+    const double *d_U = nullptr;
+    std::vector<const int *> d_level;   // node lists per level
+    std::vector<int> level_off;
+    size_t bytes = 0;   // device memory held
+};
+
+#define JIT_HIP(expr)                                                                                       \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) {                                                                             \
+            set_error(ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return false;                                                                                   \
+        }                                                                                                   \
+    } while (0)
+
+void jit_release(JitState *&js) {
+    if (!js) return;
+    void *ps[] = {js->d_ints, js->d_dbls, js->d_uoff, js->Y0, js->Y1, js->YJ, js->YD};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    delete js;
+    js = nullptr;
+}
+
+size_t jit_resident_bytes(const JitState *js) { return js ? js->bytes : 0; }
+
+bool jit_upload(H2ERI_p h, JitState *&js) {
+    if (js && js->ready) return true;
+    jit_release(js);
+    js = new JitState();
+    JitPlan &P = js->P;
+    if (!jit_plan(h, P)) { jit_release(js); return false; }
+    // int arrays, concatenated
+    std::vector<const std::vector<int> *> iv = {&P.pinfo, &P.kl_start, &P.kl_count, &P.kl_pair, &P.kl_moff, &P.kmap,
+                                                &P.t_pair, &P.t_omap, &P.t_s0, &P.t_s1, &P.segs, &P.omap,
+                                                &P.unr, &P.unc, &P.uin0, &P.y0off, &P.uin};
+    std::vector<int> ints;
+    std::vector<size_t> ioff;
+    for (auto *v : iv) { ioff.push_back(ints.size()); ints.insert(ints.end(), v->begin(), v->end()); }
+    js->level_off.clear();
+    for (auto &l : P.level_nodes) { js->level_off.push_back((int)ints.size()); ints.insert(ints.end(), l.begin(), l.end()); }
+    ints.push_back(0);
+    std::vector<double> dbls;
+    const size_t o_geo = 0;
+    dbls.insert(dbls.end(), P.pgeo.begin(), P.pgeo.end());
+    const size_t o_prim = dbls.size();
+    dbls.insert(dbls.end(), P.prim.begin(), P.prim.end());
+    const size_t o_boys = dbls.size();
+    dbls.insert(dbls.end(), P.boys, P.boys + (size_t)P.bgrid * P.bcol);
+    const size_t o_U = dbls.size();
+    dbls.insert(dbls.end(), P.U.begin(), P.U.end());
+    dbls.push_back(0.0);
+    JIT_HIP(hipMalloc(&js->d_ints, ints.size() * sizeof(int)));
+    JIT_HIP(hipMemcpy(js->d_ints, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice));
+    JIT_HIP(hipMalloc(&js->d_dbls, dbls.size() * sizeof(double)));
+    JIT_HIP(hipMemcpy(js->d_dbls, dbls.data(), dbls.size() * sizeof(double), hipMemcpyHostToDevice));
+    JIT_HIP(hipMalloc(&js->d_uoff, (P.uoff.size() + 1) * sizeof(long long)));
+    JIT_HIP(hipMemcpy(js->d_uoff, P.uoff.data(), P.uoff.size() * sizeof(long long), hipMemcpyHostToDevice));
+    const size_t ny = (size_t)std::max(P.nY, 1), nn = (size_t)std::max(P.N, 1);
+    JIT_HIP(hipMalloc(&js->Y0, ny * sizeof(double)));
+    JIT_HIP(hipMalloc(&js->Y1, ny * sizeof(double)));
+    JIT_HIP(hipMalloc(&js->YJ, nn * sizeof(double)));
+    JIT_HIP(hipMalloc(&js->YD, nn * sizeof(double)));
+    js->bytes = ints.size() * sizeof(int) + dbls.size() * sizeof(double) + (P.uoff.size() + 1) * sizeof(long long) +
+                (2 * ny + 2 * nn) * sizeof(double);
+    const int *I = js->d_ints;
+    JitView &v = js->view;
+    v.N = P.N;
+    v.pinfo = I + ioff[0];
+    v.kl_start = I + ioff[1];
+    v.kl_count = I + ioff[2];
+    v.kl_pair = I + ioff[3];
+    v.kl_moff = I + ioff[4];
+    v.kmap = I + ioff[5];
+    v.t_pair = I + ioff[6];
+    v.t_omap = I + ioff[7];
+    v.t_s0 = I + ioff[8];
+    v.t_s1 = I + ioff[9];
+    v.segs = I + ioff[10];
+    v.omap = I + ioff[11];
+    js->d_unr = I + ioff[12];
+    js->d_unc = I + ioff[13];
+    js->d_uin0 = I + ioff[14];
+    js->d_y0off = I + ioff[15];
+    js->d_uin = I + ioff[16];
+    js->d_level.clear();
+    for (int o : js->level_off) js->d_level.push_back(I + o);
+    v.pgeo = js->d_dbls + o_geo;
+    v.prim = js->d_dbls + o_prim;
+    v.boys = js->d_dbls + o_boys;
+    v.bgrid = P.bgrid;
+    v.bcol = P.bcol;
+    v.bstep = P.bstep;
+    v.btmax = P.btmax;
+    js->d_U = js->d_dbls + o_U;
+    v.Y0 = js->Y0;
+    v.Y1 = js->Y1;
+    v.YJ = js->YJ;
+    js->ready = true;
+    return true;
+}
+
+bool jit_matvec(H2ERI_p h, JitState *js, int nvec, const double *X, double *Y, void *stream) {
+    if (!js || !js->ready) { set_error(ERR_ARG, "on-the-fly B/D: not uploaded"); return false; }
+    hipStream_t s = (hipStream_t)stream;
+    const JitPlan &P = js->P;
+    const int N = P.N;
+    for (int k = 0; k < nvec; k++) {
+        const double *x = X + (size_t)k * N;
+        double *y = Y + (size_t)k * N;
+        hipLaunchKernelGGL(jit_zero_kernel, dim3(1024), dim3(256), 0, s, js->Y1, (long long)std::max(P.nY, 1));
+        hipLaunchKernelGGL(jit_zero_kernel, dim3(1024), dim3(256), 0, s, js->YJ, (long long)std::max(N, 1));
+        hipLaunchKernelGGL(jit_zero_kernel, dim3(1024), dim3(256), 0, s, js->YD, (long long)std::max(N, 1));
+        JIT_HIP(hipGetLastError());
+        for (int L = P.max_level; L >= 0; L--) {
+            const int nn = (int)P.level_nodes[L].size();
+            if (nn == 0) continue;
+            hipLaunchKernelGGL(jit_up_kernel, dim3(nn), dim3(256), 0, s, js->d_level[L], js->d_uoff, js->d_unr,
+                               js->d_unc, js->d_uin0, js->d_y0off, js->d_uin, js->d_U, x, js->Y0, N);
+            JIT_HIP(hipGetLastError());
+        }
+        JitView v = js->view;
+        v.X = x;
+        const int ntask = (int)P.t_pair.size();
+        if (ntask > 0) {
+            hipLaunchKernelGGL(jit_task_kernel, dim3(ntask), dim3(64), 0, s, v, ntask);
+            JIT_HIP(hipGetLastError());
+        }
+        for (int L = 0; L <= P.max_level; L++) {
+            const int nn = (int)P.level_nodes[L].size();
+            if (nn == 0) continue;
+            hipLaunchKernelGGL(jit_down_kernel, dim3(nn), dim3(256), 0, s, js->d_level[L], js->d_uoff, js->d_unr,
+                               js->d_unc, js->d_uin0, js->d_y0off, js->d_uin, js->d_U, js->YD, js->Y1, N);
+            JIT_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(jit_add_kernel, dim3((N + 255) / 256), dim3(256), 0, s, js->YD, js->YJ, y, N);
+        JIT_HIP(hipGetLastError());
+    }
+    (void)h;
+    return true;
+}
+
+}  // namespace h2e
+
+using namespace h2e;
+
+extern "C" {
+
+// Host execution of the on-the-fly matvec (tests only): the same row-task
+// code on one lane, the sweeps as plain loops.  Returns 0, or -1 with the
+// error set.
+int H2ERI_jit_emulate(H2ERI_p h, const double *x, double *y) {
+    if (!h || !x || !y) { set_error(ERR_ARG, "jit_emulate: null argument"); return -1; }
+    JitPlan P;
+    if (!jit_plan(h, P)) return -1;
+    const int N = P.N;
+    std::vector<double> Y0(std::max(P.nY, 1), 0.0), Y1(std::max(P.nY, 1), 0.0), YJ(N, 0.0), YD(N, 0.0);
+    for (int L = P.max_level; L >= 0; L--)
+        for (int n : P.level_nodes[L]) {
+            const double *Un = P.U.data() + P.uoff[n];
+            const int *in = P.uin.data() + P.uin0[n];
+            for (int c = 0; c < P.unc[n]; c++) {
+                double part[4] = {0, 0, 0, 0};
+                for (int r = 0; r < P.unr[n]; r++) {
+                    const int k = in[r];
+                    part[r & 3] += Un[(size_t)r * P.unc[n] + c] * (k < N ? x[k] : Y0[k - N]);
+                }
+                Y0[P.y0off[n] + c] = (part[0] + part[1]) + (part[2] + part[3]);
+            }
+        }
+    JitView v{};
+    v.N = N;
+    v.pinfo = P.pinfo.data(); v.pgeo = P.pgeo.data(); v.prim = P.prim.data();
+    v.boys = P.boys; v.bgrid = P.bgrid; v.bcol = P.bcol; v.bstep = P.bstep; v.btmax = P.btmax;
+    v.kl_start = P.kl_start.data(); v.kl_count = P.kl_count.data();
+    v.kl_pair = P.kl_pair.data(); v.kl_moff = P.kl_moff.data(); v.kmap = P.kmap.data();
+    v.t_pair = P.t_pair.data(); v.t_omap = P.t_omap.data(); v.t_s0 = P.t_s0.data(); v.t_s1 = P.t_s1.data();
+    v.segs = P.segs.data(); v.omap = P.omap.data();
+    v.X = x; v.Y0 = Y0.data(); v.YJ = YJ.data(); v.Y1 = Y1.data();
+    const int ntask = (int)P.t_pair.size();
+    std::atomic<int> next(0);
+    auto worker = [&]() {
+        std::vector<double> lds(JIT_LDS);
+        for (int t = next++; t < ntask; t = next++) jit_task<1>(v, t, 0, lds.data());
+    };
+    const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nth; i++) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+    for (int L = 0; L <= P.max_level; L++)
+        for (int n : P.level_nodes[L]) {
+            const double *Un = P.U.data() + P.uoff[n];
+            const int *out = P.uin.data() + P.uin0[n];
+            for (int r = 0; r < P.unr[n]; r++) {
+                double s = 0.0;
+                for (int c = 0; c < P.unc[n]; c++) s += Un[(size_t)r * P.unc[n] + c] * Y1[P.y0off[n] + c];
+                const int k = out[r];
+                if (k < N) YD[k] = s;
+                else Y1[k - N] += s;
+            }
+        }
+    for (int i = 0; i < N; i++) y[i] = YD[i] + YJ[i];
+    return 0;
+}
+
+// Work of one on-the-fly matvec: out4 = {row tasks, shell quartets,
+// primitive quartets, integral values}.  Host only.  Returns 0 or -1.
+int H2ERI_jit_stats(H2ERI_p h, double *out4) {
+    if (!h || !out4) { set_error(ERR_ARG, "jit_stats: null argument"); return -1; }
+    JitPlan P;
+    if (!jit_plan(h, P)) return -1;
+    out4[0] = (double)P.t_pair.size();
+    out4[1] = P.quartets;
+    out4[2] = P.prim_quartets;
+    out4[3] = P.eri_values;
+    return 0;
+}
+
+// B and D blocks stored dense (on = 0) or compressed to [U | V^T] where it
+// pays (on = 1, the default), from the next H2ERI_build_H2 on.
+void H2ERI_set_block_compression(H2ERI_p h, const int on) {
+    if (!h) { set_error(ERR_ARG, "set_block_compression: null handle"); return; }
+    host_state(h)->bd_dense = on == 0;
+}
+
+}  // extern "C"

@@ -180,6 +180,7 @@ extern "C" {
 
 int H2ERI_save_rep(H2ERI_p h, const char *fname) {
     if (!h || !h->U || !h->c_B_blks || !h->c_D_blks) { set_error(ERR_ARG, "save_rep: nothing built"); return -1; }
+    if (h->BD_JIT) { set_error(ERR_UNSUPPORTED, "save_rep: a BD_JIT representation stores no B or D blocks"); return -1; }
     FILE *f = fopen(fname, "wb");
     if (!f) { set_error(ERR_IO, "save_rep: cannot open %s", fname); return -1; }
     W w{f};

@@ -220,6 +220,27 @@ class H2ERI:
         _check(self.lib)
         return y, r
 
+    def jit_emulate(self, x):
+        """Host execution of the on-the-fly (BD_JIT) matvec (tests only)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self.lib.H2ERI_jit_emulate(self.h, _dptr(x), _dptr(y))
+        _check(self.lib)
+        return y
+
+    def jit_stats(self):
+        """Work of one on-the-fly matvec (host count)."""
+        o = np.zeros(4)
+        self.lib.H2ERI_jit_stats(self.h, _dptr(o))
+        _check(self.lib)
+        return {"tasks": int(o[0]), "quartets": o[1], "prim_quartets": o[2], "eri_values": o[3]}
+
+    def set_block_compression(self, on):
+        """B and D blocks compressed where it pays (1, default) or stored dense
+        (0), from the next build on."""
+        self.lib.H2ERI_set_block_compression(self.h, int(on))
+        _check(self.lib)
+
     def set_schedule(self, mode):
         self.lib.H2ERI_set_schedule(self.h, mode)
         _check(self.lib)

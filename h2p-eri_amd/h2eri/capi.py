@@ -120,6 +120,9 @@ SIGNATURES = {
     "H2ERI_set_wave_tile_bytes": (None, [H2ERI_p, C.c_double]),
     "H2ERI_set_near_field_grid": (None, [H2ERI_p, C.c_int]),
     "H2ERI_set_exchange_scratch": (None, [H2ERI_p, C.c_double]),
+    "H2ERI_set_block_compression": (None, [H2ERI_p, C.c_int]),
+    "H2ERI_jit_emulate": (C.c_int, [H2ERI_p, c_dbl_p, c_dbl_p]),
+    "H2ERI_jit_stats": (C.c_int, [H2ERI_p, c_dbl_p]),
     "H2ERI_get_executor": (C.c_int, [H2ERI_p]),
     "H2ERI_device_status": (C.c_int, [H2ERI_p]),
     "H2ERI_set_spin_limit": (None, [H2ERI_p, C.c_int]),
