@@ -55,7 +55,8 @@
 namespace h2e {
 
 static const int JIT_LMAX = 2;     // shell angular momentum (d)
-static const int JIT_LDS = 3200;   // LDS doubles of one task's wave
+static const int JIT_LDS = 3200;   // LDS doubles of a task whose bra pair has la + lb >= 3
+static const int JIT_LDS_S = 1280; // LDS doubles of a task whose bra pair has la + lb <= 2
 static const int JIT_NAB = 36;     // Cartesian rows of a shell pair, at most
 
 // Everything a row task reads or writes; the pointers are device or, in the
@@ -68,11 +69,13 @@ struct JitView {
     const double *boys;   // Boys table [grid][bcol]
     int bgrid, bcol;
     double bstep, btmax;
-    const int *kl_start, *kl_count;   // per partner list
-    const int *kl_pair, *kl_moff;     // per list entry: shell pair, its input map
+    const int *kl_pair, *kl_moff;     // per partner-list entry: shell pair, its input map
     const int *kmap;                  // input index per column (-1: not used)
-    const int *t_pair, *t_omap, *t_s0, *t_s1;   // per task
-    const int *segs;                  // partner lists of the tasks
+    const int *t_pair, *t_e0, *t_e1;  // per task: bra pair, its range of gent
+    const int *gent;                  // partner list entries of the tasks
+    double *part;                     // per task: the partial rows (JIT_NAB doubles)
+    // output rows of one bra pair on one side = the sum of its tasks' partials
+    const int *r_pair, *r_omap, *r_t0, *r_t1, *r_tasks;
     const int *omap;                  // output index per row (-1: not written)
     const double *X, *Y0;             // inputs: index < N in X, else Y0[index - N]
     double *YJ, *Y1;                  // outputs: index < N in YJ, else Y1[index - N]
@@ -147,8 +150,14 @@ __host__ __device__ inline double jit_in(const JitView &v, int k) {
 }
 
 // One row task on NL lanes (64 on the device, 1 in the emulator), lds of
-// JIT_LDS doubles.
-template <int NL>
+// LDSN doubles (enough for every ket class against the task's bra class):
+// bra pair a against the partner entries gent[t_e0, t_e1), its partial rows
+// into part[task].
+// Per quartet: the partial sums of each component over the primitive
+// quartets live in nsl slices (NL / ncomp of them when a quartet has fewer
+// components than lanes, so that the lanes left over share the primitive
+// loop), added in slice order at the end of the quartet.
+template <int NL, int LDSN>
 __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *lds) {
     const int a = v.t_pair[task];
     const int la = v.pinfo[4 * a], lb = v.pinfo[4 * a + 1], npa = v.pinfo[4 * a + 2], pa0 = v.pinfo[4 * a + 3];
@@ -158,10 +167,9 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
     constexpr int RPL = NL >= JIT_NAB ? 1 : JIT_NAB;   // rows per lane
     double racc[RPL];
     for (int k = 0; k < RPL; k++) racc[k] = 0.0;
-    for (int sg = v.t_s0[task]; sg < v.t_s1[task]; sg++) {
-        const int L = v.segs[sg];
-        const int e0 = v.kl_start[L], e1 = e0 + v.kl_count[L];
-        for (int e = e0; e < e1; e++) {
+    {
+        for (int ge = v.t_e0[task]; ge < v.t_e1[task]; ge++) {
+            const int e = v.gent[ge];
             const int b = v.kl_pair[e];
             const int *km = v.kmap + v.kl_moff[e];
             const int lc = v.pinfo[4 * b], ld = v.pinfo[4 * b + 1], npb = v.pinfo[4 * b + 2], pb0 = v.pinfo[4 * b + 3];
@@ -171,10 +179,12 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
             const int nH = jit_nherm(Lt), rsz = (Lt + 1) * nH;
             const int ncomp = nab * ncd;
             const int perq = esa + esb + rsz;
-            double *acc = lds, *vin = lds + ncomp, *qb = vin + ncd;
-            int bq = (JIT_LDS - ncomp - ncd) / perq;
+            const int nsl = ncomp >= NL ? 1 : NL / ncomp, npart = nsl * ncomp;
+            const int preg = ncomp > 64 ? ncomp : 64;   // partial-sum region (npart <= preg)
+            double *acc = lds, *vin = lds + preg, *qb = vin + ncd;
+            int bq = (LDSN - preg - ncd) / perq;
             if (bq > NL) bq = NL;
-            for (int c = lane; c < ncomp; c += NL) acc[c] = 0.0;
+            for (int c = lane; c < npart; c += NL) acc[c] = 0.0;
             for (int c = lane; c < ncd; c += NL) vin[c] = jit_in(v, km[c]);
             const int nq = npa * npb;
             for (int q0 = 0; q0 < nq; q0 += bq) {
@@ -214,7 +224,8 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                             }
                 }
                 jit_sync();
-                for (int cmp = lane; cmp < ncomp; cmp += NL) {   // components over the primitive quartets
+                for (int it = lane; it < npart; it += NL) {   // components over the primitive quartets
+                    const int sl = it / ncomp, cmp = it - sl * ncomp;
                     const int ab = cmp / ncd, cd = cmp - ab * ncd;
                     int ax, ay, az, bx, by, bz, cx, cy, cz, dx, dy, dz;
                     jit_cart(la, ab / nb, ax, ay, az);
@@ -225,8 +236,8 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                               oz = (az * (lb + 1) + bz) * (Lab + 1) + 2 * esa_d;
                     const int px = (cx * (ld + 1) + dx) * (Lcd + 1), py = (cy * (ld + 1) + dy) * (Lcd + 1) + esb_d,
                               pz = (cz * (ld + 1) + dz) * (Lcd + 1) + 2 * esb_d;
-                    double sum = acc[cmp];
-                    for (int q = 0; q < nqb; q++) {
+                    double sum = acc[it];
+                    for (int q = sl; q < nqb; q += nsl) {
                         const double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
                         double val = 0.0;
                         for (int t = 0; t <= ax + bx; t++)
@@ -245,10 +256,18 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                                 }
                         sum += val;
                     }
-                    acc[cmp] = sum;
+                    acc[it] = sum;
                 }
             }
             jit_sync();
+            if (nsl > 1) {   // slices in order; lane cmp alone reads and writes column cmp
+                for (int cmp = lane; cmp < ncomp; cmp += NL) {
+                    double t = acc[cmp];
+                    for (int sl = 1; sl < nsl; sl++) t += acc[sl * ncomp + cmp];
+                    acc[cmp] = t;
+                }
+                jit_sync();
+            }
             for (int k = 0; k < RPL; k++) {   // contract the quartet with the partner's input
                 const int r = lane + k * NL;
                 if (r < nab) {
@@ -260,22 +279,38 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
             jit_sync();
         }
     }
-    const int *om = v.omap + v.t_omap[task];
     for (int k = 0; k < RPL; k++) {
         const int r = lane + k * NL;
-        if (r < nab && om[r] >= 0) {
-            const int o = om[r];
-            if (o < v.N) v.YJ[o] = racc[k];
-            else v.Y1[o - v.N] = racc[k];
-        }
+        if (r < nab) v.part[(size_t)task * JIT_NAB + r] = racc[k];
     }
 }
 
-__global__ void __launch_bounds__(64) jit_task_kernel(JitView v, int ntask) {
-    __shared__ double lds[JIT_LDS];
-    const int task = blockIdx.x;
-    if (task >= ntask) return;
-    jit_task<64>(v, task, threadIdx.x, lds);
+// Output rows of group g: the partials of its tasks, added in entry order.
+__host__ __device__ inline void jit_reduce(const JitView &v, int g) {
+    const int a = v.r_pair[g];
+    const int nab = jit_ncart(v.pinfo[4 * a]) * jit_ncart(v.pinfo[4 * a + 1]);
+    const int *om = v.omap + v.r_omap[g];
+    for (int r = 0; r < nab; r++) {
+        const int o = om[r];
+        if (o < 0) continue;
+        double s = 0.0;
+        for (int k = v.r_t0[g]; k < v.r_t1[g]; k++) s += v.part[(size_t)v.r_tasks[k] * JIT_NAB + r];
+        if (o < v.N) v.YJ[o] = s;
+        else v.Y1[o - v.N] = s;
+    }
+}
+
+__global__ void jit_reduce_kernel(JitView v, int ngroup) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ngroup) jit_reduce(v, g);
+}
+
+template <int LDSN>
+__global__ void __launch_bounds__(64) jit_task_kernel(JitView v, int t0, int t1) {
+    __shared__ double lds[LDSN];
+    const int task = t0 + blockIdx.x;
+    if (task >= t1) return;
+    jit_task<64, LDSN>(v, task, threadIdx.x, lds);
 }
 
 // Up sweep of one level: y0[n] = U_n^T in_n for the listed nodes, where the
@@ -347,7 +382,8 @@ struct JitPlan {
     std::vector<int> pinfo;
     std::vector<double> pgeo, prim;
     std::vector<int> kl_start, kl_count, kl_pair, kl_moff, kmap;
-    std::vector<int> t_pair, t_omap, t_s0, t_s1, segs, omap;
+    std::vector<int> t_pair, t_e0, t_e1, gent, omap;
+    std::vector<int> r_pair, r_omap, r_t0, r_t1, r_tasks;
     // sweeps
     std::vector<long long> uoff;
     std::vector<int> unr, unc, uin0, y0off, uin;
@@ -355,6 +391,7 @@ struct JitPlan {
     std::vector<std::vector<int>> level_nodes;   // nodes with a non-empty U, per level
     // work per matvec
     double quartets = 0, prim_quartets = 0, eri_values = 0;
+    int n_big = 0;   // tasks [0, n_big) have bra pairs with la + lb >= 3
     const double *boys = nullptr;
     int bgrid = 0, bcol = 0;
     double bstep = 0, btmax = 0;
@@ -505,35 +542,36 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
         nearf[n1].push_back(n0);
     }
     auto lv = [&](int n) { return h->node_level[n]; };
-    struct TaskRec { int pair, omap, s0, s1; double cost; };
-    std::vector<TaskRec> tasks;
-    auto list_cost = [&](int L, int a) {
-        double c = 0;
-        const int na = J.pinfo[4 * a + 2] * h->sp_nbfp[a];
-        for (int e = J.kl_start[L]; e < J.kl_start[L] + J.kl_count[L]; e++) {
-            const int b = J.kl_pair[e];
-            c += (double)na * J.pinfo[4 * b + 2] * h->sp_nbfp[b] + 64.0;
-        }
-        return c;
+    // output groups: one bra pair on one side against the side's partner
+    // entries gent[e0, e1); split into tasks of bounded cost below
+    struct GroupRec { int pair, omap, e0, e1; };
+    std::vector<GroupRec> groups;
+    // cost of one quartet in rough lane-steps: components x primitive quartets
+    // over the 64 lanes, plus a fixed cost (loads, barriers)
+    auto qcost = [&](int a, int b) {
+        return (double)J.pinfo[4 * a + 2] * J.pinfo[4 * b + 2] * h->sp_nbfp[a] * h->sp_nbfp[b] / 64.0 + 24.0;
     };
-    auto count_work = [&](int L, int a) {
-        for (int e = J.kl_start[L]; e < J.kl_start[L] + J.kl_count[L]; e++) {
-            const int b = J.kl_pair[e];
+    int side_e0 = 0, side_e1 = 0;
+    auto open_side = [&](const std::vector<int> &lists) {
+        side_e0 = (int)J.gent.size();
+        for (int L : lists)
+            for (int e = J.kl_start[L]; e < J.kl_start[L] + J.kl_count[L]; e++) J.gent.push_back(e);
+        side_e1 = (int)J.gent.size();
+    };
+    auto add_tasks = [&](const std::vector<int> &, int a, const std::vector<int> &om) {
+        GroupRec g;
+        g.pair = a;
+        g.omap = (int)J.omap.size();
+        J.omap.insert(J.omap.end(), om.begin(), om.end());
+        g.e0 = side_e0;
+        g.e1 = side_e1;
+        groups.push_back(g);
+        for (int ge = g.e0; ge < g.e1; ge++) {
+            const int b = J.kl_pair[J.gent[ge]];
             J.quartets += 1;
             J.prim_quartets += (double)J.pinfo[4 * a + 2] * J.pinfo[4 * b + 2];
             J.eri_values += (double)h->sp_nbfp[a] * h->sp_nbfp[b];
         }
-    };
-    auto add_tasks = [&](const std::vector<int> &lists, int a, const std::vector<int> &om) {
-        TaskRec t;
-        t.pair = a;
-        t.omap = (int)J.omap.size();
-        J.omap.insert(J.omap.end(), om.begin(), om.end());
-        t.s0 = (int)J.segs.size();
-        t.cost = 0;
-        for (int L : lists) { J.segs.push_back(L); t.cost += list_cost(L, a); count_work(L, a); }
-        t.s1 = (int)J.segs.size();
-        tasks.push_back(t);
     };
     for (int n = 0; n < n_node; n++) {
         if (is_leaf(n)) {   // full rows: near field and case (2)/(3) coupling with n uncompressed
@@ -544,6 +582,7 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
             }
             for (int m : adm[n])
                 if (lv(n) < lv(m)) lists.push_back(2 * m + 1);
+            open_side(lists);
             if (!lists.empty())
                 for (int a = h->pt_cluster[2 * n]; a <= h->pt_cluster[2 * n + 1]; a++) {
                     std::vector<int> om(h->sp_nbfp[a]);
@@ -561,6 +600,7 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
             lists.push_back(lv(m) >= lv(n) ? 2 * m + 1 : 2 * m);
         }
         if (lists.empty()) continue;
+        open_side(lists);
         const H2E_int_vec_p jp = h->J_pair[n];
         if (!jp || rank(n) == 0) { set_error(ERR_ARG, "on-the-fly B/D: node %d has coupling blocks and no skeleton", n); return false; }
         int rr = 0;
@@ -576,15 +616,57 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
             rr += h->sp_nbfp[a];
         }
     }
-    // longest tasks first
+    // tasks: each group's entries cut into runs of at most `cap` cost, so that
+    // no task holds the whole pass (a bra pair of contracted s shells against
+    // a full near field is the longest); cap = total / 32768
+    double total = 0;
+    for (const GroupRec &g : groups)
+        for (int ge = g.e0; ge < g.e1; ge++) total += qcost(g.pair, J.kl_pair[J.gent[ge]]);
+    const double cap = std::max(total / 32768.0, 256.0);
+    struct TaskRec { int pair, e0, e1, group; double cost; };
+    std::vector<TaskRec> tasks;
+    for (int gi = 0; gi < (int)groups.size(); gi++) {
+        const GroupRec &g = groups[gi];
+        int e0 = g.e0;
+        double c = 0;
+        for (int ge = g.e0; ge < g.e1; ge++) {
+            c += qcost(g.pair, J.kl_pair[J.gent[ge]]);
+            if (c >= cap || ge + 1 == g.e1) {
+                tasks.push_back(TaskRec{g.pair, e0, ge + 1, gi, c});
+                e0 = ge + 1;
+                c = 0;
+            }
+        }
+        if (g.e0 == g.e1) tasks.push_back(TaskRec{g.pair, g.e0, g.e1, gi, 0.0});   // writes zero rows
+    }
+    // bra pairs with la + lb >= 3 first (the launch with the larger LDS), each
+    // launch longest tasks first
+    auto big = [&](int t) { return J.pinfo[4 * tasks[t].pair] + J.pinfo[4 * tasks[t].pair + 1] >= 3; };
     std::vector<int> order(tasks.size());
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tasks[x].cost > tasks[y].cost; });
-    for (int i : order) {
-        J.t_pair.push_back(tasks[i].pair);
-        J.t_omap.push_back(tasks[i].omap);
-        J.t_s0.push_back(tasks[i].s0);
-        J.t_s1.push_back(tasks[i].s1);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (big(x) != big(y)) return big(x);
+        return tasks[x].cost > tasks[y].cost;
+    });
+    J.n_big = 0;
+    std::vector<int> newid(tasks.size());
+    for (size_t k = 0; k < order.size(); k++) {
+        const TaskRec &t = tasks[order[k]];
+        newid[order[k]] = (int)k;
+        J.n_big += big(order[k]) ? 1 : 0;
+        J.t_pair.push_back(t.pair);
+        J.t_e0.push_back(t.e0);
+        J.t_e1.push_back(t.e1);
+    }
+    // each group's tasks in entry order (they were created in that order)
+    std::vector<std::vector<int>> gtasks(groups.size());
+    for (size_t t = 0; t < tasks.size(); t++) gtasks[tasks[t].group].push_back(newid[t]);
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        J.r_pair.push_back(groups[gi].pair);
+        J.r_omap.push_back(groups[gi].omap);
+        J.r_t0.push_back((int)J.r_tasks.size());
+        J.r_tasks.insert(J.r_tasks.end(), gtasks[gi].begin(), gtasks[gi].end());
+        J.r_t1.push_back((int)J.r_tasks.size());
     }
     // ---- every index within its array
     const long long space = (long long)N + J.nY;
@@ -600,20 +682,40 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
             return false;
         }
     for (size_t t = 0; t < J.t_pair.size(); t++)
-        if (J.t_omap[t] + h->sp_nbfp[J.t_pair[t]] > (long long)J.omap.size() || J.t_s1[t] > (int)J.segs.size()) {
+        if (J.t_e0[t] < 0 || J.t_e0[t] > J.t_e1[t] || J.t_e1[t] > (int)J.gent.size()) {
             set_error(ERR_ARG, "on-the-fly B/D: task %zu out of range", t);
             return false;
         }
+    for (int e : J.gent)
+        if (e < 0 || e >= (int)J.kl_pair.size()) { set_error(ERR_ARG, "on-the-fly B/D: entry %d out of range", e); return false; }
+    for (size_t g = 0; g < J.r_pair.size(); g++)
+        if (J.r_omap[g] + h->sp_nbfp[J.r_pair[g]] > (long long)J.omap.size() || J.r_t1[g] > (int)J.r_tasks.size()) {
+            set_error(ERR_ARG, "on-the-fly B/D: output group %zu out of range", g);
+            return false;
+        }
+    for (int t : J.r_tasks)
+        if (t < 0 || t >= (int)J.t_pair.size()) { set_error(ERR_ARG, "on-the-fly B/D: task id %d out of range", t); return false; }
     for (int i = 0; i < np; i++)
         if ((size_t)(J.pinfo[4 * i + 3] + J.pinfo[4 * i + 2]) * 5 > J.prim.size()) {
             set_error(ERR_ARG, "on-the-fly B/D: primitives of pair %d out of range", i);
             return false;
         }
-    // every class fits one wave's LDS (shells up to d: at most 3,087 doubles)
-    {
-        const int esz = 3 * 3 * 3 * 5, ncomp = JIT_NAB * JIT_NAB, rsz = 9 * jit_nherm(8);
-        if (ncomp + JIT_NAB + 2 * esz + rsz > JIT_LDS) { set_error(ERR_ARG, "on-the-fly B/D: LDS budget"); return false; }
-    }
+    // every quartet class fits its launch's LDS with at least one primitive
+    // quartet per batch (shells up to d)
+    for (int la = 0; la <= JIT_LMAX; la++)
+        for (int lb = 0; lb <= JIT_LMAX; lb++)
+            for (int lc = 0; lc <= JIT_LMAX; lc++)
+                for (int ld = 0; ld <= JIT_LMAX; ld++) {
+                    const int ncomp = jit_ncart(la) * jit_ncart(lb) * jit_ncart(lc) * jit_ncart(ld);
+                    const int ncd = jit_ncart(lc) * jit_ncart(ld), Lt = la + lb + lc + ld;
+                    const int perq = 3 * (la + 1) * (lb + 1) * (la + lb + 1) + 3 * (lc + 1) * (ld + 1) * (lc + ld + 1) +
+                                     (Lt + 1) * jit_nherm(Lt);
+                    const int need = (ncomp > 64 ? ncomp : 64) + ncd + perq;
+                    if (need > (la + lb >= 3 ? JIT_LDS : JIT_LDS_S)) {
+                        set_error(ERR_ARG, "on-the-fly B/D: LDS budget of class (%d%d|%d%d)", la, lb, lc, ld);
+                        return false;
+                    }
+                }
     J.boys = boys_table(&J.bgrid, &J.bcol, &J.bstep, &J.btmax);
     if (J.bcol < 2 * JIT_LMAX * 2 + 8) { set_error(ERR_ARG, "on-the-fly B/D: Boys table has too few orders"); return false; }
     return true;
@@ -628,7 +730,7 @@ struct JitState {
     int *d_ints = nullptr;      // every int array, one allocation
     double *d_dbls = nullptr;   // pgeo, prim, boys, U
     long long *d_uoff = nullptr;
-    double *Y0 = nullptr, *Y1 = nullptr, *YJ = nullptr, *YD = nullptr;
+    double *Y0 = nullptr, *Y1 = nullptr, *YJ = nullptr, *YD = nullptr, *part = nullptr;
     JitView view{};
     const int *d_unr = nullptr, *d_unc = nullptr, *d_uin0 = nullptr, *d_y0off = nullptr, *d_uin = nullptr;
     const double *d_U = nullptr;
@@ -648,7 +750,7 @@ struct JitState {
 
 void jit_release(JitState *&js) {
     if (!js) return;
-    void *ps[] = {js->d_ints, js->d_dbls, js->d_uoff, js->Y0, js->Y1, js->YJ, js->YD};
+    void *ps[] = {js->d_ints, js->d_dbls, js->d_uoff, js->Y0, js->Y1, js->YJ, js->YD, js->part};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete js;
@@ -664,9 +766,9 @@ bool jit_upload(H2ERI_p h, JitState *&js) {
     JitPlan &P = js->P;
     if (!jit_plan(h, P)) { jit_release(js); return false; }
     // int arrays, concatenated
-    std::vector<const std::vector<int> *> iv = {&P.pinfo, &P.kl_start, &P.kl_count, &P.kl_pair, &P.kl_moff, &P.kmap,
-                                                &P.t_pair, &P.t_omap, &P.t_s0, &P.t_s1, &P.segs, &P.omap,
-                                                &P.unr, &P.unc, &P.uin0, &P.y0off, &P.uin};
+    std::vector<const std::vector<int> *> iv = {&P.pinfo, &P.r_pair, &P.r_omap, &P.kl_pair, &P.kl_moff, &P.kmap,
+                                                &P.t_pair, &P.t_e0, &P.t_e1, &P.gent, &P.r_t0, &P.omap,
+                                                &P.unr, &P.unc, &P.uin0, &P.y0off, &P.uin, &P.r_t1, &P.r_tasks};
     std::vector<int> ints;
     std::vector<size_t> ioff;
     for (auto *v : iv) { ioff.push_back(ints.size()); ints.insert(ints.end(), v->begin(), v->end()); }
@@ -694,28 +796,33 @@ bool jit_upload(H2ERI_p h, JitState *&js) {
     JIT_HIP(hipMalloc(&js->Y1, ny * sizeof(double)));
     JIT_HIP(hipMalloc(&js->YJ, nn * sizeof(double)));
     JIT_HIP(hipMalloc(&js->YD, nn * sizeof(double)));
+    const size_t npart = (size_t)std::max<size_t>(P.t_pair.size(), 1) * JIT_NAB;
+    JIT_HIP(hipMalloc(&js->part, npart * sizeof(double)));
     js->bytes = ints.size() * sizeof(int) + dbls.size() * sizeof(double) + (P.uoff.size() + 1) * sizeof(long long) +
-                (2 * ny + 2 * nn) * sizeof(double);
+                (2 * ny + 2 * nn + npart) * sizeof(double);
     const int *I = js->d_ints;
     JitView &v = js->view;
     v.N = P.N;
     v.pinfo = I + ioff[0];
-    v.kl_start = I + ioff[1];
-    v.kl_count = I + ioff[2];
+    v.r_pair = I + ioff[1];
+    v.r_omap = I + ioff[2];
     v.kl_pair = I + ioff[3];
     v.kl_moff = I + ioff[4];
     v.kmap = I + ioff[5];
     v.t_pair = I + ioff[6];
-    v.t_omap = I + ioff[7];
-    v.t_s0 = I + ioff[8];
-    v.t_s1 = I + ioff[9];
-    v.segs = I + ioff[10];
+    v.t_e0 = I + ioff[7];
+    v.t_e1 = I + ioff[8];
+    v.gent = I + ioff[9];
+    v.r_t0 = I + ioff[10];
     v.omap = I + ioff[11];
     js->d_unr = I + ioff[12];
     js->d_unc = I + ioff[13];
     js->d_uin0 = I + ioff[14];
     js->d_y0off = I + ioff[15];
     js->d_uin = I + ioff[16];
+    v.r_t1 = I + ioff[17];
+    v.r_tasks = I + ioff[18];
+    v.part = js->part;
     js->d_level.clear();
     for (int o : js->level_off) js->d_level.push_back(I + o);
     v.pgeo = js->d_dbls + o_geo;
@@ -755,8 +862,17 @@ bool jit_matvec(H2ERI_p h, JitState *js, int nvec, const double *X, double *Y, v
         JitView v = js->view;
         v.X = x;
         const int ntask = (int)P.t_pair.size();
-        if (ntask > 0) {
-            hipLaunchKernelGGL(jit_task_kernel, dim3(ntask), dim3(64), 0, s, v, ntask);
+        if (P.n_big > 0) {
+            hipLaunchKernelGGL(jit_task_kernel<JIT_LDS>, dim3(P.n_big), dim3(64), 0, s, v, 0, P.n_big);
+            JIT_HIP(hipGetLastError());
+        }
+        if (ntask > P.n_big) {
+            hipLaunchKernelGGL(jit_task_kernel<JIT_LDS_S>, dim3(ntask - P.n_big), dim3(64), 0, s, v, P.n_big, ntask);
+            JIT_HIP(hipGetLastError());
+        }
+        const int ngroup = (int)P.r_pair.size();
+        if (ngroup > 0) {
+            hipLaunchKernelGGL(jit_reduce_kernel, dim3((ngroup + 255) / 256), dim3(256), 0, s, v, ngroup);
             JIT_HIP(hipGetLastError());
         }
         for (int L = 0; L <= P.max_level; L++) {
@@ -805,22 +921,28 @@ int H2ERI_jit_emulate(H2ERI_p h, const double *x, double *y) {
     v.N = N;
     v.pinfo = P.pinfo.data(); v.pgeo = P.pgeo.data(); v.prim = P.prim.data();
     v.boys = P.boys; v.bgrid = P.bgrid; v.bcol = P.bcol; v.bstep = P.bstep; v.btmax = P.btmax;
-    v.kl_start = P.kl_start.data(); v.kl_count = P.kl_count.data();
     v.kl_pair = P.kl_pair.data(); v.kl_moff = P.kl_moff.data(); v.kmap = P.kmap.data();
-    v.t_pair = P.t_pair.data(); v.t_omap = P.t_omap.data(); v.t_s0 = P.t_s0.data(); v.t_s1 = P.t_s1.data();
-    v.segs = P.segs.data(); v.omap = P.omap.data();
+    v.t_pair = P.t_pair.data(); v.t_e0 = P.t_e0.data(); v.t_e1 = P.t_e1.data(); v.gent = P.gent.data();
+    v.r_pair = P.r_pair.data(); v.r_omap = P.r_omap.data(); v.r_t0 = P.r_t0.data(); v.r_t1 = P.r_t1.data();
+    v.r_tasks = P.r_tasks.data(); v.omap = P.omap.data();
+    std::vector<double> part(std::max<size_t>(P.t_pair.size(), 1) * JIT_NAB, 0.0);
+    v.part = part.data();
     v.X = x; v.Y0 = Y0.data(); v.YJ = YJ.data(); v.Y1 = Y1.data();
     const int ntask = (int)P.t_pair.size();
     std::atomic<int> next(0);
     auto worker = [&]() {
         std::vector<double> lds(JIT_LDS);
-        for (int t = next++; t < ntask; t = next++) jit_task<1>(v, t, 0, lds.data());
+        for (int t = next++; t < ntask; t = next++) {
+            if (t < P.n_big) jit_task<1, JIT_LDS>(v, t, 0, lds.data());
+            else jit_task<1, JIT_LDS_S>(v, t, 0, lds.data());
+        }
     };
     const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
     std::vector<std::thread> pool;
     for (int i = 1; i < nth; i++) pool.emplace_back(worker);
     worker();
     for (auto &th : pool) th.join();
+    for (int g = 0; g < (int)P.r_pair.size(); g++) jit_reduce(v, g);
     for (int L = 0; L <= P.max_level; L++)
         for (int n : P.level_nodes[L]) {
             const double *Un = P.U.data() + P.uoff[n];
