@@ -65,7 +65,7 @@ struct JitView {
     int N;
     const int *pinfo;     // per shell pair: la, lb, primitive pairs, first primitive
     const double *pgeo;   // per shell pair: A (3), B (3)
-    const double *prim;   // per primitive pair: p, P (3), K
+    const double *prim;   // per primitive pair: p, P (3), K, 1/(2p)
     const double *boys;   // Boys table [grid][bcol]
     int bgrid, bcol;
     double bstep, btmax;
@@ -91,7 +91,7 @@ __host__ __device__ inline int jit_ncart(int L) { return (L + 1) * (L + 2) / 2; 
 __host__ __device__ inline int jit_nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
 // index of the Hermite (or Cartesian) triple (t, u, v) among all triples of
 // total order <= t+u+v, lexicographic within an order
-__host__ __device__ inline int jit_herm(int t, int u, int v) {
+__host__ __device__ constexpr int jit_herm(int t, int u, int v) {
     const int k = t + u + v, a = k - t;
     return k * (k + 1) * (k + 2) / 6 + a * (a + 1) / 2 + v;
 }
@@ -104,6 +104,45 @@ __host__ __device__ inline void jit_cart(int L, int k, int &lx, int &ly, int &lz
     lz = L - lx - ly;
 }
 
+// Index tables of the Hermite recursion, fixed at compile time: for each
+// Hermite triple h = 1..nherm(8)-1 in jit_herm order, the step that builds
+// it (R^{(n)}_h = X_dir R^{(n+1)}_{h1} + c R^{(n+1)}_{h2}); jit_herm of every
+// triple of order <= 8 (-1 above); 1/k! and 1/(2m-1).
+struct JitTabs {
+    int rstep[165][4];   // h1, h2 (-1: none), c, direction
+    short herm[9][9][9];
+    double inv_fact[8];
+    double inv_odd[10];
+    constexpr JitTabs() : rstep{}, herm{}, inv_fact{}, inv_odd{} {
+        for (int t = 0; t < 9; t++)
+            for (int u = 0; u < 9; u++)
+                for (int w = 0; w < 9; w++) herm[t][u][w] = (short)(t + u + w <= 8 ? jit_herm(t, u, w) : -1);
+        for (int k = 1; k <= 8; k++)
+            for (int t = k; t >= 0; t--)
+                for (int u = k - t; u >= 0; u--) {
+                    const int w = k - t - u, h = jit_herm(t, u, w);
+                    int h1, h2 = -1, c, d;
+                    if (t > 0) { h1 = jit_herm(t - 1, u, w); c = t - 1; d = 0; if (c > 0) h2 = jit_herm(t - 2, u, w); }
+                    else if (u > 0) { h1 = jit_herm(t, u - 1, w); c = u - 1; d = 1; if (c > 0) h2 = jit_herm(t, u - 2, w); }
+                    else { h1 = jit_herm(t, u, w - 1); c = w - 1; d = 2; if (c > 0) h2 = jit_herm(t, u, w - 2); }
+                    rstep[h][0] = h1; rstep[h][1] = h2; rstep[h][2] = c; rstep[h][3] = d;
+                }
+        double f = 1.0;
+        for (int k = 0; k < 8; k++) { inv_fact[k] = 1.0 / f; f *= (double)(k + 1); }
+        inv_odd[0] = 0.0;
+        for (int m = 1; m < 10; m++) inv_odd[m] = 1.0 / (double)(2 * m - 1);
+    }
+};
+__constant__ JitTabs jit_tabs_dev = JitTabs();
+static constexpr JitTabs jit_tabs_host = JitTabs();
+__host__ __device__ inline const JitTabs &jit_tabs() {
+#ifdef __HIP_DEVICE_COMPILE__
+    return jit_tabs_dev;
+#else
+    return jit_tabs_host;
+#endif
+}
+
 // F_m(T) for m = 0..mmax into out[m * stride]: the host engine's method
 // (Taylor expansion of the tabulated grid for T <= tmax, downward
 // recursion; erf and upward recursion above).
@@ -112,15 +151,15 @@ __host__ __device__ inline void jit_boys(const JitView &v, int mmax, double T, d
         const int gi = (int)(T / v.bstep + 0.5);
         const double dT = gi * v.bstep - T;
         const double *row = v.boys + (size_t)gi * v.bcol;
-        double f = 0.0, pw = 1.0, fac = 1.0;
+        const JitTabs &tb = jit_tabs();
+        double f = 0.0, pw = 1.0;
         for (int k = 0; k < 8; k++) {
-            f += row[mmax + k] * pw / fac;
+            f += row[mmax + k] * pw * tb.inv_fact[k];
             pw *= dT;
-            fac *= (double)(k + 1);
         }
         out[mmax * stride] = f;
         const double eT = exp(-T);
-        for (int m = mmax; m > 0; m--) out[(m - 1) * stride] = (2.0 * T * out[m * stride] + eT) / (double)(2 * m - 1);
+        for (int m = mmax; m > 0; m--) out[(m - 1) * stride] = (2.0 * T * out[m * stride] + eT) * tb.inv_odd[m];
     } else {
         const double eT = exp(-T), inv2T = 0.5 / T;
         out[0] = 0.5 * sqrt(M_PI / T) * erf(sqrt(T));
@@ -158,7 +197,7 @@ __host__ __device__ inline double jit_in(const JitView &v, int k) {
 // components than lanes, so that the lanes left over share the primitive
 // loop), added in slice order at the end of the quartet.
 template <int NL, int LDSN>
-__host__ __device__ void jit_task(const JitView &v, int task, int lane, double *lds) {
+__host__ __device__ void jit_task(const JitView &v, int task, int lane, double *lds, const short *hm) {
     const int a = v.t_pair[task];
     const int la = v.pinfo[4 * a], lb = v.pinfo[4 * a + 1], npa = v.pinfo[4 * a + 2], pa0 = v.pinfo[4 * a + 3];
     const double *Ag = v.pgeo + 6 * a;
@@ -192,13 +231,13 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                 jit_sync();
                 for (int q = lane; q < nqb; q += NL) {   // E tables and R of one primitive quartet
                     const int i = (q0 + q) / npb, j = (q0 + q) - ((q0 + q) / npb) * npb;
-                    const double *pi = v.prim + 5 * (size_t)(pa0 + i), *pj = v.prim + 5 * (size_t)(pb0 + j);
+                    const double *pi = v.prim + 6 * (size_t)(pa0 + i), *pj = v.prim + 6 * (size_t)(pb0 + j);
                     const double p = pi[0], qq = pj[0];
                     double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
                     for (int k = 0; k < esa + esb; k++) Ea[k] = 0.0;
                     for (int d = 0; d < 3; d++) {
-                        jit_hermite_e(la, lb, pi[1 + d] - Ag[d], pi[1 + d] - Ag[3 + d], 0.5 / p, Ea + d * esa_d);
-                        jit_hermite_e(lc, ld, pj[1 + d] - Cg[d], pj[1 + d] - Cg[3 + d], 0.5 / qq, Eb + d * esb_d);
+                        jit_hermite_e(la, lb, pi[1 + d] - Ag[d], pi[1 + d] - Ag[3 + d], pi[5], Ea + d * esa_d);
+                        jit_hermite_e(lc, ld, pj[1 + d] - Cg[d], pj[1 + d] - Cg[3 + d], pj[5], Eb + d * esb_d);
                     }
                     const double pq = p + qq, alpha = p * qq / pq;
                     const double X = pi[1] - pj[1], Y = pi[2] - pj[2], Z = pi[3] - pj[3];
@@ -207,21 +246,19 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                     jit_boys(v, Lt, T, R, nH);
                     double s = pre;
                     for (int n = 0; n <= Lt; n++) { R[n * nH] *= s; s *= -2.0 * alpha; }
-                    for (int k = 1; k <= Lt; k++)
-                        for (int t = k; t >= 0; t--)
-                            for (int u = k - t; u >= 0; u--) {
-                                const int w = k - t - u, h = jit_herm(t, u, w);
-                                int h1, h2 = -1, c;
-                                double xx;
-                                if (t > 0) { h1 = jit_herm(t - 1, u, w); c = t - 1; xx = X; if (c > 0) h2 = jit_herm(t - 2, u, w); }
-                                else if (u > 0) { h1 = jit_herm(t, u - 1, w); c = u - 1; xx = Y; if (c > 0) h2 = jit_herm(t, u - 2, w); }
-                                else { h1 = jit_herm(t, u, w - 1); c = w - 1; xx = Z; if (c > 0) h2 = jit_herm(t, u, w - 2); }
-                                for (int n = 0; n + k <= Lt; n++) {
-                                    double r = xx * R[(n + 1) * nH + h1];
-                                    if (h2 >= 0) r += c * R[(n + 1) * nH + h2];
-                                    R[n * nH + h] = r;
-                                }
-                            }
+                    const JitTabs &tb = jit_tabs();
+                    for (int k = 1, h = 1; k <= Lt; k++) {
+                        const int hend = (k + 1) * (k + 2) * (k + 3) / 6;
+                        for (; h < hend; h++) {
+                            const int h1 = tb.rstep[h][0], h2 = tb.rstep[h][1], d = tb.rstep[h][3];
+                            const double c = (double)tb.rstep[h][2];
+                            const double xx = d == 0 ? X : (d == 1 ? Y : Z);
+                            if (h2 >= 0)
+                                for (int n = 0; n + k <= Lt; n++) R[n * nH + h] = xx * R[(n + 1) * nH + h1] + c * R[(n + 1) * nH + h2];
+                            else
+                                for (int n = 0; n + k <= Lt; n++) R[n * nH + h] = xx * R[(n + 1) * nH + h1];
+                        }
+                    }
                 }
                 jit_sync();
                 for (int it = lane; it < npart; it += NL) {   // components over the primitive quartets
@@ -249,7 +286,7 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                                         for (int uu = 0; uu <= cy + dy; uu++)
                                             for (int ww = 0; ww <= cz + dz; ww++) {
                                                 const double ecd = Eb[px + tt] * Eb[py + uu] * Eb[pz + ww];
-                                                const double r = R[jit_herm(t + tt, u + uu, w + ww)];
+                                                const double r = R[hm[((t + tt) * 9 + (u + uu)) * 9 + (w + ww)]];
                                                 inner += ((tt + uu + ww) & 1) ? -ecd * r : ecd * r;
                                             }
                                     val += eab * inner;
@@ -308,9 +345,12 @@ __global__ void jit_reduce_kernel(JitView v, int ngroup) {
 template <int LDSN>
 __global__ void __launch_bounds__(64) jit_task_kernel(JitView v, int t0, int t1) {
     __shared__ double lds[LDSN];
+    __shared__ short hm[729];   // jit_herm by (t, u, w), for lane-dependent lookups
     const int task = t0 + blockIdx.x;
     if (task >= t1) return;
-    jit_task<64, LDSN>(v, task, threadIdx.x, lds);
+    for (int k = threadIdx.x; k < 729; k += 64) hm[k] = (&jit_tabs_dev.herm[0][0][0])[k];
+    __syncthreads();
+    jit_task<64, LDSN>(v, task, threadIdx.x, lds, hm);
 }
 
 // Up sweep of one level: y0[n] = U_n^T in_n for the listed nodes, where the
@@ -428,15 +468,16 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
         J.pinfo[4 * i] = pv.la;
         J.pinfo[4 * i + 1] = pv.lb;
         J.pinfo[4 * i + 2] = pv.np;
-        J.pinfo[4 * i + 3] = (int)(J.prim.size() / 5);
+        J.pinfo[4 * i + 3] = (int)(J.prim.size() / 6);
         for (int d = 0; d < 3; d++) { J.pgeo[6 * i + d] = pv.A[d]; J.pgeo[6 * i + 3 + d] = pv.B[d]; }
         for (int k = 0; k < pv.np; k++) {
             J.prim.push_back(pv.p[k]);
             for (int d = 0; d < 3; d++) J.prim.push_back(pv.P[3 * k + d]);
             J.prim.push_back(pv.K[k]);
+            J.prim.push_back(0.5 / pv.p[k]);
         }
     }
-    if (J.prim.size() / 5 >= (size_t)INT32_MAX) { set_error(ERR_UNSUPPORTED, "on-the-fly B/D: too many primitive pairs"); return false; }
+    if (J.prim.size() / 6 >= (size_t)INT32_MAX) { set_error(ERR_UNSUPPORTED, "on-the-fly B/D: too many primitive pairs"); return false; }
     // ---- nodes: ranks, y0 offsets, U and the sweep index maps
     auto rank = [&](int n) { return h->U[n] ? h->U[n]->ncol : 0; };
     auto urows = [&](int n) { return h->U[n] ? h->U[n]->nrow : 0; };
@@ -546,10 +587,34 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
     // entries gent[e0, e1); split into tasks of bounded cost below
     struct GroupRec { int pair, omap, e0, e1; };
     std::vector<GroupRec> groups;
-    // cost of one quartet in rough lane-steps: components x primitive quartets
-    // over the 64 lanes, plus a fixed cost (loads, barriers)
+    // cost of one quartet in rough lane-steps: per primitive quartet, the
+    // Hermite terms of all its components and the R table, over the 64
+    // lanes, plus a fixed cost (loads, barriers)
+    double cls[3][3][3][3];
+    for (int la = 0; la <= JIT_LMAX; la++)
+        for (int lb = 0; lb <= JIT_LMAX; lb++)
+            for (int lc = 0; lc <= JIT_LMAX; lc++)
+                for (int ld = 0; ld <= JIT_LMAX; ld++) {
+                    double terms = 0;
+                    int e[4][3];
+                    for (int ia = 0; ia < jit_ncart(la); ia++)
+                        for (int ib = 0; ib < jit_ncart(lb); ib++)
+                            for (int ic = 0; ic < jit_ncart(lc); ic++)
+                                for (int id = 0; id < jit_ncart(ld); id++) {
+                                    jit_cart(la, ia, e[0][0], e[0][1], e[0][2]);
+                                    jit_cart(lb, ib, e[1][0], e[1][1], e[1][2]);
+                                    jit_cart(lc, ic, e[2][0], e[2][1], e[2][2]);
+                                    jit_cart(ld, id, e[3][0], e[3][1], e[3][2]);
+                                    double t = 1;
+                                    for (int d = 0; d < 3; d++) t *= (e[0][d] + e[1][d] + 1) * (e[2][d] + e[3][d] + 1);
+                                    terms += t;
+                                }
+                    const int Lt = la + lb + lc + ld;
+                    cls[la][lb][lc][ld] = (3.0 * terms + 40.0 + (Lt + 1) * jit_nherm(Lt)) / 64.0;
+                }
     auto qcost = [&](int a, int b) {
-        return (double)J.pinfo[4 * a + 2] * J.pinfo[4 * b + 2] * h->sp_nbfp[a] * h->sp_nbfp[b] / 64.0 + 24.0;
+        return (double)J.pinfo[4 * a + 2] * J.pinfo[4 * b + 2] *
+                   cls[J.pinfo[4 * a]][J.pinfo[4 * a + 1]][J.pinfo[4 * b]][J.pinfo[4 * b + 1]] + 40.0;
     };
     int side_e0 = 0, side_e1 = 0;
     auto open_side = [&](const std::vector<int> &lists) {
@@ -696,7 +761,7 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
     for (int t : J.r_tasks)
         if (t < 0 || t >= (int)J.t_pair.size()) { set_error(ERR_ARG, "on-the-fly B/D: task id %d out of range", t); return false; }
     for (int i = 0; i < np; i++)
-        if ((size_t)(J.pinfo[4 * i + 3] + J.pinfo[4 * i + 2]) * 5 > J.prim.size()) {
+        if ((size_t)(J.pinfo[4 * i + 3] + J.pinfo[4 * i + 2]) * 6 > J.prim.size()) {
             set_error(ERR_ARG, "on-the-fly B/D: primitives of pair %d out of range", i);
             return false;
         }
@@ -933,8 +998,8 @@ int H2ERI_jit_emulate(H2ERI_p h, const double *x, double *y) {
     auto worker = [&]() {
         std::vector<double> lds(JIT_LDS);
         for (int t = next++; t < ntask; t = next++) {
-            if (t < P.n_big) jit_task<1, JIT_LDS>(v, t, 0, lds.data());
-            else jit_task<1, JIT_LDS_S>(v, t, 0, lds.data());
+            if (t < P.n_big) jit_task<1, JIT_LDS>(v, t, 0, lds.data(), &jit_tabs_host.herm[0][0][0]);
+            else jit_task<1, JIT_LDS_S>(v, t, 0, lds.data(), &jit_tabs_host.herm[0][0][0]);
         }
     };
     const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
