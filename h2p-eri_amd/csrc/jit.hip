@@ -48,6 +48,7 @@
 #include <cmath>
 #include <numeric>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "h2eri_internal.h"
@@ -88,7 +89,7 @@ __host__ __device__ inline void jit_sync() {
 }
 
 __host__ __device__ inline int jit_ncart(int L) { return (L + 1) * (L + 2) / 2; }
-__host__ __device__ inline int jit_nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+__host__ __device__ constexpr int jit_nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
 // index of the Hermite (or Cartesian) triple (t, u, v) among all triples of
 // total order <= t+u+v, lexicographic within an order
 __host__ __device__ constexpr int jit_herm(int t, int u, int v) {
@@ -121,7 +122,7 @@ struct JitTabs {
             for (int t = k; t >= 0; t--)
                 for (int u = k - t; u >= 0; u--) {
                     const int w = k - t - u, h = jit_herm(t, u, w);
-                    int h1, h2 = -1, c, d;
+                    int h1 = 0, h2 = -1, c = 0, d = 0;
                     if (t > 0) { h1 = jit_herm(t - 1, u, w); c = t - 1; d = 0; if (c > 0) h2 = jit_herm(t - 2, u, w); }
                     else if (u > 0) { h1 = jit_herm(t, u - 1, w); c = u - 1; d = 1; if (c > 0) h2 = jit_herm(t, u - 2, w); }
                     else { h1 = jit_herm(t, u, w - 1); c = w - 1; d = 2; if (c > 0) h2 = jit_herm(t, u, w - 2); }
@@ -181,6 +182,83 @@ __host__ __device__ inline void jit_hermite_e(int la, int lb, double xpa, double
         for (int j = 0; j < lb; j++)
             for (int t = 0; t <= i + j + 1; t++)
                 at(i, j + 1, t) = i2p * get(i, j, t - 1) + xpb * get(i, j, t) + (t + 1) * get(i, j, t + 1);
+}
+
+// Components of the Hermite triple of index h (inverse of jit_herm).
+__host__ __device__ constexpr int jit_hcomp(int h, int which) {
+    for (int k = 0; k <= 8; k++)
+        for (int t = k; t >= 0; t--)
+            for (int u = k - t; u >= 0; u--)
+                if (jit_herm(t, u, k - t - u) == h) return which == 0 ? t : (which == 1 ? u : k - t - u);
+    return 0;
+}
+template <int H1, int H2>
+struct JitSum {   // index of the triple sum and the sign of the ket triple
+    static constexpr int t1 = jit_hcomp(H1, 0), u1 = jit_hcomp(H1, 1), w1 = jit_hcomp(H1, 2);
+    static constexpr int t2 = jit_hcomp(H2, 0), u2 = jit_hcomp(H2, 1), w2 = jit_hcomp(H2, 2);
+    static constexpr int idx = jit_herm(t1 + t2, u1 + u2, w1 + w2);
+    static constexpr bool neg = ((t2 + u2 + w2) & 1) != 0;
+};
+template <int H1, int... H2>
+__host__ __device__ inline double jit_inner(const double *R, const double *ec, std::integer_sequence<int, H2...>) {
+    return (0.0 + ... + (ec[H2] * R[JitSum<H1, H2>::idx]));
+}
+template <int... H2>
+__host__ __device__ inline void jit_ket_e(const double *Eb, int px, int py, int pz, double *ec,
+                                          std::integer_sequence<int, H2...>) {
+    ((ec[H2] = (JitSum<0, H2>::neg ? -1.0 : 1.0) * Eb[px + JitSum<0, H2>::t2] * Eb[py + JitSum<0, H2>::u2] *
+               Eb[pz + JitSum<0, H2>::w2]),
+     ...);
+}
+template <int LCD, int... H1>
+__host__ __device__ inline double jit_outer(const double *Ea, int ox, int oy, int oz, const double *R, const double *ec,
+                                            std::integer_sequence<int, H1...>) {
+    return (0.0 + ... +
+            (Ea[ox + JitSum<H1, 0>::t1] * Ea[oy + JitSum<H1, 0>::u1] * Ea[oz + JitSum<H1, 0>::w1] *
+             jit_inner<H1>(R, ec, std::make_integer_sequence<int, jit_nherm(LCD)>{})));
+}
+// One component on one primitive quartet: sum over the bra Hermite triples
+// of total order <= LAB and the ket triples of order <= LCD, every R index a
+// compile-time constant.  The E rows hold zeros beyond each component's own
+// order, so the triples outside a component's box add nothing.
+template <int LAB, int LCD>
+__host__ __device__ inline double jit_comp(const double *Ea, int ox, int oy, int oz, const double *Eb, int px, int py,
+                                           int pz, const double *R) {
+    double ec[jit_nherm(LCD)];
+    jit_ket_e(Eb, px, py, pz, ec, std::make_integer_sequence<int, jit_nherm(LCD)>{});
+    return jit_outer<LCD>(Ea, ox, oy, oz, R, ec, std::make_integer_sequence<int, jit_nherm(LAB)>{});
+}
+template <int LAB>
+__host__ __device__ inline double jit_comp_l(int Lcd, const double *Ea, int ox, int oy, int oz, const double *Eb, int px,
+                                             int py, int pz, const double *R) {
+    switch (Lcd) {
+    case 0: return jit_comp<LAB, 0>(Ea, ox, oy, oz, Eb, px, py, pz, R);
+    case 1: return jit_comp<LAB, 1>(Ea, ox, oy, oz, Eb, px, py, pz, R);
+    default: return jit_comp<LAB, 2>(Ea, ox, oy, oz, Eb, px, py, pz, R);
+    }
+}
+// The same sum for the larger classes (either side of total order 3 or 4):
+// each component's own box of Hermite terms, lane-dependent loop bounds and
+// indices through the LDS copy hm of jit_herm.
+__host__ __device__ inline double jit_comp_box(const double *Ea, int ox, int oy, int oz, int ex, int ey, int ez,
+                                               const double *Eb, int px, int py, int pz, int fx, int fy, int fz,
+                                               const double *R, const short *hm) {
+    double val = 0.0;
+    for (int t = 0; t <= ex; t++)
+        for (int u = 0; u <= ey; u++)
+            for (int w = 0; w <= ez; w++) {
+                const double eab = Ea[ox + t] * Ea[oy + u] * Ea[oz + w];
+                double inner = 0.0;
+                for (int tt = 0; tt <= fx; tt++)
+                    for (int uu = 0; uu <= fy; uu++)
+                        for (int ww = 0; ww <= fz; ww++) {
+                            const double ecd = Eb[px + tt] * Eb[py + uu] * Eb[pz + ww];
+                            const double r = R[hm[((t + tt) * 9 + (u + uu)) * 9 + (w + ww)]];
+                            inner += ((tt + uu + ww) & 1) ? -ecd * r : ecd * r;
+                        }
+                val += eab * inner;
+            }
+    return val;
 }
 
 // Input value of index k (k < 0: none).
@@ -276,21 +354,13 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                     double sum = acc[it];
                     for (int q = sl; q < nqb; q += nsl) {
                         const double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
-                        double val = 0.0;
-                        for (int t = 0; t <= ax + bx; t++)
-                            for (int u = 0; u <= ay + by; u++)
-                                for (int w = 0; w <= az + bz; w++) {
-                                    const double eab = Ea[ox + t] * Ea[oy + u] * Ea[oz + w];
-                                    double inner = 0.0;
-                                    for (int tt = 0; tt <= cx + dx; tt++)
-                                        for (int uu = 0; uu <= cy + dy; uu++)
-                                            for (int ww = 0; ww <= cz + dz; ww++) {
-                                                const double ecd = Eb[px + tt] * Eb[py + uu] * Eb[pz + ww];
-                                                const double r = R[hm[((t + tt) * 9 + (u + uu)) * 9 + (w + ww)]];
-                                                inner += ((tt + uu + ww) & 1) ? -ecd * r : ecd * r;
-                                            }
-                                    val += eab * inner;
-                                }
+                        double val;
+                        if (Lab > 2 || Lcd > 2)
+                            val = jit_comp_box(Ea, ox, oy, oz, ax + bx, ay + by, az + bz, Eb, px, py, pz, cx + dx,
+                                               cy + dy, cz + dz, R, hm);
+                        else if (Lab == 0) val = jit_comp_l<0>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
+                        else if (Lab == 1) val = jit_comp_l<1>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
+                        else val = jit_comp_l<2>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
                         sum += val;
                     }
                     acc[it] = sum;
@@ -342,8 +412,10 @@ __global__ void jit_reduce_kernel(JitView v, int ngroup) {
     if (g < ngroup) jit_reduce(v, g);
 }
 
+// The small-LDS launch asks for 3 waves per SIMD (168 VGPRs); the other is
+// held to 2 by its LDS anyway.
 template <int LDSN>
-__global__ void __launch_bounds__(64) jit_task_kernel(JitView v, int t0, int t1) {
+__global__ void __launch_bounds__(64, LDSN == JIT_LDS_S ? 3 : 2) jit_task_kernel(JitView v, int t0, int t1) {
     __shared__ double lds[LDSN];
     __shared__ short hm[729];   // jit_herm by (t, u, w), for lane-dependent lookups
     const int task = t0 + blockIdx.x;
