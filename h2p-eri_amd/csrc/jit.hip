@@ -228,9 +228,11 @@ __host__ __device__ inline double jit_comp(const double *Ea, int ox, int oy, int
     jit_ket_e(Eb, px, py, pz, ec, std::make_integer_sequence<int, jit_nherm(LCD)>{});
     return jit_outer<LCD>(Ea, ox, oy, oz, R, ec, std::make_integer_sequence<int, jit_nherm(LAB)>{});
 }
-template <int LAB>
+template <int LAB, int TMAX>
 __host__ __device__ inline double jit_comp_l(int Lcd, const double *Ea, int ox, int oy, int oz, const double *Eb, int px,
                                              int py, int pz, const double *R) {
+    if constexpr (TMAX >= 3)
+        if (Lcd == 3) return jit_comp<LAB, 3>(Ea, ox, oy, oz, Eb, px, py, pz, R);
     switch (Lcd) {
     case 0: return jit_comp<LAB, 0>(Ea, ox, oy, oz, Eb, px, py, pz, R);
     case 1: return jit_comp<LAB, 1>(Ea, ox, oy, oz, Eb, px, py, pz, R);
@@ -274,7 +276,7 @@ __host__ __device__ inline double jit_in(const JitView &v, int k) {
 // quartets live in nsl slices (NL / ncomp of them when a quartet has fewer
 // components than lanes, so that the lanes left over share the primitive
 // loop), added in slice order at the end of the quartet.
-template <int NL, int LDSN>
+template <int NL, int LDSN, int TMAX>
 __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *lds, const short *hm) {
     const int a = v.t_pair[task];
     const int la = v.pinfo[4 * a], lb = v.pinfo[4 * a + 1], npa = v.pinfo[4 * a + 2], pa0 = v.pinfo[4 * a + 3];
@@ -355,12 +357,16 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                     for (int q = sl; q < nqb; q += nsl) {
                         const double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
                         double val;
-                        if (Lab > 2 || Lcd > 2)
+                        if (Lab > TMAX || Lcd > TMAX)
                             val = jit_comp_box(Ea, ox, oy, oz, ax + bx, ay + by, az + bz, Eb, px, py, pz, cx + dx,
                                                cy + dy, cz + dz, R, hm);
-                        else if (Lab == 0) val = jit_comp_l<0>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
-                        else if (Lab == 1) val = jit_comp_l<1>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
-                        else val = jit_comp_l<2>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
+                        else if (Lab == 0) val = jit_comp_l<0, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
+                        else if (Lab == 1) val = jit_comp_l<1, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
+                        else if (Lab == 2) val = jit_comp_l<2, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
+                        else {
+                            if constexpr (TMAX >= 3) val = jit_comp_l<3, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
+                            else val = 0.0;
+                        }
                         sum += val;
                     }
                     acc[it] = sum;
@@ -422,7 +428,7 @@ __global__ void __launch_bounds__(64, LDSN == JIT_LDS_S ? 3 : 2) jit_task_kernel
     if (task >= t1) return;
     for (int k = threadIdx.x; k < 729; k += 64) hm[k] = (&jit_tabs_dev.herm[0][0][0])[k];
     __syncthreads();
-    jit_task<64, LDSN>(v, task, threadIdx.x, lds, hm);
+    jit_task<64, LDSN, LDSN == JIT_LDS_S ? 2 : 3>(v, task, threadIdx.x, lds, hm);
 }
 
 // Up sweep of one level: y0[n] = U_n^T in_n for the listed nodes, where the
@@ -1070,8 +1076,8 @@ int H2ERI_jit_emulate(H2ERI_p h, const double *x, double *y) {
     auto worker = [&]() {
         std::vector<double> lds(JIT_LDS);
         for (int t = next++; t < ntask; t = next++) {
-            if (t < P.n_big) jit_task<1, JIT_LDS>(v, t, 0, lds.data(), &jit_tabs_host.herm[0][0][0]);
-            else jit_task<1, JIT_LDS_S>(v, t, 0, lds.data(), &jit_tabs_host.herm[0][0][0]);
+            if (t < P.n_big) jit_task<1, JIT_LDS, 3>(v, t, 0, lds.data(), &jit_tabs_host.herm[0][0][0]);
+            else jit_task<1, JIT_LDS_S, 2>(v, t, 0, lds.data(), &jit_tabs_host.herm[0][0][0]);
         }
     };
     const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
