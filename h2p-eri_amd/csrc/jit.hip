@@ -56,8 +56,8 @@
 namespace h2e {
 
 static const int JIT_LMAX = 2;     // shell angular momentum (d)
-static const int JIT_LDS = 3200;   // LDS doubles of a task whose bra pair has la + lb >= 3
-static const int JIT_LDS_S = 1280; // LDS doubles of a task whose bra pair has la + lb <= 2
+static const int JIT_LDS = 2048;   // LDS doubles of a task with a bra or ket pair of la + lb >= 3
+static const int JIT_LDS_S = 1280; // LDS doubles of a task whose pairs all have la + lb <= 2
 static const int JIT_NAB = 36;     // Cartesian rows of a shell pair, at most
 
 // Everything a row task reads or writes; the pointers are device or, in the
@@ -295,7 +295,7 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
             const double *Cg = v.pgeo + 6 * b;
             const int nc = jit_ncart(lc), nd = jit_ncart(ld), ncd = nc * nd, Lcd = lc + ld, Lt = Lab + Lcd;
             const int esb_d = (lc + 1) * (ld + 1) * (Lcd + 1), esb = 3 * esb_d;
-            const int nH = jit_nherm(Lt), rsz = (Lt + 1) * nH;
+            const int nH = jit_nherm(Lt), rsz = nH + Lt + 1;   // R^{(0)} and the scaled Boys values
             const int ncomp = nab * ncd;
             const int perq = esa + esb + rsz;
             const int nsl = ncomp >= NL ? 1 : NL / ncomp, npart = nsl * ncomp;
@@ -323,21 +323,25 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                     const double X = pi[1] - pj[1], Y = pi[2] - pj[2], Z = pi[3] - pj[3];
                     const double T = alpha * (X * X + Y * Y + Z * Z);
                     const double pre = 2.0 * M_PI * M_PI * sqrt(M_PI) / (p * qq * sqrt(pq)) * pi[4] * pj[4];
-                    jit_boys(v, Lt, T, R, nH);
+                    double *Fs = R + nH;   // pre (-2 alpha)^n F_n
+                    jit_boys(v, Lt, T, Fs, 1);
                     double s = pre;
-                    for (int n = 0; n <= Lt; n++) { R[n * nH] *= s; s *= -2.0 * alpha; }
+                    for (int n = 0; n <= Lt; n++) { Fs[n] *= s; s *= -2.0 * alpha; }
+                    // R^{(n)} from R^{(n+1)} in place, n = Lt-1 .. 0: orders
+                    // from the highest down, so that the orders k-1 and k-2
+                    // an order-k entry reads still hold R^{(n+1)}
                     const JitTabs &tb = jit_tabs();
-                    for (int k = 1, h = 1; k <= Lt; k++) {
-                        const int hend = (k + 1) * (k + 2) * (k + 3) / 6;
-                        for (; h < hend; h++) {
-                            const int h1 = tb.rstep[h][0], h2 = tb.rstep[h][1], d = tb.rstep[h][3];
-                            const double c = (double)tb.rstep[h][2];
-                            const double xx = d == 0 ? X : (d == 1 ? Y : Z);
-                            if (h2 >= 0)
-                                for (int n = 0; n + k <= Lt; n++) R[n * nH + h] = xx * R[(n + 1) * nH + h1] + c * R[(n + 1) * nH + h2];
-                            else
-                                for (int n = 0; n + k <= Lt; n++) R[n * nH + h] = xx * R[(n + 1) * nH + h1];
+                    R[0] = Fs[Lt];
+                    for (int n = Lt - 1; n >= 0; n--) {
+                        for (int k = Lt - n; k >= 1; k--) {
+                            const int hb = k * (k + 1) * (k + 2) / 6, he = (k + 1) * (k + 2) * (k + 3) / 6;
+                            for (int h = hb; h < he; h++) {
+                                const int h1 = tb.rstep[h][0], h2 = tb.rstep[h][1], d = tb.rstep[h][3];
+                                const double xx = d == 0 ? X : (d == 1 ? Y : Z);
+                                R[h] = h2 >= 0 ? xx * R[h1] + (double)tb.rstep[h][2] * R[h2] : xx * R[h1];
+                            }
                         }
+                        R[0] = Fs[n];
                     }
                 }
                 jit_sync();
@@ -509,7 +513,7 @@ struct JitPlan {
     std::vector<std::vector<int>> level_nodes;   // nodes with a non-empty U, per level
     // work per matvec
     double quartets = 0, prim_quartets = 0, eri_values = 0;
-    int n_big = 0;   // tasks [0, n_big) have bra pairs with la + lb >= 3
+    int n_big = 0;   // tasks [0, n_big) have a bra or ket pair with la + lb >= 3
     const double *boys = nullptr;
     int bgrid = 0, bcol = 0;
     double bstep = 0, btmax = 0;
@@ -695,10 +699,15 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
                    cls[J.pinfo[4 * a]][J.pinfo[4 * a + 1]][J.pinfo[4 * b]][J.pinfo[4 * b + 1]] + 40.0;
     };
     int side_e0 = 0, side_e1 = 0;
+    // ket pairs of total order 3 or 4 last in each side's entries: tasks are
+    // cut where they start and run in the large-LDS launch
+    auto high_ket = [&](int e) { return J.pinfo[4 * J.kl_pair[e]] + J.pinfo[4 * J.kl_pair[e] + 1] >= 3; };
     auto open_side = [&](const std::vector<int> &lists) {
         side_e0 = (int)J.gent.size();
-        for (int L : lists)
-            for (int e = J.kl_start[L]; e < J.kl_start[L] + J.kl_count[L]; e++) J.gent.push_back(e);
+        for (int pass = 0; pass < 2; pass++)
+            for (int L : lists)
+                for (int e = J.kl_start[L]; e < J.kl_start[L] + J.kl_count[L]; e++)
+                    if (high_ket(e) == (pass == 1)) J.gent.push_back(e);
         side_e1 = (int)J.gent.size();
     };
     auto add_tasks = [&](const std::vector<int> &, int a, const std::vector<int> &om) {
@@ -766,7 +775,7 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
     for (const GroupRec &g : groups)
         for (int ge = g.e0; ge < g.e1; ge++) total += qcost(g.pair, J.kl_pair[J.gent[ge]]);
     const double cap = std::max(total / 32768.0, 256.0);
-    struct TaskRec { int pair, e0, e1, group; double cost; };
+    struct TaskRec { int pair, e0, e1, group; double cost; bool high; };
     std::vector<TaskRec> tasks;
     for (int gi = 0; gi < (int)groups.size(); gi++) {
         const GroupRec &g = groups[gi];
@@ -774,17 +783,20 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
         double c = 0;
         for (int ge = g.e0; ge < g.e1; ge++) {
             c += qcost(g.pair, J.kl_pair[J.gent[ge]]);
-            if (c >= cap || ge + 1 == g.e1) {
-                tasks.push_back(TaskRec{g.pair, e0, ge + 1, gi, c});
+            const bool cut = ge + 1 == g.e1 || high_ket(J.gent[ge + 1]) != high_ket(J.gent[ge]);
+            if (c >= cap || cut) {
+                tasks.push_back(TaskRec{g.pair, e0, ge + 1, gi, c, high_ket(J.gent[ge])});
                 e0 = ge + 1;
                 c = 0;
             }
         }
-        if (g.e0 == g.e1) tasks.push_back(TaskRec{g.pair, g.e0, g.e1, gi, 0.0});   // writes zero rows
+        if (g.e0 == g.e1) tasks.push_back(TaskRec{g.pair, g.e0, g.e1, gi, 0.0, false});   // writes zero rows
     }
-    // bra pairs with la + lb >= 3 first (the launch with the larger LDS), each
-    // launch longest tasks first
-    auto big = [&](int t) { return J.pinfo[4 * tasks[t].pair] + J.pinfo[4 * tasks[t].pair + 1] >= 3; };
+    // tasks with a bra or ket pair of total order >= 3 first (the launch with
+    // the larger LDS), each launch longest tasks first
+    auto big = [&](int t) {
+        return tasks[t].high || J.pinfo[4 * tasks[t].pair] + J.pinfo[4 * tasks[t].pair + 1] >= 3;
+    };
     std::vector<int> order(tasks.size());
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -844,7 +856,7 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
             return false;
         }
     // every quartet class fits its launch's LDS with at least one primitive
-    // quartet per batch (shells up to d)
+    // quartet per batch (shells up to d; both orders <= 2 in the small launch)
     for (int la = 0; la <= JIT_LMAX; la++)
         for (int lb = 0; lb <= JIT_LMAX; lb++)
             for (int lc = 0; lc <= JIT_LMAX; lc++)
@@ -852,9 +864,9 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
                     const int ncomp = jit_ncart(la) * jit_ncart(lb) * jit_ncart(lc) * jit_ncart(ld);
                     const int ncd = jit_ncart(lc) * jit_ncart(ld), Lt = la + lb + lc + ld;
                     const int perq = 3 * (la + 1) * (lb + 1) * (la + lb + 1) + 3 * (lc + 1) * (ld + 1) * (lc + ld + 1) +
-                                     (Lt + 1) * jit_nherm(Lt);
+                                     jit_nherm(Lt) + Lt + 1;
                     const int need = (ncomp > 64 ? ncomp : 64) + ncd + perq;
-                    if (need > (la + lb >= 3 ? JIT_LDS : JIT_LDS_S)) {
+                    if (need > (la + lb >= 3 || lc + ld >= 3 ? JIT_LDS : JIT_LDS_S)) {
                         set_error(ERR_ARG, "on-the-fly B/D: LDS budget of class (%d%d|%d%d)", la, lb, lc, ld);
                         return false;
                     }
