@@ -413,6 +413,7 @@ def single_workload(args):
     exch = None
     if args.exchange:
         exch = exchange_measure(h, torch, D, D_dev, stream)
+    jit = None if args.no_jit else jit_measure(torch, stream)
 
     # dominant kernel of the headline form
     prof = r32["prof"]
@@ -497,6 +498,7 @@ def single_workload(args):
                                   "sample": f"rows of {npair_s} seeded random bra shell pairs ({nrow_s} rows of y)"},
         "build_s": round(t_build, 2),
         "exchange_fp64": exch,
+        "bd_jit_fp64": jit,
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kernel_name, "launches_per_matvec": len(prof),
@@ -515,6 +517,46 @@ def single_workload(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def jit_measure(torch, stream, cfg=1, reps=3):
+    """The on-the-fly form (BD_JIT = 1: B and D evaluated from the shell
+    pairs in every matvec) on C30H62 at reltol 1e-6: ms per FP64 matvec
+    (HIP events, device vectors), with the integral work of one matvec."""
+    import numpy as np
+
+    import h2eri
+    from h2eri import molecules as M
+    atoms, txt, _, reltol = M.workload(cfg)
+    t0 = time.perf_counter()
+    hj = h2eri.H2ERI(1e-10, 1e-10, reltol)
+    hj.read_mol_text(txt).process_shells().partition().build_h2(1)
+    t_build = time.perf_counter() - t0
+    st = hj.jit_stats()
+    x = torch.from_numpy(np.random.default_rng(17).standard_normal(hj.n)).to("cuda:0")
+    y = torch.empty_like(x)
+    hj.upload()
+    hj.matvec_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        hj.matvec_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    best = min(ms)
+    log(f"[bench] on-the-fly B/D ({M.workload_name(cfg)}): {best:.1f} ms per matvec, "
+        f"{st['eri_values'] / best / 1e6:.2f} G integrals/s")
+    rec = {"workload": f"{M.workload_name(cfg)} cc-pVDZ, reltol {reltol:g}", "krnl_mat_size": hj.n,
+           "ms_per_matvec": round(best, 3), "ms_all": [round(m, 3) for m in ms],
+           "integrals_per_matvec": st["eri_values"], "shell_quartets_per_matvec": st["quartets"],
+           "primitive_quartets_per_matvec": st["prim_quartets"], "row_tasks": st["tasks"],
+           "G_integrals_per_s": round(st["eri_values"] / best / 1e6, 3),
+           "resident_MB": round(hj.upload() / 1e6, 1), "build_s": round(t_build, 2)}
+    hj.close()
+    return rec
 
 
 def exchange_measure(h, torch, D, D_dev, stream):
@@ -654,6 +696,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-one-core", action="store_true")
     ap.add_argument("--exchange", action="store_true", help="time the exchange build too (default with --config 4)")
+    ap.add_argument("--no-jit", action="store_true", help="skip the on-the-fly (BD_JIT) matvec record")
     ap.add_argument("--tuned", default="", help="EXECUTOR,TILE_BYTES,SCHEDULE[,WAVE_TILE_BYTES[,NEAR_FIELD_GRID]] instead of running the "
                     "autotuner (profiling runs, where the profiler distorts the tuner's timings)")
     ap.add_argument("--passes-only", type=int, default=0,

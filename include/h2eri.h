@@ -207,12 +207,24 @@ void CMS_read_mol_file(const char *mol_fname, int *natom_, int *nshell_, shell_t
 void CMS_destroy_shells(const int nshell, shell_t *shells);
 
 /* Host construction (replace src/H2ERI_shell_operations.h:25,
- * src/H2ERI_partition.h:28, src/H2ERI_build_H2.h:16).  BD_JIT must be 0:
- * this library stores B and D blocks (the JIT mode of the reference is not
- * provided and is rejected with an error). */
+ * src/H2ERI_partition.h:28, src/H2ERI_build_H2.h:16).  BD_JIT = 0 stores the
+ * B and D blocks; BD_JIT = 1 (src/H2ERI_build_H2.c:1406-1411) stores U, the
+ * skeleton lists and the block shapes only, and every matvec evaluates the B
+ * and D blocks from the shell pairs on the device (shells up to d; the
+ * reference's src/H2ERI_matvec.c:411-568,761-879).  A BD_JIT handle serves
+ * H2ERI_matvec[_multi][_dev] and H2ERI_build_Coulomb[_multi][_dev] in FP64;
+ * the entry points that plan stored blocks (graph capture, autotune, groups,
+ * the allocation-free matvec) and H2ERI_build_exchange refuse it with an
+ * error, as the reference's exchange does (src/H2ERI_build_exchange.c:1358). */
 void H2ERI_process_shells(H2ERI_p h2eri);
 void H2ERI_partition(H2ERI_p h2eri);
 void H2ERI_build_H2(H2ERI_p h2eri, const int BD_JIT);
+
+/* B and D blocks stored dense (on = 0) or compressed to [U | V^T] where it
+ * pays (on = 1, the default; src/H2ERI_build_H2.c:843-903), from the next
+ * H2ERI_build_H2 on.  Dense blocks are exactly what a BD_JIT matvec
+ * evaluates.  No counterpart in the reference. */
+void H2ERI_set_block_compression(H2ERI_p h2eri, const int on);
 
 /* ------------------------------------------------------------------ */
 /* Hot path.  Each call runs on the device; host pointers in, host pointers
@@ -486,6 +498,15 @@ int H2ERI_plan_stats(H2ERI_p h2eri, const int ub_bits, const int nv, double *out
  * intra-launch read/write conflicts plus output elements not written exactly
  * once (0 expected), -1 on failure.  The hot-path entry points never call it. */
 int H2ERI_plan_emulate(H2ERI_p h2eri, const int ub_bits, const int nv, const double *X, double *Y);
+
+/* Host execution of a BD_JIT matvec (no HIP call, tests only): the device's
+ * row-task code on one lane, the sweeps as plain loops; y = A x.  Returns 0,
+ * -1 on failure. */
+int H2ERI_jit_emulate(H2ERI_p h2eri, const double *x, double *y);
+
+/* Work of one BD_JIT matvec, counted on the host: out4 = {row tasks, shell
+ * quartets, primitive quartets, integral values}.  Returns 0, -1 on failure. */
+int H2ERI_jit_stats(H2ERI_p h2eri, double *out4);
 
 /* Host verification of the persistent executor's queue (no HIP call, tests
  * only): builds the queue for `workgroups` resident workgroups, executes the
