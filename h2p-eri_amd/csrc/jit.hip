@@ -168,22 +168,6 @@ __host__ __device__ inline void jit_boys(const JitView &v, int mmax, double T, d
     }
 }
 
-// Hermite expansion coefficients of one primitive pair along one direction:
-// E[(i * (lb + 1) + j) * (la + lb + 1) + t], i <= la, j <= lb, t <= i + j
-// (E^{00}_0 = 1: the exponential factor is in the pair prefactor K).
-__host__ __device__ inline void jit_hermite_e(int la, int lb, double xpa, double xpb, double i2p, double *E) {
-    const int nt = la + lb + 1;
-    auto at = [&](int i, int j, int t) -> double & { return E[(i * (lb + 1) + j) * nt + t]; };
-    auto get = [&](int i, int j, int t) -> double { return (t < 0 || t > i + j) ? 0.0 : E[(i * (lb + 1) + j) * nt + t]; };
-    at(0, 0, 0) = 1.0;
-    for (int i = 0; i < la; i++)
-        for (int t = 0; t <= i + 1; t++) at(i + 1, 0, t) = i2p * get(i, 0, t - 1) + xpa * get(i, 0, t) + (t + 1) * get(i, 0, t + 1);
-    for (int i = 0; i <= la; i++)
-        for (int j = 0; j < lb; j++)
-            for (int t = 0; t <= i + j + 1; t++)
-                at(i, j + 1, t) = i2p * get(i, j, t - 1) + xpb * get(i, j, t) + (t + 1) * get(i, j, t + 1);
-}
-
 // Components of the Hermite triple of index h (inverse of jit_herm).
 __host__ __device__ constexpr int jit_hcomp(int h, int which) {
     for (int k = 0; k <= 8; k++)
@@ -231,7 +215,7 @@ __host__ __device__ inline double jit_comp(const double *Ea, int ox, int oy, int
 template <int LAB, int TMAX>
 __host__ __device__ inline double jit_comp_l(int Lcd, const double *Ea, int ox, int oy, int oz, const double *Eb, int px,
                                              int py, int pz, const double *R) {
-    if constexpr (TMAX >= 3)
+    if constexpr (TMAX >= 3 && LAB <= 2)
         if (Lcd == 3) return jit_comp<LAB, 3>(Ea, ox, oy, oz, Eb, px, py, pz, R);
     switch (Lcd) {
     case 0: return jit_comp<LAB, 0>(Ea, ox, oy, oz, Eb, px, py, pz, R);
@@ -261,6 +245,59 @@ __host__ __device__ inline double jit_comp_box(const double *Ea, int ox, int oy,
                 val += eab * inner;
             }
     return val;
+}
+
+// Hermite expansion coefficients of one primitive pair along one direction,
+// for a fixed (LA, LB), unrolled: E[(i * (LB + 1) + j) * (LA + LB + 1) + t],
+// i <= LA, j <= LB, every t written (zero for t > i + j, so the rows can be
+// summed to the class's full order).  E^{00}_0 = 1: the exponential factor
+// is in the pair prefactor K.  Two-term recurrences
+// E^{i+1,j}_t = E^{ij}_{t-1} / (2p) + X_PA E^{ij}_t + (t+1) E^{ij}_{t+1}, and
+// the same in j with X_PB.
+template <int LA, int LB>
+__host__ __device__ inline void jit_hermite_e_t(double xpa, double xpb, double i2p, double *E) {
+    constexpr int NT = LA + LB + 1;
+    double e[LA + 1][LB + 1][NT];
+#pragma unroll
+    for (int i = 0; i <= LA; i++)
+#pragma unroll
+        for (int j = 0; j <= LB; j++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) e[i][j][t] = 0.0;
+    e[0][0][0] = 1.0;
+#pragma unroll
+    for (int i = 0; i < LA; i++)
+#pragma unroll
+        for (int t = 0; t <= i + 1; t++)
+            e[i + 1][0][t] = (t > 0 ? i2p * e[i][0][t - 1] : 0.0) + (t <= i ? xpa * e[i][0][t] : 0.0) +
+                             (t + 1 <= i ? (t + 1) * e[i][0][t + 1] : 0.0);
+#pragma unroll
+    for (int i = 0; i <= LA; i++)
+#pragma unroll
+        for (int j = 0; j < LB; j++)
+#pragma unroll
+            for (int t = 0; t <= i + j + 1; t++)
+                e[i][j + 1][t] = (t > 0 ? i2p * e[i][j][t - 1] : 0.0) + (t <= i + j ? xpb * e[i][j][t] : 0.0) +
+                                 (t + 1 <= i + j ? (t + 1) * e[i][j][t + 1] : 0.0);
+#pragma unroll
+    for (int i = 0; i <= LA; i++)
+#pragma unroll
+        for (int j = 0; j <= LB; j++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) E[(i * (LB + 1) + j) * NT + t] = e[i][j][t];
+}
+__host__ __device__ inline void jit_hermite_e_any(int la, int lb, double xpa, double xpb, double i2p, double *E) {
+    switch (la * 3 + lb) {
+    case 0: jit_hermite_e_t<0, 0>(xpa, xpb, i2p, E); break;
+    case 1: jit_hermite_e_t<0, 1>(xpa, xpb, i2p, E); break;
+    case 2: jit_hermite_e_t<0, 2>(xpa, xpb, i2p, E); break;
+    case 3: jit_hermite_e_t<1, 0>(xpa, xpb, i2p, E); break;
+    case 4: jit_hermite_e_t<1, 1>(xpa, xpb, i2p, E); break;
+    case 5: jit_hermite_e_t<1, 2>(xpa, xpb, i2p, E); break;
+    case 6: jit_hermite_e_t<2, 0>(xpa, xpb, i2p, E); break;
+    case 7: jit_hermite_e_t<2, 1>(xpa, xpb, i2p, E); break;
+    default: jit_hermite_e_t<2, 2>(xpa, xpb, i2p, E); break;
+    }
 }
 
 // Input value of index k (k < 0: none).
@@ -314,10 +351,9 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                     const double *pi = v.prim + 6 * (size_t)(pa0 + i), *pj = v.prim + 6 * (size_t)(pb0 + j);
                     const double p = pi[0], qq = pj[0];
                     double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
-                    for (int k = 0; k < esa + esb; k++) Ea[k] = 0.0;
                     for (int d = 0; d < 3; d++) {
-                        jit_hermite_e(la, lb, pi[1 + d] - Ag[d], pi[1 + d] - Ag[3 + d], pi[5], Ea + d * esa_d);
-                        jit_hermite_e(lc, ld, pj[1 + d] - Cg[d], pj[1 + d] - Cg[3 + d], pj[5], Eb + d * esb_d);
+                        jit_hermite_e_any(la, lb, pi[1 + d] - Ag[d], pi[1 + d] - Ag[3 + d], pi[5], Ea + d * esa_d);
+                        jit_hermite_e_any(lc, ld, pj[1 + d] - Cg[d], pj[1 + d] - Cg[3 + d], pj[5], Eb + d * esb_d);
                     }
                     const double pq = p + qq, alpha = p * qq / pq;
                     const double X = pi[1] - pj[1], Y = pi[2] - pj[2], Z = pi[3] - pj[3];
@@ -361,7 +397,7 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                     for (int q = sl; q < nqb; q += nsl) {
                         const double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
                         double val;
-                        if (Lab > TMAX || Lcd > TMAX)
+                        if (Lab > TMAX || Lcd > TMAX || Lab + Lcd > 5)
                             val = jit_comp_box(Ea, ox, oy, oz, ax + bx, ay + by, az + bz, Eb, px, py, pz, cx + dx,
                                                cy + dy, cz + dz, R, hm);
                         else if (Lab == 0) val = jit_comp_l<0, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
