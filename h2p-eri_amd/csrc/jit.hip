@@ -300,6 +300,45 @@ __host__ __device__ inline void jit_hermite_e_any(int la, int lb, double xpa, do
     }
 }
 
+// The in-place Hermite recursion for a fixed total order LT, in registers:
+// for n = LT-1 .. 0 the entries of order <= LT-n in decreasing index (so
+// the orders an entry reads still hold R^{(n+1)}), then R^{(n)}_{000};
+// R^{(0)} written out at the end.
+template <int H>
+struct JitStep {   // R_h = X_dir R_{h1} + c R_{h2}
+    static constexpr int t = jit_hcomp(H, 0), u = jit_hcomp(H, 1), w = jit_hcomp(H, 2);
+    static constexpr int dir = t > 0 ? 0 : (u > 0 ? 1 : 2);
+    static constexpr int c = dir == 0 ? t - 1 : (dir == 1 ? u - 1 : w - 1);
+    static constexpr int h1 = dir == 0 ? jit_herm(t - 1, u, w) : (dir == 1 ? jit_herm(t, u - 1, w) : jit_herm(t, u, w - 1));
+    static constexpr int h2 = c <= 0 ? -1 : (dir == 0 ? jit_herm(t - 2, u, w) : (dir == 1 ? jit_herm(t, u - 2, w) : jit_herm(t, u, w - 2)));
+};
+template <int H>
+__host__ __device__ inline void jit_rdesc(double *r, const double *xyz) {
+    if constexpr (H >= 1) {
+        using S = JitStep<H>;
+        if constexpr (S::h2 >= 0) r[H] = xyz[S::dir] * r[S::h1] + (double)S::c * r[S::h2];
+        else r[H] = xyz[S::dir] * r[S::h1];
+        jit_rdesc<H - 1>(r, xyz);
+    }
+}
+template <int LT, int N>
+__host__ __device__ inline void jit_rlevel(double *r, const double *xyz, const double *Fs) {
+    if constexpr (N >= 0) {
+        jit_rdesc<jit_nherm(LT - N) - 1>(r, xyz);
+        r[0] = Fs[N];
+        jit_rlevel<LT, N - 1>(r, xyz, Fs);
+    }
+}
+template <int LT>
+__host__ __device__ inline void jit_rrec(double X, double Y, double Z, const double *Fs, double *R) {
+    double r[jit_nherm(LT)];
+    const double xyz[3] = {X, Y, Z};
+    r[0] = Fs[LT];
+    jit_rlevel<LT, LT - 1>(r, xyz, Fs);
+#pragma unroll
+    for (int h = 0; h < jit_nherm(LT); h++) R[h] = r[h];
+}
+
 // Input value of index k (k < 0: none).
 __host__ __device__ inline double jit_in(const JitView &v, int k) {
     return k < 0 ? 0.0 : (k < v.N ? v.X[k] : v.Y0[k - v.N]);
@@ -366,9 +405,17 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                     // R^{(n)} from R^{(n+1)} in place, n = Lt-1 .. 0: orders
                     // from the highest down, so that the orders k-1 and k-2
                     // an order-k entry reads still hold R^{(n+1)}
+                    switch (Lt) {
+                    case 0: R[0] = Fs[0]; break;
+                    case 1: jit_rrec<1>(X, Y, Z, Fs, R); break;
+                    case 2: jit_rrec<2>(X, Y, Z, Fs, R); break;
+                    case 3: jit_rrec<3>(X, Y, Z, Fs, R); break;
+                    case 4: jit_rrec<4>(X, Y, Z, Fs, R); break;
+                    default: break;
+                    }
                     const JitTabs &tb = jit_tabs();
-                    R[0] = Fs[Lt];
-                    for (int n = Lt - 1; n >= 0; n--) {
+                    if (Lt > 4) R[0] = Fs[Lt];
+                    for (int n = Lt > 4 ? Lt - 1 : -1; n >= 0; n--) {
                         for (int k = Lt - n; k >= 1; k--) {
                             const int hb = k * (k + 1) * (k + 2) / 6, he = (k + 1) * (k + 2) * (k + 3) / 6;
                             for (int h = hb; h < he; h++) {
