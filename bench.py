@@ -697,6 +697,8 @@ def main():
     ap.add_argument("--no-one-core", action="store_true")
     ap.add_argument("--exchange", action="store_true", help="time the exchange build too (default with --config 4)")
     ap.add_argument("--no-jit", action="store_true", help="skip the on-the-fly (BD_JIT) matvec record")
+    ap.add_argument("--jit-only", type=int, default=0, metavar="REPS",
+                    help="profiling: only the on-the-fly (BD_JIT) record, REPS timed matvecs")
     ap.add_argument("--tuned", default="", help="EXECUTOR,TILE_BYTES,SCHEDULE[,WAVE_TILE_BYTES[,NEAR_FIELD_GRID]] instead of running the "
                     "autotuner (profiling runs, where the profiler distorts the tuner's timings)")
     ap.add_argument("--passes-only", type=int, default=0,
@@ -717,6 +719,10 @@ def main():
         sys.exit(2)
     if args.config == 4:
         args.exchange = True
+    if args.jit_only:
+        import torch
+        print(json.dumps(jit_measure(torch, torch.cuda.current_stream(), reps=args.jit_only)), flush=True)
+        return
     if args.config == 3:
         group_workload(args)
     else:
