@@ -11,7 +11,10 @@ in each matvec (src/H2ERI_matvec.c:411-568,761-879).  Checks:
 * the reference's fixture J_alk32 (tests/J_alk32_ccpvdz.bin of H2P-ERI) from
   the on-the-fly Coulomb build, bar 5e-7 as for the stored form, and that J
   against the oracle's J on the dense-stored C10H22 to <= 1e-13;
-* entry points that need stored blocks refuse a BD_JIT handle.
+* entry points that need stored blocks refuse a BD_JIT handle;
+* at the benchmark size (C30H62): symmetry and linearity to rounding, and
+  agreement with the stored (compressed) representation within the
+  compression tolerance.
 """
 import os
 
@@ -97,3 +100,26 @@ def test_jit_coulomb_alk32_fixture(tmp_path):
     p = str(tmp_path / "alk32_dense.rep")
     hd.save(p)
     assert relerr(J, oracle_bind.Oracle(p).coulomb(D)) < 1e-13
+
+
+def test_jit_c30h62_full_size():
+    """C30H62 at reltol 1e-6 (the benchmark molecule, 6.1e7 shell quartets
+    per matvec): the on-the-fly operator is symmetric and linear to rounding
+    at full size, and it differs from the stored (compressed) representation
+    of the same molecule only by the compression tolerance (reltol 1e-6 on
+    the blocks; 2e-6 on C10H22)."""
+    import h2eri
+    from h2eri import molecules as M
+    atoms, txt, _, reltol = M.workload(1)
+    hj = h2eri.H2ERI(1e-10, 1e-10, reltol)
+    hj.read_mol_text(txt).process_shells().partition().build_h2(1)
+    rng = np.random.default_rng(31)
+    u, v = rng.standard_normal(hj.n), rng.standard_normal(hj.n)
+    print(f"jit C30H62 N={hj.n}", flush=True)
+    Au, Av = hj.matvec(u), hj.matvec(v)
+    assert abs(u @ Av - v @ Au) <= 1e-12 * abs(u @ Av)
+    assert relerr(hj.matvec(2.0 * u - 3.0 * v), 2.0 * Au - 3.0 * Av) < 1e-13
+    hs = h2eri.H2ERI(1e-10, 1e-10, reltol)
+    hs.build_from_mol(text=txt)
+    e = relerr(hs.matvec(u), Au)
+    assert 0.0 < e < 1e-5, e
