@@ -97,7 +97,7 @@ __host__ __device__ constexpr int jit_herm(int t, int u, int v) {
     return k * (k + 1) * (k + 2) / 6 + a * (a + 1) / 2 + v;
 }
 // exponents of the k-th lexicographic Cartesian component of order L
-__host__ __device__ inline void jit_cart(int L, int k, int &lx, int &ly, int &lz) {
+__host__ __device__ constexpr void jit_cart(int L, int k, int &lx, int &ly, int &lz) {
     int a = 0;
     while ((a + 1) * (a + 2) / 2 <= k) a++;
     lx = L - a;
@@ -109,12 +109,38 @@ __host__ __device__ inline void jit_cart(int L, int k, int &lx, int &ly, int &lz
 // Hermite triple h = 1..nherm(8)-1 in jit_herm order, the step that builds
 // it (R^{(n)}_h = X_dir R^{(n+1)}_{h1} + c R^{(n+1)}_{h2}); jit_herm of every
 // triple of order <= 8 (-1 above); 1/k! and 1/(2m-1).
+// Also: for each pair class (la, lb) and Cartesian pair index ab, the
+// offsets of its E rows in the pair's table (x, y, z) and its orders per
+// direction (packed 4 bits each); and magic[d] = ceil(2^32 / d) (2^32 for
+// d = 1, so 64 bits), so that x / d = (x magic[d]) >> 32 exactly for
+// x d < 2^32.
 struct JitTabs {
     int rstep[165][4];   // h1, h2 (-1: none), c, direction
     short herm[9][9][9];
     double inv_fact[8];
     double inv_odd[10];
-    constexpr JitTabs() : rstep{}, herm{}, inv_fact{}, inv_odd{} {
+    short eoff[100][4];  // pair classes in (la, lb) order, each nab rows
+    short pcoff[9];      // first eoff row of pair class la * 3 + lb
+    unsigned long long magic[1297];
+    constexpr JitTabs() : rstep{}, herm{}, inv_fact{}, inv_odd{}, eoff{}, pcoff{}, magic{} {
+        int row = 0;
+        for (int la = 0; la <= 2; la++)
+            for (int lb = 0; lb <= 2; lb++) {
+                pcoff[la * 3 + lb] = (short)row;
+                const int L = la + lb, esd = (la + 1) * (lb + 1) * (L + 1);
+                const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
+                for (int ia = 0; ia < na; ia++)
+                    for (int ib = 0; ib < nb; ib++, row++) {
+                        int ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+                        jit_cart(la, ia, ax, ay, az);
+                        jit_cart(lb, ib, bx, by, bz);
+                        eoff[row][0] = (short)((ax * (lb + 1) + bx) * (L + 1));
+                        eoff[row][1] = (short)((ay * (lb + 1) + by) * (L + 1) + esd);
+                        eoff[row][2] = (short)((az * (lb + 1) + bz) * (L + 1) + 2 * esd);
+                        eoff[row][3] = (short)((ax + bx) | ((ay + by) << 4) | ((az + bz) << 8));
+                    }
+            }
+        for (int d = 1; d < 1297; d++) magic[d] = (0x100000000ULL + (unsigned long long)d - 1) / (unsigned long long)d;
         for (int t = 0; t < 9; t++)
             for (int u = 0; u < 9; u++)
                 for (int w = 0; w < 9; w++) herm[t][u][w] = (short)(t + u + w <= 8 ? jit_herm(t, u, w) : -1);
@@ -339,6 +365,11 @@ __host__ __device__ inline void jit_rrec(double X, double Y, double Z, const dou
     for (int h = 0; h < jit_nherm(LT); h++) R[h] = r[h];
 }
 
+// x / d for x d < 2^32, with m = magic[d].
+__host__ __device__ inline int jit_div(int x, unsigned long long m) {
+    return (int)(((unsigned long long)(unsigned)x * m) >> 32);
+}
+
 // Input value of index k (k < 0: none).
 __host__ __device__ inline double jit_in(const JitView &v, int k) {
     return k < 0 ? 0.0 : (k < v.N ? v.X[k] : v.Y0[k - v.N]);
@@ -353,7 +384,7 @@ __host__ __device__ inline double jit_in(const JitView &v, int k) {
 // components than lanes, so that the lanes left over share the primitive
 // loop), added in slice order at the end of the quartet.
 template <int NL, int LDSN, int TMAX>
-__host__ __device__ void jit_task(const JitView &v, int task, int lane, double *lds, const short *hm) {
+__host__ __device__ void jit_task(const JitView &v, int task, int lane, double *lds, const short *hm, const short *eo) {
     const int a = v.t_pair[task];
     const int la = v.pinfo[4 * a], lb = v.pinfo[4 * a + 1], npa = v.pinfo[4 * a + 2], pa0 = v.pinfo[4 * a + 3];
     const double *Ag = v.pgeo + 6 * a;
@@ -374,11 +405,14 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
             const int nH = jit_nherm(Lt), rsz = nH + Lt + 1;   // R^{(0)} and the scaled Boys values
             const int ncomp = nab * ncd;
             const int perq = esa + esb + rsz;
-            const int nsl = ncomp >= NL ? 1 : NL / ncomp, npart = nsl * ncomp;
+            const JitTabs &tb = jit_tabs();
+            const unsigned long long m_comp = tb.magic[ncomp], m_cd = tb.magic[ncd], m_b = tb.magic[npb];
+            const int nsl = ncomp >= NL ? 1 : jit_div(NL, m_comp), npart = nsl * ncomp;
             const int preg = ncomp > 64 ? ncomp : 64;   // partial-sum region (npart <= preg)
             double *acc = lds, *vin = lds + preg, *qb = vin + ncd;
-            int bq = (LDSN - preg - ncd) / perq;
+            int bq = jit_div(LDSN - preg - ncd, tb.magic[perq]);
             if (bq > NL) bq = NL;
+            const short *eo_a = eo + 4 * tb.pcoff[la * 3 + lb], *eo_c = eo + 4 * tb.pcoff[lc * 3 + ld];
             for (int c = lane; c < npart; c += NL) acc[c] = 0.0;
             for (int c = lane; c < ncd; c += NL) vin[c] = jit_in(v, km[c]);
             const int nq = npa * npb;
@@ -386,7 +420,7 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                 const int nqb = nq - q0 < bq ? nq - q0 : bq;
                 jit_sync();
                 for (int q = lane; q < nqb; q += NL) {   // E tables and R of one primitive quartet
-                    const int i = (q0 + q) / npb, j = (q0 + q) - ((q0 + q) / npb) * npb;
+                    const int i = jit_div(q0 + q, m_b), j = q0 + q - i * npb;
                     const double *pi = v.prim + 6 * (size_t)(pa0 + i), *pj = v.prim + 6 * (size_t)(pb0 + j);
                     const double p = pi[0], qq = pj[0];
                     double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
@@ -429,24 +463,17 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                 }
                 jit_sync();
                 for (int it = lane; it < npart; it += NL) {   // components over the primitive quartets
-                    const int sl = it / ncomp, cmp = it - sl * ncomp;
-                    const int ab = cmp / ncd, cd = cmp - ab * ncd;
-                    int ax, ay, az, bx, by, bz, cx, cy, cz, dx, dy, dz;
-                    jit_cart(la, ab / nb, ax, ay, az);
-                    jit_cart(lb, ab - (ab / nb) * nb, bx, by, bz);
-                    jit_cart(lc, cd / nd, cx, cy, cz);
-                    jit_cart(ld, cd - (cd / nd) * nd, dx, dy, dz);
-                    const int ox = (ax * (lb + 1) + bx) * (Lab + 1), oy = (ay * (lb + 1) + by) * (Lab + 1) + esa_d,
-                              oz = (az * (lb + 1) + bz) * (Lab + 1) + 2 * esa_d;
-                    const int px = (cx * (ld + 1) + dx) * (Lcd + 1), py = (cy * (ld + 1) + dy) * (Lcd + 1) + esb_d,
-                              pz = (cz * (ld + 1) + dz) * (Lcd + 1) + 2 * esb_d;
+                    const int sl = jit_div(it, m_comp), cmp = it - sl * ncomp;
+                    const int ab = jit_div(cmp, m_cd), cd = cmp - ab * ncd;
+                    const short *ea = eo_a + 4 * ab, *ec = eo_c + 4 * cd;
+                    const int ox = ea[0], oy = ea[1], oz = ea[2], px = ec[0], py = ec[1], pz = ec[2];
                     double sum = acc[it];
                     for (int q = sl; q < nqb; q += nsl) {
                         const double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
                         double val;
                         if (Lab > TMAX || Lcd > TMAX || Lab + Lcd > 5)
-                            val = jit_comp_box(Ea, ox, oy, oz, ax + bx, ay + by, az + bz, Eb, px, py, pz, cx + dx,
-                                               cy + dy, cz + dz, R, hm);
+                            val = jit_comp_box(Ea, ox, oy, oz, ea[3] & 15, (ea[3] >> 4) & 15, ea[3] >> 8, Eb, px, py,
+                                               pz, ec[3] & 15, (ec[3] >> 4) & 15, ec[3] >> 8, R, hm);
                         else if (Lab == 0) val = jit_comp_l<0, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
                         else if (Lab == 1) val = jit_comp_l<1, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
                         else if (Lab == 2) val = jit_comp_l<2, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
@@ -511,11 +538,13 @@ template <int LDSN>
 __global__ void __launch_bounds__(64, LDSN == JIT_LDS_S ? 3 : 2) jit_task_kernel(JitView v, int t0, int t1) {
     __shared__ double lds[LDSN];
     __shared__ short hm[729];   // jit_herm by (t, u, w), for lane-dependent lookups
+    __shared__ short eo[400];   // E-row offsets by pair class and Cartesian pair
     const int task = t0 + blockIdx.x;
     if (task >= t1) return;
     for (int k = threadIdx.x; k < 729; k += 64) hm[k] = (&jit_tabs_dev.herm[0][0][0])[k];
+    for (int k = threadIdx.x; k < 400; k += 64) eo[k] = (&jit_tabs_dev.eoff[0][0])[k];
     __syncthreads();
-    jit_task<64, LDSN, LDSN == JIT_LDS_S ? 2 : 3>(v, task, threadIdx.x, lds, hm);
+    jit_task<64, LDSN, LDSN == JIT_LDS_S ? 2 : 3>(v, task, threadIdx.x, lds, hm, eo);
 }
 
 // Up sweep of one level: y0[n] = U_n^T in_n for the listed nodes, where the
@@ -623,6 +652,10 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
         const PairView pv = pair_view(hs->pairs[i]);
         if (pv.la > JIT_LMAX || pv.lb > JIT_LMAX || pv.la < 0 || pv.lb < 0) {
             set_error(ERR_UNSUPPORTED, "on-the-fly B/D: shell pair %d has angular momentum above d", i);
+            return false;
+        }
+        if (pv.np > 1296) {   // the multiply-high divisions by the primitive-pair count
+            set_error(ERR_UNSUPPORTED, "on-the-fly B/D: shell pair %d has %d primitive pairs (at most 1296)", i, pv.np);
             return false;
         }
         if (jit_ncart(pv.la) * jit_ncart(pv.lb) != h->sp_nbfp[i] ||
@@ -1171,8 +1204,9 @@ int H2ERI_jit_emulate(H2ERI_p h, const double *x, double *y) {
     auto worker = [&]() {
         std::vector<double> lds(JIT_LDS);
         for (int t = next++; t < ntask; t = next++) {
-            if (t < P.n_big) jit_task<1, JIT_LDS, 3>(v, t, 0, lds.data(), &jit_tabs_host.herm[0][0][0]);
-            else jit_task<1, JIT_LDS_S, 2>(v, t, 0, lds.data(), &jit_tabs_host.herm[0][0][0]);
+            const short *hm = &jit_tabs_host.herm[0][0][0], *eo = &jit_tabs_host.eoff[0][0];
+            if (t < P.n_big) jit_task<1, JIT_LDS, 3>(v, t, 0, lds.data(), hm, eo);
+            else jit_task<1, JIT_LDS_S, 2>(v, t, 0, lds.data(), hm, eo);
         }
     };
     const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
