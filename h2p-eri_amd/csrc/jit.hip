@@ -69,7 +69,7 @@ struct JitView {
     const double *prim;   // per primitive pair: p, P (3), K, 1/(2p)
     const double *boys;   // Boys table [grid][bcol]
     int bgrid, bcol;
-    double bstep, btmax;
+    double bstep, btmax, inv_bstep;
     const int *kl_pair, *kl_moff;     // per partner-list entry: shell pair, its input map
     const int *kmap;                  // input index per column (-1: not used)
     const int *t_pair, *t_e0, *t_e1;  // per task: bra pair, its range of gent
@@ -175,7 +175,7 @@ __host__ __device__ inline const JitTabs &jit_tabs() {
 // recursion; erf and upward recursion above).
 __host__ __device__ inline void jit_boys(const JitView &v, int mmax, double T, double *out, int stride) {
     if (T <= v.btmax) {
-        const int gi = (int)(T / v.bstep + 0.5);
+        const int gi = (int)(T * v.inv_bstep + 0.5);
         const double dT = gi * v.bstep - T;
         const double *row = v.boys + (size_t)gi * v.bcol;
         const JitTabs &tb = jit_tabs();
@@ -414,7 +414,15 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
             if (bq > NL) bq = NL;
             const short *eo_a = eo + 4 * tb.pcoff[la * 3 + lb], *eo_c = eo + 4 * tb.pcoff[lc * 3 + ld];
             for (int c = lane; c < npart; c += NL) acc[c] = 0.0;
-            for (int c = lane; c < ncd; c += NL) vin[c] = jit_in(v, km[c]);
+            // the partner's input: on the device one value per lane, held in a
+            // register and stored after the first batch's E/R work, so the
+            // global load's latency hides behind it
+            double vreg = 0.0;
+            if constexpr (NL >= JIT_NAB) {
+                if (lane < ncd) vreg = jit_in(v, km[lane]);
+            } else {
+                for (int c = lane; c < ncd; c += NL) vin[c] = jit_in(v, km[c]);
+            }
             const int nq = npa * npb;
             for (int q0 = 0; q0 < nq; q0 += bq) {
                 const int nqb = nq - q0 < bq ? nq - q0 : bq;
@@ -428,10 +436,11 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                         jit_hermite_e_any(la, lb, pi[1 + d] - Ag[d], pi[1 + d] - Ag[3 + d], pi[5], Ea + d * esa_d);
                         jit_hermite_e_any(lc, ld, pj[1 + d] - Cg[d], pj[1 + d] - Cg[3 + d], pj[5], Eb + d * esb_d);
                     }
-                    const double pq = p + qq, alpha = p * qq / pq;
+                    const double pq = p + qq, inv_pq = 1.0 / pq, alpha = p * qq * inv_pq;
                     const double X = pi[1] - pj[1], Y = pi[2] - pj[2], Z = pi[3] - pj[3];
                     const double T = alpha * (X * X + Y * Y + Z * Z);
-                    const double pre = 2.0 * M_PI * M_PI * sqrt(M_PI) / (p * qq * sqrt(pq)) * pi[4] * pj[4];
+                    // 2 pi^{5/2} / (p q sqrt(p + q)) K_i K_j, with 1/p = 2 pi[5]
+                    const double pre = 2.0 * M_PI * M_PI * sqrt(M_PI) * (4.0 * pi[5] * pj[5]) * sqrt(inv_pq) * pi[4] * pj[4];
                     double *Fs = R + nH;   // pre (-2 alpha)^n F_n
                     jit_boys(v, Lt, T, Fs, 1);
                     double s = pre;
@@ -461,6 +470,8 @@ __host__ __device__ void jit_task(const JitView &v, int task, int lane, double *
                         R[0] = Fs[n];
                     }
                 }
+                if constexpr (NL >= JIT_NAB)
+                    if (q0 == 0 && lane < ncd) vin[lane] = vreg;
                 jit_sync();
                 for (int it = lane; it < npart; it += NL) {   // components over the primitive quartets
                     const int sl = jit_div(it, m_comp), cmp = it - sl * ncomp;
@@ -1103,6 +1114,7 @@ bool jit_upload(H2ERI_p h, JitState *&js) {
     v.bcol = P.bcol;
     v.bstep = P.bstep;
     v.btmax = P.btmax;
+    v.inv_bstep = 1.0 / P.bstep;
     js->d_U = js->d_dbls + o_U;
     v.Y0 = js->Y0;
     v.Y1 = js->Y1;
@@ -1191,7 +1203,7 @@ int H2ERI_jit_emulate(H2ERI_p h, const double *x, double *y) {
     JitView v{};
     v.N = N;
     v.pinfo = P.pinfo.data(); v.pgeo = P.pgeo.data(); v.prim = P.prim.data();
-    v.boys = P.boys; v.bgrid = P.bgrid; v.bcol = P.bcol; v.bstep = P.bstep; v.btmax = P.btmax;
+    v.boys = P.boys; v.bgrid = P.bgrid; v.bcol = P.bcol; v.bstep = P.bstep; v.btmax = P.btmax; v.inv_bstep = 1.0 / P.bstep;
     v.kl_pair = P.kl_pair.data(); v.kl_moff = P.kl_moff.data(); v.kmap = P.kmap.data();
     v.t_pair = P.t_pair.data(); v.t_e0 = P.t_e0.data(); v.t_e1 = P.t_e1.data(); v.gent = P.gent.data();
     v.r_pair = P.r_pair.data(); v.r_omap = P.r_omap.data(); v.r_t0 = P.r_t0.data(); v.r_t1 = P.r_t1.data();
