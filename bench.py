@@ -556,7 +556,50 @@ def jit_measure(torch, stream, cfg=1, reps=3):
            "G_integrals_per_s": round(st["eri_values"] / best / 1e6, 3),
            "resident_MB": round(hj.upload() / 1e6, 1), "build_s": round(t_build, 2)}
     hj.close()
+    rec["host_engine_same_blocks"] = jit_host_compare(torch, stream)
     return rec
+
+
+def jit_host_compare(torch, stream, cfg=0, reps=3):
+    """The reference's CPU on-the-fly path needs SIMINT, absent here; the
+    like-for-like CPU figure is this project's host integral engine
+    evaluating the same (dense) B and D blocks, which a dense build times
+    (the B and D construction timers).  Beside it, the GPU on-the-fly matvec
+    on the same molecule (C10H22)."""
+    import numpy as np
+
+    import h2eri
+    from h2eri import molecules as M
+    atoms, txt, _, reltol = M.workload(cfg)
+    hd = h2eri.H2ERI(1e-10, 1e-10, reltol)
+    hd.set_block_compression(0)
+    hd.build_from_mol(text=txt)
+    cpu_s = float(hd.s.timers[2] + hd.s.timers[3])   # B_BUILD_TIMER_IDX, D_BUILD_TIMER_IDX
+    hd.close()
+    hj = h2eri.H2ERI(1e-10, 1e-10, reltol)
+    hj.read_mol_text(txt).process_shells().partition().build_h2(1)
+    x = torch.from_numpy(np.random.default_rng(18).standard_normal(hj.n)).to("cuda:0")
+    y = torch.empty_like(x)
+    hj.matvec_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        hj.matvec_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    hj.close()
+    threads = cpu_threads()
+    log(f"[bench] {M.workload_name(cfg)} B and D blocks: host engine {cpu_s:.2f} s ({threads} threads), "
+        f"GPU on-the-fly matvec {min(ms):.1f} ms")
+    return {"workload": f"{M.workload_name(cfg)} cc-pVDZ, reltol {reltol:g}", "cpu_s": round(cpu_s, 3),
+            "cpu_threads": threads, "gpu_ms_per_matvec": round(min(ms), 3),
+            "ratio": round(cpu_s * 1e3 / min(ms), 1),
+            "note": "CPU: the host integral engine evaluating the dense B and D blocks (B and D construction "
+                    "timers of a dense build); GPU: one on-the-fly matvec, which evaluates the same blocks "
+                    "from both sides and applies them"}
 
 
 def exchange_measure(h, torch, D, D_dev, stream):
