@@ -519,7 +519,7 @@ def single_workload(args):
     print(json.dumps(out), flush=True)
 
 
-def jit_measure(torch, stream, cfg=1, reps=3):
+def jit_measure(torch, stream, cfg=1, reps=3, host_compare=True):
     """The on-the-fly form (BD_JIT = 1: B and D evaluated from the shell
     pairs in every matvec) on C30H62 at reltol 1e-6: ms per FP64 matvec
     (HIP events, device vectors), with the integral work of one matvec."""
@@ -556,7 +556,8 @@ def jit_measure(torch, stream, cfg=1, reps=3):
            "G_integrals_per_s": round(st["eri_values"] / best / 1e6, 3),
            "resident_MB": round(hj.upload() / 1e6, 1), "build_s": round(t_build, 2)}
     hj.close()
-    rec["host_engine_same_blocks"] = jit_host_compare(torch, stream)
+    if host_compare:
+        rec["host_engine_same_blocks"] = jit_host_compare(torch, stream)
     return rec
 
 
@@ -742,6 +743,8 @@ def main():
     ap.add_argument("--no-jit", action="store_true", help="skip the on-the-fly (BD_JIT) matvec record")
     ap.add_argument("--jit-only", type=int, default=0, metavar="REPS",
                     help="profiling: only the on-the-fly (BD_JIT) record, REPS timed matvecs")
+    ap.add_argument("--no-jit-cpu", action="store_true",
+                    help="skip the host-engine comparison of the on-the-fly record")
     ap.add_argument("--tuned", default="", help="EXECUTOR,TILE_BYTES,SCHEDULE[,WAVE_TILE_BYTES[,NEAR_FIELD_GRID]] instead of running the "
                     "autotuner (profiling runs, where the profiler distorts the tuner's timings)")
     ap.add_argument("--passes-only", type=int, default=0,
@@ -764,7 +767,8 @@ def main():
         args.exchange = True
     if args.jit_only:
         import torch
-        print(json.dumps(jit_measure(torch, torch.cuda.current_stream(), reps=args.jit_only)), flush=True)
+        print(json.dumps(jit_measure(torch, torch.cuda.current_stream(), reps=args.jit_only,
+                                     host_compare=not args.no_jit_cpu)), flush=True)
         return
     if args.config == 3:
         group_workload(args)
