@@ -255,6 +255,14 @@ void build_U(H2ERI_p h, uint64_t seed) {
     h->U = (H2E_dense_mat_p *)calloc(n_node, sizeof(H2E_dense_mat_p));
     h->J_pair = (H2E_int_vec_p *)calloc(n_node, sizeof(H2E_int_vec_p));
     h->J_row = (H2E_int_vec_p *)calloc(n_node, sizeof(H2E_int_vec_p));
+    if (h->n_r_adm_pair == 0) {   // no far field (e.g. a single-leaf tree): no node needs a basis
+        for (int i = 0; i < n_node; i++) {
+            h->U[i] = dmat_new(0, 0);
+            h->J_pair[i] = ivec_new(1);
+            h->J_row[i] = ivec_new(1);
+        }
+        return;
+    }
     std::vector<std::vector<int>> ff;
     calc_ovlp_ff(h, ff);
     std::vector<char> skel(np, 1);

@@ -116,3 +116,75 @@ def test_dense_blocks_option(water6):
     hc.build_from_mol(text=M.mol_text(M.water_cluster(6, rng)))
     sc = hc.s
     assert any(sc.c_D_blks[i].contents.ld < 0 for i in range(sc.n_D))
+
+
+def test_jit_refuses_shells_above_d():
+    """BD_JIT = 1 evaluates shells up to d on the device; a basis with an f
+    shell is refused at H2ERI_build_H2 with an error (nothing built), and the
+    same molecule still builds with stored blocks."""
+    import h2eri
+    mol = """2
+H 2 4 4
+0.0 0.0 0.0
+S 3 1
+13.01    0.0196850
+1.962    0.1379770
+0.4446    0.4781480
+F 1 1
+0.8    1.0
+H 2 4 4
+0.0 0.0 1.4
+S 3 1
+13.01    0.0196850
+1.962    0.1379770
+0.4446    0.4781480
+F 1 1
+0.8    1.0
+"""
+    h = h2eri.H2ERI(1e-10, 1e-10, 1e-6)
+    h.read_mol_text(mol).process_shells().partition()
+    with pytest.raises(h2eri.H2ERIError, match="up to d"):
+        h.build_h2(1)
+    assert h.s.BD_JIT == 0
+    h.build_h2(0)
+    assert h.s.BD_JIT == 0 and h.s.n_D > 0
+
+
+def test_single_leaf_molecule(tmp_path):
+    """A molecule small enough to be one leaf (no far field, one diagonal
+    near-field block): the stored and the on-the-fly representation both
+    build, and the device plan (host emulation) and the on-the-fly task code
+    (host emulation) equal the oracle."""
+    import h2eri
+    mol = """2
+H 2 4 4
+0.0 0.0 0.0
+S 3 1
+13.01    0.0196850
+1.962    0.1379770
+0.4446    0.4781480
+P 1 1
+0.727    1.0
+H 2 4 4
+0.0 0.0 1.4
+S 3 1
+13.01    0.0196850
+1.962    0.1379770
+0.4446    0.4781480
+P 1 1
+0.727    1.0
+"""
+    hd = h2eri.H2ERI(1e-10, 1e-10, 1e-6)
+    hd.set_block_compression(0)
+    hd.read_mol_text(mol).process_shells().partition().build_h2(0)
+    assert hd.s.n_node == 1 and hd.s.n_r_adm_pair == 0
+    p = str(tmp_path / "h2.rep")
+    hd.save(p)
+    o = oracle_bind.Oracle(p)
+    x = np.random.default_rng(41).standard_normal(hd.n)
+    yo = o.matvec(x)
+    y, conflicts = hd.plan_emulate(x)
+    assert conflicts == 0 and relerr(y, yo) < 1e-13
+    hj = h2eri.H2ERI(1e-10, 1e-10, 1e-6)
+    hj.read_mol_text(mol).process_shells().partition().build_h2(1)
+    assert relerr(hj.jit_emulate(x), yo) < 1e-13
