@@ -59,6 +59,8 @@ static const int JIT_LMAX = 2;     // shell angular momentum (d)
 static const int JIT_LDS = 2048;   // LDS doubles of a task with a bra or ket pair of la + lb >= 3
 static const int JIT_LDS_S = 1280; // LDS doubles of a task whose pairs all have la + lb <= 2
 static const int JIT_NAB = 36;     // Cartesian rows of a shell pair, at most
+static const int JIT_KMAX = 8;     // kets per pass, at most
+static const int JIT_PASSQ = 64;   // primitive quartets of a multi-ket pass, at most
 
 // Everything a row task reads or writes; the pointers are device or, in the
 // emulator, host arrays.
@@ -383,139 +385,239 @@ __host__ __device__ inline double jit_in(const JitView &v, int k) {
 // quartets live in nsl slices (NL / ncomp of them when a quartet has fewer
 // components than lanes, so that the lanes left over share the primitive
 // loop), added in slice order at the end of the quartet.
+// E tables of both pairs and the Hermite integrals R of one primitive
+// quartet (bra primitive pair pi, ket primitive pair pj) into Ea (esa
+// doubles), Eb = Ea + esa (esb) and R = Eb + esb (nH(Lt) + Lt + 1).
+__host__ __device__ inline void jit_prim(const JitView &v, int la, int lb, int lc, int ld, int esa_d, int esb_d,
+                                         const double *pi, const double *pj, const double *Ag, const double *Cg,
+                                         double *Ea) {
+    const int Lt = la + lb + lc + ld, nH = jit_nherm(Lt);
+    double *Eb = Ea + 3 * esa_d, *R = Eb + 3 * esb_d;
+    const double p = pi[0], qq = pj[0];
+    for (int d = 0; d < 3; d++) {
+        jit_hermite_e_any(la, lb, pi[1 + d] - Ag[d], pi[1 + d] - Ag[3 + d], pi[5], Ea + d * esa_d);
+        jit_hermite_e_any(lc, ld, pj[1 + d] - Cg[d], pj[1 + d] - Cg[3 + d], pj[5], Eb + d * esb_d);
+    }
+    const double pq = p + qq, inv_pq = 1.0 / pq, alpha = p * qq * inv_pq;
+    const double X = pi[1] - pj[1], Y = pi[2] - pj[2], Z = pi[3] - pj[3];
+    const double T = alpha * (X * X + Y * Y + Z * Z);
+    // 2 pi^{5/2} / (p q sqrt(p + q)) K_i K_j, with 1/p = 2 pi[5]
+    const double pre = 2.0 * M_PI * M_PI * sqrt(M_PI) * (4.0 * pi[5] * pj[5]) * sqrt(inv_pq) * pi[4] * pj[4];
+    double *Fs = R + nH;   // pre (-2 alpha)^n F_n
+    jit_boys(v, Lt, T, Fs, 1);
+    double s = pre;
+    for (int n = 0; n <= Lt; n++) { Fs[n] *= s; s *= -2.0 * alpha; }
+    // R^{(n)} from R^{(n+1)} in place, n = Lt-1 .. 0: orders from the
+    // highest down, so that the orders k-1 and k-2 an order-k entry reads
+    // still hold R^{(n+1)}
+    switch (Lt) {
+    case 0: R[0] = Fs[0]; break;
+    case 1: jit_rrec<1>(X, Y, Z, Fs, R); break;
+    case 2: jit_rrec<2>(X, Y, Z, Fs, R); break;
+    case 3: jit_rrec<3>(X, Y, Z, Fs, R); break;
+    case 4: jit_rrec<4>(X, Y, Z, Fs, R); break;
+    default: break;
+    }
+    const JitTabs &tb = jit_tabs();
+    if (Lt > 4) R[0] = Fs[Lt];
+    for (int n = Lt > 4 ? Lt - 1 : -1; n >= 0; n--) {
+        for (int k = Lt - n; k >= 1; k--) {
+            const int hb = k * (k + 1) * (k + 2) / 6, he = (k + 1) * (k + 2) * (k + 3) / 6;
+            for (int h = hb; h < he; h++) {
+                const int h1 = tb.rstep[h][0], h2 = tb.rstep[h][1], d = tb.rstep[h][3];
+                const double xx = d == 0 ? X : (d == 1 ? Y : Z);
+                R[h] = h2 >= 0 ? xx * R[h1] + (double)tb.rstep[h][2] * R[h2] : xx * R[h1];
+            }
+        }
+        R[0] = Fs[n];
+    }
+}
+
+// One component (E-row offsets and orders ea, ec) on one primitive quartet.
+template <int TMAX>
+__host__ __device__ inline double jit_cval(int Lab, int Lcd, const double *Ea, const double *Eb, const double *R,
+                                           const short *ea, const short *ec, const short *hm) {
+    const int ox = ea[0], oy = ea[1], oz = ea[2], px = ec[0], py = ec[1], pz = ec[2];
+    if (Lab > TMAX || Lcd > TMAX || Lab + Lcd > 5)
+        return jit_comp_box(Ea, ox, oy, oz, ea[3] & 15, (ea[3] >> 4) & 15, ea[3] >> 8, Eb, px, py, pz, ec[3] & 15,
+                            (ec[3] >> 4) & 15, ec[3] >> 8, R, hm);
+    if (Lab == 0) return jit_comp_l<0, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
+    if (Lab == 1) return jit_comp_l<1, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
+    if (Lab == 2) return jit_comp_l<2, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
+    if constexpr (TMAX >= 3) return jit_comp_l<3, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
+    return 0.0;
+}
+
+// One row task on NL lanes (64 on the device, 1 in the emulator), lds of
+// LDSN doubles (enough for every ket class against the task's bra class):
+// bra pair a against the partner entries gent[t_e0, t_e1), its partial rows
+// into part[task].
+//
+// Entries are taken in passes.  A pass is either one ket (its primitive
+// quartets in batches of up to NL, each component's sum over them split
+// into NL / ncomp slices when a quartet has fewer components than lanes,
+// the slices added in order), or up to JIT_KMAX consecutive kets of one
+// class whose primitive quartets number at most JIT_PASSQ together: their
+// E/R work shares the lanes, each (ket, component) sums its own primitive
+// quartets in order (in the small-LDS launch).  The pass boundaries depend
+// only on the task, so the summation order is fixed.  pk holds 6 * (JIT_KMAX + 1) integers of
+// per-ket data (LDS on the device).
 template <int NL, int LDSN, int TMAX>
-__host__ __device__ void jit_task(const JitView &v, int task, int lane, double *lds, const short *hm, const short *eo) {
+__host__ __device__ void jit_task(const JitView &v, int task, int lane, double *lds, const short *hm, const short *eo,
+                                  long long *pk) {
     const int a = v.t_pair[task];
     const int la = v.pinfo[4 * a], lb = v.pinfo[4 * a + 1], npa = v.pinfo[4 * a + 2], pa0 = v.pinfo[4 * a + 3];
     const double *Ag = v.pgeo + 6 * a;
     const int na = jit_ncart(la), nb = jit_ncart(lb), nab = na * nb, Lab = la + lb;
     const int esa_d = (la + 1) * (lb + 1) * (Lab + 1), esa = 3 * esa_d;
     constexpr int RPL = NL >= JIT_NAB ? 1 : JIT_NAB;   // rows per lane
+    constexpr int S = JIT_KMAX + 1;                   // pk block stride
     double racc[RPL];
     for (int k = 0; k < RPL; k++) racc[k] = 0.0;
-    {
-        for (int ge = v.t_e0[task]; ge < v.t_e1[task]; ge++) {
-            const int e = v.gent[ge];
-            const int b = v.kl_pair[e];
-            const int *km = v.kmap + v.kl_moff[e];
-            const int lc = v.pinfo[4 * b], ld = v.pinfo[4 * b + 1], npb = v.pinfo[4 * b + 2], pb0 = v.pinfo[4 * b + 3];
-            const double *Cg = v.pgeo + 6 * b;
-            const int nc = jit_ncart(lc), nd = jit_ncart(ld), ncd = nc * nd, Lcd = lc + ld, Lt = Lab + Lcd;
-            const int esb_d = (lc + 1) * (ld + 1) * (Lcd + 1), esb = 3 * esb_d;
-            const int nH = jit_nherm(Lt), rsz = nH + Lt + 1;   // R^{(0)} and the scaled Boys values
-            const int ncomp = nab * ncd;
-            const int perq = esa + esb + rsz;
-            const JitTabs &tb = jit_tabs();
-            const unsigned long long m_comp = tb.magic[ncomp], m_cd = tb.magic[ncd], m_b = tb.magic[npb];
-            const int nsl = ncomp >= NL ? 1 : jit_div(NL, m_comp), npart = nsl * ncomp;
-            const int preg = ncomp > 64 ? ncomp : 64;   // partial-sum region (npart <= preg)
-            double *acc = lds, *vin = lds + preg, *qb = vin + ncd;
-            int bq = jit_div(LDSN - preg - ncd, tb.magic[perq]);
-            if (bq > NL) bq = NL;
-            const short *eo_a = eo + 4 * tb.pcoff[la * 3 + lb], *eo_c = eo + 4 * tb.pcoff[lc * 3 + ld];
-            for (int c = lane; c < npart; c += NL) acc[c] = 0.0;
-            // the partner's input: on the device one value per lane, held in a
-            // register and stored after the first batch's E/R work, so the
-            // global load's latency hides behind it
-            double vreg = 0.0;
-            if constexpr (NL >= JIT_NAB) {
-                if (lane < ncd) vreg = jit_in(v, km[lane]);
-            } else {
-                for (int c = lane; c < ncd; c += NL) vin[c] = jit_in(v, km[c]);
+    const int te1 = v.t_e1[task];
+    for (int ge = v.t_e0[task]; ge < te1;) {
+        const int e = v.gent[ge];
+        const int b = v.kl_pair[e];
+        const int *km = v.kmap + v.kl_moff[e];
+        const int lc = v.pinfo[4 * b], ld = v.pinfo[4 * b + 1], npb = v.pinfo[4 * b + 2], pb0 = v.pinfo[4 * b + 3];
+        const double *Cg = v.pgeo + 6 * b;
+        const int nc = jit_ncart(lc), nd = jit_ncart(ld), ncd = nc * nd, Lcd = lc + ld, Lt = Lab + Lcd;
+        const int esb_d = (lc + 1) * (ld + 1) * (Lcd + 1), esb = 3 * esb_d;
+        const int rsz = jit_nherm(Lt) + Lt + 1;   // R^{(0)} and the scaled Boys values
+        const int ncomp = nab * ncd;
+        const int perq = esa + esb + rsz;
+        const JitTabs &tb = jit_tabs();
+        const unsigned long long m_comp = tb.magic[ncomp], m_cd = tb.magic[ncd], m_b = tb.magic[npb];
+        const short *eo_a = eo + 4 * tb.pcoff[la * 3 + lb], *eo_c = eo + 4 * tb.pcoff[lc * 3 + ld];
+        // ---- the pass: kets ge .. ge + K - 1 of this class
+        // (the small-LDS launch only: in the other the extra path costs spills)
+        int K = 1, sq = npa * npb;
+        if (LDSN == JIT_LDS_S && sq <= JIT_PASSQ)
+            while (ge + K < te1 && K < JIT_KMAX) {
+                const int bn = v.kl_pair[v.gent[ge + K]];
+                if (v.pinfo[4 * bn] != lc || v.pinfo[4 * bn + 1] != ld) break;
+                const int nq2 = npa * v.pinfo[4 * bn + 2];
+                if (sq + nq2 > JIT_PASSQ || (K + 1) * (ncomp + ncd) + (sq + nq2) * perq > LDSN) break;
+                sq += nq2;
+                K++;
             }
-            const int nq = npa * npb;
-            for (int q0 = 0; q0 < nq; q0 += bq) {
-                const int nqb = nq - q0 < bq ? nq - q0 : bq;
-                jit_sync();
-                for (int q = lane; q < nqb; q += NL) {   // E tables and R of one primitive quartet
-                    const int i = jit_div(q0 + q, m_b), j = q0 + q - i * npb;
-                    const double *pi = v.prim + 6 * (size_t)(pa0 + i), *pj = v.prim + 6 * (size_t)(pb0 + j);
-                    const double p = pi[0], qq = pj[0];
-                    double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
-                    for (int d = 0; d < 3; d++) {
-                        jit_hermite_e_any(la, lb, pi[1 + d] - Ag[d], pi[1 + d] - Ag[3 + d], pi[5], Ea + d * esa_d);
-                        jit_hermite_e_any(lc, ld, pj[1 + d] - Cg[d], pj[1 + d] - Cg[3 + d], pj[5], Eb + d * esb_d);
-                    }
-                    const double pq = p + qq, inv_pq = 1.0 / pq, alpha = p * qq * inv_pq;
-                    const double X = pi[1] - pj[1], Y = pi[2] - pj[2], Z = pi[3] - pj[3];
-                    const double T = alpha * (X * X + Y * Y + Z * Z);
-                    // 2 pi^{5/2} / (p q sqrt(p + q)) K_i K_j, with 1/p = 2 pi[5]
-                    const double pre = 2.0 * M_PI * M_PI * sqrt(M_PI) * (4.0 * pi[5] * pj[5]) * sqrt(inv_pq) * pi[4] * pj[4];
-                    double *Fs = R + nH;   // pre (-2 alpha)^n F_n
-                    jit_boys(v, Lt, T, Fs, 1);
-                    double s = pre;
-                    for (int n = 0; n <= Lt; n++) { Fs[n] *= s; s *= -2.0 * alpha; }
-                    // R^{(n)} from R^{(n+1)} in place, n = Lt-1 .. 0: orders
-                    // from the highest down, so that the orders k-1 and k-2
-                    // an order-k entry reads still hold R^{(n+1)}
-                    switch (Lt) {
-                    case 0: R[0] = Fs[0]; break;
-                    case 1: jit_rrec<1>(X, Y, Z, Fs, R); break;
-                    case 2: jit_rrec<2>(X, Y, Z, Fs, R); break;
-                    case 3: jit_rrec<3>(X, Y, Z, Fs, R); break;
-                    case 4: jit_rrec<4>(X, Y, Z, Fs, R); break;
-                    default: break;
-                    }
-                    const JitTabs &tb = jit_tabs();
-                    if (Lt > 4) R[0] = Fs[Lt];
-                    for (int n = Lt > 4 ? Lt - 1 : -1; n >= 0; n--) {
-                        for (int k = Lt - n; k >= 1; k--) {
-                            const int hb = k * (k + 1) * (k + 2) / 6, he = (k + 1) * (k + 2) * (k + 3) / 6;
-                            for (int h = hb; h < he; h++) {
-                                const int h1 = tb.rstep[h][0], h2 = tb.rstep[h][1], d = tb.rstep[h][3];
-                                const double xx = d == 0 ? X : (d == 1 ? Y : Z);
-                                R[h] = h2 >= 0 ? xx * R[h1] + (double)tb.rstep[h][2] * R[h2] : xx * R[h1];
-                            }
-                        }
-                        R[0] = Fs[n];
-                    }
+        if (K > 1) {
+            jit_sync();   // the previous pass's readers of pk are done
+            if (lane == 0) {
+                int q = 0;
+                for (int k = 0; k < K; k++) {
+                    const int ek = v.gent[ge + k], bk = v.kl_pair[ek], npk = v.pinfo[4 * bk + 2];
+                    pk[k] = bk;
+                    pk[S + k] = npk;
+                    pk[2 * S + k] = v.pinfo[4 * bk + 3];
+                    pk[3 * S + k] = v.kl_moff[ek];
+                    pk[4 * S + k] = q;
+                    pk[5 * S + k] = (long long)tb.magic[npk];
+                    q += npa * npk;
                 }
-                if constexpr (NL >= JIT_NAB)
-                    if (q0 == 0 && lane < ncd) vin[lane] = vreg;
-                jit_sync();
-                for (int it = lane; it < npart; it += NL) {   // components over the primitive quartets
-                    const int sl = jit_div(it, m_comp), cmp = it - sl * ncomp;
-                    const int ab = jit_div(cmp, m_cd), cd = cmp - ab * ncd;
-                    const short *ea = eo_a + 4 * ab, *ec = eo_c + 4 * cd;
-                    const int ox = ea[0], oy = ea[1], oz = ea[2], px = ec[0], py = ec[1], pz = ec[2];
-                    double sum = acc[it];
-                    for (int q = sl; q < nqb; q += nsl) {
-                        const double *Ea = qb + (size_t)q * perq, *Eb = Ea + esa, *R = Eb + esb;
-                        double val;
-                        if (Lab > TMAX || Lcd > TMAX || Lab + Lcd > 5)
-                            val = jit_comp_box(Ea, ox, oy, oz, ea[3] & 15, (ea[3] >> 4) & 15, ea[3] >> 8, Eb, px, py,
-                                               pz, ec[3] & 15, (ec[3] >> 4) & 15, ec[3] >> 8, R, hm);
-                        else if (Lab == 0) val = jit_comp_l<0, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
-                        else if (Lab == 1) val = jit_comp_l<1, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
-                        else if (Lab == 2) val = jit_comp_l<2, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
-                        else {
-                            if constexpr (TMAX >= 3) val = jit_comp_l<3, TMAX>(Lcd, Ea, ox, oy, oz, Eb, px, py, pz, R);
-                            else val = 0.0;
-                        }
-                        sum += val;
-                    }
-                    acc[it] = sum;
-                }
+                pk[4 * S + K] = q;
+            }
+            double *acc = lds, *vin = lds + K * ncomp, *qb = vin + K * ncd;
+            for (int c = lane; c < K * ncomp; c += NL) acc[c] = 0.0;
+            jit_sync();
+            for (int it = lane; it < K * ncd; it += NL) {   // every ket's input
+                const int k = jit_div(it, m_cd), c = it - k * ncd;
+                vin[it] = jit_in(v, v.kmap[pk[3 * S + k] + c]);
+            }
+            for (int qf = lane; qf < sq; qf += NL) {   // E and R of every primitive quartet of the pass
+                int k = 0;
+                while (k + 1 < K && qf >= (int)pk[4 * S + k + 1]) k++;
+                const int q = qf - (int)pk[4 * S + k];
+                const int i = jit_div(q, (unsigned long long)pk[5 * S + k]), j = q - i * (int)pk[S + k];
+                const int bk = (int)pk[k];
+                jit_prim(v, la, lb, lc, ld, esa_d, esb_d, v.prim + 6 * (size_t)(pa0 + i),
+                         v.prim + 6 * (size_t)(pk[2 * S + k] + j), Ag, v.pgeo + 6 * bk, qb + (size_t)qf * perq);
             }
             jit_sync();
-            if (nsl > 1) {   // slices in order; lane cmp alone reads and writes column cmp
-                for (int cmp = lane; cmp < ncomp; cmp += NL) {
-                    double t = acc[cmp];
-                    for (int sl = 1; sl < nsl; sl++) t += acc[sl * ncomp + cmp];
-                    acc[cmp] = t;
+            for (int it = lane; it < K * ncomp; it += NL) {   // (ket, component) over its primitive quartets
+                const int k = jit_div(it, m_comp), cmp = it - k * ncomp;
+                const int ab = jit_div(cmp, m_cd), cd = cmp - ab * ncd;
+                const short *ea = eo_a + 4 * ab, *ec = eo_c + 4 * cd;
+                double sum = 0.0;
+                for (int qf = (int)pk[4 * S + k]; qf < (int)pk[4 * S + k + 1]; qf++) {
+                    const double *Ea = qb + (size_t)qf * perq;
+                    sum += jit_cval<TMAX>(Lab, Lcd, Ea, Ea + esa, Ea + esa + esb, ea, ec, hm);
                 }
-                jit_sync();
+                acc[it] = sum;
             }
-            for (int k = 0; k < RPL; k++) {   // contract the quartet with the partner's input
+            jit_sync();
+            for (int k = 0; k < RPL; k++) {   // contract with every ket's input, kets in order
                 const int r = lane + k * NL;
-                if (r < nab) {
-                    double s = 0.0;
-                    for (int c = 0; c < ncd; c++) s += acc[r * ncd + c] * vin[c];
-                    racc[k] += s;
+                if (r < nab)
+                    for (int kk = 0; kk < K; kk++) {
+                        double s = 0.0;
+                        for (int c = 0; c < ncd; c++) s += acc[kk * ncomp + r * ncd + c] * vin[kk * ncd + c];
+                        racc[k] += s;
+                    }
+            }
+            jit_sync();
+            ge += K;
+            continue;
+        }
+        // ---- one ket
+        const int nsl = ncomp >= NL ? 1 : jit_div(NL, m_comp), npart = nsl * ncomp;
+        const int preg = ncomp > 64 ? ncomp : 64;   // partial-sum region (npart <= preg)
+        double *acc = lds, *vin = lds + preg, *qb = vin + ncd;
+        int bq = jit_div(LDSN - preg - ncd, tb.magic[perq]);
+        if (bq > NL) bq = NL;
+        for (int c = lane; c < npart; c += NL) acc[c] = 0.0;
+        // the partner's input: on the device one value per lane, held in a
+        // register and stored after the first batch's E/R work, so the
+        // global load's latency hides behind it
+        double vreg = 0.0;
+        if constexpr (NL >= JIT_NAB) {
+            if (lane < ncd) vreg = jit_in(v, km[lane]);
+        } else {
+            for (int c = lane; c < ncd; c += NL) vin[c] = jit_in(v, km[c]);
+        }
+        const int nq = npa * npb;
+        for (int q0 = 0; q0 < nq; q0 += bq) {
+            const int nqb = nq - q0 < bq ? nq - q0 : bq;
+            jit_sync();
+            for (int q = lane; q < nqb; q += NL) {   // E tables and R of one primitive quartet
+                const int i = jit_div(q0 + q, m_b), j = q0 + q - i * npb;
+                jit_prim(v, la, lb, lc, ld, esa_d, esb_d, v.prim + 6 * (size_t)(pa0 + i), v.prim + 6 * (size_t)(pb0 + j),
+                         Ag, Cg, qb + (size_t)q * perq);
+            }
+            if constexpr (NL >= JIT_NAB)
+                if (q0 == 0 && lane < ncd) vin[lane] = vreg;
+            jit_sync();
+            for (int it = lane; it < npart; it += NL) {   // components over the primitive quartets
+                const int sl = jit_div(it, m_comp), cmp = it - sl * ncomp;
+                const int ab = jit_div(cmp, m_cd), cd = cmp - ab * ncd;
+                const short *ea = eo_a + 4 * ab, *ec = eo_c + 4 * cd;
+                double sum = acc[it];
+                for (int q = sl; q < nqb; q += nsl) {
+                    const double *Ea = qb + (size_t)q * perq;
+                    sum += jit_cval<TMAX>(Lab, Lcd, Ea, Ea + esa, Ea + esa + esb, ea, ec, hm);
                 }
+                acc[it] = sum;
+            }
+        }
+        jit_sync();
+        if (nsl > 1) {   // slices in order; lane cmp alone reads and writes column cmp
+            for (int cmp = lane; cmp < ncomp; cmp += NL) {
+                double t = acc[cmp];
+                for (int sl = 1; sl < nsl; sl++) t += acc[sl * ncomp + cmp];
+                acc[cmp] = t;
             }
             jit_sync();
         }
+        for (int k = 0; k < RPL; k++) {   // contract the quartet with the partner's input
+            const int r = lane + k * NL;
+            if (r < nab) {
+                double s = 0.0;
+                for (int c = 0; c < ncd; c++) s += acc[r * ncd + c] * vin[c];
+                racc[k] += s;
+            }
+        }
+        jit_sync();
+        ge++;
     }
     for (int k = 0; k < RPL; k++) {
         const int r = lane + k * NL;
@@ -550,12 +652,13 @@ __global__ void __launch_bounds__(64, LDSN == JIT_LDS_S ? 3 : 2) jit_task_kernel
     __shared__ double lds[LDSN];
     __shared__ short hm[729];   // jit_herm by (t, u, w), for lane-dependent lookups
     __shared__ short eo[400];   // E-row offsets by pair class and Cartesian pair
+    __shared__ long long pk[6 * (JIT_KMAX + 1)];   // per-ket data of a multi-ket pass
     const int task = t0 + blockIdx.x;
     if (task >= t1) return;
     for (int k = threadIdx.x; k < 729; k += 64) hm[k] = (&jit_tabs_dev.herm[0][0][0])[k];
     for (int k = threadIdx.x; k < 400; k += 64) eo[k] = (&jit_tabs_dev.eoff[0][0])[k];
     __syncthreads();
-    jit_task<64, LDSN, LDSN == JIT_LDS_S ? 2 : 3>(v, task, threadIdx.x, lds, hm, eo);
+    jit_task<64, LDSN, LDSN == JIT_LDS_S ? 2 : 3>(v, task, threadIdx.x, lds, hm, eo, pk);
 }
 
 // Up sweep of one level: y0[n] = U_n^T in_n for the listed nodes, where the
@@ -829,12 +932,18 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
     // ket pairs of total order 3 or 4 last in each side's entries: tasks are
     // cut where they start and run in the large-LDS launch
     auto high_ket = [&](int e) { return J.pinfo[4 * J.kl_pair[e]] + J.pinfo[4 * J.kl_pair[e] + 1] >= 3; };
+    // and within each part grouped by pair class (la, lb), so that runs of
+    // one class form multi-ket passes
     auto open_side = [&](const std::vector<int> &lists) {
         side_e0 = (int)J.gent.size();
         for (int pass = 0; pass < 2; pass++)
-            for (int L : lists)
-                for (int e = J.kl_start[L]; e < J.kl_start[L] + J.kl_count[L]; e++)
-                    if (high_ket(e) == (pass == 1)) J.gent.push_back(e);
+            for (int cls = 0; cls < 9; cls++)
+                for (int L : lists)
+                    for (int e = J.kl_start[L]; e < J.kl_start[L] + J.kl_count[L]; e++) {
+                        const int bb = J.kl_pair[e];
+                        if (high_ket(e) == (pass == 1) && J.pinfo[4 * bb] * 3 + J.pinfo[4 * bb + 1] == cls)
+                            J.gent.push_back(e);
+                    }
         side_e1 = (int)J.gent.size();
     };
     auto add_tasks = [&](const std::vector<int> &, int a, const std::vector<int> &om) {
@@ -1215,10 +1324,11 @@ int H2ERI_jit_emulate(H2ERI_p h, const double *x, double *y) {
     std::atomic<int> next(0);
     auto worker = [&]() {
         std::vector<double> lds(JIT_LDS);
+        std::vector<long long> pk(6 * (JIT_KMAX + 1));
         for (int t = next++; t < ntask; t = next++) {
             const short *hm = &jit_tabs_host.herm[0][0][0], *eo = &jit_tabs_host.eoff[0][0];
-            if (t < P.n_big) jit_task<1, JIT_LDS, 3>(v, t, 0, lds.data(), hm, eo);
-            else jit_task<1, JIT_LDS_S, 2>(v, t, 0, lds.data(), hm, eo);
+            if (t < P.n_big) jit_task<1, JIT_LDS, 3>(v, t, 0, lds.data(), hm, eo, pk.data());
+            else jit_task<1, JIT_LDS_S, 2>(v, t, 0, lds.data(), hm, eo, pk.data());
         }
     };
     const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
