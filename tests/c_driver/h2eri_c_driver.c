@@ -15,8 +15,9 @@
  * src/H2ERI_build_H2.h:16, src/H2ERI_build_Coulomb.h:16,
  * src/H2ERI_build_exchange.h:16 of H2P-ERI).
  *
- * usage: h2eri_c_driver MOL D_BIN reltol J|K REF_BIN [ncall]
- * D_BIN and REF_BIN hold num_bf x num_bf float64 (row-major, raw).  Prints
+ * usage: h2eri_c_driver MOL D_BIN reltol J|K REF_BIN [ncall [BD_JIT]]
+ * D_BIN and REF_BIN hold num_bf x num_bf float64 (row-major, raw); BD_JIT
+ * (default 0) is passed to H2ERI_build_H2.  Prints
  * one line "result=J|K nbf=... relerr=... n_matvec=... ms_per_call=..." and
  * exits 0, or prints the library's error and exits 1.
  */
@@ -54,12 +55,13 @@ static double now_ms(void) {
 
 int main(int argc, char **argv) {
     if (argc < 6 || (strcmp(argv[4], "J") != 0 && strcmp(argv[4], "K") != 0)) {
-        fprintf(stderr, "usage: %s MOL D_BIN reltol J|K REF_BIN [ncall]\n", argv[0]);
+        fprintf(stderr, "usage: %s MOL D_BIN reltol J|K REF_BIN [ncall [BD_JIT]]\n", argv[0]);
         return 2;
     }
     const double reltol = atof(argv[3]);
     const int want_j = strcmp(argv[4], "J") == 0;
     const int ncall = argc > 6 ? atoi(argv[6]) : 3;
+    const int bd_jit = argc > 7 ? atoi(argv[7]) : 0;
 
     simint_init();
     H2ERI_p h2eri;
@@ -68,7 +70,7 @@ int main(int argc, char **argv) {
     if (check("CMS_read_mol_file")) return 1;
     H2ERI_process_shells(h2eri);
     H2ERI_partition(h2eri);
-    H2ERI_build_H2(h2eri, 0);
+    H2ERI_build_H2(h2eri, bd_jit);
     if (check("construction")) return 1;
 
     const int nbf = h2eri->num_bf;

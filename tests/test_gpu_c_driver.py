@@ -7,7 +7,9 @@ the reference's fixtures:
 
 * J of D_alk32 for alk32_ccpvdz.mol against J_alk32 (bar 5e-7, as the CPU
   fixture test), n_matvec counting the calls;
-* K of D_alk26 for alk26_ccpvdz.mol against K_alk26 (bar 5e-6).
+* K of D_alk26 for alk26_ccpvdz.mol against K_alk26 (bar 5e-6);
+* J of D_alk32 with H2ERI_build_H2(h2eri, 1): B and D evaluated on the
+  device in every call (bar 5e-7), print_statistic reporting the mode.
 """
 import os
 import re
@@ -52,3 +54,11 @@ def test_c_caller_exchange_alk26():
                                   "1e-6", "K", os.path.join(GOLDEN, "K_alk26_ccpvdz.bin"), "2")
     assert kind == "K" and nbf == 210 and nmv == 2
     assert err < 5e-6
+
+
+def test_c_caller_coulomb_alk32_bd_jit():
+    kind, nbf, err, nmv, out = _run(os.path.join(GOLDEN, "alk32_ccpvdz.mol"), os.path.join(GOLDEN, "D_alk32_ccpvdz.bin"),
+                                    "1e-6", "J", os.path.join(GOLDEN, "J_alk32_ccpvdz.bin"), "2", "1")
+    assert kind == "J" and nbf == 260 and nmv == 2
+    assert err < 5e-7
+    assert "Yes (B & D not allocated)" in out
