@@ -2569,7 +2569,24 @@ static void collect_phase_ms(H2ERI_p h, DevState *d) {
 
 // Batched device matvec: nvec vectors, processed in passes of 16, 8, 4, 2, 1.
 static bool matvec_multi_dev_impl(H2ERI_p h, int nvec, const double *X, double *Y, hipStream_t s) {
-    if (h->BD_JIT) return upload_storage(h) && jit_matvec(h, dstate(h)->jit, nvec, X, Y, s);
+    if (h->BD_JIT) {   // timed like the stored form: the reference's phase timer slots
+        if (!upload_storage(h)) return false;
+        DevState *d = dstate(h);
+        const bool tim = d->phase_timing && !d->capturing;
+        double ms5[5] = {0, 0, 0, 0, 0};
+        if (!jit_matvec(h, d->jit, nvec, X, Y, s, tim ? ms5 : nullptr)) return false;
+        if (tim) {
+            h->timers[MV_FWD_TIMER_IDX] += ms5[0] * 1e-3;
+            h->timers[MV_MID_TIMER_IDX] += ms5[1] * 1e-3;
+            h->timers[MV_BWD_TIMER_IDX] += ms5[2] * 1e-3;
+            h->timers[MV_DEN_TIMER_IDX] += ms5[3] * 1e-3;
+            h->timers[MV_VOP_TIMER_IDX] += ms5[4] * 1e-3;
+            const int ph[5] = {PH_UP, PH_B, PH_DOWN, PH_D, PH_GATHER};
+            for (int p = 0; p < 7; p++) d->phase_ms[p] = 0.0;
+            for (int i = 0; i < 5; i++) { d->phase_ms[ph[i]] = ms5[i]; d->phase_ms[6] += ms5[i]; }
+        }
+        return true;
+    }
     const size_t N = (size_t)h->krnl_mat_size;
     int done = 0;
     while (done < nvec) {

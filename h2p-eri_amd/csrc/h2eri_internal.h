@@ -114,8 +114,10 @@ struct JitState;
 bool jit_upload(H2ERI_p h, JitState *&js);   // builds the task lists and uploads them
 void jit_release(JitState *&js);
 size_t jit_resident_bytes(const JitState *js);
-// y = A x for nvec vectors of length krnl_mat_size (device pointers) on stream s
-bool jit_matvec(H2ERI_p h, JitState *js, int nvec, const double *X, double *Y, void *stream);
+// y = A x for nvec vectors of length krnl_mat_size (device pointers) on stream s;
+// with ms5, waits for the work and adds its milliseconds per phase: up sweep,
+// coupling (B) tasks, down sweep, near-field (D) tasks, vector work
+bool jit_matvec(H2ERI_p h, JitState *js, int nvec, const double *X, double *Y, void *stream, double *ms5);
 // exchange layer (exchange.hip): device state held in h->thread_Kmat_workbuf
 void exchange_release(H2ERI_p h);
 bool exchange_prepare(H2ERI_p h);   // plan, upload and basis products (allocates)

@@ -739,6 +739,7 @@ struct JitPlan {
     std::vector<std::vector<int>> level_nodes;   // nodes with a non-empty U, per level
     // work per matvec
     double quartets = 0, prim_quartets = 0, eri_values = 0;
+    double cost_b = 0, cost_d = 0;   // planned task cost of coupling and near-field entries
     int n_big = 0;   // tasks [0, n_big) have a bra or ket pair with la + lb >= 3
     const double *boys = nullptr;
     int bgrid = 0, bcol = 0;
@@ -929,21 +930,26 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
                    cls[J.pinfo[4 * a]][J.pinfo[4 * a + 1]][J.pinfo[4 * b]][J.pinfo[4 * b + 1]] + 40.0;
     };
     int side_e0 = 0, side_e1 = 0;
+    std::vector<char> gent_d;   // per gent entry: from a near-field (D) list
     // ket pairs of total order 3 or 4 last in each side's entries: tasks are
     // cut where they start and run in the large-LDS launch
     auto high_ket = [&](int e) { return J.pinfo[4 * J.kl_pair[e]] + J.pinfo[4 * J.kl_pair[e] + 1] >= 3; };
     // and within each part grouped by pair class (la, lb), so that runs of
     // one class form multi-ket passes
-    auto open_side = [&](const std::vector<int> &lists) {
+    auto open_side = [&](const std::vector<int> &lists, int n_near) {   // the first n_near lists are near field
         side_e0 = (int)J.gent.size();
         for (int pass = 0; pass < 2; pass++)
             for (int cls = 0; cls < 9; cls++)
-                for (int L : lists)
+                for (size_t li = 0; li < lists.size(); li++) {
+                    const int L = lists[li];
                     for (int e = J.kl_start[L]; e < J.kl_start[L] + J.kl_count[L]; e++) {
                         const int bb = J.kl_pair[e];
-                        if (high_ket(e) == (pass == 1) && J.pinfo[4 * bb] * 3 + J.pinfo[4 * bb + 1] == cls)
+                        if (high_ket(e) == (pass == 1) && J.pinfo[4 * bb] * 3 + J.pinfo[4 * bb + 1] == cls) {
                             J.gent.push_back(e);
+                            gent_d.push_back((int)li < n_near ? 1 : 0);
+                        }
                     }
+                }
         side_e1 = (int)J.gent.size();
     };
     auto add_tasks = [&](const std::vector<int> &, int a, const std::vector<int> &om) {
@@ -968,9 +974,10 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
                 if (!is_leaf(m)) { set_error(ERR_ARG, "on-the-fly B/D: near-field partner %d is not a leaf", m); return false; }
                 lists.push_back(2 * m);
             }
+            const int n_near = (int)lists.size();
             for (int m : adm[n])
                 if (lv(n) < lv(m)) lists.push_back(2 * m + 1);
-            open_side(lists);
+            open_side(lists, n_near);
             if (!lists.empty())
                 for (int a = h->pt_cluster[2 * n]; a <= h->pt_cluster[2 * n + 1]; a++) {
                     std::vector<int> om(h->sp_nbfp[a]);
@@ -988,7 +995,7 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
             lists.push_back(lv(m) >= lv(n) ? 2 * m + 1 : 2 * m);
         }
         if (lists.empty()) continue;
-        open_side(lists);
+        open_side(lists, 0);
         const H2E_int_vec_p jp = h->J_pair[n];
         if (!jp || rank(n) == 0) { set_error(ERR_ARG, "on-the-fly B/D: node %d has coupling blocks and no skeleton", n); return false; }
         int rr = 0;
@@ -1009,7 +1016,11 @@ static bool jit_plan(H2ERI_p h, JitPlan &J) {
     // a full near field is the longest); cap = total / 32768
     double total = 0;
     for (const GroupRec &g : groups)
-        for (int ge = g.e0; ge < g.e1; ge++) total += qcost(g.pair, J.kl_pair[J.gent[ge]]);
+        for (int ge = g.e0; ge < g.e1; ge++) {
+            const double c = qcost(g.pair, J.kl_pair[J.gent[ge]]);
+            total += c;
+            (gent_d[ge] ? J.cost_d : J.cost_b) += c;
+        }
     const double cap = std::max(total / 32768.0, 256.0);
     struct TaskRec { int pair, e0, e1, group; double cost; bool high; };
     std::vector<TaskRec> tasks;
@@ -1128,6 +1139,7 @@ struct JitState {
     std::vector<const int *> d_level;   // node lists per level
     std::vector<int> level_off;
     size_t bytes = 0;   // device memory held
+    std::vector<hipEvent_t> ev;   // phase timing: 6 per vector
 };
 
 #define JIT_HIP(expr)                                                                                       \
@@ -1141,6 +1153,7 @@ struct JitState {
 
 void jit_release(JitState *&js) {
     if (!js) return;
+    for (hipEvent_t e : js->ev) (void)hipEventDestroy(e);
     void *ps[] = {js->d_ints, js->d_dbls, js->d_uoff, js->Y0, js->Y1, js->YJ, js->YD, js->part};
     for (void *p : ps)
         if (p) (void)hipFree(p);
@@ -1232,18 +1245,30 @@ bool jit_upload(H2ERI_p h, JitState *&js) {
     return true;
 }
 
-bool jit_matvec(H2ERI_p h, JitState *js, int nvec, const double *X, double *Y, void *stream) {
+bool jit_matvec(H2ERI_p h, JitState *js, int nvec, const double *X, double *Y, void *stream, double *ms5) {
     if (!js || !js->ready) { set_error(ERR_ARG, "on-the-fly B/D: not uploaded"); return false; }
     hipStream_t s = (hipStream_t)stream;
     const JitPlan &P = js->P;
     const int N = P.N;
+    if (ms5)
+        while (js->ev.size() < 6 * (size_t)nvec) {
+            hipEvent_t e;
+            JIT_HIP(hipEventCreate(&e));
+            js->ev.push_back(e);
+        }
+    auto mark = [&](int k, int i) -> bool {
+        if (ms5) JIT_HIP(hipEventRecord(js->ev[6 * k + i], s));
+        return true;
+    };
     for (int k = 0; k < nvec; k++) {
         const double *x = X + (size_t)k * N;
         double *y = Y + (size_t)k * N;
+        if (!mark(k, 0)) return false;
         hipLaunchKernelGGL(jit_zero_kernel, dim3(1024), dim3(256), 0, s, js->Y1, (long long)std::max(P.nY, 1));
         hipLaunchKernelGGL(jit_zero_kernel, dim3(1024), dim3(256), 0, s, js->YJ, (long long)std::max(N, 1));
         hipLaunchKernelGGL(jit_zero_kernel, dim3(1024), dim3(256), 0, s, js->YD, (long long)std::max(N, 1));
         JIT_HIP(hipGetLastError());
+        if (!mark(k, 1)) return false;
         for (int L = P.max_level; L >= 0; L--) {
             const int nn = (int)P.level_nodes[L].size();
             if (nn == 0) continue;
@@ -1251,6 +1276,7 @@ bool jit_matvec(H2ERI_p h, JitState *js, int nvec, const double *X, double *Y, v
                                js->d_unc, js->d_uin0, js->d_y0off, js->d_uin, js->d_U, x, js->Y0, N);
             JIT_HIP(hipGetLastError());
         }
+        if (!mark(k, 2)) return false;
         JitView v = js->view;
         v.X = x;
         const int ntask = (int)P.t_pair.size();
@@ -1267,6 +1293,7 @@ bool jit_matvec(H2ERI_p h, JitState *js, int nvec, const double *X, double *Y, v
             hipLaunchKernelGGL(jit_reduce_kernel, dim3((ngroup + 255) / 256), dim3(256), 0, s, v, ngroup);
             JIT_HIP(hipGetLastError());
         }
+        if (!mark(k, 3)) return false;
         for (int L = 0; L <= P.max_level; L++) {
             const int nn = (int)P.level_nodes[L].size();
             if (nn == 0) continue;
@@ -1274,8 +1301,23 @@ bool jit_matvec(H2ERI_p h, JitState *js, int nvec, const double *X, double *Y, v
                                js->d_unc, js->d_uin0, js->d_y0off, js->d_uin, js->d_U, js->YD, js->Y1, N);
             JIT_HIP(hipGetLastError());
         }
+        if (!mark(k, 4)) return false;
         hipLaunchKernelGGL(jit_add_kernel, dim3((N + 255) / 256), dim3(256), 0, s, js->YD, js->YJ, y, N);
         JIT_HIP(hipGetLastError());
+        if (!mark(k, 5)) return false;
+    }
+    if (ms5) {   // per phase: up sweep, B and D tasks (split by planned cost), down sweep, vector work
+        JIT_HIP(hipEventSynchronize(js->ev[6 * (size_t)nvec - 1]));
+        const double fb = P.cost_b + P.cost_d > 0 ? P.cost_b / (P.cost_b + P.cost_d) : 0.0;
+        for (int k = 0; k < nvec; k++) {
+            float t[5];
+            for (int i = 0; i < 5; i++) JIT_HIP(hipEventElapsedTime(&t[i], js->ev[6 * k + i], js->ev[6 * k + i + 1]));
+            ms5[0] += t[1];
+            ms5[1] += t[2] * fb;
+            ms5[2] += t[3];
+            ms5[3] += t[2] * (1.0 - fb);
+            ms5[4] += t[0] + t[4];
+        }
     }
     (void)h;
     return true;

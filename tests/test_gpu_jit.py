@@ -123,3 +123,16 @@ def test_jit_c30h62_full_size():
     hs.build_from_mol(text=txt)
     e = relerr(hs.matvec(u), Au)
     assert 0.0 < e < 1e-5, e
+
+
+def test_jit_phase_timers(water6):
+    """A host-pointer matvec of a BD_JIT handle fills the reference's phase
+    timer slots (src/H2ERI_matvec.c:890-954): forward, intermediate,
+    backward and dense (the task time split by planned B and D cost), and
+    the vector operations."""
+    hd, hj, p = water6
+    t0 = [hj.s.timers[i] for i in range(4, 9)]
+    hj.matvec(np.random.default_rng(25).standard_normal(hj.n))
+    t1 = [hj.s.timers[i] for i in range(4, 9)]
+    d = [b - a for a, b in zip(t0, t1)]
+    assert d[0] > 0 and d[1] > 0 and d[2] > 0 and d[3] > 0 and d[4] > 0, d
