@@ -3392,6 +3392,10 @@ int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double 
         set_error(ERR_ARG, "plan_emulate: bad args");
         return -1;
     }
+    if (h->BD_JIT) {   // no stored blocks to plan (H2ERI_jit_emulate runs that form)
+        set_error(ERR_UNSUPPORTED, "plan_emulate: a BD_JIT handle has no stored blocks");
+        return -1;
+    }
     Storage S;
     S.build(h, ub_bits == 32);
     Plan P;
@@ -3508,8 +3512,8 @@ int H2ERI_plan_emulate(H2ERI_p h, const int ub_bits, const int nv, const double 
 // Returns that count, or -1 on failure.
 int H2ERI_queue_emulate(H2ERI_p h, const int ub_bits, const int nv, const int workgroups, const double *X, double *Y) {
     if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !X || !Y || !nv_valid(nv) ||
-        workgroups < 1) {
-        set_error(ERR_ARG, "queue_emulate: bad args");
+        workgroups < 1 || h->BD_JIT) {
+        set_error(ERR_ARG, "queue_emulate: bad args (or a BD_JIT handle, which has no stored blocks)");
         return -1;
     }
     Storage S;
@@ -3622,6 +3626,7 @@ int H2ERI_queue_emulate(H2ERI_p h, const int ub_bits, const int nv, const int wo
 // reduce tasks per vector, external output elements}.  0, -1 on failure.
 int H2ERI_plan_stats(H2ERI_p h, const int ub_bits, const int nv, double *out8) {
     if (!h || !h->U || !h->c_B_blks || !h->c_D_blks || !out8 || !nv_valid(nv)) { set_error(ERR_ARG, "plan_stats: bad argument"); return -1; }
+    if (h->BD_JIT) { set_error(ERR_UNSUPPORTED, "plan_stats: a BD_JIT handle has no stored blocks (H2ERI_jit_stats)"); return -1; }
     Storage S;
     S.build(h, ub_bits == 32);
     DevState *ds = dstate_get(h);
@@ -3684,6 +3689,7 @@ int H2ERI_plan_stats(H2ERI_p h, const int ub_bits, const int nv, double *out8) {
 
 int H2ERI_plan_check(H2ERI_p h, const int ub_bits) {
     if (!h || !h->U || !h->c_B_blks || !h->c_D_blks) { set_error(ERR_ARG, "plan_check: nothing built"); return -1; }
+    if (h->BD_JIT) { set_error(ERR_UNSUPPORTED, "plan_check: a BD_JIT handle has no stored blocks"); return -1; }
     Storage S;
     S.build(h, ub_bits == 32);
     DevState *ds = dstate_get(h);

@@ -100,6 +100,13 @@ def test_jit_representation_stores_no_blocks(water6, tmp_path):
         hj.save(str(tmp_path / "jit.rep"))
     st = hj.jit_stats()
     assert st["tasks"] > 0 and st["quartets"] > 0 and st["eri_values"] >= st["quartets"]
+    # the stored-block plan helpers refuse it; the dense-stored handle has no on-the-fly plan
+    with pytest.raises(h2eri.H2ERIError, match="BD_JIT"):
+        hj.plan_check()
+    with pytest.raises(h2eri.H2ERIError, match="BD_JIT"):
+        hj.plan_emulate(np.zeros(hj.n))
+    with pytest.raises(h2eri.H2ERIError):
+        hd.jit_stats()
 
 
 def test_dense_blocks_option(water6):
