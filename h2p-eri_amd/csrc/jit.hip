@@ -59,7 +59,7 @@ static const int JIT_LMAX = 2;     // shell angular momentum (d)
 static const int JIT_LDS = 2048;   // LDS doubles of a task with a bra or ket pair of la + lb >= 3
 static const int JIT_LDS_S = 1280; // LDS doubles of a task whose pairs all have la + lb <= 2
 static const int JIT_NAB = 36;     // Cartesian rows of a shell pair, at most
-static const int JIT_KMAX = 8;     // kets per pass, at most
+static const int JIT_KMAX = 4;     // kets per pass, at most (8 and 16: 1% slower)
 static const int JIT_PASSQ = 64;   // primitive quartets of a multi-ket pass, at most
 
 // Everything a row task reads or writes; the pointers are device or, in the
