@@ -28,11 +28,12 @@
 //    Coulomb integrals R from R^{(n)}_{000} = (-2 alpha)^n F_n(alpha |PQ|^2)
 //    and R^{(n)}_{t+1,u,v} = t R^{(n+1)}_{t-1,u,v} + X_PQ R^{(n+1)}_{t,u,v},
 //    and the Boys function of the host engine (the same table).  One wave
-//    evaluates one quartet: lanes over primitive quartets for the E and R
-//    tables (in LDS), then lanes over the Cartesian components, each summing
-//    its component over the primitive quartets in a fixed order.  Shells up
-//    to d (cc-pVDZ); the primitive pairs, centres and conventions are the
-//    host engine's (first shell of a pair = N, row index n * nbf(M) + m,
+//    evaluates one quartet, or a few kets of one class against its bra pair
+//    (jit_task): lanes over primitive quartets for the E and R tables (in
+//    LDS), then lanes over the Cartesian components, each summing its
+//    component over the primitive quartets in a fixed order.  Shells up to
+//    d (cc-pVDZ); the primitive pairs, centres and conventions are the host
+//    engine's (first shell of a pair = N, row index n * nbf(M) + m,
 //    lexicographic Cartesians).
 //  * Sweeps: the up sweep (y0 = U^T x, children first) and the down sweep
 //    (parents first) are one launch per tree level over the nodes of that
